@@ -1,0 +1,157 @@
+/*
+ * psgpu.h -- C ABI of the MI355X (gfx950) raster hot path for Puresoft3D.
+ *
+ * Every entry point replaces one member of the reference's C++ pipeline API
+ * (src/puresoft3d/pipeline.h:30-66 and the plugin types in proc.h, vbo.h,
+ * vao.h, fbo.h).  The reference throws C++ exceptions; here every call
+ * returns PSGPU_OK (0) or a negative status, and psgpu_last_error() gives
+ * the message.  Handles are small integers, as in the reference's pools.
+ *
+ * Ordering: calls on one context are stream-ordered.  Draws are queued and
+ * executed in submission order when the queue is flushed (explicitly by
+ * psgpu_flush/psgpu_sync, or implicitly before any call that reads or
+ * replaces a buffer the queued draws use).  Reads (psgpu_read_*) return
+ * results identical to the reference's synchronous drawVAO (drawvao.cpp:3-133).
+ * A context is used by one host thread at a time.
+ */
+#ifndef PSGPU_H
+#define PSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PSGPU_OK                 0
+#define PSGPU_ERR_INVALID       -1   /* std::invalid_argument in the reference */
+#define PSGPU_ERR_RANGE         -2   /* std::out_of_range                      */
+#define PSGPU_ERR_NOMEM         -3   /* std::bad_alloc                         */
+#define PSGPU_ERR_DEVICE        -4   /* HIP runtime error                      */
+#define PSGPU_ERR_STATE         -5   /* call not valid in the current state    */
+#define PSGPU_ERR_UNSUPPORTED   -6   /* processor without a GPU twin, etc.     */
+
+/* ---- behaviour bits (pipeline.h:17-20) ---- */
+#define PSGPU_UPDATE_DEPTH  0x1
+#define PSGPU_TEST_DEPTH    0x2
+#define PSGPU_FACE_CULLING  0x4
+#define PSGPU_ALPHABLEND    0x8
+
+/* ---- processor kinds with a GPU twin (one VP/IP/FP triple each) ---- */
+#define PSGPU_PROC_DEF01    1   /* tex1light1.cpp      */
+#define PSGPU_PROC_DEF02    2   /* colr1light1.cpp     */
+#define PSGPU_PROC_DEF03    3   /* tex1bump1light1.cpp */
+#define PSGPU_PROC_DEF04    4   /* skybox.cpp          */
+#define PSGPU_PROC_DEF05    5   /* shadow.cpp          */
+#define PSGPU_PROC_LAMBERT  6   /* Lambert x bilinear texture, directional light (slot 7 = light dir, 9 = tex) */
+#define PSGPU_PROC_BLINN    7   /* Blinn-Phong + normal map, trilinear (7 light, 8 camera, 9 albedo, 10 normal map, 11 exponent) */
+
+/* ---- wrap modes (fbo.h:18) ---- */
+#define PSGPU_CLAMP 0
+#define PSGPU_WRAP  1
+
+/* ---- render-target selectors ---- */
+#define PSGPU_DISPLAY  (-1)     /* setFBO(0, NULL): the display buffer (pipeline.cpp:267-270) */
+#define PSGPU_DEFAULT_DEPTH (-1) /* setDepth(-1) (pipeline.cpp:218-223)                       */
+
+typedef struct psgpu_ctx psgpu_ctx;
+
+typedef struct {
+    int64_t tris_in, tris_culled, tris_zrejected, tris_dropped;
+    int64_t spans, frags_rasterised, frags_shaded;
+    int64_t segments;           /* span pieces after splitting at bin-chunk borders */
+    int64_t flushes, retries;   /* pipeline executions, capacity-growth re-runs      */
+} psgpu_stats;
+
+/* tile configuration of the raster kernel; searched by the autotuner */
+typedef struct {
+    int chunk_width;    /* pixels per lane along a row: 4, 8, 16, 32 or 64 */
+    int tile_rows;      /* rows per wave tile (tile_rows * lanes-per-row = 64): 1..64, power of two */
+    int waves_per_group;/* wavefronts per workgroup of the raster kernel: 1..8 */
+} psgpu_tiling;
+
+/* ---- context: PuresoftPipeline ctor/dtor (pipeline.cpp:24-116) ---- */
+int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** out);
+int psgpu_destroy(psgpu_ctx* ctx);
+const char* psgpu_last_error(psgpu_ctx* ctx);
+/* the HIP stream the context runs on (hipStream_t), for callers that share it */
+void* psgpu_stream(psgpu_ctx* ctx);
+
+/* ---- textures: createTexture / destroyTexture / getTexture (tex.cpp:4-59) ---- */
+/* pixels may be NULL (contents then undefined, as with image->pixels == NULL).
+ * mip_levels >= 1; levels beyond 0 are filled by psgpu_texture_generate_mips. */
+int psgpu_texture_create(psgpu_ctx* ctx, int width, int scanline, int height, int elem_len,
+                         int extra_layers, int wrap_mode, int mip_levels, const void* pixels, int* out_idx);
+int psgpu_texture_upload(psgpu_ctx* ctx, int tex, int layer, int level, const void* pixels, size_t bytes);
+int psgpu_texture_download(psgpu_ctx* ctx, int tex, int layer, int level, void* pixels, size_t bytes);
+int psgpu_texture_generate_mips(psgpu_ctx* ctx, int tex);
+int psgpu_texture_destroy(psgpu_ctx* ctx, int tex);
+/* width, scanline, height, elem_len of one layer/level (getTexture) */
+int psgpu_texture_info(psgpu_ctx* ctx, int tex, int level, int* width, int* scanline, int* height, int* elem_len);
+
+/* ---- vertex streams: PuresoftVBO / PuresoftVAO (vbo.cpp, vao.cpp, pipeline.cpp:118-205) ---- */
+int psgpu_vbo_create(psgpu_ctx* ctx, size_t unit_bytes, size_t unit_count, int* out_vbo);
+int psgpu_vbo_update(psgpu_ctx* ctx, int vbo, const void* src);            /* updateContent */
+int psgpu_vbo_destroy(psgpu_ctx* ctx, int vbo);
+int psgpu_vao_create(psgpu_ctx* ctx, int* out_vao);
+int psgpu_vao_attach(psgpu_ctx* ctx, int vao, int slot, int vbo, int* out_previous);  /* -1: none */
+int psgpu_vao_detach(psgpu_ctx* ctx, int vao, int slot, int* out_previous);
+int psgpu_vao_get(psgpu_ctx* ctx, int vao, int slot, int* out_vbo);
+int psgpu_vao_destroy(psgpu_ctx* ctx, int vao);   /* also destroys attached VBOs (pipeline.cpp:183-205) */
+
+/* ---- programmes: createProgramme / useProgramme (prog.cpp:43-132) ---- */
+int psgpu_program_create(psgpu_ctx* ctx, int vp_kind, int ip_kind, int fp_kind, int* out_prog);
+int psgpu_program_destroy(psgpu_ctx* ctx, int prog);
+int psgpu_program_use(psgpu_ctx* ctx, int prog);
+
+/* ---- state (pipeline.cpp:207-343) ---- */
+int psgpu_uniform_set(psgpu_ctx* ctx, int slot, const void* data, size_t len);   /* NULL clears */
+int psgpu_behavior_enable(psgpu_ctx* ctx, int bits);
+int psgpu_behavior_disable(psgpu_ctx* ctx, int bits);
+int psgpu_behavior_get(psgpu_ctx* ctx, int* bits);
+int psgpu_viewport(psgpu_ctx* ctx, int width, int height);
+int psgpu_target_set(psgpu_ctx* ctx, int fbo_slot, int tex_or_display);   /* setFBO, slot 0 only */
+int psgpu_depth_set(psgpu_ctx* ctx, int tex_or_default);                   /* setDepth */
+int psgpu_clear_depth(psgpu_ctx* ctx, float furthest);                     /* clear16 */
+int psgpu_clear_colour(psgpu_ctx* ctx, uint32_t bgra);                     /* clear4 of the display */
+
+/* ---- drawing ---- */
+int psgpu_draw(psgpu_ctx* ctx, int vao);          /* drawVAO: queued */
+int psgpu_flush(psgpu_ctx* ctx);                  /* execute queued draws (async) */
+int psgpu_sync(psgpu_ctx* ctx);                   /* flush + wait; re-runs on capacity growth */
+/* display: top-down BGRA8 rows (the renderer's back buffer); depth: default depth buffer rows */
+int psgpu_read_colour(psgpu_ctx* ctx, void* host, size_t bytes);
+int psgpu_read_depth(psgpu_ctx* ctx, void* host, size_t bytes);
+int psgpu_display_device_ptr(psgpu_ctx* ctx, void** ptr);
+int psgpu_depth_device_ptr(psgpu_ctx* ctx, void** ptr, int* scanline_floats);
+int psgpu_get_stats(psgpu_ctx* ctx, psgpu_stats* out);
+int psgpu_reset_stats(psgpu_ctx* ctx);
+int psgpu_swap_buffers(psgpu_ctx* ctx);           /* swapBuffers: flush; display keeps its buffer */
+
+/* ---- debug / parity planes (per pixel, display rows bottom-up: index y*W+x) ---- */
+/* enable=1 makes every flush record: last depth-test-passing triangle id,
+ * fragments rasterised, xor-hash of rasterised triangle ids, depth-test passes,
+ * xor-hash of passing triangle ids.  Triangle ids count across the draws since
+ * the last psgpu_debug_reset. */
+int psgpu_debug_enable(psgpu_ctx* ctx, int enable);
+int psgpu_debug_reset(psgpu_ctx* ctx);
+int psgpu_debug_read(psgpu_ctx* ctx, int32_t* last_pass, int32_t* raster_count, uint32_t* raster_hash,
+                     int32_t* pass_count, uint32_t* pass_hash);
+
+/* ---- tuning and timing ---- */
+int psgpu_tiling_set(psgpu_ctx* ctx, const psgpu_tiling* t);
+int psgpu_tiling_get(psgpu_ctx* ctx, psgpu_tiling* t);
+/* per-kernel timing with HIP events on the context stream; names[i] valid until destroy */
+int psgpu_profile_enable(psgpu_ctx* ctx, int enable);
+int psgpu_profile_read(psgpu_ctx* ctx, int max, const char** names, double* total_ms, int64_t* launches, int* count);
+int psgpu_profile_reset(psgpu_ctx* ctx);
+
+/* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a)} ---- */
+int psgpu_selftest_math(psgpu_ctx* ctx, int op, const float* a, const float* b, float* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,32 @@
+// ps_kernels.h -- host launchers of the raster kernels (ps_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "ps_types.h"
+
+void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCounters* c);
+void psk_clear_colour(hipStream_t s, uint32_t* p, int w, int h, int scan, uint32_t v, const PsCounters* c);
+void psk_zero_counters(hipStream_t s, PsCounters* c);
+void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
+               uint32_t* nrows, PsCounters* cnt, int W, int H);
+int psk_scan_blocks(int64_t cap);
+// exclusive scan; out[n] and *total_out (if given) receive the total
+void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
+              uint32_t* partial, uint32_t* total_out, const PsCounters* c);
+void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
+                PsCounters* cnt);
+void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
+              int64_t T, PsSpan* spans, uint32_t* seg_count, int W, int chunk_w, int64_t cap, PsCounters* cnt);
+void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
+                  const PsFrame& f, uint32_t* keys, uint32_t* vals, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
+int psk_rs_blocks(int64_t cap);
+void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
+                    int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
+                    bool& result_in_second);
+void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
+                const PsCounters* c);
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* kstart,
+                const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt);
+void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,
+               const PsDraw* draws, const PsFrame& f, const PsCounters* cnt);
+void psk_mip(hipStream_t s, const uint8_t* src, int ws, int hs, int ss, uint8_t* dst, int wd, int hd, int elem);
+void psk_selftest(hipStream_t s, int op, const float* a, const float* b, float* out, int n);

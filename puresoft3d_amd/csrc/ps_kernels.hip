@@ -1,0 +1,800 @@
+// ps_kernels.hip -- gfx950 kernels of the sort-middle raster path.
+//
+//  setup   : one thread per triangle: vertex twin, 1/w divide, back-face cull,
+//            z reject, pushVertex/pushTriangle row-range and edge choice
+//            (drawvao.cpp:36-63, vertthrd.cpp, rasterizer.cpp:122-283)
+//  expand  : one wave per 64 triangles writes the triangle id of every row
+//  span    : one thread per (triangle,row): span ends, clamping and the
+//            depth / correction-factor start and step (drawvao.cpp:66-110,
+//            interp.cpp:26-80); counts the bin chunks the span touches
+//  segment : one thread per span writes (bin key, span) pairs
+//  radix   : stable LSD radix sort of the pairs by bin key (8-bit digits,
+//            wave ballots give stable in-block ranks)
+//  bounds  : first pair of every bin key
+//  raster  : one wave per tile of 64 bins, lane = bin (a row piece of
+//            chunk_w pixels); each lane walks its bin's spans in triangle
+//            order, keeping depth and the winning span in LDS
+//            (fragthrd.cpp:208-252); depth test -1 < z && z - d < -1e-4
+//  shade   : one thread per pixel whose last colour-writing depth-test pass
+//            is known: re-derives that span's varyings, walks them serially
+//            to the pixel, runs the FP twin and writes BGRA8.
+//  The depth test is order dependent, so per pixel the fragments are applied
+//  strictly in submission order; no atomics touch depth or colour.
+#include "ps_kernels.h"
+#include "ps_procs.h"
+
+#define PS_WAVE 64
+
+__device__ __forceinline__ bool ps_poisoned(const PsCounters* c)
+{
+    return *((volatile const int*)&c->overflow) != 0;
+}
+
+__device__ __forceinline__ uint64_t ps_lt_mask()
+{
+    return __lanemask_lt();
+}
+
+__device__ __forceinline__ void ps_wave_add(unsigned long long* dst, unsigned long long v)
+{
+    // one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PS_WAVE);
+    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v) atomicAdd(dst, v);
+}
+
+// ---------------------------------------------------------------------------
+// fills (graph friendly: no hipMemset nodes)
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v, const PsCounters* c)
+{
+    if (c && ps_poisoned(c)) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// clear4 (fbo.cpp:332-346): memory rows 0 .. height-2 of a BGRA8 buffer
+__global__ void k_clear_colour(uint32_t* p, int width, int height, int scan, uint32_t v, const PsCounters* c)
+{
+    if (c && ps_poisoned(c)) return;
+    int64_t n = (int64_t)width * (height - 1);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t y = i / width, x = i - y * width;
+        p[y * scan + x] = v;
+    }
+}
+
+__global__ void k_zero_counters(PsCounters* c)
+{
+    if (threadIdx.x == 0) {
+        c->tris_culled = c->tris_zrejected = c->tris_dropped = 0;
+        c->spans = c->frags_rasterised = c->frags_shaded = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ps_find_draw(const PsDraw* draws, int nd, int64_t t)
+{
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (draws[mid].tri_start <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int PROG>
+__global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
+                                               float* __restrict__ vary, int nvs, uint32_t* __restrict__ nrows,
+                                               PsCounters* cnt, int W, int H)
+{
+    typedef PsProc<PROG> P;
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool live = t < T;
+    int di = live ? ps_find_draw(draws, nd, t) : 0;
+    const PsDraw& d = draws[di];
+    live = live && d.prog == PROG;
+    unsigned culled = 0, zrej = 0, dropped = 0;
+    if (live) {
+        int64_t lt = t - d.tri_start;
+        float pos[3][4];
+        float u[3][P::NV > 0 ? P::NV : 1];
+        float rw[3], z[3];
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            P::vp(d, lt * 3 + v, pos[v], u[v]);
+            // perspective division (vertthrd.cpp:37-47)
+            float r = 1.0f / pos[v][3];
+            pos[v][0] *= r; pos[v][1] *= r; pos[v][2] *= r; pos[v][3] *= r;
+            z[v] = pos[v][2];
+            rw[v] = r;
+        }
+        uint32_t n = 0;
+        PsTri tr;
+        bool keep = true;
+        if ((d.behavior & PSGPU_FACE_CULLING) && ps_back_face(pos[0], pos[1], pos[2])) { keep = false; culled = 1; }
+        if (keep) {
+            // drawvao.cpp:51-56
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+                if (pos[v][2] < -1.0f || pos[v][2] > 1.0f) keep = false;
+            if (!keep) zrej = 1;
+        }
+        if (keep) {
+            // pushVertex (rasterizer.cpp:122-139)
+            const int hw = W / 2, hh = H / 2;
+            float vx[3], vy[3];
+            bool bad = false;
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                vy[v] = ((float)hh * pos[v][1]) + (float)hh;
+                vx[v] = ((float)hw * pos[v][0]) + (float)hw;
+                // rows outside the int range make the reference read rows it
+                // never wrote (or crash); the triangle is dropped
+                if (!(vy[v] >= -2147483648.0f && vy[v] < 2147483648.0f)) bad = true;
+            }
+            int first = ps_cvt_i32(vy[0]), last = first;
+#pragma unroll
+            for (int v = 1; v < 3; v++) {
+                if (vy[v] > (float)last) last = ps_cvt_i32(vy[v]);
+                else if (vy[v] < (float)first) first = ps_cvt_i32(vy[v]);
+            }
+            if (first >= H || last < 0) { first = 0; last = -1; }
+            if (first < 0) first = 0;
+            if (last >= H) last = H - 1;
+            if (bad || first == last) { keep = false; dropped = 1; }
+            if (keep) {
+                uint32_t code;
+                int split = H;
+                if (vy[0] == vy[1]) {
+                    code = (vx[0] < vx[1]) ? (0u | (1u << 2) | (2u << 4)) : (1u | (0u << 2) | (2u << 4));
+                } else if (vy[0] == vy[2]) {
+                    code = (vx[0] < vx[2]) ? (0u | (2u << 2) | (1u << 4)) : (2u | (0u << 2) | (1u << 4));
+                } else if (vy[1] == vy[2]) {
+                    code = (vx[1] < vx[2]) ? (1u | (2u << 2) | (0u << 4)) : (2u | (1u << 2) | (0u << 4));
+                } else {
+                    int top, bottom, third = 2;
+                    if (vy[0] > vy[1]) { top = 0; bottom = 1; } else { top = 1; bottom = 0; }
+                    if (vy[top] < vy[third]) { int s = top; top = third; third = s; }
+                    else if (vy[bottom] > vy[third]) { int s = bottom; bottom = third; third = s; }
+                    PsEdge tt = ps_edge(vx, vy, top, third), tb = ps_edge(vx, vy, top, bottom);
+                    bool left = ps_edge_eval(tb, vy[third]) > ps_edge_eval(tt, vy[third]);
+                    code = (uint32_t)top | ((uint32_t)bottom << 2) | ((uint32_t)third << 4) | 64u | (left ? 128u : 0u);
+                    // last row of the lower half: processTriangle's iyMax for yMax = third.y
+                    split = vy[third] > (float)(H - 1) ? H - 1 : ps_cvt_i32(vy[third]);
+                }
+#pragma unroll
+                for (int v = 0; v < 3; v++) { tr.vx[v] = vx[v]; tr.vy[v] = vy[v]; tr.rw[v] = rw[v]; tr.z[v] = z[v]; }
+                tr.first = first;
+                tr.nrows = last - first + 1;
+                tr.split = split;
+                tr.code = code | ((uint32_t)di << 8);
+                n = (uint32_t)(last - first + 1);
+                tris[t] = tr;
+                float* vo = vary + t * 3 * (int64_t)nvs;
+#pragma unroll
+                for (int v = 0; v < 3; v++)
+#pragma unroll
+                    for (int k = 0; k < P::NV; k++) vo[v * nvs + k] = u[v][k];
+            }
+        }
+        nrows[t] = n;
+    }
+    ps_wave_add(&cnt->tris_culled, culled);
+    ps_wave_add(&cnt->tris_zrejected, zrej);
+    ps_wave_add(&cnt->tris_dropped, dropped);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of uint32 with a device-side length; out[n] = total
+// ---------------------------------------------------------------------------
+#define SCAN_ITEMS 16
+#define SCAN_BLOCK (256 * SCAN_ITEMS)
+
+__device__ __forceinline__ uint32_t ps_block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total)
+{
+    // 256 threads = 4 waves
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int w = 0; w < wave; w++) wbase += lds[w];
+    total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return wbase + x - v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
+                                                     uint32_t* __restrict__ partial, const PsCounters* c)
+{
+    __shared__ uint32_t lds[4];
+    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK;
+    uint32_t s = 0;
+    if (!(c && ps_poisoned(c)) && base < n) {
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
+            if (idx < n) s += in[idx];
+        }
+    }
+    uint32_t tot;
+    ps_block_excl_scan(s, lds, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_partials(uint32_t* partial, int nb, uint32_t* out, const uint32_t* n_ptr,
+                                                        int64_t n_host, uint32_t* total_out, int64_t cap, const PsCounters* c)
+{
+    // single block: nb <= 1024 * 16
+    __shared__ uint32_t lds[16];
+    int per = (nb + 1023) / 1024;
+    uint32_t loc[16];
+    uint32_t s = 0;
+    for (int i = 0; i < per; i++) {
+        int idx = threadIdx.x * per + i;
+        loc[i] = idx < nb ? partial[idx] : 0;
+        s += loc[i];
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (int w = 0; w < 16; w++) { if (w < wave) wbase += lds[w]; tot += lds[w]; }
+    uint32_t run = wbase + x - s;
+    for (int i = 0; i < per; i++) {
+        int idx = threadIdx.x * per + i;
+        if (idx < nb) partial[idx] = run;
+        run += loc[i];
+    }
+    if (threadIdx.x == 0) {
+        int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+        bool ok = !(c && ps_poisoned(c)) && n <= cap;
+        if (ok) out[n] = tot;
+        if (total_out && ok) {
+            *total_out = tot;
+            atomicMax(total_out + 2, tot);   // largest total since the counters were reset
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
+                                                    const uint32_t* __restrict__ partial, uint32_t* __restrict__ out,
+                                                    const PsCounters* c)
+{
+    __shared__ uint32_t lds[4];
+    if (c && ps_poisoned(c)) return;
+    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK;
+    if (base >= n) return;
+    uint32_t run = partial[blockIdx.x];
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
+        uint32_t v = idx < n ? in[idx] : 0;
+        uint32_t tot;
+        uint32_t ex = ps_block_excl_scan(v, lds, tot);
+        if (idx < n) out[idx] = run + ex;
+        run += tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// expand: triangle id of every drawn row
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nrows, const uint32_t* __restrict__ row_off,
+                                                int64_t T, int* __restrict__ span_tri, int64_t cap, PsCounters* cnt)
+{
+    // row_off[T] holds the number of rows R
+    if (ps_poisoned(cnt)) return;
+    int lane = threadIdx.x & 63;
+    int64_t t0 = (blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63));
+    if (t0 >= T) return;
+    int64_t R = row_off[T];
+    if (R > cap) {
+        if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&cnt->overflow, 1);
+        return;
+    }
+    int64_t mt = t0 + lane;
+    uint32_t myn = mt < T ? nrows[mt] : 0;
+    uint32_t myo = mt < T ? row_off[mt] : 0;
+    for (int j = 0; j < 64; j++) {
+        uint32_t n = __shfl(myn, j, 64);
+        uint32_t o = __shfl(myo, j, 64);
+        for (uint32_t r = lane; r < n; r += 64) span_tri[o + r] = (int)(t0 + j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// span setup
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
+                                              const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
+                                              int64_t T, PsSpan* __restrict__ spans, uint32_t* __restrict__ seg_count,
+                                              int W, int chunk_w, PsCounters* cnt)
+{
+    if (ps_poisoned(cnt)) return;
+    int64_t R = row_off[T];
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = base + threadIdx.x;
+        unsigned long long nspan = 0, nfrag = 0;
+        if (i < R) {
+            int t = span_tri[i];
+            const PsTri tr = tris[t];
+            int row = tr.first + (int)(i - (int64_t)row_off[t]);
+            PsEdge L, Rr;
+            ps_row_edges(tr, row, L, Rr);
+            int left = ps_cvt_i32(ps_edge_eval(L, (float)row) + 0.5f);
+            int right = ps_cvt_i32(ps_edge_eval(Rr, (float)row) + 0.5f);
+            uint32_t segs = 0;
+            PsSpan sp;
+            sp.tri = -1;
+            if (left != right) {   // drawvao.cpp:72-75
+                nspan = 1;
+                int x1 = left < 0 ? 0 : left;
+                int x2 = right >= W ? W - 1 : right;
+                if (x1 <= x2) {
+                    PsSpanGeom g;
+                    ps_span_geom(tr, row, left, right, L, Rr, g);
+                    ps_span_vis(tr, g, sp.cf2, sp.cf2_step, sp.pz, sp.pz_step);
+                    const PsDraw& d = draws[tr.code >> 8];
+                    sp.tri = t;
+                    sp.y = row | ((d.behavior & 3) << 24) | (d.writes_colour ? (1 << 28) : 0);
+                    sp.x1 = x1;
+                    sp.x2 = x2;
+                    segs = (uint32_t)(x2 / chunk_w - x1 / chunk_w + 1);
+                    nfrag = (unsigned long long)(x2 - x1 + 1);
+                }
+            }
+            spans[i] = sp;
+            seg_count[i] = segs;
+        }
+        ps_wave_add(&cnt->spans, nspan);
+        ps_wave_add(&cnt->frags_rasterised, nfrag);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// segments: (bin key, span) pairs
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ps_bin_key(const PsFrame& f, int y, int cx)
+{
+    int ty = y / f.tile_rows, ry = y - ty * f.tile_rows;
+    int tx = cx / f.tile_cols, rx = cx - tx * f.tile_cols;
+    return (uint32_t)((ty * f.n_tiles_x + tx) * 64 + ry * f.tile_cols + rx);
+}
+
+__global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ seg_off,
+                                                  const uint32_t* __restrict__ row_off, int64_t T, PsFrame f,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int64_t cap,
+                                                  PsCounters* cnt)
+{
+    if (ps_poisoned(cnt)) return;
+    int64_t R = row_off[T];
+    int64_t S = seg_off[R];
+    if (S > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->overflow, 1);
+        return;
+    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t o = seg_off[i], e = seg_off[i + 1];
+        if (o == e) continue;
+        const PsSpan sp = spans[i];
+        int y = sp.y & 0xffffff;
+        int cx = sp.x1 / f.chunk_w;
+        for (uint32_t s = o; s < e; s++, cx++) {
+            keys[s] = ps_bin_key(f, y, cx);
+            vals[s] = (uint32_t)i;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits, 4096 pairs per block
+// ---------------------------------------------------------------------------
+#define RS_ITEMS 16
+#define RS_BLOCK (256 * RS_ITEMS)
+
+__global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int shift,
+                                                 uint32_t* __restrict__ hist, int nb_cap, const PsCounters* c)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t n = *n_ptr;
+    int64_t base = blockIdx.x * (int64_t)RS_BLOCK;
+    if (!ps_poisoned(c) && base < n) {
+        for (int i = 0; i < RS_ITEMS; i++) {
+            int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
+            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nb_cap + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                    const uint32_t* n_ptr, int shift, const uint32_t* __restrict__ offs,
+                                                    int nb_cap, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                    const PsCounters* c)
+{
+    __shared__ uint32_t cntl[4][256];
+    __shared__ uint32_t base[256];
+    if (ps_poisoned(c)) return;
+    int64_t n = *n_ptr;
+    int64_t b0 = blockIdx.x * (int64_t)RS_BLOCK;
+    if (b0 >= n) return;
+    int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int w = 0; w < 4; w++) cntl[w][tid] = 0;
+    base[tid] = offs[(int64_t)tid * nb_cap + blockIdx.x];
+    __syncthreads();
+    uint32_t k[RS_ITEMS], v[RS_ITEMS], rank[RS_ITEMS];
+    uint64_t lt = ps_lt_mask();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = b0 + wave * (64 * RS_ITEMS) + r * 64 + lane;
+        bool valid = idx < n;
+        k[r] = valid ? kin[idx] : 0u;
+        v[r] = valid ? vin[idx] : 0u;
+        uint32_t dig = (k[r] >> shift) & 255u;
+        uint64_t same = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t m = __ballot((dig >> bit) & 1u);
+            same &= ((dig >> bit) & 1u) ? m : ~m;
+        }
+        uint32_t before = (uint32_t)__popcll(same & lt);
+        uint32_t cur = cntl[wave][dig];
+        rank[r] = cur + before;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0) cntl[wave][dig] = cur + (uint32_t)__popcll(same);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0;
+        for (int w = 0; w < 4; w++) {
+            uint32_t t = cntl[w][tid];
+            cntl[w][tid] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = b0 + wave * (64 * RS_ITEMS) + r * 64 + lane;
+        if (idx < n) {
+            uint32_t dig = (k[r] >> shift) & 255u;
+            uint32_t pos = base[dig] + cntl[wave][dig] + rank[r];
+            kout[pos] = k[r];
+            vout[pos] = v[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bounds: first pair of each bin key (lower_bound)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int64_t n_keys,
+                                                uint32_t* __restrict__ start, const PsCounters* c)
+{
+    if (ps_poisoned(c)) return;
+    int64_t n = *n_ptr;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k <= n_keys; k += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < (uint32_t)k) lo = mid + 1; else hi = mid;
+        }
+        start[k] = (uint32_t)lo;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// raster: ordered depth test per bin
+// ---------------------------------------------------------------------------
+template <bool DEBUG>
+__global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ vals,
+                         const uint32_t* __restrict__ kstart, const PsTri* __restrict__ tris,
+                         const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles, PsCounters* cnt)
+{
+    extern __shared__ uint32_t lds[];
+    if (ps_poisoned(cnt)) return;
+    const int CW = f.chunk_w;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int waves = blockDim.x >> 6;
+    int64_t tile = blockIdx.x * (int64_t)waves + wave;
+    if (tile >= n_tiles) return;
+    // LDS per wave, column-major [c][lane]: bank = lane for every column
+    float* dep = (float*)(lds + (size_t)wave * CW * 64 * 2);
+    int* win = (int*)(dep + CW * 64);
+
+    int ty = (int)(tile / f.n_tiles_x), tx = (int)(tile - (int64_t)ty * f.n_tiles_x);
+    int ry = lane / f.tile_cols, rx = lane - ry * f.tile_cols;
+    int y = ty * f.tile_rows + ry;
+    int cx = tx * f.tile_cols + rx;
+    int X0 = cx * CW;
+    bool inside = y < f.H && X0 < f.W;
+    int ncol = inside ? min(CW, f.W - X0) : 0;
+    float* drow = f.depth + (int64_t)y * f.depth_scan;
+    for (int c = 0; c < ncol; c++) {
+        dep[c * 64 + lane] = drow[X0 + c];
+        win[c * 64 + lane] = -1;
+    }
+    unsigned long long passes = 0;
+    bool dirty = false;
+    if (inside) {
+        uint32_t key = (uint32_t)(tile * 64 + lane);
+        uint32_t s0 = kstart[key], s1 = kstart[key + 1];
+        int X1 = X0 + ncol - 1;
+        for (uint32_t e = s0; e < s1; e++) {
+            uint32_t si = vals[e];
+            const PsSpan sp = spans[si];
+            int beh = (sp.y >> 24) & 3;
+            bool wc = (sp.y >> 28) & 1;
+            int a = max(sp.x1, X0), b = min(sp.x2, X1);
+            float cf2 = sp.cf2, pz = sp.pz;
+            const float cst = sp.cf2_step, pst = sp.pz_step;
+            for (int x = sp.x1; x < a; x++) { cf2 += cst; pz += pst; }
+            uint32_t th = 0;
+            int tid = 0;
+            if (DEBUG) {
+                const PsTri tr = tris[sp.tri];
+                const PsDraw& d = draws[tr.code >> 8];
+                tid = (int)(d.id_base + (sp.tri - d.tri_start));
+                uint32_t h = (uint32_t)tid;
+                h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+                th = h;
+            }
+            for (int x = a; x <= b; x++) {
+                // interpolateNextStep (interp.cpp:82-92)
+                float r = 1.0f / cf2;
+                cf2 += cst;
+                float z = pz * r;
+                pz += pst;
+                int c = x - X0;
+                float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
+                int64_t pix = (int64_t)y * f.W + x;
+                if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                // fragthrd.cpp:227
+                if (-1.0f < z && (z - cur < -0.0001f)) {
+                    passes++;
+                    if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
+                    if (wc) win[c * 64 + lane] = (int)si;
+                    if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                }
+            }
+        }
+    }
+    int* wrow = f.winner + (int64_t)y * f.W;
+    for (int c = 0; c < ncol; c++) {
+        if (dirty) drow[X0 + c] = dep[c * 64 + lane];
+        wrow[X0 + c] = win[c * 64 + lane];
+    }
+    ps_wave_add(&cnt->frags_shaded, passes);
+}
+
+// ---------------------------------------------------------------------------
+// shade: deferred FP of the last colour-writing pass per pixel
+// ---------------------------------------------------------------------------
+template <int PROG>
+__global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans, const PsTri* __restrict__ tris,
+                                               const float* __restrict__ vary, int nvs,
+                                               const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt)
+{
+    typedef PsProc<PROG> P;
+    if (ps_poisoned(cnt)) return;
+    int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= f.W || y >= f.H) return;
+    int w = f.winner[(int64_t)y * f.W + x];
+    if (w < 0) return;
+    const PsSpan sp = spans[w];
+    const PsTri tr = tris[sp.tri];
+    const PsDraw& d = draws[tr.code >> 8];
+    if (d.prog != PROG) return;
+    PsEdge L, R;
+    ps_row_edges(tr, y, L, R);
+    int left = ps_cvt_i32(ps_edge_eval(L, (float)y) + 0.5f);
+    int right = ps_cvt_i32(ps_edge_eval(R, (float)y) + 0.5f);
+    PsSpanGeom g;
+    ps_span_geom(tr, y, left, right, L, R, g);
+    const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
+    float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
+    ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
+    float cf2 = sp.cf2;
+    for (int k = sp.x1; k < x; k++) {
+        cf2 += sp.cf2_step;
+#pragma unroll
+        for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
+    }
+    float r = 1.0f / cf2;
+    float in[P::NV > 0 ? P::NV : 1];
+#pragma unroll
+    for (int i = 0; i < P::NV; i++) in[i] = vs[i] * r;
+    float grad[6] = {0, 0, 0, 0, 0, 0};
+    if (P::GRAD) ps_tri_grad(tr, u0, u0 + nvs, u0 + 2 * nvs, 12, grad);
+    uint32_t c;
+    if (P::fp(d, in, r, grad, c)) {
+        int64_t row = f.colour_top_down ? (int64_t)(f.colour_h - 1 - y) : (int64_t)y;
+        f.colour[row * f.colour_scan + x] = c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mip chain: rounded 2x2 mean, level l-1 -> l
+// ---------------------------------------------------------------------------
+__global__ void k_mip(const uint8_t* __restrict__ src, int ws, int hs, int ss, uint8_t* __restrict__ dst, int wd, int hd,
+                      int elem)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= wd || y >= hd) return;
+    int y0 = 2 * y, y1 = min(2 * y + 1, hs - 1);
+    int x0 = 2 * x, x1 = min(2 * x + 1, ws - 1);
+    for (int c = 0; c < elem; c++) {
+        int s = src[(int64_t)y0 * ss + x0 * elem + c] + src[(int64_t)y0 * ss + x1 * elem + c] +
+                src[(int64_t)y1 * ss + x0 * elem + c] + src[(int64_t)y1 * ss + x1 * elem + c];
+        dst[(int64_t)y * wd * elem + x * elem + c] = (uint8_t)((s + 2) >> 2);
+    }
+}
+
+__global__ void k_selftest(int op, const float* a, const float* b, float* out, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float r = 0.0f;
+    switch (op) {
+    case 0: r = a[i] / b[i]; break;
+    case 1: r = sqrtf(a[i]); break;
+    case 2: r = 1.0f / a[i]; break;
+    case 3: r = (float)ps_cvt_i32(a[i]); break;
+    case 4: r = floorf(a[i]); break;
+    case 5: r = (float)(int)ps_cvt_u32(a[i]); break;
+    case 6: r = ps_log2(a[i]); break;
+    }
+    out[i] = r;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline int gsz(int64_t n, int b) { int64_t g = (n + b - 1) / b; return (int)(g < 1 ? 1 : g); }
+
+void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCounters* c)
+{
+    if (n <= 0) return;
+    int g = gsz(n, 256); if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, p, n, v, c);
+}
+
+void psk_clear_colour(hipStream_t s, uint32_t* p, int w, int h, int scan, uint32_t v, const PsCounters* c)
+{
+    int64_t n = (int64_t)w * (h - 1);
+    if (n <= 0) return;
+    int g = gsz(n, 256); if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_clear_colour, dim3(g), dim3(256), 0, s, p, w, h, scan, v, c);
+}
+
+void psk_zero_counters(hipStream_t s, PsCounters* c)
+{
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c);
+}
+
+void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
+               uint32_t* nrows, PsCounters* cnt, int W, int H)
+{
+    dim3 g(gsz(T, 256)), b(256);
+    switch (prog) {
+    case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF01>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF02>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_DEF03: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF03>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_DEF04: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF04>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_DEF05: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF05>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_LAMBERT: hipLaunchKernelGGL(k_setup<PSGPU_PROC_LAMBERT>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_BLINN: hipLaunchKernelGGL(k_setup<PSGPU_PROC_BLINN>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    }
+}
+
+int psk_scan_blocks(int64_t cap) { return gsz(cap, SCAN_BLOCK); }
+
+void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
+              uint32_t* partial, uint32_t* total_out, const PsCounters* c)
+{
+    int nb = psk_scan_blocks(cap);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb, out, n_dev, n_host, total_out, cap, c);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, out, c);
+}
+
+void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
+                PsCounters* cnt)
+{
+    hipLaunchKernelGGL(k_expand, dim3(gsz(T, 256)), dim3(256), 0, s, nrows, row_off, T, span_tri, cap, cnt);
+}
+
+void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
+              int64_t T, PsSpan* spans, uint32_t* seg_count, int W, int chunk_w, int64_t cap, PsCounters* cnt)
+{
+    int g = gsz(cap, 256); if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_span, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, spans, seg_count, W, chunk_w, cnt);
+}
+
+void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
+                  const PsFrame& f, uint32_t* keys, uint32_t* vals, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
+{
+    int g = gsz(span_cap, 256); if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, keys, vals, seg_cap, cnt);
+}
+
+int psk_rs_blocks(int64_t cap) { return gsz(cap, RS_BLOCK); }
+
+void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
+                    int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
+                    bool& result_in_second)
+{
+    int nb = psk_rs_blocks(cap);
+    uint32_t *ki = keys, *vi = vals, *ko = keys2, *vo = vals2;
+    int passes = 0;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(256), 0, s, ki, n_dev, shift, hist, nb, c);
+        psk_scan(s, hist, nullptr, (int64_t)nb * 256, (int64_t)nb * 256, hist_off, partial, nullptr, c);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, nb, ko, vo, c);
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+        passes++;
+    }
+    result_in_second = (passes & 1) != 0;
+}
+
+void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
+                const PsCounters* c)
+{
+    int g = gsz(n_keys + 1, 256); if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_bounds, dim3(g), dim3(256), 0, s, keys, n_dev, n_keys, start, c);
+}
+
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* kstart,
+                const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt)
+{
+    int64_t n_tiles = (int64_t)f.n_tiles_x * f.n_tiles_y;
+    size_t lds = (size_t)waves_per_group * f.chunk_w * 64 * 4 * 2;
+    dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
+    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, vals, kstart, tris, draws, f, n_tiles, cnt);
+    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, vals, kstart, tris, draws, f, n_tiles, cnt);
+}
+
+void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,
+               const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
+{
+    dim3 g((f.W + 15) / 16, (f.H + 15) / 16), b(256);
+    switch (prog) {
+    case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF01>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF02>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    case PSGPU_PROC_DEF03: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF03>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    case PSGPU_PROC_DEF04: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF04>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    case PSGPU_PROC_LAMBERT: hipLaunchKernelGGL(k_shade<PSGPU_PROC_LAMBERT>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    case PSGPU_PROC_BLINN: hipLaunchKernelGGL(k_shade<PSGPU_PROC_BLINN>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    default: break;
+    }
+}
+
+void psk_mip(hipStream_t s, const uint8_t* src, int ws, int hs, int ss, uint8_t* dst, int wd, int hd, int elem)
+{
+    dim3 g((wd + 255) / 256, hd), b(256);
+    hipLaunchKernelGGL(k_mip, g, b, 0, s, src, ws, hs, ss, dst, wd, hd, elem);
+}
+
+void psk_selftest(hipStream_t s, int op, const float* a, const float* b, float* out, int n)
+{
+    hipLaunchKernelGGL(k_selftest, dim3(gsz(n, 256)), dim3(256), 0, s, op, a, b, out, n);
+}

@@ -1,0 +1,85 @@
+// ps_types.h -- device-side records shared by the raster kernels and the host
+// orchestration of libpsgpu.  Plain structs: the host fills them and copies
+// them to HBM; kernels read them through global pointers.
+#pragma once
+#include <stdint.h>
+
+#define PS_MAX_NV      20   // widest varying block (DEF03: 5 float4 slots)
+#define PS_MAX_LEVELS  16
+#define PS_MAX_LAYERS  6
+
+// One texture (a PuresoftFBO used as a texture, fbo.h:6-100) in HBM.
+// Layer-major; inside a layer level 0 (rows of `scanline` bytes) then the
+// tightly packed mip levels.
+struct PsTex {
+    const uint8_t* data;
+    int width, height, scanline, elem_len;
+    int layers, wrap, levels, pad;
+    int64_t layer_bytes;
+    int64_t level_off[PS_MAX_LEVELS];
+};
+
+// Per-draw record: the snapshot of programme, behaviour, vertex streams and
+// uniforms that drawVAO would see (drawvao.cpp:12-20).
+struct PsDraw {
+    int prog;          // PSGPU_PROC_*
+    int behavior;      // PSGPU_* behaviour bits
+    int nv;            // varying floats of the programme
+    int writes_colour; // FP writes FBO slot 0
+    int64_t tri_start; // first global triangle id of this draw (flush-local)
+    int64_t tri_count;
+    int64_t id_base;   // debug triangle-id base
+    const uint8_t* slot[5];
+    int unit[5];
+    int spec_exp;
+    float PV[16], M[16], Mrot[16];
+    float invView[16];  // DEF04
+    float PVM[16];      // DEF05
+    float lightPos[4], cameraPos[4], lightDir[4];
+    PsTex tex0, tex1;
+};
+
+// Triangle after vertex processing and setup (vertthrd.cpp:14-65,
+// rasterizer.cpp:122-283).  `code` packs the row->edge mapping:
+//   bits 0-1 a, 2-3 b, 4-5 c (vertex indices), bit 6 standing, bit 7 points-left
+//   flat   : left edge (a,c), right edge (b,c) for every row
+//   standing (a=top, b=bottom, c=third): rows <= split use the lower half
+struct PsTri {
+    float vx[3], vy[3];
+    float rw[3], z[3];
+    int first;     // first drawn row
+    int nrows;     // drawn rows (0 = no rows)
+    int split;     // last row of the lower half (standing triangles)
+    uint32_t code; // edge code | draw index << 8
+};
+
+// One scanline span with at least one pixel (drawvao.cpp:66-110, interp.cpp:26-80).
+struct PsSpan {
+    int tri;        // flush-local triangle index
+    int y;          // row
+    int x1, x2;     // leftClamped, rightClamped (x1 <= x2)
+    float cf2, cf2_step, pz, pz_step;
+};
+
+// Flush-wide parameters passed by value to kernels.
+struct PsFrame {
+    int W, H;
+    int chunk_w, tile_rows, tile_cols, n_chunks;   // raster tiling
+    int n_tiles_x, n_tiles_y;
+    int64_t n_keys;
+    // targets
+    uint32_t* colour; int colour_scan; int colour_top_down; int colour_h; // scan in uint32 units
+    float* depth; int depth_scan;                           // scan in floats
+    // per-pixel winner (span index) for the shading pass
+    int* winner;
+    // optional debug planes
+    int32_t* dbg_last; int32_t* dbg_rcount; uint32_t* dbg_rhash; int32_t* dbg_pcount; uint32_t* dbg_phash;
+};
+
+// Device-side counters of one flush.
+struct PsCounters {
+    unsigned long long tris_culled, tris_zrejected, tris_dropped;
+    unsigned long long spans, frags_rasterised, frags_shaded;
+    int overflow;
+    int pad;
+};

@@ -1,0 +1,1223 @@
+// psgpu_api.hip -- C ABI of the raster hot path (include/psgpu.h).
+//
+// Host side of libpsgpu: object pools with the reference's semantics
+// (pipeline.cpp, tex.cpp, prog.cpp, vbo.cpp, vao.cpp), a per-context command
+// log of flushes and clears, and the orchestration of the kernels in
+// ps_kernels.hip on one HIP stream.
+//
+// Capacity handling: intermediate buffers (spans, bin pairs) are sized from
+// the previous frames.  Kernels that would overrun a buffer set an overflow
+// flag instead; every later kernel of the log then skips its work, and
+// psgpu_sync() grows the buffers and replays the whole log since the last
+// sync.  Clears and flushes only write depth/colour when the flag is clear,
+// so a replay starts from the state the log started from.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/psgpu.h"
+#include "ps_kernels.h"
+
+namespace {
+
+const int kMaxSlots = 16;      // config.h:4
+const int kMaxUniforms = 1024; // config.h:5
+
+struct Tex {
+    bool alive = false;
+    uint8_t* d = nullptr;
+    int w = 0, h = 0, scan = 0, elem = 0, layers = 1, wrap = 0, levels = 1;
+    int64_t layer_bytes = 0;
+    int64_t level_off[PS_MAX_LEVELS] = {0};
+};
+
+struct Vbo {
+    bool alive = false;
+    uint8_t* d = nullptr;
+    size_t unit = 0, count = 0;
+};
+
+struct Vao {
+    bool alive = false;
+    int slot[kMaxSlots];
+};
+
+struct Prog {
+    bool alive = false;
+    int vp = 0, ip = 0, fp = 0;
+};
+
+struct Targets {
+    uint32_t* colour; int colour_scan; int colour_top_down; int colour_w; int colour_h;
+    float* depth; int depth_scan; int depth_w; int depth_h;
+    int vpw, vph;
+};
+
+enum CmdType { CMD_FLUSH = 0, CMD_CLEAR_DEPTH = 1, CMD_CLEAR_COLOUR = 2 };
+
+struct Cmd {
+    CmdType type;
+    std::vector<PsDraw> draws;
+    Targets tg;
+    float fval = 0.0f;
+    uint32_t uval = 0;
+};
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+};
+
+struct ProfRec { const char* name; hipEvent_t a, b; };
+
+}  // namespace
+
+struct psgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int dev_w = 0, dev_h = 0;
+    int vp_w = 0, vp_h = 0;
+    uint32_t* display = nullptr;
+    float* depth = nullptr;
+    int depth_scan = 0;
+    std::vector<Tex> tex;
+    std::vector<Vbo> vbo;
+    std::vector<Vao> vao;
+    std::vector<Prog> prog;
+    std::vector<std::vector<uint8_t>> uniforms;
+    int behavior = PSGPU_UPDATE_DEPTH | PSGPU_TEST_DEPTH | PSGPU_FACE_CULLING;  // pipeline.cpp:34
+    int cur_prog = -1;
+    int fbo0 = PSGPU_DISPLAY;
+    int depth_sel = PSGPU_DEFAULT_DEPTH;
+    std::vector<PsDraw> queue;
+    int64_t queue_tris = 0;
+    std::vector<Cmd> log;
+    psgpu_tiling tiling = {16, 16, 1};
+    // scratch
+    DevBuf<PsDraw> d_draws;
+    DevBuf<PsTri> d_tris;
+    DevBuf<float> d_vary;
+    DevBuf<uint32_t> d_nrows, d_rowoff, d_partial;
+    DevBuf<int> d_spantri;
+    DevBuf<PsSpan> d_spans;
+    DevBuf<uint32_t> d_segcount, d_segoff;
+    DevBuf<uint32_t> d_keys, d_vals, d_keys2, d_vals2, d_hist, d_histoff, d_kstart;
+    DevBuf<int> d_winner;
+    int64_t span_cap = 0, seg_cap = 0;
+    uint32_t* d_sizes = nullptr;
+    uint32_t* h_sizes = nullptr;
+    PsCounters* d_cnt = nullptr;
+    PsCounters* h_cnt = nullptr;
+    // draw staging (pinned), ring of slots
+    static const int kStage = 4;
+    PsDraw* h_stage[kStage] = {nullptr};
+    size_t h_stage_cap[kStage] = {0};
+    hipEvent_t stage_ev[kStage] = {nullptr};
+    int stage_next = 0;
+    // debug planes
+    bool debug = false;
+    int32_t* dbg_last = nullptr; int32_t* dbg_rcount = nullptr; uint32_t* dbg_rhash = nullptr;
+    int32_t* dbg_pcount = nullptr; uint32_t* dbg_phash = nullptr;
+    int dbg_w = 0, dbg_h = 0;
+    int64_t dbg_tri_counter = 0;
+    // stats and profiling
+    psgpu_stats stats;
+    bool profiling = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, std::pair<double, int64_t>> prof;
+    std::vector<std::string> prof_names;
+};
+
+namespace {
+
+int fail(psgpu_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) return fail((c), PSGPU_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T> int ensure(psgpu_ctx* c, DevBuf<T>& b, size_t n)
+{
+    if (n <= b.cap && b.p) return PSGPU_OK;
+    size_t cap = std::max(n, (size_t)(b.cap * 3 / 2));
+    if (cap < 64) cap = 64;
+    if (b.p) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    HIPCHK(c, hipMalloc((void**)&b.p, cap * sizeof(T)));
+    b.cap = cap;
+    return PSGPU_OK;
+}
+
+hipEvent_t ev_get(psgpu_ctx* c)
+{
+    if (!c->ev_pool.empty()) { hipEvent_t e = c->ev_pool.back(); c->ev_pool.pop_back(); return e; }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+struct ProfScope {
+    psgpu_ctx* c; const char* name; hipEvent_t a = nullptr;
+    ProfScope(psgpu_ctx* c_, const char* n) : c(c_), name(n)
+    {
+        if (c->profiling) { a = ev_get(c); hipEventRecord(a, c->stream); }
+    }
+    ~ProfScope()
+    {
+        if (c->profiling) {
+            hipEvent_t b = ev_get(c);
+            hipEventRecord(b, c->stream);
+            c->prof_pending.push_back({name, a, b});
+        }
+    }
+};
+
+void prof_collect(psgpu_ctx* c)
+{
+    for (auto& r : c->prof_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& p = c->prof[r.name];
+            p.first += ms;
+            p.second += 1;
+        }
+        c->ev_pool.push_back(r.a);
+        c->ev_pool.push_back(r.b);
+    }
+    c->prof_pending.clear();
+}
+
+bool valid_tex(psgpu_ctx* c, int i) { return i >= 0 && i < (int)c->tex.size() && c->tex[i].alive; }
+
+int prog_nv(int kind)
+{
+    switch (kind) {
+    case PSGPU_PROC_DEF01: return 10;
+    case PSGPU_PROC_DEF02: return 12;
+    case PSGPU_PROC_DEF03: return 18;
+    case PSGPU_PROC_DEF04: return 3;
+    case PSGPU_PROC_DEF05: return 0;
+    case PSGPU_PROC_LAMBERT: return 5;
+    case PSGPU_PROC_BLINN: return 14;
+    default: return -1;
+    }
+}
+
+// host copy of mcemaths_transform_m4m4 (matrix.cpp:588-701), same operation order
+void host_m4m4(float* d, const float* l, const float* r)
+{
+    for (int j = 0; j < 4; j++) {
+        const float* v = r + 4 * j;
+        for (int i = 0; i < 4; i++) {
+            float a = v[0] * l[i], b = v[1] * l[4 + i], cc = v[2] * l[8 + i], dd = v[3] * l[12 + i];
+            d[4 * j + i] = ((a + b) + cc) + dd;
+        }
+    }
+}
+
+int current_targets(psgpu_ctx* c, Targets& t)
+{
+    t.vpw = c->vp_w;
+    t.vph = c->vp_h;
+    if (c->fbo0 == PSGPU_DISPLAY) {
+        t.colour = c->display; t.colour_scan = c->dev_w; t.colour_top_down = 1; t.colour_w = c->dev_w; t.colour_h = c->dev_h;
+    } else {
+        if (!valid_tex(c, c->fbo0)) return fail(c, PSGPU_ERR_STATE, "colour target texture %d was destroyed", c->fbo0);
+        Tex& x = c->tex[c->fbo0];
+        if (x.elem != 4 || x.scan % 4) return fail(c, PSGPU_ERR_INVALID, "colour target must have 4-byte texels");
+        t.colour = (uint32_t*)x.d; t.colour_scan = x.scan / 4; t.colour_top_down = 0; t.colour_w = x.w; t.colour_h = x.h;
+    }
+    if (c->depth_sel == PSGPU_DEFAULT_DEPTH) {
+        t.depth = c->depth; t.depth_scan = c->depth_scan; t.depth_w = c->dev_w; t.depth_h = c->dev_h;
+    } else {
+        if (!valid_tex(c, c->depth_sel)) return fail(c, PSGPU_ERR_STATE, "depth texture %d was destroyed", c->depth_sel);
+        Tex& x = c->tex[c->depth_sel];
+        t.depth = (float*)x.d; t.depth_scan = x.scan / 4; t.depth_w = x.w; t.depth_h = x.h;
+    }
+    return PSGPU_OK;
+}
+
+PsTex tex_desc(const Tex& x)
+{
+    PsTex d;
+    memset(&d, 0, sizeof(d));
+    d.data = x.d; d.width = x.w; d.height = x.h; d.scanline = x.scan; d.elem_len = x.elem;
+    d.layers = x.layers; d.wrap = x.wrap; d.levels = x.levels; d.layer_bytes = x.layer_bytes;
+    for (int l = 0; l < PS_MAX_LEVELS; l++) d.level_off[l] = x.level_off[l];
+    return d;
+}
+
+int stage_draws(psgpu_ctx* c, const std::vector<PsDraw>& draws, PsDraw** out)
+{
+    int s = c->stage_next;
+    c->stage_next = (s + 1) % psgpu_ctx::kStage;
+    if (c->stage_ev[s]) HIPCHK(c, hipEventSynchronize(c->stage_ev[s]));
+    if (c->h_stage_cap[s] < draws.size()) {
+        if (c->h_stage[s]) HIPCHK(c, hipHostFree(c->h_stage[s]));
+        size_t cap = std::max(draws.size(), (size_t)16);
+        HIPCHK(c, hipHostMalloc((void**)&c->h_stage[s], cap * sizeof(PsDraw)));
+        c->h_stage_cap[s] = cap;
+    }
+    memcpy(c->h_stage[s], draws.data(), draws.size() * sizeof(PsDraw));
+    int rc = ensure(c, c->d_draws, draws.size());
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_draws.p, c->h_stage[s], draws.size() * sizeof(PsDraw), hipMemcpyHostToDevice, c->stream));
+    if (!c->stage_ev[s]) HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(c->stage_ev[s], c->stream));
+    *out = c->d_draws.p;
+    return PSGPU_OK;
+}
+
+int key_bits(int64_t n_keys)
+{
+    int b = 0;
+    while (((int64_t)1 << b) < n_keys + 1) b++;
+    return b < 1 ? 1 : b;
+}
+
+// run one flush command: the whole pipeline over its draws
+int run_flush(psgpu_ctx* c, const Cmd& cmd)
+{
+    const std::vector<PsDraw>& draws = cmd.draws;
+    if (draws.empty()) return PSGPU_OK;
+    const Targets& tg = cmd.tg;
+    int64_t T = 0;
+    int nvs = 1;
+    bool progs[8] = {false}, shade_progs[8] = {false};
+    for (const PsDraw& d : draws) {
+        T += d.tri_count;
+        nvs = std::max(nvs, d.nv);
+        progs[d.prog] = true;
+        if (d.writes_colour) shade_progs[d.prog] = true;
+    }
+    if (T == 0) return PSGPU_OK;
+    int rc;
+    PsDraw* dd;
+    if ((rc = stage_draws(c, draws, &dd))) return rc;
+    if ((rc = ensure(c, c->d_tris, T)) || (rc = ensure(c, c->d_vary, (size_t)T * 3 * nvs)) ||
+        (rc = ensure(c, c->d_nrows, T)) || (rc = ensure(c, c->d_rowoff, T + 1)))
+        return rc;
+    if (c->span_cap == 0) {
+        c->span_cap = std::max<int64_t>(1 << 20, 6 * T);
+        c->seg_cap = c->span_cap * 3 / 2;
+    }
+    int64_t scan_cap = std::max<int64_t>(std::max<int64_t>(T, c->span_cap), (int64_t)psk_rs_blocks(c->seg_cap) * 256);
+    if ((rc = ensure(c, c->d_partial, psk_scan_blocks(scan_cap) + 1)) || (rc = ensure(c, c->d_spantri, c->span_cap)) ||
+        (rc = ensure(c, c->d_spans, c->span_cap)) || (rc = ensure(c, c->d_segcount, c->span_cap)) ||
+        (rc = ensure(c, c->d_segoff, c->span_cap + 1)) || (rc = ensure(c, c->d_keys, c->seg_cap)) ||
+        (rc = ensure(c, c->d_vals, c->seg_cap)) || (rc = ensure(c, c->d_keys2, c->seg_cap)) ||
+        (rc = ensure(c, c->d_vals2, c->seg_cap)))
+        return rc;
+    int nbr = psk_rs_blocks(c->seg_cap);
+    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 256)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 256 + 1)))
+        return rc;
+
+    // tiling
+    PsFrame f;
+    memset(&f, 0, sizeof(f));
+    f.W = tg.vpw;
+    f.H = tg.vph;
+    f.chunk_w = c->tiling.chunk_width;
+    f.tile_rows = c->tiling.tile_rows;
+    f.tile_cols = 64 / f.tile_rows;
+    f.n_chunks = (f.W + f.chunk_w - 1) / f.chunk_w;
+    f.n_tiles_x = (f.n_chunks + f.tile_cols - 1) / f.tile_cols;
+    f.n_tiles_y = (f.H + f.tile_rows - 1) / f.tile_rows;
+    f.n_keys = (int64_t)f.n_tiles_x * f.n_tiles_y * 64;
+    f.colour = tg.colour; f.colour_scan = tg.colour_scan; f.colour_top_down = tg.colour_top_down; f.colour_h = tg.colour_h;
+    f.depth = tg.depth; f.depth_scan = tg.depth_scan;
+    if ((rc = ensure(c, c->d_kstart, f.n_keys + 1)) || (rc = ensure(c, c->d_winner, (size_t)f.W * f.H))) return rc;
+    f.winner = c->d_winner.p;
+    if (c->debug) {
+        if (c->dbg_w != f.W || c->dbg_h != f.H) return fail(c, PSGPU_ERR_STATE, "debug planes sized for another viewport");
+        f.dbg_last = c->dbg_last; f.dbg_rcount = c->dbg_rcount; f.dbg_rhash = c->dbg_rhash;
+        f.dbg_pcount = c->dbg_pcount; f.dbg_phash = c->dbg_phash;
+    }
+
+    hipStream_t s = c->stream;
+    int nd = (int)draws.size();
+    {
+        ProfScope p(c, "setup");
+        for (int k = 1; k < 8; k++)
+            if (progs[k]) psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_nrows.p, c->d_cnt, f.W, f.H);
+    }
+    {
+        ProfScope p(c, "scan_rows");
+        // row counts never depend on buffer capacities: always scanned, so the
+        // largest row total of the log is known even after an overflow
+        psk_scan(s, c->d_nrows.p, nullptr, T, T, c->d_rowoff.p, c->d_partial.p, c->d_sizes + 0, nullptr);
+    }
+    {
+        ProfScope p(c, "expand");
+        psk_expand(s, c->d_nrows.p, c->d_rowoff.p, T, c->d_spantri.p, c->span_cap, c->d_cnt);
+    }
+    {
+        ProfScope p(c, "span");
+        psk_span(s, c->d_tris.p, dd, c->d_spantri.p, c->d_rowoff.p, T, c->d_spans.p, c->d_segcount.p, f.W, f.chunk_w,
+                 c->span_cap, c->d_cnt);
+    }
+    {
+        ProfScope p(c, "scan_segs");
+        psk_scan(s, c->d_segcount.p, c->d_sizes + 0, 0, c->span_cap, c->d_segoff.p, c->d_partial.p, c->d_sizes + 1, c->d_cnt);
+    }
+    {
+        ProfScope p(c, "segments");
+        psk_segments(s, c->d_spans.p, c->d_segoff.p, c->d_rowoff.p, T, f, c->d_keys.p, c->d_vals.p, c->span_cap, c->seg_cap,
+                     c->d_cnt);
+    }
+    bool second = false;
+    {
+        ProfScope p(c, "radix_sort");
+        psk_radix_sort(s, c->d_keys.p, c->d_vals.p, c->d_keys2.p, c->d_vals2.p, c->d_sizes + 1, c->seg_cap,
+                       key_bits(f.n_keys), c->d_hist.p, c->d_histoff.p, c->d_partial.p, c->d_cnt, second);
+    }
+    uint32_t* skeys = second ? c->d_keys2.p : c->d_keys.p;
+    uint32_t* svals = second ? c->d_vals2.p : c->d_vals.p;
+    {
+        ProfScope p(c, "bounds");
+        psk_bounds(s, skeys, c->d_sizes + 1, f.n_keys, c->d_kstart.p, c->d_cnt);
+    }
+    {
+        ProfScope p(c, "raster");
+        psk_raster(s, c->debug, c->d_spans.p, svals, c->d_kstart.p, c->d_tris.p, dd, f, c->tiling.waves_per_group, c->d_cnt);
+    }
+    {
+        ProfScope p(c, "shade");
+        for (int k = 1; k < 8; k++)
+            if (shade_progs[k]) psk_shade(s, k, c->d_spans.p, c->d_tris.p, c->d_vary.p, nvs, dd, f, c->d_cnt);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->stats.flushes++;
+    return PSGPU_OK;
+}
+
+int run_cmd(psgpu_ctx* c, const Cmd& cmd)
+{
+    switch (cmd.type) {
+    case CMD_FLUSH:
+        return run_flush(c, cmd);
+    case CMD_CLEAR_DEPTH: {
+        // clear16 fills m_bytes = scanline * height (fbo.cpp:348-371)
+        uint32_t v;
+        memcpy(&v, &cmd.fval, 4);
+        psk_fill_u32(c->stream, (uint32_t*)cmd.tg.depth, (int64_t)cmd.tg.depth_scan * cmd.tg.depth_h, v, c->d_cnt);
+        return PSGPU_OK;
+    }
+    case CMD_CLEAR_COLOUR:
+        psk_clear_colour(c->stream, c->display, c->dev_w, c->dev_h, c->dev_w, cmd.uval, c->d_cnt);
+        return PSGPU_OK;
+    }
+    return PSGPU_OK;
+}
+
+int queue_to_log(psgpu_ctx* c)
+{
+    if (c->queue.empty()) return PSGPU_OK;
+    Cmd cmd;
+    cmd.type = CMD_FLUSH;
+    cmd.draws.swap(c->queue);
+    c->queue_tris = 0;
+    int rc = current_targets(c, cmd.tg);
+    if (rc) return rc;
+    c->log.push_back(std::move(cmd));
+    return run_cmd(c, c->log.back());
+}
+
+int do_sync(psgpu_ctx* c)
+{
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    for (int attempt = 0; attempt < 16; attempt++) {
+        HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(PsCounters), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_sizes, c->d_sizes, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->profiling) prof_collect(c);
+        if (!c->h_cnt->overflow) {
+            c->stats.tris_culled += (int64_t)c->h_cnt->tris_culled;
+            c->stats.tris_zrejected += (int64_t)c->h_cnt->tris_zrejected;
+            c->stats.tris_dropped += (int64_t)c->h_cnt->tris_dropped;
+            c->stats.spans += (int64_t)c->h_cnt->spans;
+            c->stats.frags_rasterised += (int64_t)c->h_cnt->frags_rasterised;
+            c->stats.frags_shaded += (int64_t)c->h_cnt->frags_shaded;
+            c->log.clear();
+            HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
+            HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
+            return PSGPU_OK;
+        }
+        // grow and replay the log
+        int64_t need_spans = (int64_t)c->h_sizes[2], need_segs = (int64_t)c->h_sizes[3];
+        if (need_spans > c->span_cap) {
+            c->span_cap = need_spans + need_spans / 4 + 1024;
+            c->seg_cap = std::max<int64_t>(c->seg_cap, c->span_cap * 3 / 2);
+        } else if (need_segs > c->seg_cap) {
+            c->seg_cap = need_segs + need_segs / 4 + 1024;
+        } else {
+            c->span_cap *= 2;
+            c->seg_cap *= 2;
+        }
+        c->stats.retries++;
+        HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
+        for (const Cmd& cmd : c->log)
+            if ((rc = run_cmd(c, cmd))) return rc;
+    }
+    return fail(c, PSGPU_ERR_NOMEM, "intermediate buffers kept overflowing");
+}
+
+int need_ctx(psgpu_ctx* c) { return c ? PSGPU_OK : PSGPU_ERR_INVALID; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** out)
+{
+    (void)flags;
+    if (!out) return PSGPU_ERR_INVALID;
+    *out = nullptr;
+    if (width < 1 || height < 1) return PSGPU_ERR_INVALID;
+    // default depth scanline (pipeline.cpp:31); it is shorter than the row when
+    // width % 4 == 1, which makes the reference alias the row ends
+    int dscan = (int)(width / 4.0f + 0.5f) * 4;
+    if (dscan < width) return PSGPU_ERR_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PSGPU_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return PSGPU_ERR_DEVICE;
+    psgpu_ctx* c = new psgpu_ctx();
+    c->device = device;
+    c->dev_w = c->vp_w = width;
+    c->dev_h = c->vp_h = height;
+    c->depth_scan = dscan;
+    c->uniforms.resize(kMaxUniforms);
+    memset(&c->stats, 0, sizeof(c->stats));
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc((void**)&c->display, (size_t)width * height * 4) == hipSuccess &&
+              hipMalloc((void**)&c->depth, (size_t)dscan * height * 4) == hipSuccess &&
+              hipMalloc((void**)&c->d_cnt, sizeof(PsCounters)) == hipSuccess &&
+              hipHostMalloc((void**)&c->h_cnt, sizeof(PsCounters)) == hipSuccess &&
+              hipMalloc((void**)&c->d_sizes, 4 * sizeof(uint32_t)) == hipSuccess &&
+              hipHostMalloc((void**)&c->h_sizes, 4 * sizeof(uint32_t)) == hipSuccess &&
+              hipMemsetAsync(c->display, 0, (size_t)width * height * 4, c->stream) == hipSuccess &&
+              hipMemsetAsync(c->depth, 0, (size_t)dscan * height * 4, c->stream) == hipSuccess &&
+              hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream) == hipSuccess &&
+              hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream) == hipSuccess &&
+              hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) {
+        psgpu_destroy(c);
+        return PSGPU_ERR_DEVICE;
+    }
+    memset(c->h_cnt, 0, sizeof(PsCounters));
+    *out = c;
+    return PSGPU_OK;
+}
+
+int psgpu_destroy(psgpu_ctx* c)
+{
+    if (!c) return PSGPU_ERR_INVALID;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    auto fr = [](void* p) { if (p) hipFree(p); };
+    fr(c->display); fr(c->depth); fr(c->d_cnt); fr(c->d_sizes);
+    if (c->h_cnt) hipHostFree(c->h_cnt);
+    if (c->h_sizes) hipHostFree(c->h_sizes);
+    for (auto& t : c->tex) fr(t.d);
+    for (auto& v : c->vbo) fr(v.d);
+    fr(c->d_draws.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
+    fr(c->d_spantri.p); fr(c->d_spans.p); fr(c->d_segcount.p); fr(c->d_segoff.p); fr(c->d_keys.p); fr(c->d_vals.p);
+    fr(c->d_keys2.p); fr(c->d_vals2.p); fr(c->d_hist.p); fr(c->d_histoff.p); fr(c->d_kstart.p); fr(c->d_winner.p);
+    fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
+    for (int i = 0; i < psgpu_ctx::kStage; i++) {
+        if (c->h_stage[i]) hipHostFree(c->h_stage[i]);
+        if (c->stage_ev[i]) hipEventDestroy(c->stage_ev[i]);
+    }
+    for (auto& r : c->prof_pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PSGPU_OK;
+}
+
+const char* psgpu_last_error(psgpu_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* psgpu_stream(psgpu_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+// ---- textures --------------------------------------------------------------
+int psgpu_texture_create(psgpu_ctx* c, int width, int scanline, int height, int elem_len, int extra_layers,
+                         int wrap_mode, int mip_levels, const void* pixels, int* out_idx)
+{
+    if (need_ctx(c) || !out_idx) return PSGPU_ERR_INVALID;
+    if (elem_len != 1 && elem_len != 4) return fail(c, PSGPU_ERR_INVALID, "elemLen must be 1 or 4 (fbo.cpp:21-24)");
+    if (width < 1 || height < 1 || scanline < width * elem_len) return fail(c, PSGPU_ERR_INVALID, "bad texture size");
+    if (extra_layers < 0 || extra_layers > PS_MAX_LAYERS - 1) return fail(c, PSGPU_ERR_RANGE, "extraLayers out of range");
+    if (wrap_mode != PSGPU_CLAMP && wrap_mode != PSGPU_WRAP) return fail(c, PSGPU_ERR_INVALID, "bad wrap mode");
+    int maxl = 1;
+    while (((width >> maxl) > 0 || (height >> maxl) > 0) && maxl < PS_MAX_LEVELS) maxl++;
+    if (mip_levels < 1 || mip_levels > maxl) return fail(c, PSGPU_ERR_RANGE, "mip levels out of range");
+    if (mip_levels > 1 && scanline != width * elem_len) return fail(c, PSGPU_ERR_INVALID, "mipmapped textures need a tight scanline");
+    Tex t;
+    t.w = width; t.h = height; t.scan = scanline; t.elem = elem_len; t.layers = 1 + extra_layers; t.wrap = wrap_mode;
+    t.levels = mip_levels;
+    int64_t off = (int64_t)scanline * height;
+    t.level_off[0] = 0;
+    for (int l = 1; l < mip_levels; l++) {
+        t.level_off[l] = off;
+        int w = std::max(1, width >> l), h = std::max(1, height >> l);
+        off += (int64_t)w * h * elem_len;
+    }
+    t.layer_bytes = (off + 255) / 256 * 256;
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMalloc((void**)&t.d, (size_t)t.layer_bytes * t.layers));
+    HIPCHK(c, hipMemsetAsync(t.d, 0, (size_t)t.layer_bytes * t.layers, c->stream));
+    if (pixels) HIPCHK(c, hipMemcpyAsync(t.d, pixels, (size_t)scanline * height, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    t.alive = true;
+    size_t i = 0;
+    for (; i < c->tex.size(); i++) if (!c->tex[i].alive) break;   // tex.cpp:6-16: first vacant slot
+    if (i == c->tex.size()) c->tex.push_back(t); else c->tex[i] = t;
+    *out_idx = (int)i;
+    return PSGPU_OK;
+}
+
+static int level_bytes(const Tex& t, int level, int* w, int* h, int* scan)
+{
+    *w = std::max(1, t.w >> level);
+    *h = std::max(1, t.h >> level);
+    *scan = level == 0 ? t.scan : *w * t.elem;
+    return PSGPU_OK;
+}
+
+int psgpu_texture_upload(psgpu_ctx* c, int tex, int layer, int level, const void* pixels, size_t bytes)
+{
+    if (need_ctx(c) || !pixels) return PSGPU_ERR_INVALID;
+    if (!valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+    Tex& t = c->tex[tex];
+    if (layer < 0 || layer >= t.layers || level < 0 || level >= t.levels) return fail(c, PSGPU_ERR_RANGE, "layer/level out of range");
+    int w, h, scan;
+    level_bytes(t, level, &w, &h, &scan);
+    if (bytes != (size_t)scan * h) return fail(c, PSGPU_ERR_INVALID, "expected %zu bytes", (size_t)scan * h);
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(t.d + t.layer_bytes * layer + t.level_off[level], pixels, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSGPU_OK;
+}
+
+int psgpu_texture_download(psgpu_ctx* c, int tex, int layer, int level, void* pixels, size_t bytes)
+{
+    if (need_ctx(c) || !pixels) return PSGPU_ERR_INVALID;
+    if (!valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+    Tex& t = c->tex[tex];
+    if (layer < 0 || layer >= t.layers || level < 0 || level >= t.levels) return fail(c, PSGPU_ERR_RANGE, "layer/level out of range");
+    int w, h, scan;
+    level_bytes(t, level, &w, &h, &scan);
+    if (bytes != (size_t)scan * h) return fail(c, PSGPU_ERR_INVALID, "expected %zu bytes", (size_t)scan * h);
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(pixels, t.d + t.layer_bytes * layer + t.level_off[level], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSGPU_OK;
+}
+
+int psgpu_texture_generate_mips(psgpu_ctx* c, int tex)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (!valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+    int rc = do_sync(c);
+    if (rc) return rc;
+    Tex& t = c->tex[tex];
+    for (int L = 0; L < t.layers; L++)
+        for (int l = 1; l < t.levels; l++) {
+            int ws, hs, ss, wd, hd, sd;
+            level_bytes(t, l - 1, &ws, &hs, &ss);
+            level_bytes(t, l, &wd, &hd, &sd);
+            psk_mip(c->stream, t.d + t.layer_bytes * L + t.level_off[l - 1], ws, hs, ss, t.d + t.layer_bytes * L + t.level_off[l],
+                    wd, hd, t.elem);
+        }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSGPU_OK;
+}
+
+int psgpu_texture_destroy(psgpu_ctx* c, int tex)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (tex < 0 || tex >= (int)c->tex.size()) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+    if (!c->tex[tex].alive) return PSGPU_OK;
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipFree(c->tex[tex].d));
+    c->tex[tex] = Tex();
+    return PSGPU_OK;
+}
+
+int psgpu_texture_info(psgpu_ctx* c, int tex, int level, int* width, int* scanline, int* height, int* elem_len)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (!valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+    Tex& t = c->tex[tex];
+    if (level < 0 || level >= t.levels) return fail(c, PSGPU_ERR_RANGE, "level out of range");
+    int w, h, s;
+    level_bytes(t, level, &w, &h, &s);
+    if (width) *width = w;
+    if (scanline) *scanline = s;
+    if (height) *height = h;
+    if (elem_len) *elem_len = t.elem;
+    return PSGPU_OK;
+}
+
+// ---- vertex streams ---------------------------------------------------------
+int psgpu_vbo_create(psgpu_ctx* c, size_t unit_bytes, size_t unit_count, int* out_vbo)
+{
+    if (need_ctx(c) || !out_vbo) return PSGPU_ERR_INVALID;
+    if (unit_bytes == 0 || unit_count == 0) return fail(c, PSGPU_ERR_INVALID, "empty VBO");
+    Vbo v;
+    v.unit = unit_bytes;
+    v.count = unit_count;
+    if (hipMalloc((void**)&v.d, unit_bytes * unit_count) != hipSuccess) return fail(c, PSGPU_ERR_NOMEM, "VBO allocation failed");
+    v.alive = true;
+    size_t i = 0;
+    for (; i < c->vbo.size(); i++) if (!c->vbo[i].alive) break;
+    if (i == c->vbo.size()) c->vbo.push_back(v); else c->vbo[i] = v;
+    *out_vbo = (int)i;
+    return PSGPU_OK;
+}
+
+int psgpu_vbo_update(psgpu_ctx* c, int vbo, const void* src)
+{
+    if (need_ctx(c) || !src) return PSGPU_ERR_INVALID;
+    if (vbo < 0 || vbo >= (int)c->vbo.size() || !c->vbo[vbo].alive) return fail(c, PSGPU_ERR_RANGE, "VBO out of range");
+    int rc = do_sync(c);
+    if (rc) return rc;
+    Vbo& v = c->vbo[vbo];
+    HIPCHK(c, hipMemcpyAsync(v.d, src, v.unit * v.count, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSGPU_OK;
+}
+
+int psgpu_vbo_destroy(psgpu_ctx* c, int vbo)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (vbo < 0 || vbo >= (int)c->vbo.size()) return fail(c, PSGPU_ERR_RANGE, "VBO out of range");
+    if (!c->vbo[vbo].alive) return PSGPU_OK;
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipFree(c->vbo[vbo].d));
+    c->vbo[vbo] = Vbo();
+    for (auto& a : c->vao)
+        if (a.alive)
+            for (int s = 0; s < kMaxSlots; s++) if (a.slot[s] == vbo) a.slot[s] = -1;
+    return PSGPU_OK;
+}
+
+int psgpu_vao_create(psgpu_ctx* c, int* out_vao)
+{
+    if (need_ctx(c) || !out_vao) return PSGPU_ERR_INVALID;
+    Vao a;
+    a.alive = true;
+    for (int s = 0; s < kMaxSlots; s++) a.slot[s] = -1;
+    size_t i = 0;
+    for (; i < c->vao.size(); i++) if (!c->vao[i].alive) break;   // pipeline.cpp:118-136
+    if (i == c->vao.size()) c->vao.push_back(a); else c->vao[i] = a;
+    *out_vao = (int)i;
+    return PSGPU_OK;
+}
+
+static int check_vao_slot(psgpu_ctx* c, int vao, int slot)
+{
+    if (vao < 0 || vao >= (int)c->vao.size() || !c->vao[vao].alive) return fail(c, PSGPU_ERR_RANGE, "vao out of range");
+    if (slot < 0 || slot >= kMaxSlots) return fail(c, PSGPU_ERR_RANGE, "VBO slot out of range");
+    return PSGPU_OK;
+}
+
+int psgpu_vao_attach(psgpu_ctx* c, int vao, int slot, int vbo, int* out_previous)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = check_vao_slot(c, vao, slot);
+    if (rc) return rc;
+    if (vbo != -1 && (vbo < 0 || vbo >= (int)c->vbo.size() || !c->vbo[vbo].alive)) return fail(c, PSGPU_ERR_RANGE, "VBO out of range");
+    if (out_previous) *out_previous = c->vao[vao].slot[slot];
+    c->vao[vao].slot[slot] = vbo;
+    return PSGPU_OK;
+}
+
+int psgpu_vao_detach(psgpu_ctx* c, int vao, int slot, int* out_previous) { return psgpu_vao_attach(c, vao, slot, -1, out_previous); }
+
+int psgpu_vao_get(psgpu_ctx* c, int vao, int slot, int* out_vbo)
+{
+    if (need_ctx(c) || !out_vbo) return PSGPU_ERR_INVALID;
+    int rc = check_vao_slot(c, vao, slot);
+    if (rc) return rc;
+    *out_vbo = c->vao[vao].slot[slot];
+    return PSGPU_OK;
+}
+
+int psgpu_vao_destroy(psgpu_ctx* c, int vao)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (vao < 0 || vao >= (int)c->vao.size()) return fail(c, PSGPU_ERR_RANGE, "vao out of range");
+    if (!c->vao[vao].alive) return PSGPU_OK;
+    int slots[kMaxSlots];
+    memcpy(slots, c->vao[vao].slot, sizeof(slots));
+    c->vao[vao].alive = false;
+    for (int s = 0; s < kMaxSlots; s++)
+        if (slots[s] >= 0) {
+            int rc = psgpu_vbo_destroy(c, slots[s]);
+            if (rc) return rc;
+        }
+    return PSGPU_OK;
+}
+
+// ---- programmes -------------------------------------------------------------
+int psgpu_program_create(psgpu_ctx* c, int vp, int ip, int fp, int* out_prog)
+{
+    if (need_ctx(c) || !out_prog) return PSGPU_ERR_INVALID;
+    if (prog_nv(vp) < 0 || prog_nv(ip) < 0 || prog_nv(fp) < 0)
+        return fail(c, PSGPU_ERR_UNSUPPORTED, "processor has no GPU twin");
+    if (vp != ip || ip != fp)
+        return fail(c, PSGPU_ERR_UNSUPPORTED, "VP/IP/FP must belong to one built-in programme (varying layouts differ)");
+    Prog p;
+    p.alive = true; p.vp = vp; p.ip = ip; p.fp = fp;
+    size_t i = 0;
+    for (; i < c->prog.size(); i++) if (!c->prog[i].alive) break;   // prog.cpp:52-60
+    if (i == c->prog.size()) c->prog.push_back(p); else c->prog[i] = p;
+    *out_prog = (int)i;
+    return PSGPU_OK;
+}
+
+int psgpu_program_destroy(psgpu_ctx* c, int prog)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (prog < 0 || prog >= (int)c->prog.size()) return fail(c, PSGPU_ERR_RANGE, "programme out of range");
+    c->prog[prog].alive = false;
+    if (c->cur_prog == prog) c->cur_prog = -1;
+    return PSGPU_OK;
+}
+
+int psgpu_program_use(psgpu_ctx* c, int prog)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (prog < 0 || prog >= (int)c->prog.size()) return fail(c, PSGPU_ERR_RANGE, "programme out of range");
+    c->cur_prog = c->prog[prog].alive ? prog : -1;
+    return PSGPU_OK;
+}
+
+// ---- state ------------------------------------------------------------------
+int psgpu_uniform_set(psgpu_ctx* c, int slot, const void* data, size_t len)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (slot < 0 || slot >= kMaxUniforms) return fail(c, PSGPU_ERR_RANGE, "uniform slot out of range");
+    auto& u = c->uniforms[slot];
+    if (!data) { u.clear(); u.shrink_to_fit(); return PSGPU_OK; }
+    u.assign((const uint8_t*)data, (const uint8_t*)data + len);
+    return PSGPU_OK;
+}
+
+int psgpu_behavior_enable(psgpu_ctx* c, int bits) { if (need_ctx(c)) return PSGPU_ERR_INVALID; c->behavior |= bits; return PSGPU_OK; }
+int psgpu_behavior_disable(psgpu_ctx* c, int bits) { if (need_ctx(c)) return PSGPU_ERR_INVALID; c->behavior &= ~bits; return PSGPU_OK; }
+int psgpu_behavior_get(psgpu_ctx* c, int* bits)
+{
+    if (need_ctx(c) || !bits) return PSGPU_ERR_INVALID;
+    *bits = c->behavior;
+    return PSGPU_OK;
+}
+
+int psgpu_viewport(psgpu_ctx* c, int width, int height)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (width < 1 || height < 1) return fail(c, PSGPU_ERR_INVALID, "bad viewport");
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    c->vp_w = width;
+    c->vp_h = height;
+    return PSGPU_OK;
+}
+
+int psgpu_target_set(psgpu_ctx* c, int fbo_slot, int tex)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (fbo_slot < 0 || fbo_slot >= 16) return fail(c, PSGPU_ERR_RANGE, "FBO slot out of range");
+    if (fbo_slot != 0) return fail(c, PSGPU_ERR_UNSUPPORTED, "the built-in processors write FBO slot 0 only");
+    if (tex != PSGPU_DISPLAY && !valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    c->fbo0 = tex;
+    return PSGPU_OK;
+}
+
+int psgpu_depth_set(psgpu_ctx* c, int tex)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (tex != PSGPU_DEFAULT_DEPTH) {
+        if (!valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "PuresoftPipeline::setDepth");
+        // pipeline.cpp:231-235
+        if (c->tex[tex].elem != 4 || c->tex[tex].scan % 4) return fail(c, PSGPU_ERR_INVALID, "PuresoftPipeline::setDepth");
+    }
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    c->depth_sel = tex;
+    return PSGPU_OK;
+}
+
+int psgpu_clear_depth(psgpu_ctx* c, float furthest)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    Cmd cmd;
+    cmd.type = CMD_CLEAR_DEPTH;
+    cmd.fval = furthest;
+    if ((rc = current_targets(c, cmd.tg))) return rc;
+    c->log.push_back(std::move(cmd));
+    return run_cmd(c, c->log.back());
+}
+
+int psgpu_clear_colour(psgpu_ctx* c, uint32_t bgra)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    Cmd cmd;
+    cmd.type = CMD_CLEAR_COLOUR;
+    cmd.uval = bgra;
+    if ((rc = current_targets(c, cmd.tg))) return rc;
+    c->log.push_back(std::move(cmd));
+    return run_cmd(c, c->log.back());
+}
+
+// ---- drawing ------------------------------------------------------------------
+static int uniform_bytes(psgpu_ctx* c, int slot, size_t need, const uint8_t** out)
+{
+    const auto& u = c->uniforms[slot];
+    if (u.size() < need) return fail(c, PSGPU_ERR_STATE, "uniform slot %d not set (needs %zu bytes)", slot, need);
+    *out = u.data();
+    return PSGPU_OK;
+}
+
+static int uniform_tex(psgpu_ctx* c, int slot, PsTex* out)
+{
+    const uint8_t* p;
+    int rc = uniform_bytes(c, slot, 4, &p);
+    if (rc) return rc;
+    int idx;
+    memcpy(&idx, p, 4);
+    if (!valid_tex(c, idx)) return fail(c, PSGPU_ERR_STATE, "uniform %d names texture %d, which does not exist", slot, idx);
+    if (c->tex[idx].elem != 4) return fail(c, PSGPU_ERR_STATE, "sampled textures need 4-byte texels");
+    *out = tex_desc(c->tex[idx]);
+    return PSGPU_OK;
+}
+
+int psgpu_draw(psgpu_ctx* c, int vao)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    // drawvao.cpp:12-15: silently nothing without a programme or with a bad VAO
+    if (c->cur_prog < 0 || vao < 0 || vao >= (int)c->vao.size() || !c->vao[vao].alive) return PSGPU_OK;
+    if (c->behavior & PSGPU_ALPHABLEND)
+        return fail(c, PSGPU_ERR_UNSUPPORTED, "alpha blending is not supported by the GPU path");
+    const Prog& pg = c->prog[c->cur_prog];
+    const Vao& a = c->vao[vao];
+    PsDraw d;
+    memset(&d, 0, sizeof(d));
+    d.prog = pg.vp;
+    d.behavior = c->behavior;
+    d.nv = prog_nv(pg.vp);
+    d.writes_colour = pg.vp != PSGPU_PROC_DEF05;
+    // processVertices stops at the first empty VBO (vertthrd.cpp:14-51)
+    int64_t nverts = -1;
+    for (int s = 0; s < kMaxSlots; s++)
+        if (a.slot[s] >= 0) {
+            int64_t n = (int64_t)c->vbo[a.slot[s]].count;
+            if (nverts < 0 || n < nverts) nverts = n;
+        }
+    if (nverts < 0) return PSGPU_OK;
+    d.tri_count = nverts / 3;
+    // vertex slots read by the VP twins, with their element sizes
+    int need[5] = {16, 0, 0, 0, 0};
+    switch (d.prog) {
+    case PSGPU_PROC_DEF01: need[3] = 16; need[4] = 8; break;
+    case PSGPU_PROC_DEF02: need[1] = 16; need[2] = 16; break;
+    case PSGPU_PROC_DEF03: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
+    case PSGPU_PROC_LAMBERT: need[3] = 16; need[4] = 8; break;
+    case PSGPU_PROC_BLINN: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
+    default: break;
+    }
+    for (int s = 0; s < 5; s++) {
+        if (!need[s]) continue;
+        if (a.slot[s] < 0) return fail(c, PSGPU_ERR_STATE, "VBO slot %d is not attached", s);
+        const Vbo& v = c->vbo[a.slot[s]];
+        if ((int)v.unit < need[s]) return fail(c, PSGPU_ERR_STATE, "VBO slot %d elements must hold %d bytes", s, need[s]);
+        d.slot[s] = v.d;
+        d.unit[s] = (int)v.unit;
+    }
+    const uint8_t* p;
+    int rc;
+    switch (d.prog) {
+    case PSGPU_PROC_DEF01: case PSGPU_PROC_DEF02: case PSGPU_PROC_DEF03: case PSGPU_PROC_BLINN:
+        if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(d.PV, p, 64);
+        if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(d.M, p, 64);
+        if ((rc = uniform_bytes(c, 5, 64, &p))) return rc; memcpy(d.Mrot, p, 64);
+        if ((rc = uniform_bytes(c, 7, 16, &p))) return rc; memcpy(d.lightPos, p, 16);
+        if ((rc = uniform_bytes(c, 8, 16, &p))) return rc; memcpy(d.cameraPos, p, 16);
+        if (d.prog != PSGPU_PROC_DEF02 && (rc = uniform_tex(c, 9, &d.tex0))) return rc;
+        if ((d.prog == PSGPU_PROC_DEF03 || d.prog == PSGPU_PROC_BLINN) && (rc = uniform_tex(c, 10, &d.tex1))) return rc;
+        d.spec_exp = 32;
+        if (d.prog == PSGPU_PROC_BLINN && c->uniforms[11].size() >= 4) memcpy(&d.spec_exp, c->uniforms[11].data(), 4);
+        if (d.spec_exp < 0) return fail(c, PSGPU_ERR_INVALID, "specular exponent must be >= 0");
+        break;
+    case PSGPU_PROC_DEF04: {
+        if ((rc = uniform_bytes(c, 1, 64, &p))) return rc;
+        float V[16];
+        memcpy(V, p, 64);
+        // skybox.cpp:34-37: transpose, zero the last column
+        for (int r = 0; r < 4; r++)
+            for (int cc = 0; cc < 4; cc++) d.invView[cc * 4 + r] = V[r * 4 + cc];
+        d.invView[12] = d.invView[13] = d.invView[14] = d.invView[15] = 0.0f;
+        if ((rc = uniform_tex(c, 2, &d.tex0))) return rc;
+        break;
+    }
+    case PSGPU_PROC_DEF05: {
+        float PV[16], M[16];
+        if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(PV, p, 64);
+        if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(M, p, 64);
+        host_m4m4(d.PVM, PV, M);
+        break;
+    }
+    case PSGPU_PROC_LAMBERT:
+        if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(d.PV, p, 64);
+        if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(d.M, p, 64);
+        if ((rc = uniform_bytes(c, 5, 64, &p))) return rc; memcpy(d.Mrot, p, 64);
+        if ((rc = uniform_bytes(c, 7, 12, &p))) return rc; memcpy(d.lightDir, p, 12);
+        if ((rc = uniform_tex(c, 9, &d.tex0))) return rc;
+        break;
+    }
+    // targets must cover the viewport
+    Targets tg;
+    if ((rc = current_targets(c, tg))) return rc;
+    if (tg.colour_w < c->vp_w || tg.colour_h < c->vp_h || tg.depth_w < c->vp_w || tg.depth_h < c->vp_h)
+        return fail(c, PSGPU_ERR_STATE, "render targets are smaller than the viewport");
+    if (d.tri_count == 0) return PSGPU_OK;
+    d.tri_start = c->queue_tris;
+    d.id_base = c->dbg_tri_counter;
+    c->dbg_tri_counter += d.tri_count;
+    c->queue_tris += d.tri_count;
+    c->stats.tris_in += d.tri_count;
+    c->queue.push_back(d);
+    return PSGPU_OK;
+}
+
+int psgpu_flush(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    return queue_to_log(c);
+}
+
+int psgpu_sync(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    return do_sync(c);
+}
+
+int psgpu_swap_buffers(psgpu_ctx* c) { return psgpu_flush(c); }
+
+int psgpu_read_colour(psgpu_ctx* c, void* host, size_t bytes)
+{
+    if (need_ctx(c) || !host) return PSGPU_ERR_INVALID;
+    if (bytes != (size_t)c->dev_w * c->dev_h * 4) return fail(c, PSGPU_ERR_INVALID, "display is %d x %d BGRA8", c->dev_w, c->dev_h);
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(host, c->display, bytes, hipMemcpyDeviceToHost));
+    return PSGPU_OK;
+}
+
+int psgpu_read_depth(psgpu_ctx* c, void* host, size_t bytes)
+{
+    if (need_ctx(c) || !host) return PSGPU_ERR_INVALID;
+    if (bytes != (size_t)c->depth_scan * c->dev_h * 4) return fail(c, PSGPU_ERR_INVALID, "depth is %d x %d floats", c->depth_scan, c->dev_h);
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(host, c->depth, bytes, hipMemcpyDeviceToHost));
+    return PSGPU_OK;
+}
+
+int psgpu_display_device_ptr(psgpu_ctx* c, void** ptr)
+{
+    if (need_ctx(c) || !ptr) return PSGPU_ERR_INVALID;
+    *ptr = c->display;
+    return PSGPU_OK;
+}
+
+int psgpu_depth_device_ptr(psgpu_ctx* c, void** ptr, int* scan)
+{
+    if (need_ctx(c) || !ptr) return PSGPU_ERR_INVALID;
+    *ptr = c->depth;
+    if (scan) *scan = c->depth_scan;
+    return PSGPU_OK;
+}
+
+int psgpu_get_stats(psgpu_ctx* c, psgpu_stats* out)
+{
+    if (need_ctx(c) || !out) return PSGPU_ERR_INVALID;
+    *out = c->stats;
+    return PSGPU_OK;
+}
+
+int psgpu_reset_stats(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = do_sync(c);
+    memset(&c->stats, 0, sizeof(c->stats));
+    return rc;
+}
+
+// ---- debug planes -----------------------------------------------------------
+int psgpu_debug_enable(psgpu_ctx* c, int enable)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    c->debug = enable != 0;
+    if (c->debug) return psgpu_debug_reset(c);
+    return PSGPU_OK;
+}
+
+int psgpu_debug_reset(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = do_sync(c);
+    if (rc) return rc;
+    size_t n = (size_t)c->vp_w * c->vp_h;
+    if (c->dbg_w != c->vp_w || c->dbg_h != c->vp_h || !c->dbg_last) {
+        for (void* p : {(void*)c->dbg_last, (void*)c->dbg_rcount, (void*)c->dbg_rhash, (void*)c->dbg_pcount, (void*)c->dbg_phash})
+            if (p) hipFree(p);
+        HIPCHK(c, hipMalloc((void**)&c->dbg_last, n * 4));
+        HIPCHK(c, hipMalloc((void**)&c->dbg_rcount, n * 4));
+        HIPCHK(c, hipMalloc((void**)&c->dbg_rhash, n * 4));
+        HIPCHK(c, hipMalloc((void**)&c->dbg_pcount, n * 4));
+        HIPCHK(c, hipMalloc((void**)&c->dbg_phash, n * 4));
+        c->dbg_w = c->vp_w;
+        c->dbg_h = c->vp_h;
+    }
+    psk_fill_u32(c->stream, (uint32_t*)c->dbg_last, (int64_t)n, 0xffffffffu, nullptr);
+    psk_fill_u32(c->stream, (uint32_t*)c->dbg_rcount, (int64_t)n, 0, nullptr);
+    psk_fill_u32(c->stream, c->dbg_rhash, (int64_t)n, 0, nullptr);
+    psk_fill_u32(c->stream, (uint32_t*)c->dbg_pcount, (int64_t)n, 0, nullptr);
+    psk_fill_u32(c->stream, c->dbg_phash, (int64_t)n, 0, nullptr);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dbg_tri_counter = 0;
+    return PSGPU_OK;
+}
+
+int psgpu_debug_read(psgpu_ctx* c, int32_t* last_pass, int32_t* raster_count, uint32_t* raster_hash, int32_t* pass_count,
+                     uint32_t* pass_hash)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (!c->dbg_last) return fail(c, PSGPU_ERR_STATE, "debug planes are not enabled");
+    int rc = do_sync(c);
+    if (rc) return rc;
+    size_t n = (size_t)c->dbg_w * c->dbg_h * 4;
+    if (last_pass) HIPCHK(c, hipMemcpy(last_pass, c->dbg_last, n, hipMemcpyDeviceToHost));
+    if (raster_count) HIPCHK(c, hipMemcpy(raster_count, c->dbg_rcount, n, hipMemcpyDeviceToHost));
+    if (raster_hash) HIPCHK(c, hipMemcpy(raster_hash, c->dbg_rhash, n, hipMemcpyDeviceToHost));
+    if (pass_count) HIPCHK(c, hipMemcpy(pass_count, c->dbg_pcount, n, hipMemcpyDeviceToHost));
+    if (pass_hash) HIPCHK(c, hipMemcpy(pass_hash, c->dbg_phash, n, hipMemcpyDeviceToHost));
+    return PSGPU_OK;
+}
+
+// ---- tuning and timing ------------------------------------------------------
+int psgpu_tiling_set(psgpu_ctx* c, const psgpu_tiling* t)
+{
+    if (need_ctx(c) || !t) return PSGPU_ERR_INVALID;
+    int cw = t->chunk_width, tr = t->tile_rows, w = t->waves_per_group;
+    bool okc = cw == 4 || cw == 8 || cw == 16 || cw == 32 || cw == 64;
+    bool okr = tr >= 1 && tr <= 64 && (tr & (tr - 1)) == 0;
+    if (!okc || !okr || w < 1 || w > 8) return fail(c, PSGPU_ERR_INVALID, "unsupported tiling");
+    if ((size_t)w * cw * 64 * 8 > 65536) return fail(c, PSGPU_ERR_INVALID, "tiling needs more than 64 KiB of LDS");
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    c->tiling = *t;
+    return PSGPU_OK;
+}
+
+int psgpu_tiling_get(psgpu_ctx* c, psgpu_tiling* t)
+{
+    if (need_ctx(c) || !t) return PSGPU_ERR_INVALID;
+    *t = c->tiling;
+    return PSGPU_OK;
+}
+
+int psgpu_profile_enable(psgpu_ctx* c, int enable)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = do_sync(c);
+    c->profiling = enable != 0;
+    return rc;
+}
+
+int psgpu_profile_read(psgpu_ctx* c, int max, const char** names, double* total_ms, int64_t* launches, int* count)
+{
+    if (need_ctx(c) || !count) return PSGPU_ERR_INVALID;
+    int rc = do_sync(c);
+    if (rc) return rc;
+    c->prof_names.clear();
+    for (auto& kv : c->prof) c->prof_names.push_back(kv.first);
+    int i = 0;
+    for (auto& kv : c->prof) {
+        if (i >= max) break;
+        if (names) names[i] = c->prof_names[i].c_str();
+        if (total_ms) total_ms[i] = kv.second.first;
+        if (launches) launches[i] = kv.second.second;
+        i++;
+    }
+    *count = i;
+    return PSGPU_OK;
+}
+
+int psgpu_profile_reset(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    int rc = do_sync(c);
+    c->prof.clear();
+    return rc;
+}
+
+int psgpu_selftest_math(psgpu_ctx* c, int op, const float* a, const float* b, float* out, int n)
+{
+    if (need_ctx(c) || !a || !out || n <= 0) return PSGPU_ERR_INVALID;
+    float *da = nullptr, *db = nullptr, *dout = nullptr;
+    HIPCHK(c, hipMalloc((void**)&da, (size_t)n * 4));
+    HIPCHK(c, hipMalloc((void**)&db, (size_t)n * 4));
+    HIPCHK(c, hipMalloc((void**)&dout, (size_t)n * 4));
+    HIPCHK(c, hipMemcpy(da, a, (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(db, b ? b : a, (size_t)n * 4, hipMemcpyHostToDevice));
+    psk_selftest(c->stream, op, da, db, dout, n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost));
+    hipFree(da); hipFree(db); hipFree(dout);
+    return PSGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,348 @@
+"""Seeded procedural scenes for the benchmark workloads and the parity tests.
+
+A Scene holds non-indexed triangle streams in the reference's VBO slot
+convention (test/scenobj.cpp:135-157: 0 position float4, 1 tangent, 2 binormal,
+3 normal float4, 4 texcoord float2), level-0 texture images (BGRA8 bytes),
+uniform slots and an ordered list of draws.  Matrices are column-major float32,
+as mcemaths stores them (matrix.cpp:515-557).
+"""
+import numpy as np
+
+from . import native as N
+
+
+class TextureSpec:
+    def __init__(self, images, wrap=N.WRAP, levels=1):
+        """images: list (one per layer) of uint8 arrays (h, w, 4) in BGRA order."""
+        self.images = [np.ascontiguousarray(im, np.uint8) for im in images]
+        self.height, self.width = self.images[0].shape[:2]
+        self.wrap = wrap
+        self.levels = levels
+        self.layers = len(self.images)
+
+
+class DrawSpec:
+    def __init__(self, mesh, prog, uniforms=None, behavior=None):
+        self.mesh = mesh
+        self.prog = prog
+        self.uniforms = uniforms or {}
+        self.behavior = behavior
+
+
+class Scene:
+    def __init__(self, width, height):
+        self.width, self.height = width, height
+        self.meshes = {}
+        self.textures = []
+        self.uniforms = {}
+        self.draws = []
+        self.clear_depth = 1.0
+        self.clear_colour = 0
+        self.name = ""
+
+    def triangles(self):
+        return sum(min(a.shape[0] for a in self.meshes[d.mesh].values()) // 3 for d in self.draws)
+
+    def vertex_bytes(self):
+        tot = 0
+        for d in self.draws:
+            m = self.meshes[d.mesh]
+            n = min(a.shape[0] for a in m.values()) // 3 * 3
+            tot += sum(n * a.shape[1] * 4 for a in m.values())
+        return tot
+
+
+# ---------------------------------------------------------------- matrices
+def perspective(fov_rad, aspect, near, far):
+    """mcemaths_make_proj_perspective (matrxgl.cpp:9-22), as a 4x4 row-major math matrix."""
+    m = np.eye(4)
+    h = 1.0 / np.tan(fov_rad / 2.0)
+    m[0, 0] = h / aspect
+    m[1, 1] = h
+    m[2, 2] = (far + near) / (near - far)
+    m[3, 2] = -1.0
+    m[2, 3] = 2.0 * (near * far) / (near - far)
+    m[3, 3] = 0.0
+    return m
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)):
+    eye, target, up = (np.asarray(v, np.float64) for v in (eye, target, up))
+    f = target - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    m[0, 3], m[1, 3], m[2, 3] = -s @ eye, -u @ eye, f @ eye
+    return m
+
+
+def translate(x, y, z):
+    m = np.eye(4)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def colmajor(m):
+    """row-major 4x4 math matrix -> 16 float32 in mcemaths column-major order"""
+    return np.ascontiguousarray(np.asarray(m, np.float64).T.ravel(), np.float32)
+
+
+def vec4(x, y, z, w=0.0):
+    return np.array([x, y, z, w], np.float32)
+
+
+# ---------------------------------------------------------------- noise
+def value_noise(h, w, cells, rng, channels=1, octaves=1, persistence=0.5):
+    """Seeded value noise in [0,1]: random lattice values, smoothstep interpolation,
+    optional fBm octaves; tiles periodically."""
+    out = np.zeros((h, w, channels), np.float64)
+    amp, tot = 1.0, 0.0
+    for o in range(octaves):
+        c = cells * (2 ** o)
+        lat = rng.random((c + 1, c + 1, channels))
+        lat[c, :] = lat[0, :]
+        lat[:, c] = lat[:, 0]
+        fy = (np.arange(h) + 0.5) / h * c
+        fx = (np.arange(w) + 0.5) / w * c
+        iy, ix = np.floor(fy).astype(int), np.floor(fx).astype(int)
+        ty, tx = fy - iy, fx - ix
+        ty, tx = ty * ty * (3 - 2 * ty), tx * tx * (3 - 2 * tx)
+        a = lat[iy][:, ix]
+        b = lat[iy][:, ix + 1]
+        cc = lat[iy + 1][:, ix]
+        d = lat[iy + 1][:, ix + 1]
+        tx3 = tx[None, :, None]
+        ty3 = ty[:, None, None]
+        top = a + (b - a) * tx3
+        bot = cc + (d - cc) * tx3
+        out += amp * (top + (bot - top) * ty3)
+        tot += amp
+        amp *= persistence
+    return out / tot
+
+
+def noise_texture(size, seed, cells=16, octaves=1):
+    rng = np.random.default_rng(seed)
+    v = value_noise(size, size, cells, rng, channels=3, octaves=octaves)
+    img = np.empty((size, size, 4), np.uint8)
+    img[..., :3] = np.clip(v * 255.0 + 0.5, 0, 255).astype(np.uint8)
+    img[..., 3] = 255
+    return img
+
+
+def normal_map(size, seed, cells=32, strength=4.0):
+    """tangent-space normal map of a seeded value-noise height field; r,g,b = x,y,z"""
+    rng = np.random.default_rng(seed)
+    hgt = value_noise(size, size, cells, rng, channels=1, octaves=3)[..., 0] * strength
+    dx = (np.roll(hgt, -1, 1) - np.roll(hgt, 1, 1)) * 0.5 * (cells / 8.0)
+    dy = (np.roll(hgt, -1, 0) - np.roll(hgt, 1, 0)) * 0.5 * (cells / 8.0)
+    n = np.stack([-dx, -dy, np.ones_like(dx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    img = np.empty((size, size, 4), np.uint8)
+    enc = np.clip((n * 0.5 + 0.5) * 255.0 + 0.5, 0, 255).astype(np.uint8)
+    img[..., 2], img[..., 1], img[..., 0] = enc[..., 0], enc[..., 1], enc[..., 2]
+    img[..., 3] = 255
+    return img
+
+
+def mip_levels(w, h):
+    n = 1
+    while (w >> n) > 0 or (h >> n) > 0:
+        n += 1
+    return n
+
+
+# ---------------------------------------------------------------- meshes
+def _orient_ccw(tris_pos, outward):
+    """flip triangles whose winding is not counter-clockwise seen from `outward`"""
+    a, b, c = tris_pos[:, 0, :3], tris_pos[:, 1, :3], tris_pos[:, 2, :3]
+    n = np.cross(b - a, c - a)
+    return (n * outward).sum(-1) < 0
+
+
+def uv_sphere(stacks, slices, radius=1.0, center=(0, 0, 0), seed=None, tangents=False):
+    """Lat-long sphere, two triangles per quad including the poles."""
+    i = np.arange(stacks + 1)[:, None]
+    j = np.arange(slices + 1)[None, :]
+    th = np.pi * i / stacks
+    ph = 2 * np.pi * j / slices
+    nx = np.sin(th) * np.cos(ph)
+    ny = np.cos(th) * np.ones_like(ph)
+    nz = np.sin(th) * np.sin(ph)
+    nrm = np.stack(np.broadcast_arrays(nx, ny, nz), -1)
+    uv = np.stack(np.broadcast_arrays(j / slices + 0 * i, i / stacks + 0 * j), -1)
+    tan = np.stack(np.broadcast_arrays(-np.sin(ph) + 0 * th, 0 * th + 0 * ph, np.cos(ph) + 0 * th), -1)
+    q = []
+    for a in range(stacks):
+        for b in range(slices):
+            q.append(((a, b), (a + 1, b), (a + 1, b + 1)))
+            q.append(((a, b), (a + 1, b + 1), (a, b + 1)))
+    idx = np.array(q)  # (T, 3, 2)
+    n = nrm[idx[..., 0], idx[..., 1]]
+    t_uv = uv[idx[..., 0], idx[..., 1]]
+    t_tan = tan[idx[..., 0], idx[..., 1]]
+    pos = n * radius + np.asarray(center, np.float64)
+    flip = _orient_ccw(pos, (pos - np.asarray(center, np.float64)).mean(1))
+    for arr in (n, t_uv, t_tan, pos):
+        arr[flip] = arr[flip][:, [0, 2, 1]]
+    if seed is not None:
+        perm = np.random.default_rng(seed).permutation(len(idx))
+        n, t_uv, t_tan, pos = n[perm], t_uv[perm], t_tan[perm], pos[perm]
+    nv = n.shape[0] * 3
+    P = np.ones((nv, 4), np.float32)
+    P[:, :3] = pos.reshape(-1, 3)
+    Nn = np.zeros((nv, 4), np.float32)
+    Nn[:, :3] = n.reshape(-1, 3)
+    UV = t_uv.reshape(-1, 2).astype(np.float32)
+    mesh = {0: P, 3: Nn, 4: UV}
+    if tangents:
+        T = np.zeros((nv, 4), np.float32)
+        T[:, :3] = t_tan.reshape(-1, 3)
+        B = np.zeros((nv, 4), np.float32)
+        B[:, :3] = np.cross(Nn[:, :3], T[:, :3])
+        mesh[1], mesh[2] = T, B
+    return mesh
+
+
+def terrain(cells, size, seed, height=8.0, uv_repeat=8.0):
+    rng = np.random.default_rng(seed)
+    n = cells + 1
+    hgt = value_noise(n, n, 6, rng, channels=1, octaves=6)[..., 0]
+    hgt = (hgt - hgt.mean()) * height * 2.0
+    xs = np.linspace(-size / 2, size / 2, n)
+    zs = np.linspace(-size / 2, size / 2, n)
+    X, Z = np.meshgrid(xs, zs)
+    step = size / cells
+    dhdx = np.gradient(hgt, step, axis=1)
+    dhdz = np.gradient(hgt, step, axis=0)
+    nrm = np.stack([-dhdx, np.ones_like(hgt), -dhdz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    tan = np.stack([np.ones_like(hgt), dhdx, np.zeros_like(hgt)], -1)
+    tan /= np.linalg.norm(tan, axis=-1, keepdims=True)
+    bin_ = np.cross(nrm, tan)
+    pos = np.stack([X, hgt, Z], -1)
+    uv = np.stack([np.arange(n)[None, :] / cells * uv_repeat + 0 * Z, np.arange(n)[:, None] / cells * uv_repeat + 0 * X], -1)
+    a = np.arange(cells)
+    ii, jj = np.meshgrid(a, a, indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    # two triangles per cell, counter-clockwise seen from +y
+    t1 = np.stack([np.stack([ii, jj], -1), np.stack([ii + 1, jj], -1), np.stack([ii + 1, jj + 1], -1)], 1)
+    t2 = np.stack([np.stack([ii, jj], -1), np.stack([ii + 1, jj + 1], -1), np.stack([ii, jj + 1], -1)], 1)
+    idx = np.stack([t1, t2], 1).reshape(-1, 3, 2)
+    P3 = pos[idx[..., 0], idx[..., 1]]
+    flip = _orient_ccw(P3, np.array([0.0, 1.0, 0.0]))
+    idx[flip] = idx[flip][:, [0, 2, 1]]
+    nv = idx.shape[0] * 3
+    out = {}
+    for slot, arr, w in ((0, pos, 1.0), (1, tan, 0.0), (2, bin_, 0.0), (3, nrm, 0.0)):
+        a4 = np.full((nv, 4), w, np.float32)
+        a4[:, :3] = arr[idx[..., 0], idx[..., 1]].reshape(-1, 3)
+        out[slot] = a4
+    out[4] = uv[idx[..., 0], idx[..., 1]].reshape(-1, 2).astype(np.float32)
+    return out, hgt, xs, zs
+
+
+# ---------------------------------------------------------------- workloads
+def sphere_scene(width=640, height=480, seed=1, tex_size=512):
+    """640x480 UV sphere (32x32, 2,048 triangles), Lambert x bilinear 512^2
+    value-noise texture, directional light normalize(1,1,1), camera at z=3, 60 deg."""
+    sc = Scene(width, height)
+    sc.name = "uv_sphere_2048_%dx%d" % (width, height)
+    sc.meshes["sphere"] = uv_sphere(32, 32, 1.0, seed=seed)
+    sc.textures.append(TextureSpec([noise_texture(tex_size, seed)], wrap=N.WRAP, levels=1))
+    P = perspective(np.radians(60.0), width / height, 0.1, 10.0)
+    V = look_at((0, 0, 3), (0, 0, 0))
+    sc.uniforms[3] = colmajor(P @ V)
+    sc.uniforms[4] = colmajor(np.eye(4))
+    sc.uniforms[5] = colmajor(np.eye(4))
+    L = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
+    sc.uniforms[7] = vec4(*L)
+    sc.uniforms[9] = np.array([0], np.int32)
+    sc.draws.append(DrawSpec("sphere", N.PROC_LAMBERT))
+    return sc
+
+
+def terrain_scene(width=1920, height=1080, seed=7, cells=300, n_spheres=64, tex_size=2048, spec_exp=32):
+    """1080p terrain (300x300 cells, 180k triangles) + 64 instanced spheres of
+    320 triangles, 2048^2 albedo + normal map with full mip chains, trilinear,
+    per-pixel Blinn-Phong with normal mapping, one point light."""
+    sc = Scene(width, height)
+    sc.name = "terrain_%dk_%dx%d" % ((cells * cells * 2 + n_spheres * 320) // 1000, width, height)
+    size = 100.0
+    ter, hgt, xs, zs = terrain(cells, size, seed)
+    sc.meshes["terrain"] = ter
+    sc.meshes["sphere"] = uv_sphere(10, 16, 1.5, tangents=True)
+    lv = mip_levels(tex_size, tex_size)
+    sc.textures.append(TextureSpec([noise_texture(tex_size, seed + 1, cells=32, octaves=4)], wrap=N.WRAP, levels=lv))
+    sc.textures.append(TextureSpec([normal_map(tex_size, seed + 2)], wrap=N.WRAP, levels=lv))
+    eye = np.array([0.0, 22.0, 62.0])
+    P = perspective(np.radians(60.0), width / height, 0.5, 200.0)
+    V = look_at(eye, (0, 0, -5))
+    sc.uniforms[3] = colmajor(P @ V)
+    sc.uniforms[4] = colmajor(np.eye(4))
+    sc.uniforms[5] = colmajor(np.eye(4))
+    sc.uniforms[7] = vec4(30.0, 60.0, 40.0, 0.0)
+    sc.uniforms[8] = vec4(*eye, 0.0)
+    sc.uniforms[9] = np.array([0], np.int32)
+    sc.uniforms[10] = np.array([1], np.int32)
+    sc.uniforms[11] = np.array([spec_exp], np.int32)
+    sc.draws.append(DrawSpec("terrain", N.PROC_BLINN))
+    rng = np.random.default_rng(seed + 3)
+    for k in range(n_spheres):
+        x, z = rng.uniform(-size * 0.45, size * 0.45, 2)
+        iz = int(np.clip(np.searchsorted(zs, z), 0, len(zs) - 1))
+        ix = int(np.clip(np.searchsorted(xs, x), 0, len(xs) - 1))
+        y = hgt[iz, ix] + 2.0
+        sc.draws.append(DrawSpec("sphere", N.PROC_BLINN, {4: colmajor(translate(x, y, z))}))
+    return sc
+
+
+# ---------------------------------------------------------------- GPU upload
+class GpuScene:
+    """A Scene resident on one psgpu context (VBOs, VAOs, textures, programmes)."""
+
+    def __init__(self, ctx, scene, gen_mips=True):
+        self.ctx, self.scene = ctx, scene
+        self.tex = []
+        for t in scene.textures:
+            idx = ctx.texture_create(t.width, t.height, 4, extra_layers=t.layers - 1, wrap=t.wrap,
+                                     mip_levels=t.levels, pixels=t.images[0])
+            for L in range(1, t.layers):
+                ctx.texture_upload(idx, t.images[L], layer=L)
+            if t.levels > 1 and gen_mips:
+                ctx.texture_generate_mips(idx)
+            self.tex.append(idx)
+        self.vao = {}
+        for name, slots in scene.meshes.items():
+            v = ctx.vao_create()
+            for s, arr in slots.items():
+                ctx.vao_attach(v, s, ctx.vbo_create(arr))
+            self.vao[name] = v
+        self.progs = {}
+        for d in scene.draws:
+            if d.prog not in self.progs:
+                self.progs[d.prog] = ctx.program_create(d.prog)
+
+    def frame(self, clear=True):
+        """clearDepth + clearColour + the scene's drawVAO calls (queued)"""
+        ctx, sc = self.ctx, self.scene
+        if clear:
+            ctx.clear_depth(sc.clear_depth)
+            ctx.clear_colour(sc.clear_colour)
+        for k, v in sc.uniforms.items():
+            ctx.uniform(k, v)
+        cur = None
+        for d in sc.draws:
+            if d.prog != cur:
+                ctx.program_use(self.progs[d.prog])
+                cur = d.prog
+            for k, v in d.uniforms.items():
+                ctx.uniform(k, v)
+            ctx.draw(self.vao[d.mesh])
+            for k in d.uniforms:
+                if k in sc.uniforms:
+                    ctx.uniform(k, sc.uniforms[k])
+        ctx.flush()

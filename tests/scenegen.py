@@ -1,0 +1,115 @@
+"""Seeded micro-scenes for parity tests: random triangles in clip space for each
+built-in processor, with the edge cases the reference's rasterizer and depth
+test are sensitive to (flat rows, off-screen spans, z at the +-1 limits,
+coplanar overlaps for the 1e-4 depth epsilon, degenerate triangles)."""
+import numpy as np
+
+from puresoft3d_amd import native as N
+from puresoft3d_amd import scenes as S
+
+
+def _slots_for(prog, pos, rng):
+    n = pos.shape[0]
+    def f4(w=0.0):
+        a = rng.uniform(-1, 1, (n, 4)).astype(np.float32)
+        a[:, 3] = w
+        return a
+    uv = rng.uniform(-0.25, 1.25, (n, 2)).astype(np.float32)
+    if prog in (N.PROC_DEF01, N.PROC_LAMBERT):
+        return {0: pos, 3: f4(), 4: uv}
+    if prog == N.PROC_DEF02:
+        col = rng.uniform(0, 255, (n, 4)).astype(np.float32)
+        return {0: pos, 1: f4(), 2: col}
+    if prog in (N.PROC_DEF03, N.PROC_BLINN):
+        return {0: pos, 1: f4(), 2: f4(), 3: f4(), 4: uv}
+    return {0: pos}
+
+
+def random_clip_triangles(rng, n, kind="mixed"):
+    """n triangles as (3n, 4) clip positions (w > 0)"""
+    P = np.zeros((n, 3, 4), np.float32)
+    for t in range(n):
+        c = rng.integers(0, 6) if kind == "mixed" else kind
+        if c == 0:      # ordinary, partially off screen
+            xy = rng.uniform(-1.3, 1.3, (3, 2))
+            z = rng.uniform(-0.95, 0.95, 3)
+            w = rng.uniform(0.5, 2.0, 3)
+        elif c == 1:    # tiny
+            ctr = rng.uniform(-1, 1, 2)
+            xy = ctr + rng.uniform(-0.02, 0.02, (3, 2))
+            z = np.full(3, rng.uniform(-0.9, 0.9))
+            w = np.ones(3)
+        elif c == 2:    # flat top/bottom on a pixel row
+            xy = rng.uniform(-1, 1, (3, 2))
+            xy[1, 1] = xy[0, 1]
+            z = rng.uniform(-0.9, 0.9, 3)
+            w = np.ones(3)
+        elif c == 3:    # z touching the limits
+            xy = rng.uniform(-1, 1, (3, 2))
+            z = rng.choice([-1.0, 1.0, -0.99999, 0.99999, 0.0], 3)
+            w = np.ones(3)
+        elif c == 4:    # huge, extends far off screen
+            xy = rng.uniform(-40, 40, (3, 2))
+            z = rng.uniform(-0.5, 0.5, 3)
+            w = np.ones(3)
+        else:           # degenerate: repeated vertex
+            xy = rng.uniform(-1, 1, (3, 2))
+            xy[2] = xy[0]
+            z = rng.uniform(-0.9, 0.9, 3)
+            w = np.ones(3)
+        P[t, :, 0] = xy[:, 0] * w
+        P[t, :, 1] = xy[:, 1] * w
+        P[t, :, 2] = z * w
+        P[t, :, 3] = w
+    return P.reshape(-1, 4)
+
+
+def coplanar_pairs(rng, n):
+    """pairs of overlapping triangles at nearly equal depth (exercise z - d < -1e-4)"""
+    out = []
+    for _ in range(n):
+        xy = rng.uniform(-0.8, 0.8, (3, 2))
+        z = rng.uniform(-0.5, 0.5)
+        dz = rng.choice([0.0, 5e-5, -5e-5, 1.5e-4, -1.5e-4, 1e-4])
+        for zz in (z, z + dz):
+            tri = np.zeros((3, 4), np.float32)
+            tri[:, :2] = xy + rng.uniform(-0.05, 0.05, (3, 2))
+            tri[:, 2] = zz
+            tri[:, 3] = 1.0
+            out.append(tri)
+    return np.concatenate(out, 0)
+
+
+def micro_scene(prog, seed, width=96, height=64, n=120, tex=32):
+    rng = np.random.default_rng(seed)
+    sc = S.Scene(width, height)
+    sc.name = "micro_p%d_s%d_%dx%d" % (prog, seed, width, height)
+    pos = np.concatenate([random_clip_triangles(rng, n), coplanar_pairs(rng, n // 6)], 0)
+    sc.meshes["m"] = _slots_for(prog, pos, rng)
+    I = S.colmajor(np.eye(4))
+    if prog == N.PROC_DEF04:
+        layers = [S.noise_texture(tex, seed + 10 + L, cells=4) for L in range(6)]
+        sc.textures.append(S.TextureSpec(layers, wrap=N.CLAMP))
+        R = np.eye(4)
+        ang = rng.uniform(0, 2 * np.pi)
+        R[0, 0], R[0, 2], R[2, 0], R[2, 2] = np.cos(ang), np.sin(ang), -np.sin(ang), np.cos(ang)
+        sc.uniforms[0] = I
+        sc.uniforms[1] = S.colmajor(R)
+        sc.uniforms[2] = np.array([0], np.int32)
+    else:
+        M = np.eye(4)
+        M[:3, :3] += rng.uniform(-0.05, 0.05, (3, 3))
+        sc.uniforms[3] = I
+        sc.uniforms[4] = S.colmajor(M)
+        sc.uniforms[5] = S.colmajor(np.eye(4) + np.diag(rng.uniform(-0.2, 0.2, 4)))
+        sc.uniforms[7] = S.vec4(*rng.uniform(-3, 3, 3), 0.0) if prog != N.PROC_LAMBERT else S.vec4(0.48, 0.6, 0.64)
+        sc.uniforms[8] = S.vec4(*rng.uniform(-3, 3, 3), 0.0)
+        lv = S.mip_levels(tex, tex) if prog == N.PROC_BLINN else 1
+        sc.textures.append(S.TextureSpec([S.noise_texture(tex, seed + 1, cells=4)], wrap=int(rng.integers(0, 2)), levels=lv))
+        sc.textures.append(S.TextureSpec([S.normal_map(tex, seed + 2, cells=4)], wrap=N.WRAP, levels=lv))
+        sc.uniforms[9] = np.array([0], np.int32)
+        sc.uniforms[10] = np.array([1], np.int32)
+        sc.uniforms[11] = np.array([int(rng.integers(1, 64))], np.int32)
+    sc.draws.append(S.DrawSpec("m", prog))
+    sc.clear_colour = 0x00203040
+    return sc

@@ -1,0 +1,138 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact: colour (BGRA8), depth (FP32 bits), and per pixel the last
+depth-test-passing triangle, the number of rasterised fragments, the number of
+depth-test passes, and xor-hashes of the rasterised and passing triangle ids.
+"""
+import numpy as np
+import pytest
+
+from conftest import announce
+import scenegen
+
+pytestmark = pytest.mark.gpu
+
+N = pytest.importorskip("puresoft3d_amd.native")
+from puresoft3d_amd import scenes as S  # noqa: E402
+
+
+def gpu_render(scene, tiling=None, debug=True, frames=1):
+    ctx = N.Context(scene.width, scene.height)
+    try:
+        if tiling:
+            ctx.tiling(*tiling)
+        gs = S.GpuScene(ctx, scene)
+        if debug:
+            ctx.debug_enable(True)
+        for _ in range(frames):
+            if debug:
+                ctx.debug_reset()
+            gs.frame()
+        col = ctx.read_colour()
+        dep = ctx.read_depth()
+        dbg = ctx.debug_read(scene.width, scene.height) if debug else None
+        st = ctx.stats()
+        return col, dep, dbg, st
+    finally:
+        ctx.close()
+
+
+def assert_same(scene, oracle, tiling=None, frames=1):
+    announce("%s tiling=%s" % (scene.name, tiling))
+    col, dep, dbg, st = gpu_render(scene, tiling, frames=frames)
+    f, ost = oracle.render_scene(scene, debug=True)
+    assert dbg["raster_count"].sum() == ost.frags_rasterised
+    np.testing.assert_array_equal(dbg["raster_count"], f.raster_count)
+    np.testing.assert_array_equal(dbg["raster_hash"], f.raster_hash)
+    np.testing.assert_array_equal(dbg["pass_count"], f.pass_count)
+    np.testing.assert_array_equal(dbg["pass_hash"], f.pass_hash)
+    np.testing.assert_array_equal(dbg["last_pass"], f.last_pass)
+    np.testing.assert_array_equal(dep.view(np.uint32), f.depth.view(np.uint32))
+    np.testing.assert_array_equal(col, f.colour)
+    assert st["frags_shaded"] == ost.frags_shaded * frames
+    assert st["frags_rasterised"] == ost.frags_rasterised * frames
+    assert st["tris_culled"] == ost.tris_culled * frames
+    return st
+
+
+def test_math_selftest():
+    announce("selftest_math")
+    rng = np.random.default_rng(0)
+    ctx = N.Context(8, 8)
+    try:
+        a = np.concatenate([rng.uniform(-1e3, 1e3, 4000), rng.uniform(-1, 1, 2000) * 1e-38,
+                            [0.0, -0.0, 1e-45, 3e38, np.inf, -np.inf, np.nan]]).astype(np.float32)
+        b = np.concatenate([rng.uniform(-10, 10, 4000), rng.uniform(-1, 1, 2000),
+                            [1.0, -3.0, 7.0, 1e-38, 2.0, np.inf, 1.0]]).astype(np.float32)
+        with np.errstate(all="ignore"):
+            np.testing.assert_array_equal(ctx.selftest_math(0, a, b).view(np.uint32), (a / b).view(np.uint32))
+            sq = np.abs(a)
+            np.testing.assert_array_equal(ctx.selftest_math(1, sq).view(np.uint32), np.sqrt(sq).view(np.uint32))
+            np.testing.assert_array_equal(ctx.selftest_math(2, b).view(np.uint32),
+                                          (np.float32(1.0) / b).view(np.uint32))
+            x = np.concatenate([a, np.array([2.2e9, -2.2e9, 2147483520.0, -2147483648.0, 1e20], np.float32)])
+            ref = np.where((x >= -2147483648.0) & (x < 2147483648.0), np.trunc(np.nan_to_num(x)), -2147483648.0)
+            np.testing.assert_array_equal(ctx.selftest_math(3, x), ref.astype(np.float32))
+            np.testing.assert_array_equal(ctx.selftest_math(4, a).view(np.uint32), np.floor(a).view(np.uint32))
+    finally:
+        ctx.close()
+
+
+def test_log2_matches_oracle(oracle):
+    announce("selftest log2")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([10.0 ** rng.uniform(-44, 38, 5000), [1.0, 2.0, 0.5, 1e-45, 0.0, np.inf]]).astype(np.float32)
+    ctx = N.Context(8, 8)
+    try:
+        g = ctx.selftest_math(6, x)
+    finally:
+        ctx.close()
+    o = np.array([oracle.lib().or_log2_approx(float(v)) for v in x], np.float32)
+    np.testing.assert_array_equal(g.view(np.uint32), o.view(np.uint32))
+
+
+def test_sphere_640x480_bitexact(oracle):
+    sc = S.sphere_scene()
+    st = assert_same(sc, oracle)
+    assert st["frags_shaded"] > 50000
+
+
+@pytest.mark.parametrize("tiling", [(16, 16, 1), (8, 8, 4), (32, 64, 2), (4, 1, 8), (64, 32, 1)])
+def test_sphere_tilings_identical(oracle, tiling):
+    sc = S.sphere_scene(width=320, height=240)
+    assert_same(sc, oracle, tiling=tiling)
+
+
+@pytest.mark.parametrize("prog", [N.PROC_DEF01, N.PROC_DEF02, N.PROC_DEF03, N.PROC_DEF04, N.PROC_DEF05,
+                                  N.PROC_LAMBERT, N.PROC_BLINN])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_micro_scenes(oracle, prog, seed):
+    sc = scenegen.micro_scene(prog, seed)
+    assert_same(sc, oracle)
+
+
+def test_terrain_reduced_bitexact(oracle):
+    sc = S.terrain_scene(width=480, height=270, cells=60, n_spheres=8, tex_size=256)
+    assert_same(sc, oracle)
+
+
+def test_two_frames_accumulate(oracle):
+    sc = S.sphere_scene(width=200, height=150)
+    assert_same(sc, oracle, frames=2)
+
+
+def test_mips_match_oracle(oracle):
+    announce("mips")
+    spec = S.TextureSpec([S.noise_texture(40, 3)], levels=S.mip_levels(40, 40))
+    spec2 = S.TextureSpec([np.random.default_rng(5).integers(0, 256, (37, 23, 4), dtype=np.uint8)],
+                          levels=S.mip_levels(23, 37))
+    ctx = N.Context(8, 8)
+    try:
+        for sp in (spec, spec2):
+            t = ctx.texture_create(sp.width, sp.height, 4, mip_levels=sp.levels, pixels=sp.images[0])
+            ctx.texture_generate_mips(t)
+            ot = oracle.OracleTexture(sp)
+            for l in range(sp.levels):
+                np.testing.assert_array_equal(ctx.texture_download(t, 0, l).ravel(), ot.level(0, l))
+    finally:
+        ctx.close()
