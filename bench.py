@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the Puresoft3D raster hot path on one MI355X.
+
+Workload (one step = one frame): 1920x1080, seeded terrain of 300x300 cells
+(180,000 triangles) plus 64 instanced spheres of 320 triangles, 2048^2 albedo
+and normal maps with full mip chains, trilinear sampling, per-pixel
+Blinn-Phong with normal mapping, one point light, FP32 depth.  A frame is
+clearDepth + clearColour + 65 drawVAO calls, with scenes and textures resident
+in HBM when the timed region starts.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the oracle
+(a C restatement of the reference pipeline with the reference's threading:
+one producer thread plus row-interleaved workers), run on a bounded number
+of frames of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fragments shaded/s + 1080p frames/s vs host CPU; p50/p99 view latency when batched"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(scene):
+    """B = 3*T*S_v + W*H*(4 colour + 4 depth) + texture bytes (all mip levels)"""
+    vb = scene.vertex_bytes()
+    fb = scene.width * scene.height * 8
+    tb = 0
+    for t in scene.textures:
+        for l in range(t.levels):
+            tb += max(1, t.width >> l) * max(1, t.height >> l) * 4 * t.layers
+    return vb, fb, tb
+
+
+def kernel_alg_bytes(name, scene):
+    vb, fb, tb = algorithmic_bytes(scene)
+    px = scene.width * scene.height
+    return {"setup": vb, "raster": px * 4, "shade": px * 4 + tb}.get(name)
+
+
+def cpu_baseline(scene, seconds, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    frames, t_frames = 0, []
+    frag = 0
+    pyoracle.render_scene(scene, n_threads=threads)   # warm-up (page-in, thread pool)
+    t_start = time.perf_counter()
+    while frames < 30 and time.perf_counter() - t_start < seconds:
+        t0 = time.perf_counter()
+        _, st = pyoracle.render_scene(scene, n_threads=threads)
+        t_frames.append(time.perf_counter() - t0)
+        frag += st.frags_shaded
+        frames += 1
+    pyoracle.pool_shutdown()
+    tot = sum(t_frames)
+    return {"value": frames / tot, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "%d frames of the same 1080p workload (clear + 65 draws each), oracle with 1 producer + %d "
+                      "row-interleaved worker threads" % (frames, threads - 1),
+            "ms_per_frame_p50": float(np.median(t_frames) * 1e3),
+            "fragments_shaded_per_s": frag / tot}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="reduced scene for quick checks")
+    ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (views would be sharded "
+              "across GPUs with scenes replicated; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from puresoft3d_amd import native as N
+    from puresoft3d_amd import scenes as S
+
+    if args.small:
+        scene = S.terrain_scene(width=960, height=540, cells=150, n_spheres=16, tex_size=1024)
+    else:
+        scene = S.terrain_scene()
+    T = scene.triangles()
+    print("[bench] scene %s: %d triangles" % (scene.name, T), flush=True)
+    ctx = N.Context(scene.width, scene.height)
+    gs = S.GpuScene(ctx, scene)
+    print("[bench] resident; warming up", flush=True)
+
+    def frame():
+        gs.frame()
+
+    graph = None
+    if args.mode == "graph":
+        ctx.capture_begin()
+        gs.frame()
+        graph = ctx.capture_end()
+
+        def frame():  # noqa: F811
+            ctx.graph_launch(graph, 1)
+
+    for _ in range(args.warmup):
+        frame()
+    ctx.sync()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    fps = args.steps / dt
+    print("[bench] timed %d frames in %.3f s" % (args.steps, dt), flush=True)
+
+    # per-kernel durations with HIP events on the context stream
+    ctx.profile(True)
+    ctx.profile_reset()
+    nprof = max(10, min(args.steps, 50))
+    for _ in range(nprof):
+        gs.frame()
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    per_kernel = {k: v[0] / max(1, v[1]) for k, v in prof.items()}
+    dom = max(per_kernel, key=per_kernel.get)
+    alg = kernel_alg_bytes(dom, scene)
+    vb, fb, tb = algorithmic_bytes(scene)
+    B = vb + fb + tb
+    roof = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+            "traffic": None}
+    if alg:
+        achieved = alg / (per_kernel[dom] * 1e-3) / 1e9
+        roof.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), alg_bytes=alg,
+                    avg_ms=round(per_kernel[dom], 4))
+
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural terrain, spheres and noise textures)",
+        "config": {"workload": scene.name, "width": scene.width, "height": scene.height, "triangles": T,
+                   "shading": "Blinn-Phong + normal map, trilinear 2048^2 mipmapped", "mode": args.mode},
+        "fragments_shaded_per_s": round(st["frags_shaded"] / dt, 1),
+        "fragments_rasterised_per_s": round(st["frags_rasterised"] / dt, 1),
+        "frame_alg_bytes": B,
+        "frame_alg_GBs": round(B * fps / 1e9, 1),
+        "kernel_ms": {k: round(v, 4) for k, v in sorted(per_kernel.items())},
+        "roofline": roof,
+    }
+    ctx.close()
+    if not args.no_cpu:
+        thr = args.cpu_threads or min(os.cpu_count() or 2, 16)
+        print("[bench] cpu baseline: oracle with %d threads for <= %.0f s" % (thr, args.cpu_seconds), flush=True)
+        cb = cpu_baseline(scene, args.cpu_seconds, thr)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = round(fps / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -148,6 +148,21 @@ int psgpu_profile_enable(psgpu_ctx* ctx, int enable);
 int psgpu_profile_read(psgpu_ctx* ctx, int max, const char** names, double* total_ms, int64_t* launches, int* count);
 int psgpu_profile_reset(psgpu_ctx* ctx);
 
+/* ---- recorded command lists (hipGraph replay) ---- */
+/* Between capture_begin and capture_end, clears, draws and flushes are recorded
+ * instead of run.  capture_end executes the recording once (which sizes the
+ * scratch buffers), then captures its kernels into a hipGraph.  graph_launch
+ * replays it `count` times, stream-ordered; the snapshot of programme,
+ * uniforms, behaviour and targets taken at record time is reused.  Resource
+ * updates (textures, VBOs) are refused while recording.  Graphs are captured
+ * again automatically when scratch buffers have been reallocated.  No extern
+ * counterpart in the reference: it replaces the per-frame drawVAO calls of a
+ * static scene by one launch. */
+int psgpu_capture_begin(psgpu_ctx* ctx);
+int psgpu_capture_end(psgpu_ctx* ctx, int* out_graph);
+int psgpu_graph_launch(psgpu_ctx* ctx, int graph, int count);
+int psgpu_graph_destroy(psgpu_ctx* ctx, int graph);
+
 /* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a)} ---- */
 int psgpu_selftest_math(psgpu_ctx* ctx, int op, const float* a, const float* b, float* out, int n);
 

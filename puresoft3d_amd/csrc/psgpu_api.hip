@@ -58,7 +58,7 @@ struct Targets {
     int vpw, vph;
 };
 
-enum CmdType { CMD_FLUSH = 0, CMD_CLEAR_DEPTH = 1, CMD_CLEAR_COLOUR = 2 };
+enum CmdType { CMD_FLUSH = 0, CMD_CLEAR_DEPTH = 1, CMD_CLEAR_COLOUR = 2, CMD_GRAPH = 3 };
 
 struct Cmd {
     CmdType type;
@@ -66,6 +66,19 @@ struct Cmd {
     Targets tg;
     float fval = 0.0f;
     uint32_t uval = 0;
+    int graph = -1, count = 0;
+};
+
+// A recorded command list replayed as one hipGraph (stream capture of the
+// kernels of every command).  The draw records live in a device buffer owned
+// by the graph, so replays need no host-to-device copies.
+struct Graph {
+    bool alive = false;
+    std::vector<Cmd> cmds;
+    std::vector<PsDraw*> dev_draws;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+    int64_t flushes = 0;
 };
 
 template <typename T> struct DevBuf {
@@ -98,6 +111,11 @@ struct psgpu_ctx {
     std::vector<PsDraw> queue;
     int64_t queue_tris = 0;
     std::vector<Cmd> log;
+    bool capturing = false;        // between psgpu_capture_begin and _end
+    bool in_stream_capture = false;
+    std::vector<Cmd> capture_list;
+    std::vector<Graph> graphs;
+    uint64_t buf_gen = 1;          // bumped whenever a scratch buffer moves
     psgpu_tiling tiling = {16, 16, 1};
     // scratch
     DevBuf<PsDraw> d_draws;
@@ -159,6 +177,7 @@ template <typename T> int ensure(psgpu_ctx* c, DevBuf<T>& b, size_t n)
     if (n <= b.cap && b.p) return PSGPU_OK;
     size_t cap = std::max(n, (size_t)(b.cap * 3 / 2));
     if (cap < 64) cap = 64;
+    if (c->in_stream_capture) return fail(c, PSGPU_ERR_STATE, "scratch buffer too small during graph capture");
     if (b.p) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipFree(b.p));
@@ -167,6 +186,7 @@ template <typename T> int ensure(psgpu_ctx* c, DevBuf<T>& b, size_t n)
     }
     HIPCHK(c, hipMalloc((void**)&b.p, cap * sizeof(T)));
     b.cap = cap;
+    c->buf_gen++;
     return PSGPU_OK;
 }
 
@@ -182,11 +202,11 @@ struct ProfScope {
     psgpu_ctx* c; const char* name; hipEvent_t a = nullptr;
     ProfScope(psgpu_ctx* c_, const char* n) : c(c_), name(n)
     {
-        if (c->profiling) { a = ev_get(c); hipEventRecord(a, c->stream); }
+        if (c->profiling && !c->in_stream_capture) { a = ev_get(c); hipEventRecord(a, c->stream); }
     }
     ~ProfScope()
     {
-        if (c->profiling) {
+        if (c->profiling && !c->in_stream_capture) {
             hipEvent_t b = ev_get(c);
             hipEventRecord(b, c->stream);
             c->prof_pending.push_back({name, a, b});
@@ -298,7 +318,7 @@ int key_bits(int64_t n_keys)
 }
 
 // run one flush command: the whole pipeline over its draws
-int run_flush(psgpu_ctx* c, const Cmd& cmd)
+int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
 {
     const std::vector<PsDraw>& draws = cmd.draws;
     if (draws.empty()) return PSGPU_OK;
@@ -314,8 +334,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd)
     }
     if (T == 0) return PSGPU_OK;
     int rc;
-    PsDraw* dd;
-    if ((rc = stage_draws(c, draws, &dd))) return rc;
+    PsDraw* dd = staged;
+    if (!dd && (rc = stage_draws(c, draws, &dd))) return rc;
     if ((rc = ensure(c, c->d_tris, T)) || (rc = ensure(c, c->d_vary, (size_t)T * 3 * nvs)) ||
         (rc = ensure(c, c->d_nrows, T)) || (rc = ensure(c, c->d_rowoff, T + 1)))
         return rc;
@@ -413,11 +433,15 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd)
     return PSGPU_OK;
 }
 
-int run_cmd(psgpu_ctx* c, const Cmd& cmd)
+int graph_launch_n(psgpu_ctx* c, int id, int count);
+
+int run_cmd(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged = nullptr)
 {
     switch (cmd.type) {
     case CMD_FLUSH:
-        return run_flush(c, cmd);
+        return run_flush(c, cmd, staged);
+    case CMD_GRAPH:
+        return graph_launch_n(c, cmd.graph, cmd.count);
     case CMD_CLEAR_DEPTH: {
         // clear16 fills m_bytes = scanline * height (fbo.cpp:348-371)
         uint32_t v;
@@ -432,6 +456,17 @@ int run_cmd(psgpu_ctx* c, const Cmd& cmd)
     return PSGPU_OK;
 }
 
+// a command is either recorded (between capture begin/end) or logged and run
+int submit(psgpu_ctx* c, Cmd&& cmd)
+{
+    if (c->capturing) {
+        c->capture_list.push_back(std::move(cmd));
+        return PSGPU_OK;
+    }
+    c->log.push_back(std::move(cmd));
+    return run_cmd(c, c->log.back());
+}
+
 int queue_to_log(psgpu_ctx* c)
 {
     if (c->queue.empty()) return PSGPU_OK;
@@ -441,12 +476,12 @@ int queue_to_log(psgpu_ctx* c)
     c->queue_tris = 0;
     int rc = current_targets(c, cmd.tg);
     if (rc) return rc;
-    c->log.push_back(std::move(cmd));
-    return run_cmd(c, c->log.back());
+    return submit(c, std::move(cmd));
 }
 
 int do_sync(psgpu_ctx* c)
 {
+    if (c->capturing) return fail(c, PSGPU_ERR_STATE, "not allowed while recording a command list");
     int rc = queue_to_log(c);
     if (rc) return rc;
     for (int attempt = 0; attempt < 16; attempt++) {
@@ -484,6 +519,37 @@ int do_sync(psgpu_ctx* c)
             if ((rc = run_cmd(c, cmd))) return rc;
     }
     return fail(c, PSGPU_ERR_NOMEM, "intermediate buffers kept overflowing");
+}
+
+int graph_capture(psgpu_ctx* c, Graph& g)
+{
+    if (g.exec) { HIPCHK(c, hipGraphExecDestroy(g.exec)); g.exec = nullptr; }
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    c->in_stream_capture = true;
+    int rc = PSGPU_OK;
+    for (size_t i = 0; i < g.cmds.size() && !rc; i++) rc = run_cmd(c, g.cmds[i], g.dev_draws[i]);
+    c->in_stream_capture = false;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(c, PSGPU_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(c, PSGPU_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    g.gen = c->buf_gen;
+    return PSGPU_OK;
+}
+
+int graph_launch_n(psgpu_ctx* c, int id, int count)
+{
+    Graph& g = c->graphs[id];
+    if (!g.exec || g.gen != c->buf_gen) {
+        int rc = graph_capture(c, g);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < count; i++) HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+    c->stats.flushes += g.flushes * count;
+    return PSGPU_OK;
 }
 
 int need_ctx(psgpu_ctx* c) { return c ? PSGPU_OK : PSGPU_ERR_INVALID; }
@@ -553,6 +619,10 @@ int psgpu_destroy(psgpu_ctx* c)
     for (int i = 0; i < psgpu_ctx::kStage; i++) {
         if (c->h_stage[i]) hipHostFree(c->h_stage[i]);
         if (c->stage_ev[i]) hipEventDestroy(c->stage_ev[i]);
+    }
+    for (auto& g : c->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        for (auto p : g.dev_draws) if (p) hipFree(p);
     }
     for (auto& r : c->prof_pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -892,8 +962,7 @@ int psgpu_clear_depth(psgpu_ctx* c, float furthest)
     cmd.type = CMD_CLEAR_DEPTH;
     cmd.fval = furthest;
     if ((rc = current_targets(c, cmd.tg))) return rc;
-    c->log.push_back(std::move(cmd));
-    return run_cmd(c, c->log.back());
+    return submit(c, std::move(cmd));
 }
 
 int psgpu_clear_colour(psgpu_ctx* c, uint32_t bgra)
@@ -905,8 +974,7 @@ int psgpu_clear_colour(psgpu_ctx* c, uint32_t bgra)
     cmd.type = CMD_CLEAR_COLOUR;
     cmd.uval = bgra;
     if ((rc = current_targets(c, cmd.tg))) return rc;
-    c->log.push_back(std::move(cmd));
-    return run_cmd(c, c->log.back());
+    return submit(c, std::move(cmd));
 }
 
 // ---- drawing ------------------------------------------------------------------
@@ -1202,6 +1270,85 @@ int psgpu_profile_reset(psgpu_ctx* c)
     int rc = do_sync(c);
     c->prof.clear();
     return rc;
+}
+
+// ---- recorded command lists replayed as hipGraphs ---------------------------
+int psgpu_capture_begin(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (c->capturing) return fail(c, PSGPU_ERR_STATE, "already recording");
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    c->capturing = true;
+    c->capture_list.clear();
+    return PSGPU_OK;
+}
+
+int psgpu_capture_end(psgpu_ctx* c, int* out_graph)
+{
+    if (need_ctx(c) || !out_graph) return PSGPU_ERR_INVALID;
+    if (!c->capturing) return fail(c, PSGPU_ERR_STATE, "not recording");
+    int rc = queue_to_log(c);      // pending draws join the recording
+    c->capturing = false;
+    if (rc) return rc;
+    Graph g;
+    g.alive = true;
+    g.cmds.swap(c->capture_list);
+    // first execution runs eagerly: it sizes every scratch buffer
+    for (const Cmd& cmd : g.cmds) {
+        Cmd copy = cmd;
+        c->log.push_back(std::move(copy));
+        if ((rc = run_cmd(c, c->log.back()))) return rc;
+    }
+    if ((rc = do_sync(c))) return rc;
+    for (const Cmd& cmd : g.cmds) {
+        PsDraw* p = nullptr;
+        if (cmd.type == CMD_FLUSH && !cmd.draws.empty()) {
+            HIPCHK(c, hipMalloc((void**)&p, cmd.draws.size() * sizeof(PsDraw)));
+            HIPCHK(c, hipMemcpy(p, cmd.draws.data(), cmd.draws.size() * sizeof(PsDraw), hipMemcpyHostToDevice));
+            g.flushes++;
+        }
+        g.dev_draws.push_back(p);
+    }
+    if ((rc = graph_capture(c, g))) {
+        for (auto p : g.dev_draws) if (p) hipFree(p);
+        return rc;
+    }
+    size_t i = 0;
+    for (; i < c->graphs.size(); i++) if (!c->graphs[i].alive) break;
+    if (i == c->graphs.size()) c->graphs.push_back(std::move(g)); else c->graphs[i] = std::move(g);
+    *out_graph = (int)i;
+    return PSGPU_OK;
+}
+
+int psgpu_graph_launch(psgpu_ctx* c, int graph, int count)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (graph < 0 || graph >= (int)c->graphs.size() || !c->graphs[graph].alive) return fail(c, PSGPU_ERR_RANGE, "graph out of range");
+    if (c->capturing) return fail(c, PSGPU_ERR_STATE, "not allowed while recording a command list");
+    if (count <= 0) return PSGPU_OK;
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    Cmd cmd;
+    cmd.type = CMD_GRAPH;
+    cmd.graph = graph;
+    cmd.count = count;
+    c->log.push_back(std::move(cmd));
+    return run_cmd(c, c->log.back());
+}
+
+int psgpu_graph_destroy(psgpu_ctx* c, int graph)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (graph < 0 || graph >= (int)c->graphs.size()) return fail(c, PSGPU_ERR_RANGE, "graph out of range");
+    if (!c->graphs[graph].alive) return PSGPU_OK;
+    int rc = do_sync(c);
+    if (rc) return rc;
+    Graph& g = c->graphs[graph];
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    for (auto p : g.dev_draws) if (p) hipFree(p);
+    c->graphs[graph] = Graph();
+    return PSGPU_OK;
 }
 
 int psgpu_selftest_math(psgpu_ctx* c, int op, const float* a, const float* b, float* out, int n)

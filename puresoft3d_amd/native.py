@@ -104,6 +104,10 @@ _SIGS = {
     "psgpu_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                             C.POINTER(C.c_int64), C.POINTER(C.c_int)], C.c_int),
     "psgpu_profile_reset": ([C.c_void_p], C.c_int),
+    "psgpu_capture_begin": ([C.c_void_p], C.c_int),
+    "psgpu_capture_end": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+    "psgpu_graph_launch": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "psgpu_graph_destroy": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_selftest_math": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
 }
 
@@ -354,6 +358,20 @@ class Context:
         k = C.c_int()
         self._chk(self.L.psgpu_profile_read(self.h, n, names, ms, cnt, C.byref(k)))
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(k.value)}
+
+    def capture_begin(self):
+        self._chk(self.L.psgpu_capture_begin(self.h))
+
+    def capture_end(self):
+        out = C.c_int()
+        self._chk(self.L.psgpu_capture_end(self.h, C.byref(out)))
+        return out.value
+
+    def graph_launch(self, graph, count=1):
+        self._chk(self.L.psgpu_graph_launch(self.h, graph, count))
+
+    def graph_destroy(self, graph):
+        self._chk(self.L.psgpu_graph_destroy(self.h, graph))
 
     def selftest_math(self, op, a, b=None):
         a = np.ascontiguousarray(a, np.float32)

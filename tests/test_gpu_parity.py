@@ -136,3 +136,52 @@ def test_mips_match_oracle(oracle):
                 np.testing.assert_array_equal(ctx.texture_download(t, 0, l).ravel(), ot.level(0, l))
     finally:
         ctx.close()
+
+
+def test_graph_replay_matches_eager(oracle):
+    """a recorded frame replayed as a hipGraph gives the same image, depth and
+    fragment planes as the eager path, replay after replay"""
+    sc = S.terrain_scene(width=320, height=180, cells=40, n_spheres=4, tex_size=128)
+    announce("graph replay " + sc.name)
+    f, ost = oracle.render_scene(sc, debug=True)
+    ctx = N.Context(sc.width, sc.height)
+    try:
+        gs = S.GpuScene(ctx, sc)
+        ctx.debug_enable(True)
+        ctx.capture_begin()
+        gs.frame()
+        g = ctx.capture_end()          # runs the recording once
+        for rep in range(3):
+            ctx.debug_reset()
+            ctx.graph_launch(g, 1)
+            col, dep = ctx.read_colour(), ctx.read_depth()
+            dbg = ctx.debug_read(sc.width, sc.height)
+            np.testing.assert_array_equal(col, f.colour)
+            np.testing.assert_array_equal(dep.view(np.uint32), f.depth.view(np.uint32))
+            np.testing.assert_array_equal(dbg["pass_hash"], f.pass_hash)
+            np.testing.assert_array_equal(dbg["last_pass"], f.last_pass)
+        ctx.reset_stats()
+        ctx.graph_launch(g, 5)
+        ctx.sync()
+        assert ctx.stats()["frags_shaded"] == 5 * ost.frags_shaded
+    finally:
+        ctx.close()
+
+
+def test_capacity_growth_replays_log(oracle):
+    """tiny initial scratch buffers overflow; the sync replays the log and the
+    result is unchanged"""
+    sc = S.terrain_scene(width=640, height=360, cells=120, n_spheres=8, tex_size=128)
+    announce("capacity growth " + sc.name)
+    f, ost = oracle.render_scene(sc)
+    ctx = N.Context(sc.width, sc.height)
+    try:
+        gs = S.GpuScene(ctx, sc)
+        ctx.clear_colour(0x11223344)
+        gs.frame()
+        gs.frame()
+        col, dep = ctx.read_colour(), ctx.read_depth()
+        np.testing.assert_array_equal(col, f.colour)
+        np.testing.assert_array_equal(dep.view(np.uint32), f.depth.view(np.uint32))
+    finally:
+        ctx.close()
