@@ -73,6 +73,9 @@ typedef struct {
 } psgpu_tiling;
 
 /* ---- context: PuresoftPipeline ctor/dtor (pipeline.cpp:24-116) ---- */
+/* flags: PSGPU_CREATE_TINY_SCRATCH starts the intermediate buffers at a few
+ * hundred entries, so the first flushes overflow and are replayed (tests). */
+#define PSGPU_CREATE_TINY_SCRATCH 0x1
 int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** out);
 int psgpu_destroy(psgpu_ctx* ctx);
 const char* psgpu_last_error(psgpu_ctx* ctx);

@@ -5,7 +5,6 @@
 
 void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCounters* c);
 void psk_clear_colour(hipStream_t s, uint32_t* p, int w, int h, int scan, uint32_t v, const PsCounters* c);
-void psk_zero_counters(hipStream_t s, PsCounters* c);
 void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                uint32_t* nrows, PsCounters* cnt, int W, int H);
 int psk_scan_blocks(int64_t cap);
@@ -24,6 +23,7 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
                     bool& result_in_second);
 void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
                 const PsCounters* c);
+size_t psk_raster_lds(int chunk_w, int waves);
 void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* kstart,
                 const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt);
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,

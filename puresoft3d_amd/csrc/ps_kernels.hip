@@ -35,11 +35,11 @@ __device__ __forceinline__ uint64_t ps_lt_mask()
     return __lanemask_lt();
 }
 
-__device__ __forceinline__ void ps_wave_add(unsigned long long* dst, unsigned long long v)
+__device__ __forceinline__ void ps_wave_add(PsCounters* c, int field, unsigned long long v)
 {
-    // one atomic per wave
+    // one atomic per wave, on a per-block stripe
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PS_WAVE);
-    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v) atomicAdd(dst, v);
+    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v) atomicAdd(&c->stat[blockIdx.x & (PS_STRIPES - 1)][field], v);
 }
 
 // ---------------------------------------------------------------------------
@@ -63,13 +63,6 @@ __global__ void k_clear_colour(uint32_t* p, int width, int height, int scan, uin
     }
 }
 
-__global__ void k_zero_counters(PsCounters* c)
-{
-    if (threadIdx.x == 0) {
-        c->tris_culled = c->tris_zrejected = c->tris_dropped = 0;
-        c->spans = c->frags_rasterised = c->frags_shaded = 0;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // setup
@@ -181,9 +174,9 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
         }
         nrows[t] = n;
     }
-    ps_wave_add(&cnt->tris_culled, culled);
-    ps_wave_add(&cnt->tris_zrejected, zrej);
-    ps_wave_add(&cnt->tris_dropped, dropped);
+    ps_wave_add(cnt, PS_ST_CULLED, culled);
+    ps_wave_add(cnt, PS_ST_ZREJ, zrej);
+    ps_wave_add(cnt, PS_ST_DROPPED, dropped);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,11 +185,12 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
 #define SCAN_ITEMS 16
 #define SCAN_BLOCK (256 * SCAN_ITEMS)
 
+// exclusive scan of one value per thread over a 256-thread block
 __device__ __forceinline__ uint32_t ps_block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total)
 {
-    // 256 threads = 4 waves
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
+#pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         uint32_t y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
@@ -204,10 +198,27 @@ __device__ __forceinline__ uint32_t ps_block_excl_scan(uint32_t v, uint32_t* lds
     if (lane == 63) lds[wave] = x;
     __syncthreads();
     uint32_t wbase = 0;
-    for (int w = 0; w < wave; w++) wbase += lds[w];
+#pragma unroll
+    for (int w = 0; w < 4; w++) wbase += (w < wave) ? lds[w] : 0u;
     total = lds[0] + lds[1] + lds[2] + lds[3];
     __syncthreads();
     return wbase + x - v;
+}
+
+// thread t of block b owns items [b*SCAN_BLOCK + t*SCAN_ITEMS, +SCAN_ITEMS)
+__device__ __forceinline__ void ps_load_items(const uint32_t* in, int64_t base, int64_t n, uint32_t* v)
+{
+    if (base + SCAN_ITEMS <= n) {
+        const uint4* p = (const uint4*)(in + base);
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; q++) {
+            uint4 u = p[q];
+            v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0u;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
@@ -215,13 +226,13 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict_
 {
     __shared__ uint32_t lds[4];
     int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK;
+    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t s = 0;
     if (!(c && ps_poisoned(c)) && base < n) {
-        for (int i = 0; i < SCAN_ITEMS; i++) {
-            int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
-            if (idx < n) s += in[idx];
-        }
+        uint32_t v[SCAN_ITEMS];
+        ps_load_items(in, base, n, v);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     }
     uint32_t tot;
     ps_block_excl_scan(s, lds, tot);
@@ -275,16 +286,31 @@ __global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__
     __shared__ uint32_t lds[4];
     if (c && ps_poisoned(c)) return;
     int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK;
-    if (base >= n) return;
-    uint32_t run = partial[blockIdx.x];
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
-        uint32_t v = idx < n ? in[idx] : 0;
-        uint32_t tot;
-        uint32_t ex = ps_block_excl_scan(v, lds, tot);
-        if (idx < n) out[idx] = run + ex;
-        run += tot;
+    if (blockIdx.x * (int64_t)SCAN_BLOCK >= n) return;
+    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    ps_load_items(in, base, n, v);
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
+    uint32_t tot;
+    uint32_t run = partial[blockIdx.x] + ps_block_excl_scan(s, lds, tot);
+    if (base + SCAN_ITEMS <= n) {
+        uint4* p = (uint4*)(out + base);
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; q++) {
+            uint4 u;
+            u.x = run; run += v[4 * q];
+            u.y = run; run += v[4 * q + 1];
+            u.z = run; run += v[4 * q + 2];
+            u.w = run; run += v[4 * q + 3];
+            p[q] = u;
+        }
+    } else {
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            if (base + i < n) out[base + i] = run;
+            run += v[i];
+        }
     }
 }
 
@@ -358,19 +384,22 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
             spans[i] = sp;
             seg_count[i] = segs;
         }
-        ps_wave_add(&cnt->spans, nspan);
-        ps_wave_add(&cnt->frags_rasterised, nfrag);
+        ps_wave_add(cnt, PS_ST_SPANS, nspan);
+        ps_wave_add(cnt, PS_ST_FRAGS, nfrag);
     }
 }
 
 // ---------------------------------------------------------------------------
 // segments: (bin key, span) pairs
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t ps_bin_key(const PsFrame& f, int y, int cx)
+// tile of a (row, chunk) bin, and the bin's lane inside the tile's wave
+__device__ __forceinline__ uint32_t ps_bin_tile(const PsFrame& f, int y, int cx)
 {
-    int ty = y / f.tile_rows, ry = y - ty * f.tile_rows;
-    int tx = cx / f.tile_cols, rx = cx - tx * f.tile_cols;
-    return (uint32_t)((ty * f.n_tiles_x + tx) * 64 + ry * f.tile_cols + rx);
+    return (uint32_t)((y / f.tile_rows) * f.n_tiles_x + cx / f.tile_cols);
+}
+__device__ __forceinline__ uint32_t ps_bin_lane(const PsFrame& f, int y, int cx)
+{
+    return (uint32_t)((y % f.tile_rows) * f.tile_cols + cx % f.tile_cols);
 }
 
 __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ seg_off,
@@ -385,57 +414,71 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->overflow, 1);
         return;
     }
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t o = seg_off[i], e = seg_off[i + 1];
-        if (o == e) continue;
-        const PsSpan sp = spans[i];
-        int y = sp.y & 0xffffff;
-        int cx = sp.x1 / f.chunk_w;
-        for (uint32_t s = o; s < e; s++, cx++) {
-            keys[s] = ps_bin_key(f, y, cx);
-            vals[s] = (uint32_t)i;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = base + threadIdx.x;
+        unsigned long long nseg = 0;
+        if (i < R) {
+            uint32_t o = seg_off[i], e = seg_off[i + 1];
+            if (o != e) {
+                const PsSpan sp = spans[i];
+                int y = sp.y & 0xffffff;
+                int cx = sp.x1 / f.chunk_w;
+                for (uint32_t s = o; s < e; s++, cx++) {
+                    keys[s] = ps_bin_tile(f, y, cx);
+                    vals[s] = ((uint32_t)i << 6) | ps_bin_lane(f, y, cx);
+                }
+                nseg = e - o;
+            }
         }
+        ps_wave_add(cnt, PS_ST_SEGS, nseg);
     }
 }
 
 // ---------------------------------------------------------------------------
-// stable LSD radix sort, 8-bit digits, 4096 pairs per block
+// stable LSD radix sort, digits of up to 11 bits, 4096 pairs per block
 // ---------------------------------------------------------------------------
 #define RS_ITEMS 16
 #define RS_BLOCK (256 * RS_ITEMS)
+#define RS_MAXB 2048
 
-__global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int shift,
+__global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int shift, int bits,
                                                  uint32_t* __restrict__ hist, int nb_cap, const PsCounters* c)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[RS_MAXB];
+    const int nbk = 1 << bits;
+    const uint32_t mask = (uint32_t)nbk - 1u;
+    for (int d = threadIdx.x; d < nbk; d += 256) h[d] = 0;
     __syncthreads();
     int64_t n = *n_ptr;
     int64_t base = blockIdx.x * (int64_t)RS_BLOCK;
     if (!ps_poisoned(c) && base < n) {
         for (int i = 0; i < RS_ITEMS; i++) {
             int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
-            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
         }
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nb_cap + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < nbk; d += 256) hist[(int64_t)d * nb_cap + blockIdx.x] = h[d];
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     const uint32_t* n_ptr, int shift, const uint32_t* __restrict__ offs,
                                                     int nb_cap, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                     const PsCounters* c)
 {
-    __shared__ uint32_t cntl[4][256];
-    __shared__ uint32_t base[256];
+    constexpr int NBK = 1 << BITS;
+    __shared__ uint32_t cntl[4][NBK];
+    __shared__ uint32_t base[NBK];
     if (ps_poisoned(c)) return;
     int64_t n = *n_ptr;
     int64_t b0 = blockIdx.x * (int64_t)RS_BLOCK;
     if (b0 >= n) return;
     int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (int w = 0; w < 4; w++) cntl[w][tid] = 0;
-    base[tid] = offs[(int64_t)tid * nb_cap + blockIdx.x];
+    for (int d = tid; d < NBK; d += 256) {
+        cntl[0][d] = cntl[1][d] = cntl[2][d] = cntl[3][d] = 0;
+        base[d] = offs[(int64_t)d * nb_cap + blockIdx.x];
+    }
     __syncthreads();
     uint32_t k[RS_ITEMS], v[RS_ITEMS], rank[RS_ITEMS];
     uint64_t lt = ps_lt_mask();
@@ -445,10 +488,10 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
         bool valid = idx < n;
         k[r] = valid ? kin[idx] : 0u;
         v[r] = valid ? vin[idx] : 0u;
-        uint32_t dig = (k[r] >> shift) & 255u;
+        uint32_t dig = (k[r] >> shift) & (NBK - 1);
         uint64_t same = __ballot(valid);
 #pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
+        for (int bit = 0; bit < BITS; bit++) {
             uint64_t m = __ballot((dig >> bit) & 1u);
             same &= ((dig >> bit) & 1u) ? m : ~m;
         }
@@ -460,11 +503,11 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    {
+    for (int d = tid; d < NBK; d += 256) {
         uint32_t run = 0;
         for (int w = 0; w < 4; w++) {
-            uint32_t t = cntl[w][tid];
-            cntl[w][tid] = run;
+            uint32_t t = cntl[w][d];
+            cntl[w][d] = run;
             run += t;
         }
     }
@@ -473,7 +516,7 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
     for (int r = 0; r < RS_ITEMS; r++) {
         int64_t idx = b0 + wave * (64 * RS_ITEMS) + r * 64 + lane;
         if (idx < n) {
-            uint32_t dig = (k[r] >> shift) & 255u;
+            uint32_t dig = (k[r] >> shift) & (NBK - 1);
             uint32_t pos = base[dig] + cntl[wave][dig] + rank[r];
             kout[pos] = k[r];
             vout[pos] = v[r];
@@ -502,9 +545,11 @@ __global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ key
 // ---------------------------------------------------------------------------
 // raster: ordered depth test per bin
 // ---------------------------------------------------------------------------
+#define RA_CHUNK 1024   // tile-list entries split per pass
+
 template <bool DEBUG>
 __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ vals,
-                         const uint32_t* __restrict__ kstart, const PsTri* __restrict__ tris,
+                         const uint32_t* __restrict__ tstart, const PsTri* __restrict__ tris,
                          const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles, PsCounters* cnt)
 {
     extern __shared__ uint32_t lds[];
@@ -514,9 +559,14 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
     int waves = blockDim.x >> 6;
     int64_t tile = blockIdx.x * (int64_t)waves + wave;
     if (tile >= n_tiles) return;
-    // LDS per wave, column-major [c][lane]: bank = lane for every column
-    float* dep = (float*)(lds + (size_t)wave * CW * 64 * 2);
+    // LDS per wave: depth and winner columns [c][lane] (bank = lane), the
+    // per-lane span lists of the current chunk, per-lane counts and offsets
+    uint32_t* wl = lds + (size_t)wave * (CW * 64 * 2 + RA_CHUNK + 128);
+    float* dep = (float*)wl;
     int* win = (int*)(dep + CW * 64);
+    uint32_t* list = (uint32_t*)(win + CW * 64);
+    uint32_t* lcnt = list + RA_CHUNK;
+    uint32_t* loff = lcnt + 64;
 
     int ty = (int)(tile / f.n_tiles_x), tx = (int)(tile - (int64_t)ty * f.n_tiles_x);
     int ry = lane / f.tile_cols, rx = lane - ry * f.tile_cols;
@@ -532,12 +582,50 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
     }
     unsigned long long passes = 0;
     bool dirty = false;
-    if (inside) {
-        uint32_t key = (uint32_t)(tile * 64 + lane);
-        uint32_t s0 = kstart[key], s1 = kstart[key + 1];
-        int X1 = X0 + ncol - 1;
-        for (uint32_t e = s0; e < s1; e++) {
-            uint32_t si = vals[e];
+    const uint64_t lt = ps_lt_mask();
+    const int X1 = X0 + ncol - 1;
+    uint32_t s0 = tstart[tile], s1 = tstart[tile + 1];
+    for (uint32_t cb = s0; cb < s1; cb += RA_CHUNK) {
+        uint32_t ce = min(s1, cb + RA_CHUNK);
+        // stable split of [cb, ce) by lane: count, offsets, place
+        lcnt[lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int pass = 0; pass < 2; pass++) {
+            if (pass == 1) {
+                uint32_t cntv = lcnt[lane];
+                uint32_t x = cntv;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint32_t yv = __shfl_up(x, o, 64);
+                    if (lane >= o) x += yv;
+                }
+                loff[lane] = x - cntv;
+                lcnt[lane] = 0;
+                __builtin_amdgcn_wave_barrier();
+            }
+            for (uint32_t rb = cb; rb < ce; rb += 64) {
+                uint32_t e = rb + lane;
+                bool valid = e < ce;
+                uint32_t v = valid ? vals[e] : 0u;
+                uint32_t key = v & 63u;
+                uint64_t same = __ballot(valid);
+#pragma unroll
+                for (int bit = 0; bit < 6; bit++) {
+                    uint64_t m = __ballot((key >> bit) & 1u);
+                    same &= ((key >> bit) & 1u) ? m : ~m;
+                }
+                uint32_t before = (uint32_t)__popcll(same & lt);
+                uint32_t cur = lcnt[key];
+                if (pass == 1 && valid) list[loff[key] + cur + before] = v >> 6;
+                __builtin_amdgcn_wave_barrier();
+                if (valid && before == 0) lcnt[key] = cur + (uint32_t)__popcll(same);
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (!inside) continue;
+        uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
+        for (uint32_t q = l0; q < l1; q++) {
+            uint32_t si = list[q];
             const PsSpan sp = spans[si];
             int beh = (sp.y >> 24) & 3;
             bool wc = (sp.y >> 28) & 1;
@@ -574,13 +662,14 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
                 }
             }
         }
+        __builtin_amdgcn_wave_barrier();
     }
     int* wrow = f.winner + (int64_t)y * f.W;
     for (int c = 0; c < ncol; c++) {
         if (dirty) drow[X0 + c] = dep[c * 64 + lane];
         wrow[X0 + c] = win[c * 64 + lane];
     }
-    ps_wave_add(&cnt->frags_shaded, passes);
+    ps_wave_add(cnt, PS_ST_SHADED, passes);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,10 +773,6 @@ void psk_clear_colour(hipStream_t s, uint32_t* p, int w, int h, int scan, uint32
     hipLaunchKernelGGL(k_clear_colour, dim3(g), dim3(256), 0, s, p, w, h, scan, v, c);
 }
 
-void psk_zero_counters(hipStream_t s, PsCounters* c)
-{
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, c);
-}
 
 void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                uint32_t* nrows, PsCounters* cnt, int W, int H)
@@ -742,16 +827,21 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
                     bool& result_in_second)
 {
     int nb = psk_rs_blocks(cap);
+    int passes = (key_bits + 10) / 11;
+    if (passes < 1) passes = 1;
+    int bits = (key_bits + passes - 1) / passes;
     uint32_t *ki = keys, *vi = vals, *ko = keys2, *vo = vals2;
-    int passes = 0;
-    for (int shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(256), 0, s, ki, n_dev, shift, hist, nb, c);
-        psk_scan(s, hist, nullptr, (int64_t)nb * 256, (int64_t)nb * 256, hist_off, partial, nullptr, c);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, nb, ko, vo, c);
+    for (int p = 0, shift = 0; p < passes; p++, shift += bits) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(256), 0, s, ki, n_dev, shift, bits, hist, nb, c);
+        psk_scan(s, hist, nullptr, (int64_t)nb << bits, (int64_t)nb << bits, hist_off, partial, nullptr, c);
+        switch (bits) {
+#define RS_CASE(B) case B: hipLaunchKernelGGL(k_rs_scatter<B>, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, nb, ko, vo, c); break;
+        RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4) RS_CASE(5) RS_CASE(6) RS_CASE(7) RS_CASE(8) RS_CASE(9) RS_CASE(10) RS_CASE(11)
+#undef RS_CASE
+        }
         uint32_t* t;
         t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;
-        passes++;
     }
     result_in_second = (passes & 1) != 0;
 }
@@ -763,14 +853,16 @@ void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int6
     hipLaunchKernelGGL(k_bounds, dim3(g), dim3(256), 0, s, keys, n_dev, n_keys, start, c);
 }
 
-void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* kstart,
+size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * (chunk_w * 64 * 2 + RA_CHUNK + 128) * 4; }
+
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* tstart,
                 const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt)
 {
     int64_t n_tiles = (int64_t)f.n_tiles_x * f.n_tiles_y;
-    size_t lds = (size_t)waves_per_group * f.chunk_w * 64 * 4 * 2;
+    size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
     dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
-    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, vals, kstart, tris, draws, f, n_tiles, cnt);
-    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, vals, kstart, tris, draws, f, n_tiles, cnt);
+    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, vals, tstart, tris, draws, f, n_tiles, cnt);
+    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, vals, tstart, tris, draws, f, n_tiles, cnt);
 }
 
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,

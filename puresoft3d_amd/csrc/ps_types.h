@@ -76,10 +76,12 @@ struct PsFrame {
     int32_t* dbg_last; int32_t* dbg_rcount; uint32_t* dbg_rhash; int32_t* dbg_pcount; uint32_t* dbg_phash;
 };
 
-// Device-side counters of one flush.
+// Device-side counters.  Statistics are striped over 64 slots (by block
+// index) so that waves of many blocks do not serialise on one address.
+enum { PS_ST_CULLED = 0, PS_ST_ZREJ, PS_ST_DROPPED, PS_ST_SPANS, PS_ST_FRAGS, PS_ST_SHADED, PS_ST_SEGS, PS_ST_N = 8 };
+#define PS_STRIPES 64
 struct PsCounters {
-    unsigned long long tris_culled, tris_zrejected, tris_dropped;
-    unsigned long long spans, frags_rasterised, frags_shaded;
+    unsigned long long stat[PS_STRIPES][PS_ST_N];
     int overflow;
-    int pad;
+    int pad[15];
 };

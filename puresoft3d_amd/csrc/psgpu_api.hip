@@ -130,6 +130,9 @@ struct psgpu_ctx {
     int64_t span_cap = 0, seg_cap = 0;
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
+    // per logged command: the overflow flag as it stood after the command ran
+    static const int kFlagCap = 16384;
+    int* h_flags = nullptr;
     PsCounters* d_cnt = nullptr;
     PsCounters* h_cnt = nullptr;
     // draw staging (pinned), ring of slots
@@ -343,7 +346,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
         c->span_cap = std::max<int64_t>(1 << 20, 6 * T);
         c->seg_cap = c->span_cap * 3 / 2;
     }
-    int64_t scan_cap = std::max<int64_t>(std::max<int64_t>(T, c->span_cap), (int64_t)psk_rs_blocks(c->seg_cap) * 256);
+    if (c->span_cap > ((int64_t)1 << 26)) return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans in one flush");
+    int64_t scan_cap = std::max<int64_t>(std::max<int64_t>(T, c->span_cap), (int64_t)psk_rs_blocks(c->seg_cap) * 2048);
     if ((rc = ensure(c, c->d_partial, psk_scan_blocks(scan_cap) + 1)) || (rc = ensure(c, c->d_spantri, c->span_cap)) ||
         (rc = ensure(c, c->d_spans, c->span_cap)) || (rc = ensure(c, c->d_segcount, c->span_cap)) ||
         (rc = ensure(c, c->d_segoff, c->span_cap + 1)) || (rc = ensure(c, c->d_keys, c->seg_cap)) ||
@@ -351,7 +355,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
         (rc = ensure(c, c->d_vals2, c->seg_cap)))
         return rc;
     int nbr = psk_rs_blocks(c->seg_cap);
-    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 256)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 256 + 1)))
+    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 1)))
         return rc;
 
     // tiling
@@ -365,7 +369,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
     f.n_chunks = (f.W + f.chunk_w - 1) / f.chunk_w;
     f.n_tiles_x = (f.n_chunks + f.tile_cols - 1) / f.tile_cols;
     f.n_tiles_y = (f.H + f.tile_rows - 1) / f.tile_rows;
-    f.n_keys = (int64_t)f.n_tiles_x * f.n_tiles_y * 64;
+    f.n_keys = (int64_t)f.n_tiles_x * f.n_tiles_y;   // bin keys are tiles
     f.colour = tg.colour; f.colour_scan = tg.colour_scan; f.colour_top_down = tg.colour_top_down; f.colour_h = tg.colour_h;
     f.depth = tg.depth; f.depth_scan = tg.depth_scan;
     if ((rc = ensure(c, c->d_kstart, f.n_keys + 1)) || (rc = ensure(c, c->d_winner, (size_t)f.W * f.H))) return rc;
@@ -456,6 +460,16 @@ int run_cmd(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged = nullptr)
     return PSGPU_OK;
 }
 
+int do_sync(psgpu_ctx* c);
+
+// remember whether the overflow flag was already set when command i finished:
+// the first command that finished with it set is where a replay starts
+int mark_done(psgpu_ctx* c, int i)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_flags + i, &c->d_cnt->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return PSGPU_OK;
+}
+
 // a command is either recorded (between capture begin/end) or logged and run
 int submit(psgpu_ctx* c, Cmd&& cmd)
 {
@@ -463,8 +477,14 @@ int submit(psgpu_ctx* c, Cmd&& cmd)
         c->capture_list.push_back(std::move(cmd));
         return PSGPU_OK;
     }
+    if ((int)c->log.size() >= psgpu_ctx::kFlagCap - 1) {
+        int rc = do_sync(c);
+        if (rc) return rc;
+    }
     c->log.push_back(std::move(cmd));
-    return run_cmd(c, c->log.back());
+    int rc = run_cmd(c, c->log.back());
+    if (rc) return rc;
+    return mark_done(c, (int)c->log.size() - 1);
 }
 
 int queue_to_log(psgpu_ctx* c)
@@ -490,12 +510,16 @@ int do_sync(psgpu_ctx* c)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->profiling) prof_collect(c);
         if (!c->h_cnt->overflow) {
-            c->stats.tris_culled += (int64_t)c->h_cnt->tris_culled;
-            c->stats.tris_zrejected += (int64_t)c->h_cnt->tris_zrejected;
-            c->stats.tris_dropped += (int64_t)c->h_cnt->tris_dropped;
-            c->stats.spans += (int64_t)c->h_cnt->spans;
-            c->stats.frags_rasterised += (int64_t)c->h_cnt->frags_rasterised;
-            c->stats.frags_shaded += (int64_t)c->h_cnt->frags_shaded;
+            int64_t sum[PS_ST_N] = {0};
+            for (int k = 0; k < PS_STRIPES; k++)
+                for (int f = 0; f < PS_ST_N; f++) sum[f] += (int64_t)c->h_cnt->stat[k][f];
+            c->stats.tris_culled += sum[PS_ST_CULLED];
+            c->stats.tris_zrejected += sum[PS_ST_ZREJ];
+            c->stats.tris_dropped += sum[PS_ST_DROPPED];
+            c->stats.spans += sum[PS_ST_SPANS];
+            c->stats.frags_rasterised += sum[PS_ST_FRAGS];
+            c->stats.frags_shaded += sum[PS_ST_SHADED];
+            c->stats.segments += sum[PS_ST_SEGS];
             c->log.clear();
             HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
             HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
@@ -513,10 +537,16 @@ int do_sync(psgpu_ctx* c)
             c->seg_cap *= 2;
         }
         c->stats.retries++;
+        // commands that finished before the overflow are complete; replay the rest
+        size_t first = 0;
+        while (first < c->log.size() && !c->h_flags[first]) first++;
+        c->log.erase(c->log.begin(), c->log.begin() + first);
         HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
-        for (const Cmd& cmd : c->log)
-            if ((rc = run_cmd(c, cmd))) return rc;
+        for (size_t i = 0; i < c->log.size(); i++) {
+            if ((rc = run_cmd(c, c->log[i]))) return rc;
+            if ((rc = mark_done(c, (int)i))) return rc;
+        }
     }
     return fail(c, PSGPU_ERR_NOMEM, "intermediate buffers kept overflowing");
 }
@@ -563,7 +593,6 @@ extern "C" {
 
 int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** out)
 {
-    (void)flags;
     if (!out) return PSGPU_ERR_INVALID;
     *out = nullptr;
     if (width < 1 || height < 1) return PSGPU_ERR_INVALID;
@@ -579,6 +608,10 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
     c->dev_w = c->vp_w = width;
     c->dev_h = c->vp_h = height;
     c->depth_scan = dscan;
+    if (flags & PSGPU_CREATE_TINY_SCRATCH) {   // start tiny: exercises growth and log replay
+        c->span_cap = 256;
+        c->seg_cap = 256;
+    }
     c->uniforms.resize(kMaxUniforms);
     memset(&c->stats, 0, sizeof(c->stats));
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
@@ -588,6 +621,7 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
               hipHostMalloc((void**)&c->h_cnt, sizeof(PsCounters)) == hipSuccess &&
               hipMalloc((void**)&c->d_sizes, 4 * sizeof(uint32_t)) == hipSuccess &&
               hipHostMalloc((void**)&c->h_sizes, 4 * sizeof(uint32_t)) == hipSuccess &&
+              hipHostMalloc((void**)&c->h_flags, psgpu_ctx::kFlagCap * sizeof(int)) == hipSuccess &&
               hipMemsetAsync(c->display, 0, (size_t)width * height * 4, c->stream) == hipSuccess &&
               hipMemsetAsync(c->depth, 0, (size_t)dscan * height * 4, c->stream) == hipSuccess &&
               hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream) == hipSuccess &&
@@ -610,6 +644,7 @@ int psgpu_destroy(psgpu_ctx* c)
     fr(c->display); fr(c->depth); fr(c->d_cnt); fr(c->d_sizes);
     if (c->h_cnt) hipHostFree(c->h_cnt);
     if (c->h_sizes) hipHostFree(c->h_sizes);
+    if (c->h_flags) hipHostFree(c->h_flags);
     for (auto& t : c->tex) fr(t.d);
     for (auto& v : c->vbo) fr(v.d);
     fr(c->d_draws.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
@@ -1223,7 +1258,7 @@ int psgpu_tiling_set(psgpu_ctx* c, const psgpu_tiling* t)
     bool okc = cw == 4 || cw == 8 || cw == 16 || cw == 32 || cw == 64;
     bool okr = tr >= 1 && tr <= 64 && (tr & (tr - 1)) == 0;
     if (!okc || !okr || w < 1 || w > 8) return fail(c, PSGPU_ERR_INVALID, "unsupported tiling");
-    if ((size_t)w * cw * 64 * 8 > 65536) return fail(c, PSGPU_ERR_INVALID, "tiling needs more than 64 KiB of LDS");
+    if (psk_raster_lds(cw, w) > 65536) return fail(c, PSGPU_ERR_INVALID, "tiling needs more than 64 KiB of LDS");
     int rc = queue_to_log(c);
     if (rc) return rc;
     c->tiling = *t;
@@ -1299,6 +1334,7 @@ int psgpu_capture_end(psgpu_ctx* c, int* out_graph)
         Cmd copy = cmd;
         c->log.push_back(std::move(copy));
         if ((rc = run_cmd(c, c->log.back()))) return rc;
+        if ((rc = mark_done(c, (int)c->log.size() - 1))) return rc;
     }
     if ((rc = do_sync(c))) return rc;
     for (const Cmd& cmd : g.cmds) {
@@ -1329,12 +1365,13 @@ int psgpu_graph_launch(psgpu_ctx* c, int graph, int count)
     if (count <= 0) return PSGPU_OK;
     int rc = queue_to_log(c);
     if (rc) return rc;
+    // replays were sized by capture_end and buffers only grow: a replay
+    // cannot overflow, so a launch of `count` replays is one log entry
     Cmd cmd;
     cmd.type = CMD_GRAPH;
     cmd.graph = graph;
     cmd.count = count;
-    c->log.push_back(std::move(cmd));
-    return run_cmd(c, c->log.back());
+    return submit(c, std::move(cmd));
 }
 
 int psgpu_graph_destroy(psgpu_ctx* c, int graph)

@@ -146,10 +146,10 @@ def _ptr(a):
 class Context:
     """One psgpu context (a PuresoftPipeline bound to one GPU and one stream)."""
 
-    def __init__(self, width, height, device=0):
+    def __init__(self, width, height, device=0, flags=0):
         self.L = lib()
         h = C.c_void_p()
-        rc = self.L.psgpu_create(device, width, height, 0, C.byref(h))
+        rc = self.L.psgpu_create(device, width, height, flags, C.byref(h))
         if rc != OK:
             raise PsGpuError(rc, "psgpu_create(%d, %d, %d) failed" % (device, width, height))
         self.h = h

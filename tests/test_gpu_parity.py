@@ -174,7 +174,7 @@ def test_capacity_growth_replays_log(oracle):
     sc = S.terrain_scene(width=640, height=360, cells=120, n_spheres=8, tex_size=128)
     announce("capacity growth " + sc.name)
     f, ost = oracle.render_scene(sc)
-    ctx = N.Context(sc.width, sc.height)
+    ctx = N.Context(sc.width, sc.height, flags=1)
     try:
         gs = S.GpuScene(ctx, sc)
         ctx.clear_colour(0x11223344)
@@ -183,5 +183,8 @@ def test_capacity_growth_replays_log(oracle):
         col, dep = ctx.read_colour(), ctx.read_depth()
         np.testing.assert_array_equal(col, f.colour)
         np.testing.assert_array_equal(dep.view(np.uint32), f.depth.view(np.uint32))
+        st = ctx.stats()
+        assert st["retries"] >= 1
+        assert st["frags_shaded"] == 2 * ost.frags_shaded
     finally:
         ctx.close()
