@@ -1,0 +1,295 @@
+"""Pins of the oracle parts the reference's compiled rasterizer cannot check:
+interpolation, depth test, culling, samplers, mip chain, log2, shading.  Each
+check compares with something the mathematics fixes (a float64 closed form,
+an invariant, a special case), chosen so that a dropped term, a wrong index
+or a missing 1/w correction fails it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from puresoft3d_amd import native as N
+from puresoft3d_amd import scenes as S
+
+I4 = S.colmajor(np.eye(4))
+
+
+def _single_tri_scene(pos, prog=N.PROC_DEF05, w=64, h=48, extra=None, uniforms=None, textures=()):
+    sc = S.Scene(w, h)
+    slots = {0: np.asarray(pos, np.float32)}
+    if extra:
+        slots.update(extra)
+    sc.meshes["m"] = slots
+    sc.uniforms.update({3: I4, 4: I4, 5: I4})
+    if uniforms:
+        sc.uniforms.update(uniforms)
+    sc.textures.extend(textures)
+    sc.draws.append(S.DrawSpec("m", prog, behavior=N.UPDATE_DEPTH | N.TEST_DEPTH))
+    return sc
+
+
+def _screen(pos, W, H, def05=False):
+    p = np.asarray(pos, np.float64).copy()
+    if def05:   # shadow.cpp:26 scales x, y, z by 0.9 before the transform
+        p[:, :3] *= np.float64(np.float32(0.9))
+    ndc = p[:, :3] / p[:, 3:4]
+    X = (W // 2) * ndc[:, 0] + (W // 2)
+    Y = (H // 2) * ndc[:, 1] + (H // 2)
+    return X, Y, ndc[:, 2], 1.0 / p[:, 3]
+
+
+def _box(x, y, n=7):
+    """points of the half-pixel box around (x, y)"""
+    o = np.linspace(-0.5, 0.5, n)
+    gx, gy = np.meshgrid(x + o, y + o)
+    return gx.ravel(), gy.ravel()
+
+
+def _persp(X, Y, rw, vals, x, y):
+    """1/w-weighted (perspective-correct) interpolation at screen points"""
+    b = np.array(_bary(X, Y, x, y))
+    q = b * rw[:, None]
+    return (q * np.asarray(vals)[:, None]).sum(0) / q.sum(0)
+
+
+def _bary(X, Y, x, y):
+    d = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0])
+    l1 = ((x - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (y - Y[0])) / d
+    l2 = ((X[1] - X[0]) * (y - Y[0]) - (x - X[0]) * (Y[1] - Y[0])) / d
+    return 1 - l1 - l2, l1, l2
+
+
+@pytest.mark.parametrize("vary_w", [False, True])
+def test_depth_closed_form(oracle, vary_w):
+    """The reference interpolates depth like a varying: projZ = sum(c_i rw_i z_i)
+    and cf2 = sum(c_i rw_i) are stepped linearly along a span and z = projZ/cf2
+    (interp.cpp:45-66, 82-92).  Span ends are taken at the rounded end columns
+    on the edge lines, so the depth at a pixel is the exact 1/w-weighted value
+    at a point within half a pixel of it (the affine NDC-z plane when w is
+    constant).  A missing 1/w factor or a wrong edge vertex breaks the bound."""
+    rng = np.random.default_rng(4)
+    W, H = 64, 48
+    checked = 0
+    for _ in range(40):
+        w = rng.uniform(0.5, 3.0, 3) if vary_w else np.full(3, rng.uniform(0.5, 3.0))
+        ndc = np.c_[rng.uniform(-0.9, 0.9, (3, 2)), rng.uniform(-0.9, 0.9, 3)]
+        pos = np.c_[ndc * w[:, None], w].astype(np.float32)
+        sc = _single_tri_scene(pos, w=W, h=H)
+        sc.draws[0].behavior = N.UPDATE_DEPTH | N.TEST_DEPTH
+        f, st = oracle.render_scene(sc)
+        X, Y, Z, rw = _screen(pos, W, H, def05=True)
+        d = f.depth_image()
+        ys, xs = np.where(d != 1.0)
+        for y, x in zip(ys, xs):
+            bx, by = _box(float(x), float(y))
+            z = _persp(X, Y, rw, Z, bx, by)
+            assert z.min() - 1e-4 <= d[y, x] <= z.max() + 1e-4, (x, y, d[y, x], z.min(), z.max())
+            if not vary_w:
+                zp = (np.array(_bary(X, Y, bx, by)) * Z[:, None]).sum(0)
+                assert zp.min() - 1e-4 <= d[y, x] <= zp.max() + 1e-4
+            checked += 1
+    assert checked > 1000
+
+
+def test_perspective_correct_texcoords(oracle):
+    """texcoords interpolated with the 1/w correction (interp.cpp:36-41,
+    correctInterpolation) equal the perspective-correct float64 value; a ramp
+    texture makes the sampled byte = 256*u"""
+    rng = np.random.default_rng(5)
+    W, H = 96, 64
+    ramp = np.zeros((256, 256, 4), np.uint8)
+    ramp[..., 0] = np.arange(256)[None, :]
+    ramp[..., 1] = np.arange(256)[:, None]
+    tex = S.TextureSpec([ramp], wrap=N.CLAMP)
+    L = np.array([0.0, 0.0, 1.0])
+    checked = 0
+    worst = 0.0
+    for _ in range(20):
+        w = rng.uniform(0.4, 4.0, 3)
+        ndc = np.c_[rng.uniform(-0.95, 0.95, (3, 2)), rng.uniform(-0.5, 0.5, 3)]
+        pos = np.c_[ndc * w[:, None], w].astype(np.float32)
+        uv = rng.uniform(0.05, 0.95, (3, 2)).astype(np.float32)
+        nrm = np.tile(np.r_[L, 0.0], (3, 1)).astype(np.float32)
+        sc = _single_tri_scene(pos, N.PROC_LAMBERT, W, H, {3: nrm, 4: uv},
+                               {7: S.vec4(*L), 9: np.array([0], np.int32)}, [tex])
+        f, st = oracle.render_scene(sc, debug=True)
+        X, Y, Z, rw = _screen(pos, W, H)
+        hit = np.where(f.pass_count.reshape(H, W) > 0)
+        for y, x in zip(*hit):
+            bx, by = _box(float(x), float(y))
+            u = 256 * _persp(X, Y, rw, uv[:, 0], bx, by)
+            v = 256 * _persp(X, Y, rw, uv[:, 1], bx, by)
+            px = f.colour[H - 1 - y, x].astype(float)
+            # the byte is the truncated texel coordinate of a point within half a pixel
+            err = max(u.min() - 1.0 - px[0], px[0] - u.max(), v.min() - 1.0 - px[1], px[1] - v.max())
+            worst = max(worst, err)
+            checked += 1
+    assert checked > 2000
+    assert worst < 0.05, worst
+
+
+def test_back_face_is_sign_of_cross_z(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(6)
+    fp = lambda a: np.ascontiguousarray(a, np.float32).ctypes.data_as(C.POINTER(C.c_float))
+    for _ in range(5000):
+        v = rng.uniform(-2, 2, (3, 4)).astype(np.float32)
+        v[:, 3] = 1
+        a, b = v[1] - v[0], v[2] - v[1]
+        cz = float(a[0]) * float(b[1]) - float(a[1]) * float(b[0])
+        if abs(cz) < 1e-6:
+            continue
+        assert L.or_is_back_face(fp(v[0]), fp(v[1]), fp(v[2])) == (1 if cz < 0 else 0)
+    # |c|^2 == 0 -> rsqrt = inf -> NaN -> not culled (vertthrd.cpp:56-65)
+    z = np.zeros(4, np.float32)
+    assert L.or_is_back_face(fp(z), fp(z), fp(z)) == 0
+    # collinear
+    assert L.or_is_back_face(fp([0, 0, 0, 1]), fp([1, 1, 0, 1]), fp([2, 2, 0, 1])) == 0
+
+
+def _x86_u32(f):
+    f = np.float32(f)
+    if not (-9.223372e18 <= f < 9.223372e18):
+        return 0
+    return int(np.int64(f)) & 0xffffffff
+
+
+def test_nearest_sampler_closed_form(oracle):
+    """samplr2d.cpp:19-25 + clampCoord (fbo.cpp:552-594)"""
+    L = oracle.lib()
+    rng = np.random.default_rng(7)
+    for wrap in (0, 1):
+        Wt, Ht = 5, 4
+        img = rng.integers(0, 256, (Ht, Wt, 4), dtype=np.uint8)
+        t = oracle.texture_struct(np.ascontiguousarray(img.ravel()), wrap=wrap, width=Wt, height=Ht)
+        out = np.zeros(4, np.uint8)
+        for u, v in rng.uniform(-2.5, 2.5, (3000, 2)).astype(np.float32):
+            L.or_sample_nearest(C.byref(t), 0, float(u), float(v), out.ctypes.data)
+            row = np.int32(np.uint32(_x86_u32(np.float32(Ht) * v + np.float32(0.5))).view(np.int32))
+            col = np.int32(np.uint32(_x86_u32(np.float32(Wt) * u + np.float32(0.5))).view(np.int32))
+            if wrap == 0:
+                row, col = min(max(row, 0), Ht - 1), min(max(col, 0), Wt - 1)
+            else:
+                row = int(np.fmod(row, Ht - 1)); row += (Ht - 1) if row < 0 else 0
+                col = int(np.fmod(col, Wt - 1)); col += (Wt - 1) if col < 0 else 0
+            np.testing.assert_array_equal(out, img[row, col])
+
+
+def test_cube_face_table(oracle):
+    """OpenGL major-axis table with strict > comparisons (samplrcube.cpp:29-97);
+    each face holds a distinct colour so the face is readable from the texel"""
+    L = oracle.lib()
+    Wt = 4
+    faces = np.zeros((6, Wt, Wt, 4), np.uint8)
+    for k in range(6):
+        faces[k, ..., 0] = 10 * (k + 1)
+    t = oracle.texture_struct(np.ascontiguousarray(faces.ravel()), width=Wt, height=Wt, layers=6)
+    out = np.zeros(4, np.uint8)
+
+    def face(d):
+        d = np.asarray(d, np.float32)
+        L.or_sample_cube(C.byref(t), d.ctypes.data_as(C.POINTER(C.c_float)), out.ctypes.data)
+        return out[0] // 10 - 1
+
+    assert face([1, 0, 0, 0]) == 0 and face([-1, 0, 0, 0]) == 1
+    assert face([0, 0, 1, 0]) == 4 and face([0, 0, -1, 0]) == 5
+    # Y wins: the reference chooses +Y/-Y by the sign of Z (samplrcube.cpp:78-95)
+    assert face([0, 1, 0.0, 0]) == 3 and face([0, 1, 0.1, 0]) == 2 and face([0, -1, -0.1, 0]) == 3
+    # ties fall through: |X| == |Y| goes to the Z/Y branch; |X| == |Z| > |Y| goes to Z
+    assert face([1, 0, 1, 0]) == 4
+    assert face([1, 1, 0, 0]) == 3
+
+
+def test_bilinear_reproduces_affine_texture(oracle):
+    L = oracle.lib()
+    Wt, Ht = 64, 48
+    img = np.zeros((Ht, Wt, 4), np.uint8)
+    img[..., 0] = (2 * np.arange(Wt))[None, :] % 256
+    img[..., 1] = (3 * np.arange(Ht))[:, None] % 256
+    t = oracle.texture_struct(np.ascontiguousarray(img.ravel()), width=Wt, height=Ht)
+    out = np.zeros(4, np.float32)
+    rng = np.random.default_rng(8)
+    for u, v in rng.uniform(0.0, 0.9, (2000, 2)).astype(np.float32):
+        fx, fy = np.float32(Wt) * u, np.float32(Ht) * v
+        if fx > Wt - 1 or fy > Ht - 1:
+            continue
+        L.or_sample_bilinear(C.byref(t), 0, 0, float(u), float(v), out.ctypes.data_as(C.POINTER(C.c_float)))
+        assert abs(out[0] - 2 * float(fx)) < 1e-3 and abs(out[1] - 3 * float(fy)) < 1e-3
+
+
+def test_mip_chain(oracle):
+    rng = np.random.default_rng(9)
+    const = S.TextureSpec([np.full((32, 32, 4), 77, np.uint8)], levels=6)
+    ot = oracle.OracleTexture(const)
+    for l in range(6):
+        assert (ot.level(0, l) == 77).all()
+    img = rng.integers(0, 256, (16, 16, 4), dtype=np.uint8)
+    ot = oracle.OracleTexture(S.TextureSpec([img], levels=5))
+    a = img.astype(np.int32)
+    l1 = (a[0::2, 0::2] + a[0::2, 1::2] + a[1::2, 0::2] + a[1::2, 1::2] + 2) >> 2
+    np.testing.assert_array_equal(ot.level(0, 1).reshape(8, 8, 4), l1)
+
+
+def test_log2_approximation(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(10)
+    for e in range(-140, 128, 7):
+        assert L.or_log2_approx(float(2.0 ** e)) == pytest.approx(e, abs=0)
+    xs = (10.0 ** rng.uniform(-40, 38, 5000)).astype(np.float32)
+    err = max(abs(L.or_log2_approx(float(x)) - np.log2(np.float64(x))) for x in xs)
+    assert err < 2e-5
+
+
+def test_depth_epsilon_keeps_first_triangle(oracle):
+    """-1 < z && z - d < -1e-4 (fragthrd.cpp:227): a coplanar duplicate never
+    passes; a copy 2e-4 nearer passes"""
+    tri = np.array([[-0.8, -0.8, 0.3, 1], [0.8, -0.8, 0.3, 1], [0.0, 0.8, 0.3, 1]], np.float32)
+    near = tri.copy()
+    near[:, 2] -= 2e-4
+    pos = np.concatenate([tri, tri, near])
+    sc = _single_tri_scene(pos)
+    f, st = oracle.render_scene(sc, debug=True)
+    pc = f.pass_count[f.raster_count > 0]
+    assert (pc <= 2).all() and (f.last_pass[f.pass_count > 0] != 1).all()
+    both = (f.raster_count == 3)
+    assert (f.pass_count[both] == 2).all() and (f.last_pass[both] == 2).all()
+
+
+def test_z_reject_whole_triangle(oracle):
+    """drawvao.cpp:51-56: any vertex with z outside [-1, 1] drops the triangle"""
+    # the depth-only processor scales z by 0.9 (shadow.cpp:26): 1.1 -> 0.99 kept, 1.12 -> 1.008 dropped
+    base = np.array([[-0.5, -0.5, 0.0, 1], [0.5, -0.5, 0.0, 1], [0.0, 0.5, 1.1, 1]], np.float32)
+    over = base.copy()
+    over[2, 2] = 1.12
+    f, st = oracle.render_scene(_single_tri_scene(np.concatenate([base, over])))
+    assert st.tris_zrejected == 1 and st.frags_shaded > 0
+
+
+def test_clear4_skips_last_memory_row(oracle):
+    f = oracle.Frame(8, 6)
+    f.colour[:] = 9
+    f.clear_colour(0x01020304)
+    assert (f.colour[:-1].view(np.uint32) == 0x01020304).all()
+    assert (f.colour[-1] == 9).all()
+
+
+def test_threads_do_not_change_results(oracle):
+    sc = S.sphere_scene(width=320, height=240)
+    a, sa = oracle.render_scene(sc, n_threads=1)
+    b, sb = oracle.render_scene(sc, n_threads=4)
+    np.testing.assert_array_equal(a.colour, b.colour)
+    np.testing.assert_array_equal(a.depth, b.depth)
+    assert sa.as_dict() == sb.as_dict()
+
+
+@pytest.mark.parametrize("which", ["sphere", "terrain"])
+def test_fp32_shading_within_one_lsb_of_fp64(oracle, which):
+    if which == "sphere":
+        sc = S.sphere_scene(width=320, height=240)
+    else:
+        sc = S.terrain_scene(width=320, height=180, cells=40, n_spheres=4, tex_size=128)
+    a, _ = oracle.render_scene(sc)
+    b, _ = oracle.render_scene(sc, fp64=True)
+    d = np.abs(a.colour.astype(int) - b.colour.astype(int))
+    assert d.max() <= 1, d.max()
+    assert (d > 0).mean() < 0.05
