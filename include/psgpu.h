@@ -119,8 +119,15 @@ int psgpu_target_set(psgpu_ctx* ctx, int fbo_slot, int tex_or_display);   /* set
 int psgpu_depth_set(psgpu_ctx* ctx, int tex_or_default);                   /* setDepth */
 int psgpu_clear_depth(psgpu_ctx* ctx, float furthest);                     /* clear16 */
 int psgpu_clear_colour(psgpu_ctx* ctx, uint32_t bgra);                     /* clear4 of the display */
+/* clears every row of the current colour target (FBO slot 0); the reference
+ * can only clear the display (clearColour), this serves off-screen views */
+int psgpu_clear_target(psgpu_ctx* ctx, uint32_t bgra);
 
 /* ---- drawing ---- */
+/* Draws are queued per view (render targets + viewport).  Changing targets or
+ * viewport between draws starts another view of the same queued flush, so
+ * many views render in one pass of the kernels; a view whose buffers an
+ * earlier queued view already uses makes the queue flush first. */
 int psgpu_draw(psgpu_ctx* ctx, int vao);          /* drawVAO: queued */
 int psgpu_flush(psgpu_ctx* ctx);                  /* execute queued draws (async) */
 int psgpu_sync(psgpu_ctx* ctx);                   /* flush + wait; re-runs on capacity growth */

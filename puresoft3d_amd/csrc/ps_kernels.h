@@ -4,9 +4,9 @@
 #include "ps_types.h"
 
 void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCounters* c);
-void psk_clear_colour(hipStream_t s, uint32_t* p, int w, int h, int scan, uint32_t v, const PsCounters* c);
+void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_elems, const PsCounters* c);
 void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
-               uint32_t* nrows, PsCounters* cnt, int W, int H);
+               uint32_t* nrows, PsCounters* cnt, const PsView* views);
 int psk_scan_blocks(int64_t cap);
 // exclusive scan; out[n] and *total_out (if given) receive the total
 void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
@@ -14,9 +14,10 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
                 PsCounters* cnt);
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
-              int64_t T, PsSpan* spans, uint32_t* seg_count, int W, int chunk_w, int64_t cap, PsCounters* cnt);
+              int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt);
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
-                  const PsFrame& f, uint32_t* keys, uint32_t* vals, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
+                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, uint32_t* keys, uint32_t* vals,
+                  int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
 int psk_rs_blocks(int64_t cap);
 void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
                     int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,

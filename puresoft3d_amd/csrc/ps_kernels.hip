@@ -42,6 +42,26 @@ __device__ __forceinline__ void ps_wave_add(PsCounters* c, int field, unsigned l
     if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v) atomicAdd(&c->stat[blockIdx.x & (PS_STRIPES - 1)][field], v);
 }
 
+__device__ __forceinline__ int ps_view_of_tile(const PsFrame& f, int64_t t)
+{
+    int lo = 0, hi = f.n_views - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (f.views[mid].tile_base <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int ps_view_of_stile(const PsFrame& f, int64_t t)
+{
+    int lo = 0, hi = f.n_views - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (f.views[mid].stile_base <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // fills (graph friendly: no hipMemset nodes)
 // ---------------------------------------------------------------------------
@@ -52,14 +72,16 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v, const PsCounters*
         p[i] = v;
 }
 
-// clear4 (fbo.cpp:332-346): memory rows 0 .. height-2 of a BGRA8 buffer
-__global__ void k_clear_colour(uint32_t* p, int width, int height, int scan, uint32_t v, const PsCounters* c)
+// a clear command: fill f = blockIdx.y; clear4 (fbo.cpp:332-346) is a fill of
+// rows 0 .. height-2, clear16 (fbo.cpp:348-371) one of the whole buffer
+__global__ void k_fills(const PsFill* __restrict__ fills, const PsCounters* c)
 {
     if (c && ps_poisoned(c)) return;
-    int64_t n = (int64_t)width * (height - 1);
+    const PsFill fl = fills[blockIdx.y];
+    int64_t n = fl.rows * fl.width;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t y = i / width, x = i - y * width;
-        p[y * scan + x] = v;
+        int64_t y = i / fl.width, x = i - y * fl.width;
+        fl.ptr[y * fl.scan + x] = fl.value;
     }
 }
 
@@ -80,7 +102,7 @@ __device__ __forceinline__ int ps_find_draw(const PsDraw* draws, int nd, int64_t
 template <int PROG>
 __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
                                                float* __restrict__ vary, int nvs, uint32_t* __restrict__ nrows,
-                                               PsCounters* cnt, int W, int H)
+                                               PsCounters* cnt, const PsView* __restrict__ views)
 {
     typedef PsProc<PROG> P;
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -90,6 +112,7 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
     live = live && d.prog == PROG;
     unsigned culled = 0, zrej = 0, dropped = 0;
     if (live) {
+        const int W = views[d.view].W, H = views[d.view].H;
         int64_t lt = t - d.tri_start;
         float pos[3][4];
         float u[3][P::NV > 0 ? P::NV : 1];
@@ -346,7 +369,7 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nro
 __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
                                               const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
                                               int64_t T, PsSpan* __restrict__ spans, uint32_t* __restrict__ seg_count,
-                                              int W, int chunk_w, PsCounters* cnt)
+                                              const PsView* __restrict__ views, int chunk_w, PsCounters* cnt)
 {
     if (ps_poisoned(cnt)) return;
     int64_t R = row_off[T];
@@ -356,6 +379,8 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         if (i < R) {
             int t = span_tri[i];
             const PsTri tr = tris[t];
+            const PsDraw& d = draws[tr.code >> 8];
+            const int W = views[d.view].W;
             int row = tr.first + (int)(i - (int64_t)row_off[t]);
             PsEdge L, Rr;
             ps_row_edges(tr, row, L, Rr);
@@ -372,7 +397,6 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
                     PsSpanGeom g;
                     ps_span_geom(tr, row, left, right, L, Rr, g);
                     ps_span_vis(tr, g, sp.cf2, sp.cf2_step, sp.pz, sp.pz_step);
-                    const PsDraw& d = draws[tr.code >> 8];
                     sp.tri = t;
                     sp.y = row | ((d.behavior & 3) << 24) | (d.writes_colour ? (1 << 28) : 0);
                     sp.x1 = x1;
@@ -393,9 +417,9 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
 // segments: (bin key, span) pairs
 // ---------------------------------------------------------------------------
 // tile of a (row, chunk) bin, and the bin's lane inside the tile's wave
-__device__ __forceinline__ uint32_t ps_bin_tile(const PsFrame& f, int y, int cx)
+__device__ __forceinline__ uint32_t ps_bin_tile(const PsFrame& f, const PsView& v, int y, int cx)
 {
-    return (uint32_t)((y / f.tile_rows) * f.n_tiles_x + cx / f.tile_cols);
+    return (uint32_t)(v.tile_base + (y / f.tile_rows) * v.n_tiles_x + cx / f.tile_cols);
 }
 __device__ __forceinline__ uint32_t ps_bin_lane(const PsFrame& f, int y, int cx)
 {
@@ -404,6 +428,7 @@ __device__ __forceinline__ uint32_t ps_bin_lane(const PsFrame& f, int y, int cx)
 
 __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ seg_off,
                                                   const uint32_t* __restrict__ row_off, int64_t T, PsFrame f,
+                                                  const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int64_t cap,
                                                   PsCounters* cnt)
 {
@@ -421,10 +446,11 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
             uint32_t o = seg_off[i], e = seg_off[i + 1];
             if (o != e) {
                 const PsSpan sp = spans[i];
+                const PsView& v = f.views[draws[tris[sp.tri].code >> 8].view];
                 int y = sp.y & 0xffffff;
                 int cx = sp.x1 / f.chunk_w;
                 for (uint32_t s = o; s < e; s++, cx++) {
-                    keys[s] = ps_bin_tile(f, y, cx);
+                    keys[s] = ps_bin_tile(f, v, y, cx);
                     vals[s] = ((uint32_t)i << 6) | ps_bin_lane(f, y, cx);
                 }
                 nseg = e - o;
@@ -568,14 +594,17 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
     uint32_t* lcnt = list + RA_CHUNK;
     uint32_t* loff = lcnt + 64;
 
-    int ty = (int)(tile / f.n_tiles_x), tx = (int)(tile - (int64_t)ty * f.n_tiles_x);
+    const PsView& V = f.views[ps_view_of_tile(f, tile)];
+    int64_t ltile = tile - V.tile_base;
+    int ty = (int)(ltile / V.n_tiles_x), tx = (int)(ltile - (int64_t)ty * V.n_tiles_x);
     int ry = lane / f.tile_cols, rx = lane - ry * f.tile_cols;
     int y = ty * f.tile_rows + ry;
     int cx = tx * f.tile_cols + rx;
     int X0 = cx * CW;
-    bool inside = y < f.H && X0 < f.W;
-    int ncol = inside ? min(CW, f.W - X0) : 0;
-    float* drow = f.depth + (int64_t)y * f.depth_scan;
+    const int VW = V.W;
+    bool inside = y < V.H && X0 < VW;
+    int ncol = inside ? min(CW, VW - X0) : 0;
+    float* drow = V.depth + (int64_t)y * V.depth_scan;
     for (int c = 0; c < ncol; c++) {
         dep[c * 64 + lane] = drow[X0 + c];
         win[c * 64 + lane] = -1;
@@ -651,7 +680,7 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
                 pz += pst;
                 int c = x - X0;
                 float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
-                int64_t pix = (int64_t)y * f.W + x;
+                int64_t pix = (int64_t)y * VW + x;
                 if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
                 // fragthrd.cpp:227
                 if (-1.0f < z && (z - cur < -0.0001f)) {
@@ -664,7 +693,7 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
         }
         __builtin_amdgcn_wave_barrier();
     }
-    int* wrow = f.winner + (int64_t)y * f.W;
+    int* wrow = V.winner + (int64_t)y * VW;
     for (int c = 0; c < ncol; c++) {
         if (dirty) drow[X0 + c] = dep[c * 64 + lane];
         wrow[X0 + c] = win[c * 64 + lane];
@@ -682,10 +711,13 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
 {
     typedef PsProc<PROG> P;
     if (ps_poisoned(cnt)) return;
-    int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= f.W || y >= f.H) return;
-    int w = f.winner[(int64_t)y * f.W + x];
+    const PsView& V = f.views[ps_view_of_stile(f, blockIdx.x)];
+    int64_t st = blockIdx.x - V.stile_base;
+    int sy = (int)(st / V.n_stiles_x), sx = (int)(st - (int64_t)sy * V.n_stiles_x);
+    int x = sx * 16 + (threadIdx.x & 15);
+    int y = sy * 16 + (threadIdx.x >> 4);
+    if (x >= V.W || y >= V.H) return;
+    int w = V.winner[(int64_t)y * V.W + x];
     if (w < 0) return;
     const PsSpan sp = spans[w];
     const PsTri tr = tris[sp.tri];
@@ -714,8 +746,8 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
     if (P::GRAD) ps_tri_grad(tr, u0, u0 + nvs, u0 + 2 * nvs, 12, grad);
     uint32_t c;
     if (P::fp(d, in, r, grad, c)) {
-        int64_t row = f.colour_top_down ? (int64_t)(f.colour_h - 1 - y) : (int64_t)y;
-        f.colour[row * f.colour_scan + x] = c;
+        int64_t row = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
+        V.colour[row * V.colour_scan + x] = c;
     }
 }
 
@@ -765,27 +797,26 @@ void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCou
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, p, n, v, c);
 }
 
-void psk_clear_colour(hipStream_t s, uint32_t* p, int w, int h, int scan, uint32_t v, const PsCounters* c)
+void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_elems, const PsCounters* c)
 {
-    int64_t n = (int64_t)w * (h - 1);
     if (n <= 0) return;
-    int g = gsz(n, 256); if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_clear_colour, dim3(g), dim3(256), 0, s, p, w, h, scan, v, c);
+    int g = gsz(max_elems, 256); if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_fills, dim3(g, n), dim3(256), 0, s, fills, c);
 }
 
 
 void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
-               uint32_t* nrows, PsCounters* cnt, int W, int H)
+               uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
     dim3 g(gsz(T, 256)), b(256);
     switch (prog) {
-    case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF01>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
-    case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF02>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
-    case PSGPU_PROC_DEF03: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF03>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
-    case PSGPU_PROC_DEF04: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF04>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
-    case PSGPU_PROC_DEF05: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF05>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
-    case PSGPU_PROC_LAMBERT: hipLaunchKernelGGL(k_setup<PSGPU_PROC_LAMBERT>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
-    case PSGPU_PROC_BLINN: hipLaunchKernelGGL(k_setup<PSGPU_PROC_BLINN>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, W, H); break;
+    case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF01>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF02>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    case PSGPU_PROC_DEF03: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF03>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    case PSGPU_PROC_DEF04: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF04>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    case PSGPU_PROC_DEF05: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF05>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    case PSGPU_PROC_LAMBERT: hipLaunchKernelGGL(k_setup<PSGPU_PROC_LAMBERT>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    case PSGPU_PROC_BLINN: hipLaunchKernelGGL(k_setup<PSGPU_PROC_BLINN>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
     }
 }
 
@@ -807,17 +838,18 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
 }
 
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
-              int64_t T, PsSpan* spans, uint32_t* seg_count, int W, int chunk_w, int64_t cap, PsCounters* cnt)
+              int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_span, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, spans, seg_count, W, chunk_w, cnt);
+    hipLaunchKernelGGL(k_span, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, spans, seg_count, views, chunk_w, cnt);
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
-                  const PsFrame& f, uint32_t* keys, uint32_t* vals, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
+                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, uint32_t* keys, uint32_t* vals,
+                  int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
 {
     int g = gsz(span_cap, 256); if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, keys, vals, seg_cap, cnt);
+    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, tris, draws, keys, vals, seg_cap, cnt);
 }
 
 int psk_rs_blocks(int64_t cap) { return gsz(cap, RS_BLOCK); }
@@ -858,7 +890,7 @@ size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * (chunk_w 
 void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* tstart,
                 const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt)
 {
-    int64_t n_tiles = (int64_t)f.n_tiles_x * f.n_tiles_y;
+    int64_t n_tiles = f.n_tiles;
     size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
     dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
     if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, vals, tstart, tris, draws, f, n_tiles, cnt);
@@ -868,7 +900,7 @@ void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* 
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,
                const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
 {
-    dim3 g((f.W + 15) / 16, (f.H + 15) / 16), b(256);
+    dim3 g((unsigned)f.n_stiles), b(256);
     switch (prog) {
     case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF01>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
     case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF02>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;

@@ -22,6 +22,7 @@ struct PsTex {
 // Per-draw record: the snapshot of programme, behaviour, vertex streams and
 // uniforms that drawVAO would see (drawvao.cpp:12-20).
 struct PsDraw {
+    int view;          // index of the view (render target + viewport) in the flush
     int prog;          // PSGPU_PROC_*
     int behavior;      // PSGPU_* behaviour bits
     int nv;            // varying floats of the programme
@@ -61,19 +62,36 @@ struct PsSpan {
     float cf2, cf2_step, pz, pz_step;
 };
 
+// One view of a flush: a viewport and its colour/depth targets.  A flush
+// renders any number of views with the same kernels; tiles and shading
+// blocks of all views are numbered consecutively.
+struct PsView {
+    int W, H;
+    int n_chunks, n_tiles_x, n_tiles_y;
+    int n_stiles_x;                 // 16x16 shading blocks per row
+    int64_t tile_base;              // first raster tile of the view
+    int64_t stile_base;             // first shading block of the view
+    uint32_t* colour; int colour_scan; int colour_top_down; int colour_h;  // scan in uint32
+    float* depth; int depth_scan;   // scan in floats
+    int* winner;                    // W*H: span index of the last colour-writing pass
+};
+
 // Flush-wide parameters passed by value to kernels.
 struct PsFrame {
-    int W, H;
-    int chunk_w, tile_rows, tile_cols, n_chunks;   // raster tiling
-    int n_tiles_x, n_tiles_y;
-    int64_t n_keys;
-    // targets
-    uint32_t* colour; int colour_scan; int colour_top_down; int colour_h; // scan in uint32 units
-    float* depth; int depth_scan;                           // scan in floats
-    // per-pixel winner (span index) for the shading pass
-    int* winner;
-    // optional debug planes
+    int chunk_w, tile_rows, tile_cols;   // raster tiling
+    int n_views;
+    int64_t n_tiles, n_stiles;
+    const PsView* views;
+    // optional debug planes (single-view flushes), index y*W+x
     int32_t* dbg_last; int32_t* dbg_rcount; uint32_t* dbg_rhash; int32_t* dbg_pcount; uint32_t* dbg_phash;
+};
+
+// One fill of a clear command: rows x width elements from ptr, rows `scan` apart.
+struct PsFill {
+    uint32_t* ptr;
+    int64_t rows, width, scan;
+    uint32_t value;
+    int pad;
 };
 
 // Device-side counters.  Statistics are striped over 64 slots (by block

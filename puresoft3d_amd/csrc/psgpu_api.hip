@@ -56,26 +56,35 @@ struct Targets {
     uint32_t* colour; int colour_scan; int colour_top_down; int colour_w; int colour_h;
     float* depth; int depth_scan; int depth_w; int depth_h;
     int vpw, vph;
+    bool same(const Targets& o) const
+    {
+        return colour == o.colour && depth == o.depth && vpw == o.vpw && vph == o.vph && colour_scan == o.colour_scan &&
+               depth_scan == o.depth_scan;
+    }
 };
 
-enum CmdType { CMD_FLUSH = 0, CMD_CLEAR_DEPTH = 1, CMD_CLEAR_COLOUR = 2, CMD_GRAPH = 3 };
+// pending draws of one view (render targets + viewport) of the next flush
+struct Segment {
+    Targets tg;
+    std::vector<PsDraw> draws;
+};
+
+enum CmdType { CMD_FLUSH = 0, CMD_CLEARS = 1, CMD_GRAPH = 3 };
 
 struct Cmd {
     CmdType type;
-    std::vector<PsDraw> draws;
-    Targets tg;
-    float fval = 0.0f;
-    uint32_t uval = 0;
-    int graph = -1, count = 0;
+    std::vector<Segment> views;   // CMD_FLUSH
+    std::vector<PsFill> fills;    // CMD_CLEARS
+    int graph = -1, count = 0;    // CMD_GRAPH
 };
 
 // A recorded command list replayed as one hipGraph (stream capture of the
-// kernels of every command).  The draw records live in a device buffer owned
-// by the graph, so replays need no host-to-device copies.
+// kernels of every command).  Each command's device records (draws, views,
+// fills) live in a buffer owned by the graph, so replays copy nothing.
 struct Graph {
     bool alive = false;
     std::vector<Cmd> cmds;
-    std::vector<PsDraw*> dev_draws;
+    std::vector<uint8_t*> blobs;
     hipGraphExec_t exec = nullptr;
     uint64_t gen = 0;
     int64_t flushes = 0;
@@ -87,6 +96,13 @@ template <typename T> struct DevBuf {
 };
 
 struct ProfRec { const char* name; hipEvent_t a, b; };
+
+// device image of one command: draws, then views, then fills
+struct Blob {
+    std::vector<uint8_t> bytes;
+    size_t off_views = 0, off_fills = 0;
+    int nd = 0, nv = 0, nf = 0;
+};
 
 }  // namespace
 
@@ -108,7 +124,8 @@ struct psgpu_ctx {
     int cur_prog = -1;
     int fbo0 = PSGPU_DISPLAY;
     int depth_sel = PSGPU_DEFAULT_DEPTH;
-    std::vector<PsDraw> queue;
+    std::vector<Segment> queue;        // pending views with draws
+    std::vector<PsFill> pending_fills; // pending clears (run before the queued draws)
     int64_t queue_tris = 0;
     std::vector<Cmd> log;
     bool capturing = false;        // between psgpu_capture_begin and _end
@@ -118,7 +135,7 @@ struct psgpu_ctx {
     uint64_t buf_gen = 1;          // bumped whenever a scratch buffer moves
     psgpu_tiling tiling = {16, 16, 1};
     // scratch
-    DevBuf<PsDraw> d_draws;
+    DevBuf<uint8_t> d_blob;
     DevBuf<PsTri> d_tris;
     DevBuf<float> d_vary;
     DevBuf<uint32_t> d_nrows, d_rowoff, d_partial;
@@ -135,9 +152,9 @@ struct psgpu_ctx {
     int* h_flags = nullptr;
     PsCounters* d_cnt = nullptr;
     PsCounters* h_cnt = nullptr;
-    // draw staging (pinned), ring of slots
+    // command staging (pinned), ring of slots
     static const int kStage = 4;
-    PsDraw* h_stage[kStage] = {nullptr};
+    uint8_t* h_stage[kStage] = {nullptr};
     size_t h_stage_cap[kStage] = {0};
     hipEvent_t stage_ev[kStage] = {nullptr};
     int stage_next = 0;
@@ -292,27 +309,6 @@ PsTex tex_desc(const Tex& x)
     return d;
 }
 
-int stage_draws(psgpu_ctx* c, const std::vector<PsDraw>& draws, PsDraw** out)
-{
-    int s = c->stage_next;
-    c->stage_next = (s + 1) % psgpu_ctx::kStage;
-    if (c->stage_ev[s]) HIPCHK(c, hipEventSynchronize(c->stage_ev[s]));
-    if (c->h_stage_cap[s] < draws.size()) {
-        if (c->h_stage[s]) HIPCHK(c, hipHostFree(c->h_stage[s]));
-        size_t cap = std::max(draws.size(), (size_t)16);
-        HIPCHK(c, hipHostMalloc((void**)&c->h_stage[s], cap * sizeof(PsDraw)));
-        c->h_stage_cap[s] = cap;
-    }
-    memcpy(c->h_stage[s], draws.data(), draws.size() * sizeof(PsDraw));
-    int rc = ensure(c, c->d_draws, draws.size());
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->d_draws.p, c->h_stage[s], draws.size() * sizeof(PsDraw), hipMemcpyHostToDevice, c->stream));
-    if (!c->stage_ev[s]) HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming));
-    HIPCHK(c, hipEventRecord(c->stage_ev[s], c->stream));
-    *out = c->d_draws.p;
-    return PSGPU_OK;
-}
-
 int key_bits(int64_t n_keys)
 {
     int b = 0;
@@ -320,25 +316,115 @@ int key_bits(int64_t n_keys)
     return b < 1 ? 1 : b;
 }
 
-// run one flush command: the whole pipeline over its draws
-int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// lay out a command's device records: draws (view-tagged, in submission
+// order), views (tiling and scratch offsets are filled in here), fills
+int build_blob(psgpu_ctx* c, const Cmd& cmd, Blob& b, PsFrame& f, int64_t& T, int& nvs, bool* progs, bool* shade_progs,
+               int64_t& n_pix)
 {
-    const std::vector<PsDraw>& draws = cmd.draws;
-    if (draws.empty()) return PSGPU_OK;
-    const Targets& tg = cmd.tg;
-    int64_t T = 0;
-    int nvs = 1;
-    bool progs[8] = {false}, shade_progs[8] = {false};
-    for (const PsDraw& d : draws) {
-        T += d.tri_count;
-        nvs = std::max(nvs, d.nv);
-        progs[d.prog] = true;
-        if (d.writes_colour) shade_progs[d.prog] = true;
+    std::vector<PsDraw> draws;
+    std::vector<PsView> views;
+    T = 0;
+    nvs = 1;
+    n_pix = 0;
+    memset(&f, 0, sizeof(f));
+    f.chunk_w = c->tiling.chunk_width;
+    f.tile_rows = c->tiling.tile_rows;
+    f.tile_cols = 64 / f.tile_rows;
+    int64_t tile_base = 0, stile_base = 0;
+    for (size_t v = 0; v < cmd.views.size(); v++) {
+        const Segment& sg = cmd.views[v];
+        PsView pv;
+        memset(&pv, 0, sizeof(pv));
+        pv.W = sg.tg.vpw;
+        pv.H = sg.tg.vph;
+        pv.n_chunks = (pv.W + f.chunk_w - 1) / f.chunk_w;
+        pv.n_tiles_x = (pv.n_chunks + f.tile_cols - 1) / f.tile_cols;
+        pv.n_tiles_y = (pv.H + f.tile_rows - 1) / f.tile_rows;
+        pv.n_stiles_x = (pv.W + 15) / 16;
+        pv.tile_base = tile_base;
+        pv.stile_base = stile_base;
+        tile_base += (int64_t)pv.n_tiles_x * pv.n_tiles_y;
+        stile_base += (int64_t)pv.n_stiles_x * ((pv.H + 15) / 16);
+        pv.colour = sg.tg.colour; pv.colour_scan = sg.tg.colour_scan; pv.colour_top_down = sg.tg.colour_top_down;
+        pv.colour_h = sg.tg.colour_h;
+        pv.depth = sg.tg.depth; pv.depth_scan = sg.tg.depth_scan;
+        pv.winner = (int*)(intptr_t)n_pix;   // offset; rebased once the winner buffer is known
+        n_pix += (int64_t)pv.W * pv.H;
+        views.push_back(pv);
+        for (PsDraw d : sg.draws) {
+            d.view = (int)v;
+            d.tri_start = T;
+            T += d.tri_count;
+            nvs = std::max(nvs, d.nv);
+            progs[d.prog] = true;
+            if (d.writes_colour) shade_progs[d.prog] = true;
+            draws.push_back(d);
+        }
     }
+    f.n_views = (int)views.size();
+    f.n_tiles = tile_base;
+    f.n_stiles = stile_base;
+    int rc = ensure(c, c->d_winner, (size_t)std::max<int64_t>(n_pix, 1));
+    if (rc) return rc;
+    for (PsView& pv : views) pv.winner = c->d_winner.p + (intptr_t)pv.winner;
+    b.nd = (int)draws.size();
+    b.nv = (int)views.size();
+    b.nf = (int)cmd.fills.size();
+    b.off_views = align256(draws.size() * sizeof(PsDraw));
+    b.off_fills = b.off_views + align256(views.size() * sizeof(PsView));
+    b.bytes.assign(b.off_fills + cmd.fills.size() * sizeof(PsFill) + 16, 0);
+    if (!draws.empty()) memcpy(b.bytes.data(), draws.data(), draws.size() * sizeof(PsDraw));
+    if (!views.empty()) memcpy(b.bytes.data() + b.off_views, views.data(), views.size() * sizeof(PsView));
+    if (!cmd.fills.empty()) memcpy(b.bytes.data() + b.off_fills, cmd.fills.data(), cmd.fills.size() * sizeof(PsFill));
+    return PSGPU_OK;
+}
+
+// copy a blob to the device through the pinned staging ring
+int stage_blob(psgpu_ctx* c, const Blob& b, uint8_t** out)
+{
+    int s = c->stage_next;
+    c->stage_next = (s + 1) % psgpu_ctx::kStage;
+    if (c->stage_ev[s]) HIPCHK(c, hipEventSynchronize(c->stage_ev[s]));
+    if (c->h_stage_cap[s] < b.bytes.size()) {
+        if (c->h_stage[s]) HIPCHK(c, hipHostFree(c->h_stage[s]));
+        size_t cap = std::max(b.bytes.size(), (size_t)4096);
+        HIPCHK(c, hipHostMalloc((void**)&c->h_stage[s], cap));
+        c->h_stage_cap[s] = cap;
+    }
+    memcpy(c->h_stage[s], b.bytes.data(), b.bytes.size());
+    // one device blob per staging slot, so copies in flight never alias
+    int rc = ensure(c, c->d_blob, psgpu_ctx::kStage * align256(std::max(b.bytes.size(), (size_t)1)) + 256);
+    if (rc) return rc;
+    size_t slot_bytes = c->d_blob.cap / psgpu_ctx::kStage / 256 * 256;
+    if (slot_bytes < b.bytes.size()) {
+        if ((rc = ensure(c, c->d_blob, psgpu_ctx::kStage * align256(b.bytes.size()) * 2))) return rc;
+        slot_bytes = c->d_blob.cap / psgpu_ctx::kStage / 256 * 256;
+    }
+    uint8_t* dst = c->d_blob.p + (size_t)s * slot_bytes;
+    HIPCHK(c, hipMemcpyAsync(dst, c->h_stage[s], b.bytes.size(), hipMemcpyHostToDevice, c->stream));
+    if (!c->stage_ev[s]) HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(c->stage_ev[s], c->stream));
+    *out = dst;
+    return PSGPU_OK;
+}
+
+// run one flush: the whole pipeline over the draws of all its views
+int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
+{
+    Blob blob;
+    PsFrame f;
+    int64_t T, n_pix;
+    int nvs;
+    bool progs[8] = {false}, shade_progs[8] = {false};
+    int rc = build_blob(c, cmd, blob, f, T, nvs, progs, shade_progs, n_pix);
+    if (rc) return rc;
     if (T == 0) return PSGPU_OK;
-    int rc;
-    PsDraw* dd = staged;
-    if (!dd && (rc = stage_draws(c, draws, &dd))) return rc;
+    uint8_t* dev = staged;
+    if (!dev && (rc = stage_blob(c, blob, &dev))) return rc;
+    PsDraw* dd = (PsDraw*)dev;
+    f.views = (const PsView*)(dev + blob.off_views);
     if ((rc = ensure(c, c->d_tris, T)) || (rc = ensure(c, c->d_vary, (size_t)T * 3 * nvs)) ||
         (rc = ensure(c, c->d_nrows, T)) || (rc = ensure(c, c->d_rowoff, T + 1)))
         return rc;
@@ -355,37 +441,23 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
         (rc = ensure(c, c->d_vals2, c->seg_cap)))
         return rc;
     int nbr = psk_rs_blocks(c->seg_cap);
-    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 1)))
+    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 1)) ||
+        (rc = ensure(c, c->d_kstart, f.n_tiles + 1)))
         return rc;
-
-    // tiling
-    PsFrame f;
-    memset(&f, 0, sizeof(f));
-    f.W = tg.vpw;
-    f.H = tg.vph;
-    f.chunk_w = c->tiling.chunk_width;
-    f.tile_rows = c->tiling.tile_rows;
-    f.tile_cols = 64 / f.tile_rows;
-    f.n_chunks = (f.W + f.chunk_w - 1) / f.chunk_w;
-    f.n_tiles_x = (f.n_chunks + f.tile_cols - 1) / f.tile_cols;
-    f.n_tiles_y = (f.H + f.tile_rows - 1) / f.tile_rows;
-    f.n_keys = (int64_t)f.n_tiles_x * f.n_tiles_y;   // bin keys are tiles
-    f.colour = tg.colour; f.colour_scan = tg.colour_scan; f.colour_top_down = tg.colour_top_down; f.colour_h = tg.colour_h;
-    f.depth = tg.depth; f.depth_scan = tg.depth_scan;
-    if ((rc = ensure(c, c->d_kstart, f.n_keys + 1)) || (rc = ensure(c, c->d_winner, (size_t)f.W * f.H))) return rc;
-    f.winner = c->d_winner.p;
     if (c->debug) {
-        if (c->dbg_w != f.W || c->dbg_h != f.H) return fail(c, PSGPU_ERR_STATE, "debug planes sized for another viewport");
+        if (f.n_views != 1 || c->dbg_w != cmd.views[0].tg.vpw || c->dbg_h != cmd.views[0].tg.vph)
+            return fail(c, PSGPU_ERR_STATE, "debug planes need a single view of the debug viewport");
         f.dbg_last = c->dbg_last; f.dbg_rcount = c->dbg_rcount; f.dbg_rhash = c->dbg_rhash;
         f.dbg_pcount = c->dbg_pcount; f.dbg_phash = c->dbg_phash;
     }
 
     hipStream_t s = c->stream;
-    int nd = (int)draws.size();
+    int nd = blob.nd;
+    const PsView* dv = f.views;
     {
         ProfScope p(c, "setup");
         for (int k = 1; k < 8; k++)
-            if (progs[k]) psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_nrows.p, c->d_cnt, f.W, f.H);
+            if (progs[k]) psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_nrows.p, c->d_cnt, dv);
     }
     {
         ProfScope p(c, "scan_rows");
@@ -399,7 +471,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
     }
     {
         ProfScope p(c, "span");
-        psk_span(s, c->d_tris.p, dd, c->d_spantri.p, c->d_rowoff.p, T, c->d_spans.p, c->d_segcount.p, f.W, f.chunk_w,
+        psk_span(s, c->d_tris.p, dd, c->d_spantri.p, c->d_rowoff.p, T, c->d_spans.p, c->d_segcount.p, dv, f.chunk_w,
                  c->span_cap, c->d_cnt);
     }
     {
@@ -408,20 +480,20 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
     }
     {
         ProfScope p(c, "segments");
-        psk_segments(s, c->d_spans.p, c->d_segoff.p, c->d_rowoff.p, T, f, c->d_keys.p, c->d_vals.p, c->span_cap, c->seg_cap,
-                     c->d_cnt);
+        psk_segments(s, c->d_spans.p, c->d_segoff.p, c->d_rowoff.p, T, f, c->d_tris.p, dd, c->d_keys.p, c->d_vals.p,
+                     c->span_cap, c->seg_cap, c->d_cnt);
     }
     bool second = false;
     {
         ProfScope p(c, "radix_sort");
         psk_radix_sort(s, c->d_keys.p, c->d_vals.p, c->d_keys2.p, c->d_vals2.p, c->d_sizes + 1, c->seg_cap,
-                       key_bits(f.n_keys), c->d_hist.p, c->d_histoff.p, c->d_partial.p, c->d_cnt, second);
+                       key_bits(f.n_tiles), c->d_hist.p, c->d_histoff.p, c->d_partial.p, c->d_cnt, second);
     }
     uint32_t* skeys = second ? c->d_keys2.p : c->d_keys.p;
     uint32_t* svals = second ? c->d_vals2.p : c->d_vals.p;
     {
         ProfScope p(c, "bounds");
-        psk_bounds(s, skeys, c->d_sizes + 1, f.n_keys, c->d_kstart.p, c->d_cnt);
+        psk_bounds(s, skeys, c->d_sizes + 1, f.n_tiles, c->d_kstart.p, c->d_cnt);
     }
     {
         ProfScope p(c, "raster");
@@ -437,25 +509,34 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged)
     return PSGPU_OK;
 }
 
+int run_clears(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
+{
+    if (cmd.fills.empty()) return PSGPU_OK;
+    Blob blob;
+    blob.off_views = blob.off_fills = 0;
+    blob.nf = (int)cmd.fills.size();
+    blob.bytes.assign(cmd.fills.size() * sizeof(PsFill), 0);
+    memcpy(blob.bytes.data(), cmd.fills.data(), blob.bytes.size());
+    uint8_t* dev = staged;
+    int rc;
+    if (!dev && (rc = stage_blob(c, blob, &dev))) return rc;
+    int64_t mx = 1;
+    for (const PsFill& fl : cmd.fills) mx = std::max<int64_t>(mx, fl.rows * fl.width);
+    ProfScope p(c, "clear");
+    for (size_t i = 0; i < cmd.fills.size(); i += 65535)
+        psk_fills(c->stream, (const PsFill*)dev + i, (int)std::min<size_t>(65535, cmd.fills.size() - i), mx, c->d_cnt);
+    HIPCHK(c, hipGetLastError());
+    return PSGPU_OK;
+}
+
 int graph_launch_n(psgpu_ctx* c, int id, int count);
 
-int run_cmd(psgpu_ctx* c, const Cmd& cmd, PsDraw* staged = nullptr)
+int run_cmd(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged = nullptr)
 {
     switch (cmd.type) {
-    case CMD_FLUSH:
-        return run_flush(c, cmd, staged);
-    case CMD_GRAPH:
-        return graph_launch_n(c, cmd.graph, cmd.count);
-    case CMD_CLEAR_DEPTH: {
-        // clear16 fills m_bytes = scanline * height (fbo.cpp:348-371)
-        uint32_t v;
-        memcpy(&v, &cmd.fval, 4);
-        psk_fill_u32(c->stream, (uint32_t*)cmd.tg.depth, (int64_t)cmd.tg.depth_scan * cmd.tg.depth_h, v, c->d_cnt);
-        return PSGPU_OK;
-    }
-    case CMD_CLEAR_COLOUR:
-        psk_clear_colour(c->stream, c->display, c->dev_w, c->dev_h, c->dev_w, cmd.uval, c->d_cnt);
-        return PSGPU_OK;
+    case CMD_FLUSH: return run_flush(c, cmd, staged);
+    case CMD_CLEARS: return run_clears(c, cmd, staged);
+    case CMD_GRAPH: return graph_launch_n(c, cmd.graph, cmd.count);
     }
     return PSGPU_OK;
 }
@@ -487,16 +568,49 @@ int submit(psgpu_ctx* c, Cmd&& cmd)
     return mark_done(c, (int)c->log.size() - 1);
 }
 
+// pending clears first, then the pending draws of every view as one flush
 int queue_to_log(psgpu_ctx* c)
 {
-    if (c->queue.empty()) return PSGPU_OK;
-    Cmd cmd;
-    cmd.type = CMD_FLUSH;
-    cmd.draws.swap(c->queue);
-    c->queue_tris = 0;
-    int rc = current_targets(c, cmd.tg);
-    if (rc) return rc;
-    return submit(c, std::move(cmd));
+    int rc;
+    if (!c->pending_fills.empty()) {
+        Cmd cmd;
+        cmd.type = CMD_CLEARS;
+        cmd.fills.swap(c->pending_fills);
+        if ((rc = submit(c, std::move(cmd)))) return rc;
+    }
+    if (!c->queue.empty()) {
+        Cmd cmd;
+        cmd.type = CMD_FLUSH;
+        cmd.views.swap(c->queue);
+        c->queue_tris = 0;
+        if ((rc = submit(c, std::move(cmd)))) return rc;
+    }
+    return PSGPU_OK;
+}
+
+bool queue_uses(psgpu_ctx* c, const void* buf)
+{
+    for (const Segment& sg : c->queue)
+        if ((const void*)sg.tg.colour == buf || (const void*)sg.tg.depth == buf) return true;
+    return false;
+}
+
+int add_fill(psgpu_ctx* c, void* ptr, int64_t rows, int64_t width, int64_t scan, uint32_t value)
+{
+    if (queue_uses(c, ptr)) {       // draws already queued for this buffer come first
+        int rc = queue_to_log(c);
+        if (rc) return rc;
+    }
+    if (rows <= 0 || width <= 0) return PSGPU_OK;
+    PsFill fl;
+    memset(&fl, 0, sizeof(fl));
+    fl.ptr = (uint32_t*)ptr;
+    fl.rows = rows;
+    fl.width = width;
+    fl.scan = scan;
+    fl.value = value;
+    c->pending_fills.push_back(fl);
+    return PSGPU_OK;
 }
 
 int do_sync(psgpu_ctx* c)
@@ -558,7 +672,7 @@ int graph_capture(psgpu_ctx* c, Graph& g)
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     c->in_stream_capture = true;
     int rc = PSGPU_OK;
-    for (size_t i = 0; i < g.cmds.size() && !rc; i++) rc = run_cmd(c, g.cmds[i], g.dev_draws[i]);
+    for (size_t i = 0; i < g.cmds.size() && !rc; i++) rc = run_cmd(c, g.cmds[i], g.blobs[i]);
     c->in_stream_capture = false;
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
@@ -574,6 +688,19 @@ int graph_launch_n(psgpu_ctx* c, int id, int count)
 {
     Graph& g = c->graphs[id];
     if (!g.exec || g.gen != c->buf_gen) {
+        // the blobs hold winner-buffer addresses: rebuild them for the current buffers
+        for (size_t i = 0; i < g.cmds.size(); i++) {
+            if (g.cmds[i].type != CMD_FLUSH || !g.blobs[i]) continue;
+            Blob b;
+            PsFrame f;
+            int64_t T, npx;
+            int nvs;
+            bool p1[8] = {false}, p2[8] = {false};
+            int rc = build_blob(c, g.cmds[i], b, f, T, nvs, p1, p2, npx);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(g.blobs[i], b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
         int rc = graph_capture(c, g);
         if (rc) return rc;
     }
@@ -647,7 +774,7 @@ int psgpu_destroy(psgpu_ctx* c)
     if (c->h_flags) hipHostFree(c->h_flags);
     for (auto& t : c->tex) fr(t.d);
     for (auto& v : c->vbo) fr(v.d);
-    fr(c->d_draws.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
+    fr(c->d_blob.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
     fr(c->d_spantri.p); fr(c->d_spans.p); fr(c->d_segcount.p); fr(c->d_segoff.p); fr(c->d_keys.p); fr(c->d_vals.p);
     fr(c->d_keys2.p); fr(c->d_vals2.p); fr(c->d_hist.p); fr(c->d_histoff.p); fr(c->d_kstart.p); fr(c->d_winner.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
@@ -657,7 +784,7 @@ int psgpu_destroy(psgpu_ctx* c)
     }
     for (auto& g : c->graphs) {
         if (g.exec) hipGraphExecDestroy(g.exec);
-        for (auto p : g.dev_draws) if (p) hipFree(p);
+        for (auto p : g.blobs) if (p) hipFree(p);
     }
     for (auto& r : c->prof_pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -955,8 +1082,6 @@ int psgpu_viewport(psgpu_ctx* c, int width, int height)
 {
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
     if (width < 1 || height < 1) return fail(c, PSGPU_ERR_INVALID, "bad viewport");
-    int rc = queue_to_log(c);
-    if (rc) return rc;
     c->vp_w = width;
     c->vp_h = height;
     return PSGPU_OK;
@@ -968,8 +1093,6 @@ int psgpu_target_set(psgpu_ctx* c, int fbo_slot, int tex)
     if (fbo_slot < 0 || fbo_slot >= 16) return fail(c, PSGPU_ERR_RANGE, "FBO slot out of range");
     if (fbo_slot != 0) return fail(c, PSGPU_ERR_UNSUPPORTED, "the built-in processors write FBO slot 0 only");
     if (tex != PSGPU_DISPLAY && !valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
-    int rc = queue_to_log(c);
-    if (rc) return rc;
     c->fbo0 = tex;
     return PSGPU_OK;
 }
@@ -982,8 +1105,6 @@ int psgpu_depth_set(psgpu_ctx* c, int tex)
         // pipeline.cpp:231-235
         if (c->tex[tex].elem != 4 || c->tex[tex].scan % 4) return fail(c, PSGPU_ERR_INVALID, "PuresoftPipeline::setDepth");
     }
-    int rc = queue_to_log(c);
-    if (rc) return rc;
     c->depth_sel = tex;
     return PSGPU_OK;
 }
@@ -991,25 +1112,30 @@ int psgpu_depth_set(psgpu_ctx* c, int tex)
 int psgpu_clear_depth(psgpu_ctx* c, float furthest)
 {
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
-    int rc = queue_to_log(c);
+    Targets tg;
+    int rc = current_targets(c, tg);
     if (rc) return rc;
-    Cmd cmd;
-    cmd.type = CMD_CLEAR_DEPTH;
-    cmd.fval = furthest;
-    if ((rc = current_targets(c, cmd.tg))) return rc;
-    return submit(c, std::move(cmd));
+    uint32_t v;
+    memcpy(&v, &furthest, 4);
+    // clear16 fills m_bytes = scanline * height (fbo.cpp:348-371)
+    return add_fill(c, tg.depth, 1, (int64_t)tg.depth_scan * tg.depth_h, 0, v);
 }
 
 int psgpu_clear_colour(psgpu_ctx* c, uint32_t bgra)
 {
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
-    int rc = queue_to_log(c);
+    // clearColour clears the display with clear4, which skips the last memory row
+    // (pipeline.cpp:340-343, fbo.cpp:332-346)
+    return add_fill(c, c->display, c->dev_h - 1, c->dev_w, c->dev_w, bgra);
+}
+
+int psgpu_clear_target(psgpu_ctx* c, uint32_t bgra)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    Targets tg;
+    int rc = current_targets(c, tg);
     if (rc) return rc;
-    Cmd cmd;
-    cmd.type = CMD_CLEAR_COLOUR;
-    cmd.uval = bgra;
-    if ((rc = current_targets(c, cmd.tg))) return rc;
-    return submit(c, std::move(cmd));
+    return add_fill(c, tg.colour, tg.colour_h, tg.colour_w, tg.colour_scan, bgra);
 }
 
 // ---- drawing ------------------------------------------------------------------
@@ -1123,12 +1249,23 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     if (tg.colour_w < c->vp_w || tg.colour_h < c->vp_h || tg.depth_w < c->vp_w || tg.depth_h < c->vp_h)
         return fail(c, PSGPU_ERR_STATE, "render targets are smaller than the viewport");
     if (d.tri_count == 0) return PSGPU_OK;
+    // draws with the targets and viewport of the last pending view join it;
+    // otherwise they open a new view of the same flush, unless a pending view
+    // already renders into one of these buffers (then that work goes first)
+    if (c->queue.empty() || !c->queue.back().tg.same(tg)) {
+        if (queue_uses(c, tg.colour) || queue_uses(c, tg.depth)) {
+            if ((rc = queue_to_log(c))) return rc;
+        }
+        Segment sg;
+        sg.tg = tg;
+        c->queue.push_back(std::move(sg));
+    }
     d.tri_start = c->queue_tris;
     d.id_base = c->dbg_tri_counter;
     c->dbg_tri_counter += d.tri_count;
     c->queue_tris += d.tri_count;
     c->stats.tris_in += d.tri_count;
-    c->queue.push_back(d);
+    c->queue.back().draws.push_back(d);
     return PSGPU_OK;
 }
 
@@ -1338,16 +1475,27 @@ int psgpu_capture_end(psgpu_ctx* c, int* out_graph)
     }
     if ((rc = do_sync(c))) return rc;
     for (const Cmd& cmd : g.cmds) {
-        PsDraw* p = nullptr;
-        if (cmd.type == CMD_FLUSH && !cmd.draws.empty()) {
-            HIPCHK(c, hipMalloc((void**)&p, cmd.draws.size() * sizeof(PsDraw)));
-            HIPCHK(c, hipMemcpy(p, cmd.draws.data(), cmd.draws.size() * sizeof(PsDraw), hipMemcpyHostToDevice));
+        uint8_t* p = nullptr;
+        Blob b;
+        if (cmd.type == CMD_FLUSH) {
+            PsFrame f;
+            int64_t T, npx;
+            int nvs;
+            bool p1[8] = {false}, p2[8] = {false};
+            if ((rc = build_blob(c, cmd, b, f, T, nvs, p1, p2, npx))) return rc;
             g.flushes++;
+        } else if (cmd.type == CMD_CLEARS) {
+            b.bytes.assign(cmd.fills.size() * sizeof(PsFill), 0);
+            memcpy(b.bytes.data(), cmd.fills.data(), b.bytes.size());
         }
-        g.dev_draws.push_back(p);
+        if (!b.bytes.empty()) {
+            HIPCHK(c, hipMalloc((void**)&p, b.bytes.size()));
+            HIPCHK(c, hipMemcpy(p, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
+        }
+        g.blobs.push_back(p);
     }
     if ((rc = graph_capture(c, g))) {
-        for (auto p : g.dev_draws) if (p) hipFree(p);
+        for (auto p : g.blobs) if (p) hipFree(p);
         return rc;
     }
     size_t i = 0;
@@ -1383,7 +1531,7 @@ int psgpu_graph_destroy(psgpu_ctx* c, int graph)
     if (rc) return rc;
     Graph& g = c->graphs[graph];
     if (g.exec) hipGraphExecDestroy(g.exec);
-    for (auto p : g.dev_draws) if (p) hipFree(p);
+    for (auto p : g.blobs) if (p) hipFree(p);
     c->graphs[graph] = Graph();
     return PSGPU_OK;
 }

@@ -85,6 +85,7 @@ _SIGS = {
     "psgpu_depth_set": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_clear_depth": ([C.c_void_p, C.c_float], C.c_int),
     "psgpu_clear_colour": ([C.c_void_p, C.c_uint32], C.c_int),
+    "psgpu_clear_target": ([C.c_void_p, C.c_uint32], C.c_int),
     "psgpu_draw": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_flush": ([C.c_void_p], C.c_int),
     "psgpu_sync": ([C.c_void_p], C.c_int),
@@ -289,6 +290,9 @@ class Context:
 
     def clear_colour(self, bgra=0):
         self._chk(self.L.psgpu_clear_colour(self.h, bgra))
+
+    def clear_target(self, bgra=0):
+        self._chk(self.L.psgpu_clear_target(self.h, bgra))
 
     def draw(self, vao):
         self._chk(self.L.psgpu_draw(self.h, vao))

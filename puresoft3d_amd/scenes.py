@@ -346,3 +346,132 @@ class GpuScene:
                 if k in sc.uniforms:
                     ctx.uniform(k, sc.uniforms[k])
         ctx.flush()
+
+
+# ---------------------------------------------------------------- batches of views
+class SceneAsset:
+    """A resident scene for batched views: one mesh, albedo + normal textures."""
+
+    def __init__(self, mesh, textures, center, radius, name):
+        self.mesh, self.textures, self.center, self.radius, self.name = mesh, textures, center, radius, name
+
+    def triangles(self):
+        return min(a.shape[0] for a in self.mesh.values()) // 3
+
+
+def batch_assets(n_scenes=16, seed=11, tri_lo=10_000, tri_hi=500_000, tex_size=1024):
+    """n_scenes seeded terrain patches with log-uniform triangle counts and
+    log2(tex_size)+1-level mipmapped albedo + normal maps."""
+    rng = np.random.default_rng(seed)
+    assets = []
+    lv = mip_levels(tex_size, tex_size)
+    for k in range(n_scenes):
+        T = float(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        cells = max(2, int(round(np.sqrt(T / 2.0))))
+        mesh, hgt, xs, zs = terrain(cells, 40.0, seed * 100 + k, height=4.0, uv_repeat=4.0)
+        tex = [TextureSpec([noise_texture(tex_size, seed * 100 + k + 1, cells=16, octaves=3)], wrap=N.WRAP, levels=lv),
+               TextureSpec([normal_map(tex_size, seed * 100 + k + 2, cells=16)], wrap=N.WRAP, levels=lv)]
+        assets.append(SceneAsset(mesh, tex, np.array([0.0, 0.0, 0.0]), 28.0, "patch%d_%dk" % (k, cells * cells * 2 // 1000)))
+    return assets
+
+
+class ViewSpec:
+    def __init__(self, scene, width, height, eye, target, light, spec_exp=24):
+        self.scene, self.width, self.height = scene, width, height
+        self.eye, self.target, self.light, self.spec_exp = eye, target, light, spec_exp
+
+    def uniforms(self):
+        P = perspective(np.radians(60.0), self.width / self.height, 0.5, 200.0)
+        V = look_at(self.eye, self.target)
+        return {3: colmajor(P @ V), 4: colmajor(np.eye(4)), 5: colmajor(np.eye(4)),
+                7: vec4(*self.light, 0.0), 8: vec4(*self.eye, 0.0), 9: np.array([0], np.int32),
+                10: np.array([1], np.int32), 11: np.array([self.spec_exp], np.int32)}
+
+
+def batch_views(assets, n_views=256, seed=12, side_lo=256, side_hi=1024, aspect_jitter=0.25):
+    """views with area log-uniform in [side_lo^2, side_hi^2]; cameras seeded on
+    an upper sphere around their scene"""
+    rng = np.random.default_rng(seed)
+    views = []
+    for i in range(n_views):
+        a = float(np.exp(rng.uniform(np.log(side_lo * side_lo), np.log(side_hi * side_hi))))
+        asp = float(np.exp(rng.uniform(-aspect_jitter, aspect_jitter)))
+        w = max(8, int(round(np.sqrt(a * asp))))
+        h = max(8, int(round(a / w)))
+        s = int(rng.integers(0, len(assets)))
+        th = rng.uniform(0, 2 * np.pi)
+        el = rng.uniform(0.35, 1.1)
+        r = assets[s].radius * rng.uniform(1.4, 2.0)
+        eye = assets[s].center + r * np.array([np.cos(el) * np.cos(th), np.sin(el), np.cos(el) * np.sin(th)])
+        light = assets[s].center + np.array([20.0, 50.0, 10.0])
+        views.append(ViewSpec(s, w, h, eye, assets[s].center, light))
+    return views
+
+
+class GpuBatch:
+    """Assets resident on one context plus per-view colour/depth textures.
+    submit() queues every view (targets, viewport, clears, camera, draw); the
+    library renders all of them in one pass of its kernels."""
+
+    def __init__(self, ctx, assets, views, gen_mips=True):
+        self.ctx, self.assets, self.views = ctx, assets, views
+        self.tex = []
+        self.vao = []
+        for a in assets:
+            ids = []
+            for t in a.textures:
+                idx = ctx.texture_create(t.width, t.height, 4, wrap=t.wrap, mip_levels=t.levels, pixels=t.images[0])
+                if t.levels > 1 and gen_mips:
+                    ctx.texture_generate_mips(idx)
+                ids.append(idx)
+            self.tex.append(ids)
+            v = ctx.vao_create()
+            for s, arr in a.mesh.items():
+                ctx.vao_attach(v, s, ctx.vbo_create(arr))
+            self.vao.append(v)
+        self.prog = ctx.program_create(N.PROC_BLINN)
+        self.targets = []
+        for vw in views:
+            c = ctx.texture_create(vw.width, vw.height, 4, wrap=N.CLAMP)
+            d = ctx.texture_create(vw.width, vw.height, 4, wrap=N.CLAMP)
+            self.targets.append((c, d))
+
+    def submit(self, indices=None, clear_bgra=0x00302010):
+        ctx = self.ctx
+        ctx.program_use(self.prog)
+        for i in (range(len(self.views)) if indices is None else indices):
+            vw = self.views[i]
+            c, d = self.targets[i]
+            ctx.target(c)
+            ctx.depth_target(d)
+            ctx.viewport(vw.width, vw.height)
+            ctx.clear_depth(1.0)
+            ctx.clear_target(clear_bgra)
+            u = vw.uniforms()
+            u[9] = np.array([self.tex[vw.scene][0]], np.int32)
+            u[10] = np.array([self.tex[vw.scene][1]], np.int32)
+            for k, val in u.items():
+                ctx.uniform(k, val)
+            ctx.draw(self.vao[vw.scene])
+        ctx.target(N.DISPLAY)
+        ctx.depth_target(N.DEFAULT_DEPTH)
+        ctx.viewport(ctx.width, ctx.height)
+
+    def read(self, i):
+        c, d = self.targets[i]
+        vw = self.views[i]
+        col = self.ctx.texture_download(c).reshape(vw.height, vw.width, 4)
+        dep = self.ctx.texture_download(d).view(np.float32).reshape(vw.height, vw.width)
+        return col, dep
+
+    def view_scene(self, i):
+        """the view as a stand-alone Scene (for the oracle or a one-view render)"""
+        vw = self.views[i]
+        a = self.assets[vw.scene]
+        sc = Scene(vw.width, vw.height)
+        sc.name = "view%d_%s_%dx%d" % (i, a.name, vw.width, vw.height)
+        sc.meshes["m"] = a.mesh
+        sc.textures = list(a.textures)
+        sc.uniforms = vw.uniforms()
+        sc.draws.append(DrawSpec("m", N.PROC_BLINN))
+        return sc
