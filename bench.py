@@ -67,6 +67,63 @@ def cpu_baseline(scene, seconds, threads):
             "fragments_shaded_per_s": frag / tot}
 
 
+def batch_section(args, N, S, log):
+    """256 views (area log-uniform 256^2..1024^2) over 16 resident scenes of
+    10k-500k triangles, 1024^2 mipmapped textures, Blinn-Phong trilinear;
+    the whole batch is one recorded command list replayed as a hipGraph."""
+    from puresoft3d_amd import autotune as AT
+    t0 = time.perf_counter()
+    assets = S.batch_assets(n_scenes=16, seed=11, tri_lo=10_000, tri_hi=500_000, tex_size=1024)
+    views = S.batch_views(assets, n_views=256, seed=12, side_lo=256, side_hi=1024)
+    log("[bench] batch: %d scenes (%d..%d tris), %d views, generated in %.1f s" % (
+        len(assets), min(a.triangles() for a in assets), max(a.triangles() for a in assets), len(views),
+        time.perf_counter() - t0))
+    ctx = N.Context(64, 64)
+    b = S.GpuBatch(ctx, assets, views)
+    ctx.capture_begin()
+    b.submit()
+    g = ctx.capture_end()
+    run = lambda: ctx.graph_launch(g, 1)  # noqa: E731
+    tuned = None
+    if args.tune:
+        best, table = AT.autotune(ctx, run, reps=3, log=log)
+        tuned = {"best": {"chunk_width": best[0], "tile_rows": best[1], "waves_per_group": best[2]},
+                 "searched": len(table),
+                 "table_ms": {"%dx%d/c%d/w%d" % (r[1], r[2], r[0][0], r[0][2]): round(r[3], 3) for r in table}}
+    for _ in range(3):
+        run()
+    ctx.sync()
+    ctx.reset_stats()
+    lat = []
+    for _ in range(args.batch_steps):
+        t1 = time.perf_counter()
+        run()
+        ctx.sync()
+        lat.append(time.perf_counter() - t1)
+    st = ctx.stats()
+    tot = sum(lat)
+    px = sum(v.width * v.height for v in views)
+    # one view per launch, same views: the plain path's per-view latency
+    single = []
+    for i in range(0, len(views), 8):
+        t1 = time.perf_counter()
+        b.submit([i])
+        ctx.sync()
+        single.append(time.perf_counter() - t1)
+    ctx.close()
+    out = {"views": len(views), "scenes": len(assets), "pixels_per_batch": px,
+           "triangles_per_batch": int(sum(assets[v.scene].triangles() for v in views)),
+           "batch_ms_p50": round(float(np.median(lat)) * 1e3, 3),
+           "batch_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "views_per_s": round(len(views) * args.batch_steps / tot, 1),
+           "fragments_shaded_per_s": round(st["frags_shaded"] / tot, 1),
+           "single_view_ms_p50": round(float(np.median(single)) * 1e3, 3),
+           "single_view_views_per_s": round(1.0 / float(np.mean(single)), 1)}
+    if tuned:
+        out["autotune"] = tuned
+    return out, assets, views
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +134,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced scene for quick checks")
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
+    ap.add_argument("--batch", action="store_true", help="also run the 256-view batch workload")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--tune", action="store_true", help="autotune the raster tiling on the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (views would be sharded "
@@ -165,6 +225,9 @@ def main():
         "roofline": roof,
     }
     ctx.close()
+    if args.batch:
+        log = lambda m: print(m, flush=True)  # noqa: E731
+        out["batched"], _, _ = batch_section(args, N, S, log)
     if not args.no_cpu:
         thr = args.cpu_threads or min(os.cpu_count() or 2, 16)
         print("[bench] cpu baseline: oracle with %d threads for <= %.0f s" % (thr, args.cpu_seconds), flush=True)

@@ -67,7 +67,7 @@ typedef struct {
 
 /* tile configuration of the raster kernel; searched by the autotuner */
 typedef struct {
-    int chunk_width;    /* pixels per lane along a row: 4, 8, 16, 32 or 64 */
+    int chunk_width;    /* pixels per lane along a row: 1, 2, 4, 8, 16, 32 or 64 */
     int tile_rows;      /* rows per wave tile (tile_rows * lanes-per-row = 64): 1..64, power of two */
     int waves_per_group;/* wavefronts per workgroup of the raster kernel: 1..8 */
 } psgpu_tiling;

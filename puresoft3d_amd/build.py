@@ -1,6 +1,6 @@
 """Build recipe of the native libraries (hipcc for gfx950, in-tree outputs).
 
-  python -m puresoft3d_amd.build        # builds libpsgpu.so
+  python -m puresoft3d_amd.build        # libpsgpu.so, libpuresoft3d.so, host/psrender
 """
 import os
 import subprocess
@@ -38,8 +38,27 @@ def build_psgpu():
     return out
 
 
+HOST = os.path.join(HERE, "host")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+
+
+def build_facade():
+    """C++ PuresoftPipeline facade (host code only) over libpsgpu, and the
+    psrender program that drives it."""
+    lib = os.path.join(HERE, "libpuresoft3d.so")
+    run(["g++"] + CXX_FLAGS + ["-shared", "-o", lib + ".tmp", os.path.join(HOST, "puresoft3d.cpp"),
+                               "-L" + HERE, "-lpsgpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(lib + ".tmp", lib)
+    exe = os.path.join(HOST, "psrender")
+    run(["g++"] + CXX_FLAGS + ["-o", exe + ".tmp", os.path.join(HOST, "psrender.cpp"),
+                               "-L" + HERE, "-lpuresoft3d", "-lpsgpu", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(exe + ".tmp", exe)
+    return lib, exe
+
+
 def main():
     build_psgpu()
+    build_facade()
 
 
 if __name__ == "__main__":

@@ -27,6 +27,7 @@ namespace {
 
 const int kMaxSlots = 16;      // config.h:4
 const int kMaxUniforms = 1024; // config.h:5
+const int64_t kMaxFlushTris = (int64_t)8 << 20;  // triangles queued before a flush is forced
 
 struct Tex {
     bool alive = false;
@@ -684,24 +685,47 @@ int graph_capture(psgpu_ctx* c, Graph& g)
     return PSGPU_OK;
 }
 
+// upload the device records of every command of a graph for the current
+// scratch buffers, then capture it
+int graph_rebuild(psgpu_ctx* c, Graph& g)
+{
+    for (size_t i = 0; i < g.cmds.size(); i++) {
+        if (g.cmds[i].type != CMD_FLUSH || !g.blobs[i]) continue;
+        Blob b;
+        PsFrame f;
+        int64_t T, npx;
+        int nvs;
+        bool p1[8] = {false}, p2[8] = {false};
+        int rc = build_blob(c, g.cmds[i], b, f, T, nvs, p1, p2, npx);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(g.blobs[i], b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return graph_capture(c, g);
+}
+
+// a graph whose capture is stale is first run once eagerly: that sizes the
+// scratch buffers for the current tiling (overflow replays included)
+int graph_prepare(psgpu_ctx* c, Graph& g)
+{
+    int rc = do_sync(c);
+    if (rc) return rc;
+    for (const Cmd& cmd : g.cmds) {
+        Cmd copy = cmd;
+        c->log.push_back(std::move(copy));
+        if ((rc = run_cmd(c, c->log.back()))) return rc;
+        if ((rc = mark_done(c, (int)c->log.size() - 1))) return rc;
+    }
+    if ((rc = do_sync(c))) return rc;
+    return graph_rebuild(c, g);
+}
+
 int graph_launch_n(psgpu_ctx* c, int id, int count)
 {
     Graph& g = c->graphs[id];
     if (!g.exec || g.gen != c->buf_gen) {
-        // the blobs hold winner-buffer addresses: rebuild them for the current buffers
-        for (size_t i = 0; i < g.cmds.size(); i++) {
-            if (g.cmds[i].type != CMD_FLUSH || !g.blobs[i]) continue;
-            Blob b;
-            PsFrame f;
-            int64_t T, npx;
-            int nvs;
-            bool p1[8] = {false}, p2[8] = {false};
-            int rc = build_blob(c, g.cmds[i], b, f, T, nvs, p1, p2, npx);
-            if (rc) return rc;
-            HIPCHK(c, hipMemcpyAsync(g.blobs[i], b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
-        int rc = graph_capture(c, g);
+        // buffers only grew since this launch was logged: records and capture suffice
+        int rc = graph_rebuild(c, g);
         if (rc) return rc;
     }
     for (int i = 0; i < count; i++) HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
@@ -1252,6 +1276,9 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     // draws with the targets and viewport of the last pending view join it;
     // otherwise they open a new view of the same flush, unless a pending view
     // already renders into one of these buffers (then that work goes first)
+    if (c->queue_tris + d.tri_count > kMaxFlushTris && !c->queue.empty()) {
+        if ((rc = queue_to_log(c))) return rc;     // bound the scratch of one flush
+    }
     if (c->queue.empty() || !c->queue.back().tg.same(tg)) {
         if (queue_uses(c, tg.colour) || queue_uses(c, tg.depth)) {
             if ((rc = queue_to_log(c))) return rc;
@@ -1392,13 +1419,14 @@ int psgpu_tiling_set(psgpu_ctx* c, const psgpu_tiling* t)
 {
     if (need_ctx(c) || !t) return PSGPU_ERR_INVALID;
     int cw = t->chunk_width, tr = t->tile_rows, w = t->waves_per_group;
-    bool okc = cw == 4 || cw == 8 || cw == 16 || cw == 32 || cw == 64;
+    bool okc = cw >= 1 && cw <= 64 && (cw & (cw - 1)) == 0;
     bool okr = tr >= 1 && tr <= 64 && (tr & (tr - 1)) == 0;
     if (!okc || !okr || w < 1 || w > 8) return fail(c, PSGPU_ERR_INVALID, "unsupported tiling");
     if (psk_raster_lds(cw, w) > 65536) return fail(c, PSGPU_ERR_INVALID, "tiling needs more than 64 KiB of LDS");
-    int rc = queue_to_log(c);
+    int rc = do_sync(c);    // logged commands are replayed with the tiling they ran with
     if (rc) return rc;
     c->tiling = *t;
+    c->buf_gen++;     // graphs bake the tiling in: capture them again
     return PSGPU_OK;
 }
 
@@ -1513,7 +1541,11 @@ int psgpu_graph_launch(psgpu_ctx* c, int graph, int count)
     if (count <= 0) return PSGPU_OK;
     int rc = queue_to_log(c);
     if (rc) return rc;
-    // replays were sized by capture_end and buffers only grow: a replay
+    Graph& g = c->graphs[graph];
+    if (!g.exec || g.gen != c->buf_gen) {
+        if ((rc = graph_prepare(c, g))) return rc;
+    }
+    // a prepared graph fits the scratch buffers, which only grow: a replay
     // cannot overflow, so a launch of `count` replays is one log entry
     Cmd cmd;
     cmd.type = CMD_GRAPH;

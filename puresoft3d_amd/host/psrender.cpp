@@ -1,0 +1,194 @@
+// psrender -- renders a scene file through the PuresoftPipeline facade and
+// writes the display (BGRA8, top row first) to a file.
+//
+//   psrender <scene.pssc> <out.bgra> [frames]
+//
+// The scene file layout is documented at puresoft3d_amd/scenes.py:save_scene.
+// Each frame is what the reference's demo loop does (test/main.cpp): clear
+// depth and colour, set uniforms, one drawVAO per draw, swapBuffers.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "puresoft3d.h"
+
+namespace {
+
+struct Reader {
+    std::vector<unsigned char> buf;
+    size_t at = 0;
+    template <class T> T get()
+    {
+        T v;
+        need(sizeof(T));
+        memcpy(&v, buf.data() + at, sizeof(T));
+        at += sizeof(T);
+        return v;
+    }
+    const unsigned char* take(size_t n)
+    {
+        need(n);
+        at += n;
+        return buf.data() + at - n;
+    }
+    void need(size_t n)
+    {
+        if (at + n > buf.size()) throw std::runtime_error("scene file truncated");
+    }
+};
+
+struct Uniform {
+    int slot;
+    std::vector<unsigned char> bytes;
+};
+
+std::vector<Uniform> read_uniforms(Reader& r)
+{
+    std::vector<Uniform> u(r.get<int>());
+    for (auto& x : u) {
+        x.slot = r.get<int>();
+        int n = r.get<int>();
+        const unsigned char* p = r.take(n);
+        x.bytes.assign(p, p + n);
+    }
+    return u;
+}
+
+PuresoftProcessor* make_proc(int kind, int role)
+{
+#define PS_CASE(K, NAME)                                         \
+    case K:                                                      \
+        if (role == 0) return new VertexProcesser##NAME();       \
+        if (role == 1) return new InterpolationProcessor##NAME(); \
+        return new FragmentProcessor##NAME();
+    switch (kind) {
+        PS_CASE(PSGPU_PROC_DEF01, DEF01)
+        PS_CASE(PSGPU_PROC_DEF02, DEF02)
+        PS_CASE(PSGPU_PROC_DEF03, DEF03)
+        PS_CASE(PSGPU_PROC_DEF04, DEF04)
+        PS_CASE(PSGPU_PROC_DEF05, DEF05)
+        PS_CASE(PSGPU_PROC_LAMBERT, LAMBERT)
+        PS_CASE(PSGPU_PROC_BLINN, BLINN)
+    }
+#undef PS_CASE
+    throw std::invalid_argument("unknown processor kind " + std::to_string(kind));
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s <scene.pssc> <out.bgra> [frames]\n", argv[0]);
+        return 2;
+    }
+    int frames = argc > 3 ? atoi(argv[3]) : 1;
+    try {
+        Reader r;
+        {
+            FILE* f = fopen(argv[1], "rb");
+            if (!f) throw std::runtime_error(std::string("cannot open ") + argv[1]);
+            fseek(f, 0, SEEK_END);
+            r.buf.resize(ftell(f));
+            fseek(f, 0, SEEK_SET);
+            if (fread(r.buf.data(), 1, r.buf.size(), f) != r.buf.size()) throw std::runtime_error("short read");
+            fclose(f);
+        }
+        if (r.get<unsigned>() != 0x43535350u || r.get<unsigned>() != 1u) throw std::runtime_error("not a scene file");
+        int W = r.get<int>(), H = r.get<int>();
+        float clear_depth = r.get<float>();
+        PURESOFTBGRA clear_colour;
+        clear_colour.i32 = r.get<unsigned>();
+
+        PuresoftPipeline pipe(0, W, H);
+        int n_tex = r.get<int>();
+        for (int t = 0; t < n_tex; t++) {
+            int w = r.get<int>(), h = r.get<int>(), layers = r.get<int>(), wrap = r.get<int>(), levels = r.get<int>();
+            size_t bytes = (size_t)w * h * 4;
+            PURESOFTIMGBUFF32 img = {(unsigned)w, (unsigned)w * 4, (unsigned)h, 4, (void*)r.take(bytes)};
+            int idx = pipe.createTexture(&img, layers - 1, (PuresoftFBO::WRAPMODE)wrap, levels);
+            if (idx != t) throw std::runtime_error("texture indices are expected in creation order");
+            for (int L = 1; L < layers; L++) pipe.setTexturePixels(idx, r.take(bytes), (PuresoftFBO::LAYER)L);
+            if (levels > 1) pipe.generateMipmaps(idx);
+        }
+        std::vector<int> vaos(r.get<int>());
+        for (auto& vao : vaos) {
+            vao = pipe.createVAO();
+            int n_slots = r.get<int>();
+            for (int s = 0; s < n_slots; s++) {
+                int slot = r.get<int>(), comps = r.get<int>(), count = r.get<int>();
+                PuresoftVBO* vbo = new PuresoftVBO(comps * sizeof(float), count);
+                vbo->updateContent(r.take((size_t)comps * count * sizeof(float)));
+                pipe.attachVBO(vao, slot, vbo);
+            }
+        }
+        std::vector<Uniform> globals = read_uniforms(r);
+        struct Draw {
+            int mesh, prog, behavior;
+            std::vector<Uniform> uniforms;
+        };
+        std::vector<Draw> draws(r.get<int>());
+        std::map<int, int> programmes;
+        for (auto& d : draws) {
+            d.mesh = r.get<int>();
+            d.prog = r.get<int>();
+            d.behavior = r.get<int>();
+            d.uniforms = read_uniforms(r);
+            if (d.mesh < 0 || d.mesh >= (int)vaos.size()) throw std::out_of_range("draw mesh index");
+            if (!programmes.count(d.prog)) {
+                int v = pipe.addProcessor(make_proc(d.prog, 0));
+                int i = pipe.addProcessor(make_proc(d.prog, 1));
+                int f = pipe.addProcessor(make_proc(d.prog, 2));
+                programmes[d.prog] = pipe.createProgramme(v, i, f);
+            }
+        }
+        const int all = BEHAVIOR_UPDATE_DEPTH | BEHAVIOR_TEST_DEPTH | BEHAVIOR_FACE_CULLING;
+        const int deflt = all;
+
+        auto frame = [&]() {
+            pipe.clearDepth(clear_depth);
+            pipe.clearColour(clear_colour);
+            for (auto& u : globals) pipe.setUniform(u.slot, u.bytes.data(), u.bytes.size());
+            int cur = -1, beh = deflt;
+            for (auto& d : draws) {
+                if (d.prog != cur) pipe.useProgramme(programmes[cur = d.prog]);
+                int want = d.behavior < 0 ? deflt : d.behavior;
+                if (want != beh) {
+                    pipe.enable(want);
+                    pipe.disable(~want & all);
+                    beh = want;
+                }
+                for (auto& u : d.uniforms) pipe.setUniform(u.slot, u.bytes.data(), u.bytes.size());
+                pipe.drawVAO(vaos[d.mesh]);
+                for (auto& u : d.uniforms)
+                    for (auto& g : globals)
+                        if (g.slot == u.slot) pipe.setUniform(g.slot, g.bytes.data(), g.bytes.size());
+            }
+            if (beh != deflt) {
+                pipe.enable(deflt);
+                pipe.disable(~deflt & all);
+            }
+            pipe.swapBuffers();
+        };
+        frame();   // first frame: uploads settle, scratch sized
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 1; i < frames; i++) frame();
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (frames > 1) printf("psrender: %d frames in %.3f s (%.1f frames/s incl. readback)\n", frames - 1, dt, (frames - 1) / dt);
+
+        std::vector<unsigned char> pixels((size_t)W * H * 4);
+        pipe.readDisplay(pixels.data());
+        FILE* f = fopen(argv[2], "wb");
+        if (!f || fwrite(pixels.data(), 1, pixels.size(), f) != pixels.size()) throw std::runtime_error("cannot write output");
+        fclose(f);
+    } catch (const std::exception& e) {
+        fprintf(stderr, "psrender: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,166 @@
+// puresoft3d.h -- C++ host facade with the reference's pipeline API
+// (src/puresoft3d/pipeline.h:24-156, vbo.h, proc.h, defs.h) over the C ABI of
+// include/psgpu.h.  Code written against PuresoftPipeline keeps its calls;
+// the processors it adds must be the built-in ones, which have GPU twins.
+// Errors are raised as the reference raises them: std::out_of_range,
+// std::invalid_argument, std::bad_alloc, and std::runtime_error for device
+// failures.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/psgpu.h"
+
+// ---- defs.h -----------------------------------------------------------------
+typedef struct {
+    size_t capacity;
+    void* data;
+} PURESOFTUNIFORM;
+
+typedef struct {
+    union {
+        struct { unsigned char bgra[4]; } ary;
+        struct { unsigned char b, g, r, a; } elems;
+        unsigned int i32;
+    };
+} PURESOFTBGRA;
+
+typedef struct {
+    unsigned int width;
+    unsigned int scanline;
+    unsigned int height;
+    unsigned int elemLen;
+    void* pixels;
+} PURESOFTIMGBUFF32;
+
+const PURESOFTBGRA PURESOFTBGRA_BLACK = {{{{0, 0, 0, 0}}}};
+
+// ---- pipeline.h:17-20 ---------------------------------------------------------
+const int BEHAVIOR_UPDATE_DEPTH = PSGPU_UPDATE_DEPTH;
+const int BEHAVIOR_TEST_DEPTH = PSGPU_TEST_DEPTH;
+const int BEHAVIOR_FACE_CULLING = PSGPU_FACE_CULLING;
+const int BEHAVIOR_ALPHABLEND = PSGPU_ALPHABLEND;
+
+// ---- fbo.h: the enums the texture API uses ---------------------------------------
+struct PuresoftFBO {
+    enum WRAPMODE { CLAMP = PSGPU_CLAMP, WRAP = PSGPU_WRAP };
+    enum LAYER { LAYER_DEFAULT, LAYER_XPOS = LAYER_DEFAULT, LAYER_XNEG, LAYER_YPOS, LAYER_YNEG, LAYER_ZPOS, LAYER_ZNEG,
+                 LAYER_MAX };
+};
+
+// ---- vbo.h: a vertex stream; contents live on the host until attached ----------
+class PuresoftVBO {
+public:
+    PuresoftVBO(size_t unitBytes, size_t unitCount);
+    void updateContent(const void* src);
+    size_t unitBytes() const { return m_unitBytes; }
+    size_t unitCount() const { return m_unitCount; }
+
+private:
+    friend class PuresoftPipeline;
+    size_t m_unitBytes, m_unitCount;
+    std::vector<unsigned char> m_host;
+    psgpu_ctx* m_ctx = nullptr;   // set once attached
+    int m_handle = -1;
+};
+
+// ---- proc.h: processors are the built-in types with a GPU twin ------------------
+class PuresoftProcessor {
+public:
+    explicit PuresoftProcessor(int kind) : m_kind(kind) {}
+    virtual ~PuresoftProcessor() {}
+    int gpuKind() const { return m_kind; }
+
+private:
+    int m_kind;
+};
+class PuresoftVertexProcessor : public PuresoftProcessor { public: using PuresoftProcessor::PuresoftProcessor; };
+class PuresoftInterpolationProcessor : public PuresoftProcessor { public: using PuresoftProcessor::PuresoftProcessor; };
+class PuresoftFragmentProcessor : public PuresoftProcessor { public: using PuresoftProcessor::PuresoftProcessor; };
+
+#define PS_DECLARE_PROCS(NAME, KIND)                                                                           \
+    class VertexProcesser##NAME : public PuresoftVertexProcessor { public: VertexProcesser##NAME() : PuresoftVertexProcessor(KIND) {} }; \
+    class InterpolationProcessor##NAME : public PuresoftInterpolationProcessor { public: InterpolationProcessor##NAME() : PuresoftInterpolationProcessor(KIND) {} }; \
+    class FragmentProcessor##NAME : public PuresoftFragmentProcessor { public: FragmentProcessor##NAME() : PuresoftFragmentProcessor(KIND) {} };
+
+PS_DECLARE_PROCS(DEF01, PSGPU_PROC_DEF01)      // tex1light1.h
+PS_DECLARE_PROCS(DEF02, PSGPU_PROC_DEF02)      // colr1light1.h
+PS_DECLARE_PROCS(DEF03, PSGPU_PROC_DEF03)      // tex1bump1light1.h
+PS_DECLARE_PROCS(DEF04, PSGPU_PROC_DEF04)      // skybox.h
+PS_DECLARE_PROCS(DEF05, PSGPU_PROC_DEF05)      // shadow.h
+PS_DECLARE_PROCS(LAMBERT, PSGPU_PROC_LAMBERT)  // Lambert x bilinear, directional light
+PS_DECLARE_PROCS(BLINN, PSGPU_PROC_BLINN)      // Blinn-Phong + normal map, trilinear
+#undef PS_DECLARE_PROCS
+
+// ---- rndr.h: a headless renderer owning the host copy of the display ------------
+class PuresoftHeadlessRenderer {
+public:
+    PuresoftHeadlessRenderer(int w, int h) : m_w(w), m_h(h), m_pixels((size_t)w * h * 4) {}
+    void getDesc(PURESOFTIMGBUFF32* d) { d->width = m_w; d->scanline = m_w * 4; d->height = m_h; d->elemLen = 4; d->pixels = m_pixels.data(); }
+    void* buffer() { return m_pixels.data(); }
+
+private:
+    int m_w, m_h;
+    std::vector<unsigned char> m_pixels;
+};
+
+// ---- pipeline.h ------------------------------------------------------------------
+class PuresoftPipeline {
+public:
+    PuresoftPipeline(uintptr_t canvasWindow, int deviceWidth, int deviceHeight, PuresoftHeadlessRenderer* rndr = nullptr,
+                     int device = 0);
+    ~PuresoftPipeline();
+
+    // texture api
+    int createTexture(const PURESOFTIMGBUFF32* image, int extraLayers = 0,
+                      PuresoftFBO::WRAPMODE mode = PuresoftFBO::CLAMP, int mipLevels = 1);
+    void destroyTexture(int idx);
+    // copies the layer into image->pixels when it is non-null, else reports the layout
+    void getTexture(int idx, PURESOFTIMGBUFF32* image, PuresoftFBO::LAYER layer = PuresoftFBO::LAYER_DEFAULT);
+    void setTexturePixels(int idx, const void* pixels, PuresoftFBO::LAYER layer = PuresoftFBO::LAYER_DEFAULT);
+    void generateMipmaps(int idx);
+
+    // processor api
+    int addProcessor(PuresoftProcessor* proc);
+    void destroyProcessor(int idx);
+    int createProgramme(int vid, int iid, int fid);
+    void destroyProgramme(int idx);
+    void useProgramme(int idx);
+
+    // vao api
+    int createVAO(void);
+    PuresoftVBO* attachVBO(int vao, int idx, PuresoftVBO* vbo);
+    PuresoftVBO* detachVBO(int vao, int idx);
+    PuresoftVBO* getVBO(int vao, int idx);
+    void destroyVAO(int vao);
+
+    // rendering api
+    void setViewport(int width, int height, uintptr_t canvasWindow = 0);
+    void setDepth(int idx = -1);
+    void setFBO(int idx, int textureOrDisplay);      // texture index, or -1 for the display
+    void setUniform(int idx, const void* data, size_t len);
+    void drawVAO(int vao, bool callerThrdForFragProc = false);
+    void swapBuffers(void);                            // display -> renderer buffer
+    void enable(int behavior);
+    void disable(int behavior);
+    void clearDepth(float furthest = 1.0f);
+    void clearColour(PURESOFTBGRA bkgnd = PURESOFTBGRA_BLACK);
+
+    // access for callers that read results
+    void finish(void);
+    void readDisplay(void* bgraTopDown);
+    psgpu_ctx* context() { return m_ctx; }
+
+private:
+    void check(int rc) const;
+    psgpu_ctx* m_ctx = nullptr;
+    int m_width, m_height;
+    std::unique_ptr<PuresoftHeadlessRenderer> m_ownRenderer;
+    PuresoftHeadlessRenderer* m_renderer;
+    std::vector<PuresoftProcessor*> m_processors;
+    std::vector<std::vector<PuresoftVBO*>> m_vaoSlots;
+};

@@ -334,17 +334,25 @@ class GpuScene:
             ctx.clear_colour(sc.clear_colour)
         for k, v in sc.uniforms.items():
             ctx.uniform(k, v)
-        cur = None
+        cur, beh = None, N.DEFAULT_BEHAVIOR
         for d in sc.draws:
             if d.prog != cur:
                 ctx.program_use(self.progs[d.prog])
                 cur = d.prog
+            want = N.DEFAULT_BEHAVIOR if d.behavior is None else d.behavior
+            if want != beh:
+                ctx.enable(want)
+                ctx.disable(~want & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING))
+                beh = want
             for k, v in d.uniforms.items():
                 ctx.uniform(k, v)
             ctx.draw(self.vao[d.mesh])
             for k in d.uniforms:
                 if k in sc.uniforms:
                     ctx.uniform(k, sc.uniforms[k])
+        if beh != N.DEFAULT_BEHAVIOR:
+            ctx.enable(N.DEFAULT_BEHAVIOR)
+            ctx.disable(~N.DEFAULT_BEHAVIOR & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING))
         ctx.flush()
 
 
@@ -475,3 +483,53 @@ class GpuBatch:
         sc.uniforms = vw.uniforms()
         sc.draws.append(DrawSpec("m", N.PROC_BLINN))
         return sc
+
+
+# ---------------------------------------------------------------- scene files
+SCENE_MAGIC = 0x43535350   # "PSSC"
+
+
+def save_scene(scene, path):
+    """Write a Scene as a flat little-endian file that the C++ facade's
+    psrender program reads (puresoft3d_amd/host/psrender.cpp):
+
+      u32 magic, u32 version=1, i32 width, height, f32 clear_depth, u32 clear_colour
+      i32 n_textures; per texture: i32 width, height, layers, wrap, levels, then
+          layers * height * width * 4 bytes (BGRA8, layer-major)
+      i32 n_meshes; per mesh: i32 n_slots; per slot: i32 slot, comps, count,
+          then count * comps float32
+      i32 n_uniforms; per uniform: i32 slot, i32 bytes, then the bytes
+      i32 n_draws; per draw: i32 mesh, prog, behavior (-1 = default),
+          i32 n_uniforms; per uniform as above
+    """
+    import struct
+    names = list(scene.meshes)
+    out = [struct.pack("<IIiifI", SCENE_MAGIC, 1, scene.width, scene.height, scene.clear_depth,
+                       scene.clear_colour & 0xFFFFFFFF)]
+
+    def uniforms(u):
+        parts = [struct.pack("<i", len(u))]
+        for k, v in sorted(u.items()):
+            b = np.ascontiguousarray(v).tobytes()
+            parts += [struct.pack("<ii", k, len(b)), b]
+        return parts
+
+    out.append(struct.pack("<i", len(scene.textures)))
+    for t in scene.textures:
+        out.append(struct.pack("<iiiii", t.width, t.height, t.layers, t.wrap, t.levels))
+        out += [im.tobytes() for im in t.images]
+    out.append(struct.pack("<i", len(names)))
+    for nm in names:
+        m = scene.meshes[nm]
+        out.append(struct.pack("<i", len(m)))
+        for s, a in sorted(m.items()):
+            a = np.ascontiguousarray(a, np.float32)
+            out += [struct.pack("<iii", s, a.shape[1], a.shape[0]), a.tobytes()]
+    out += uniforms(scene.uniforms)
+    out.append(struct.pack("<i", len(scene.draws)))
+    for d in scene.draws:
+        beh = -1 if d.behavior is None else d.behavior
+        out.append(struct.pack("<iii", names.index(d.mesh), d.prog, beh))
+        out += uniforms(d.uniforms)
+    with open(path, "wb") as f:
+        f.write(b"".join(out))

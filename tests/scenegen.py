@@ -113,3 +113,21 @@ def micro_scene(prog, seed, width=96, height=64, n=120, tex=32):
     sc.draws.append(S.DrawSpec("m", prog))
     sc.clear_colour = 0x00203040
     return sc
+
+
+def behavior_scene(seed, width=96, height=64, n=60):
+    """draws of random triangles under every combination of the depth-update,
+    depth-test and face-culling behaviours (pipeline.cpp:324-332), DEF02 and
+    Blinn-Phong interleaved, so later draws land on the earlier ones' depth"""
+    rng = np.random.default_rng(seed)
+    sc = micro_scene(N.PROC_BLINN, seed, width, height, n)
+    sc.name = "behaviors_s%d_%dx%d" % (seed, width, height)
+    sc.draws = []
+    for beh in range(8):
+        for prog in (N.PROC_DEF02, N.PROC_BLINN):
+            name = "m%d_%d" % (beh, prog)
+            pos = random_clip_triangles(rng, n // 4)
+            sc.meshes[name] = _slots_for(prog, pos, rng)
+            sc.draws.append(S.DrawSpec(name, prog, behavior=beh))
+    sc.draws.insert(0, S.DrawSpec("m", N.PROC_BLINN))
+    return sc

@@ -1,0 +1,85 @@
+"""The C++ PuresoftPipeline facade (puresoft3d_amd/host) and its psrender
+program: scenes rendered through the reference-shaped API must equal the
+oracle's frames bit for bit, as the C-ABI path does."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import announce
+import scenegen
+
+from puresoft3d_amd import native as N
+from puresoft3d_amd import scenes as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(os.path.dirname(HERE), "puresoft3d_amd")
+PSRENDER = os.path.join(PKG, "host", "psrender")
+FACADE = os.path.join(PKG, "libpuresoft3d.so")
+
+# PuresoftPipeline members the facade keeps from pipeline.h:24-156
+FACADE_METHODS = ["createTexture", "destroyTexture", "getTexture", "addProcessor", "destroyProcessor",
+                  "createProgramme", "destroyProgramme", "useProgramme", "createVAO", "attachVBO", "detachVBO",
+                  "getVBO", "destroyVAO", "setViewport", "setDepth", "setFBO", "setUniform", "drawVAO",
+                  "swapBuffers", "enable", "disable", "clearDepth", "clearColour"]
+
+
+def _built():
+    if not (os.path.exists(PSRENDER) and os.path.exists(FACADE)):
+        pytest.skip("facade not built (python -m puresoft3d_amd.build)")
+
+
+def test_facade_exports_pipeline_api():
+    _built()
+    out = subprocess.run(["nm", "-DC", "--defined-only", FACADE], capture_output=True, text=True, check=True).stdout
+    for m in FACADE_METHODS:
+        assert "PuresoftPipeline::%s(" % m in out, m
+
+
+def test_psrender_usage():
+    _built()
+    r = subprocess.run([PSRENDER], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def _read_scene_header(path):
+    raw = open(path, "rb").read()
+    magic, ver, w, h, cd, cc = np.frombuffer(raw[:24], "<u4").tolist()
+    return magic, ver, w, h, raw
+
+
+def test_scene_file_layout(tmp_path):
+    sc = scenegen.micro_scene(N.PROC_BLINN, 1)
+    p = str(tmp_path / "s.pssc")
+    S.save_scene(sc, p)
+    magic, ver, w, h, raw = _read_scene_header(p)
+    assert (magic, ver, w, h) == (S.SCENE_MAGIC, 1, sc.width, sc.height)
+    tex = sum(20 + t.layers * t.width * t.height * 4 for t in sc.textures)
+    mesh = sum(4 + sum(12 + a.size * 4 for a in m.values()) for m in sc.meshes.values())
+    uni = 4 + sum(8 + np.asarray(v).nbytes for v in sc.uniforms.values())
+    draws = 4 + sum(16 + sum(8 + np.asarray(v).nbytes for v in d.uniforms.values()) for d in sc.draws)
+    assert len(raw) == 24 + 4 + tex + 4 + mesh + uni + draws
+
+
+def _psrender(scene, tmp_path, frames=1):
+    p = str(tmp_path / (scene.name + ".pssc"))
+    o = str(tmp_path / (scene.name + ".bgra"))
+    S.save_scene(scene, p)
+    announce("psrender %s frames=%d" % (scene.name, frames))
+    r = subprocess.run([PSRENDER, p, o, str(frames)], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    return np.fromfile(o, np.uint8).reshape(scene.height, scene.width, 4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["sphere", "blinn", "cube", "behaviors"])
+def test_psrender_matches_oracle(oracle, tmp_path, which):
+    _built()
+    sc = {"sphere": lambda: S.sphere_scene(width=320, height=240),
+          "blinn": lambda: scenegen.micro_scene(N.PROC_BLINN, 3),
+          "cube": lambda: scenegen.micro_scene(N.PROC_DEF04, 3),
+          "behaviors": lambda: scenegen.behavior_scene(4)}[which]()
+    got = _psrender(sc, tmp_path, frames=2)
+    f, _ = oracle.render_scene(sc)
+    np.testing.assert_array_equal(got, f.colour)

@@ -250,8 +250,26 @@ def test_batched_views_graph_replay(oracle):
             col, dep = b.read(i)
             np.testing.assert_array_equal(col, ref[i][0])
             np.testing.assert_array_equal(dep.view(np.uint32), ref[i][1].view(np.uint32))
+        # another tiling (8x8 tiles: many more bin segments) recaptures the graph
+        for tiling in [(1, 8, 2), (64, 64, 1), (4, 16, 8)]:
+            ctx.tiling(*tiling)
+            for k in range(len(views)):
+                ctx.target(b.targets[k][0])
+                ctx.clear_target(0xFFFFFFFF)
+            ctx.target(N.DISPLAY)
+            ctx.graph_launch(g, 1)
+            ctx.sync()
+            for i in range(len(views)):
+                col, dep = b.read(i)
+                np.testing.assert_array_equal(col, ref[i][0])
+                np.testing.assert_array_equal(dep.view(np.uint32), ref[i][1].view(np.uint32))
     finally:
         ctx.close()
     for i in (0, len(views) - 1):
         f, _ = _oracle_view(oracle, b.view_scene(i), 0x00302010)
         np.testing.assert_array_equal(ref[i][0], f.colour)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_behaviors_bitexact(oracle, seed):
+    assert_same(scenegen.behavior_scene(seed), oracle)

@@ -293,3 +293,39 @@ def test_fp32_shading_within_one_lsb_of_fp64(oracle, which):
     d = np.abs(a.colour.astype(int) - b.colour.astype(int))
     assert d.max() <= 1, d.max()
     assert (d > 0).mean() < 0.05
+
+
+@pytest.mark.parametrize("beh", range(8))
+def test_behaviors(oracle, beh):
+    """fragthrd.cpp:184-236 and drawvao.cpp:46: the depth-test, depth-update
+    and face-culling behaviours against a per-pixel replay of the three
+    constant-depth triangles, each triangle's coverage taken from a render of
+    it alone.  Without TEST_DEPTH the fragment is compared with 1.0."""
+    tris = [np.array([[-0.8, -0.8, 0.2, 1], [0.8, -0.8, 0.2, 1], [0.0, 0.8, 0.2, 1]], np.float32),   # front
+            np.array([[-0.6, -0.6, -0.3, 1], [0.0, 0.7, -0.3, 1], [0.6, -0.6, -0.3, 1]], np.float32),  # back face
+            np.array([[-0.7, -0.7, 0.5, 1], [0.7, -0.7, 0.5, 1], [0.0, 0.7, 0.5, 1]], np.float32)]    # farther
+    zs = [np.float32(0.9) * np.float32(z) for z in (0.2, -0.3, 0.5)]   # shadow.cpp:26 scales by 0.9
+
+    def render(pos):
+        sc = _single_tri_scene(pos)
+        sc.draws[0].behavior = beh
+        sc.clear_depth = 0.75
+        return oracle.render_scene(sc, debug=True)
+
+    cover = [render(t)[0].raster_count.reshape(48, 64) > 0 for t in tris]
+    f, st = render(np.concatenate(tris))
+    assert st.tris_culled == (1 if beh & N.FACE_CULLING else 0)
+    if beh & N.FACE_CULLING:
+        assert not cover[1].any()
+    d = np.full((48, 64), np.float32(0.75))
+    passes = np.zeros((48, 64), int)
+    for c, z in zip(cover, zs):
+        cur = d if beh & N.TEST_DEPTH else np.float32(1.0)
+        ok = c & (z - cur < np.float32(-0.0001))
+        passes += ok
+        if beh & N.UPDATE_DEPTH:
+            d = np.where(ok, z, d)
+    np.testing.assert_array_equal(f.raster_count.reshape(48, 64), sum(c.astype(int) for c in cover))
+    np.testing.assert_array_equal(f.pass_count.reshape(48, 64), passes)
+    np.testing.assert_allclose(f.depth[:48, :64], d, rtol=0, atol=1e-6)
+    assert st.frags_shaded == passes.sum()
