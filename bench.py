@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--batch", action="store_true", help="also run the 256-view batch workload")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--tune", action="store_true", help="autotune the raster tiling on the batch workload")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="also run one plain copy kernel of MIB MiB (k_selftest op 7) to calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (views would be sharded "
@@ -224,6 +226,10 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in sorted(per_kernel.items())},
         "roofline": roof,
     }
+    if args.calib_copy:
+        n = args.calib_copy * (1 << 20) // 4
+        ctx.selftest_math(7, np.ones(n, np.float32))
+        out["calib_copy_bytes"] = {"read": n * 4, "written": n * 4}
     ctx.close()
     if args.batch:
         log = lambda m: print(m, flush=True)  # noqa: E731

@@ -140,6 +140,27 @@ int psgpu_get_stats(psgpu_ctx* ctx, psgpu_stats* out);
 int psgpu_reset_stats(psgpu_ctx* ctx);
 int psgpu_swap_buffers(psgpu_ctx* ctx);           /* swapBuffers: flush; display keeps its buffer */
 
+/* ---- asynchronous readback and fences (serving) ----
+ * No counterpart in the reference, whose drawVAO returns with the frame done
+ * (drawvao.cpp:122-132).  A server queues many views, then reads each one
+ * back without waiting: psgpu_texture_read_async queues a device-to-host
+ * copy of a region (memory rows y..y+h-1, columns x..x+w-1 of one layer;
+ * PSGPU_DISPLAY for the display) into host memory, after everything queued
+ * before it.  Copies run on a second stream, so they overlap later flushes.
+ * Host memory should come from psgpu_host_alloc (pinned).  psgpu_fence marks
+ * the current point; psgpu_fence_query reports done = 1 once every command
+ * before it (copies included) has finished with correct results.  A command
+ * that overflowed its scratch buffers is re-run by the query itself (a full
+ * sync), so results are never returned from an overflowed run.  Host buffers
+ * must stay valid until the fence after their copy is done. */
+int psgpu_host_alloc(psgpu_ctx* ctx, size_t bytes, void** ptr);
+int psgpu_host_free(psgpu_ctx* ctx, void* ptr);   /* syncs first */
+int psgpu_texture_read_async(psgpu_ctx* ctx, int tex, int layer, int x, int y, int w, int h, void* host,
+                             size_t host_pitch);
+int psgpu_fence(psgpu_ctx* ctx, uint64_t* out_fence);
+int psgpu_fence_query(psgpu_ctx* ctx, uint64_t fence, int* done);
+int psgpu_fence_wait(psgpu_ctx* ctx, uint64_t fence);
+
 /* ---- debug / parity planes (per pixel, display rows bottom-up: index y*W+x) ---- */
 /* enable=1 makes every flush record: last depth-test-passing triangle id,
  * fragments rasterised, xor-hash of rasterised triangle ids, depth-test passes,
@@ -173,7 +194,9 @@ int psgpu_capture_end(psgpu_ctx* ctx, int* out_graph);
 int psgpu_graph_launch(psgpu_ctx* ctx, int graph, int count);
 int psgpu_graph_destroy(psgpu_ctx* ctx, int graph);
 
-/* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a)} ---- */
+/* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a),
+ *      5:(int)(unsigned)x86(a), 6:log2 approximation of the trilinear LOD, 7:a (a plain copy kernel,
+ *      the calibration point for HBM traffic counters)} ---- */
 int psgpu_selftest_math(psgpu_ctx* ctx, int op, const float* a, const float* b, float* out, int n);
 
 #ifdef __cplusplus

@@ -1,0 +1,90 @@
+/*
+ * psserve.h -- render-serving loop over libpsgpu (continuous batching of view
+ * requests against resident scenes, with HBM-budget admission).
+ *
+ * No counterpart in the reference, which renders one view per drawVAO call
+ * on the caller's thread (drawvao.cpp:3-133).  This is the loop that keeps
+ * many such views in flight: scenes and textures stay resident on one
+ * psgpu context; each request names a scene, a resolution and a camera.
+ *
+ * Policy (one GPU, one context, one host thread):
+ *  - arrivals join a FIFO; the head is admitted when its render targets
+ *    (colour + depth, rounded up to a size class) plus a per-pixel scratch
+ *    estimate fit in the HBM budget;
+ *  - whenever fewer than max_inflight batches are on the GPU, every admitted
+ *    request not yet launched (up to max_batch_views / max_batch_pixels) is
+ *    queued as one multi-view flush, followed by asynchronous copies of each
+ *    view into pinned host memory and a fence;
+ *  - finished batches release their targets, which admits more requests.
+ *    Requests arriving while batches run are admitted and launched as soon as
+ *    targets and a batch slot free up, so the GPU always has the next batch
+ *    queued behind the current one and readback copies overlap it.
+ * max_batch_views = 1 with max_inflight = 1 is the plain one-view-per-launch
+ * path.  A view renders to the same pixels alone or in any batch.
+ */
+#ifndef PSSERVE_H
+#define PSSERVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "psgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a resident scene: one VAO drawn with the Blinn-Phong programme */
+typedef struct {
+    int vao;
+    int albedo_tex;     /* uniform slot 9  */
+    int normal_tex;     /* uniform slot 10 */
+} psserve_scene;
+
+typedef struct {
+    int scene;
+    int width, height;
+    float pv[16];       /* slot 3: projection x view, column-major */
+    float eye[4];       /* slot 8 */
+    float light[4];     /* slot 7 */
+    int spec_exp;       /* slot 11 */
+    double arrival_s;   /* seconds after the start of the run */
+} psserve_request;
+
+typedef struct {
+    int program;                /* psgpu programme used for every view */
+    size_t hbm_budget;          /* bytes for render targets + scratch estimate */
+    int scratch_bytes_per_px;   /* scratch estimate per pixel in flight */
+    int max_batch_views;
+    int64_t max_batch_pixels;
+    int max_inflight;           /* batches queued on the GPU at once */
+    uint32_t clear_bgra;
+} psserve_config;
+
+typedef struct {
+    double arrival_s, admitted_s, launched_s, done_s;
+    int batch;
+} psserve_result;
+
+typedef struct {
+    int64_t batches, views, pixels;
+    int64_t target_allocs, target_evictions;
+    int64_t peak_bytes;          /* render targets held at once */
+    int max_batch_views_seen;
+    double wall_s;
+} psserve_stats;
+
+/* Runs the request list in real time (arrivals are waited for) and returns
+ * per-request timestamps.  images: NULL, or one host pointer per request
+ * (NULL entries allowed) that receives height rows of width*4 bytes (the
+ * colour target's memory rows). */
+int psserve_run(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_request* reqs, int n,
+                const psserve_config* cfg, psserve_result* results, uint8_t* const* images, psserve_stats* stats);
+
+/* message of the last failed psserve_run on this thread */
+const char* psserve_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

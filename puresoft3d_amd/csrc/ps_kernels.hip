@@ -781,6 +781,7 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 4: r = floorf(a[i]); break;
     case 5: r = (float)(int)ps_cvt_u32(a[i]); break;
     case 6: r = ps_log2(a[i]); break;
+    case 7: r = a[i]; break;   // plain copy: 4 bytes read + 4 written per element
     }
     out[i] = r;
 }

@@ -70,13 +70,31 @@ struct Segment {
     std::vector<PsDraw> draws;
 };
 
-enum CmdType { CMD_FLUSH = 0, CMD_CLEARS = 1, CMD_GRAPH = 3 };
+enum CmdType { CMD_FLUSH = 0, CMD_CLEARS = 1, CMD_GRAPH = 3, CMD_COPY = 4 };
+
+// a device-to-host copy of a 2D region, run on the copy stream after the
+// commands before it
+struct CopyRegion {
+    const uint8_t* src = nullptr;
+    size_t src_pitch = 0;
+    void* dst = nullptr;
+    size_t dst_pitch = 0, width_bytes = 0, rows = 0;
+};
 
 struct Cmd {
     CmdType type;
     std::vector<Segment> views;   // CMD_FLUSH
     std::vector<PsFill> fills;    // CMD_CLEARS
     int graph = -1, count = 0;    // CMD_GRAPH
+    CopyRegion copy;              // CMD_COPY
+};
+
+// a point in the command stream: done when both streams passed it and no
+// command before it overflowed its scratch buffers
+struct Fence {
+    uint64_t id;
+    hipEvent_t main_ev, copy_ev;
+    size_t log_end;   // commands [0, log_end) of the log come before it
 };
 
 // A recorded command list replayed as one hipGraph (stream capture of the
@@ -110,6 +128,11 @@ struct Blob {
 struct psgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;   // device-to-host copies overlap the next flushes
+    hipEvent_t copy_ev = nullptr;
+    std::vector<Fence> fences;
+    std::vector<hipEvent_t> fence_ev_pool;
+    uint64_t next_fence = 1;             // a completed sync clears the log and retires every fence
     std::string err;
     int dev_w = 0, dev_h = 0;
     int vp_w = 0, vp_h = 0;
@@ -532,12 +555,22 @@ int run_clears(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
 
 int graph_launch_n(psgpu_ctx* c, int id, int count);
 
+int run_copy(psgpu_ctx* c, const CopyRegion& r)
+{
+    HIPCHK(c, hipEventRecord(c->copy_ev, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->copy_stream, c->copy_ev, 0));
+    HIPCHK(c, hipMemcpy2DAsync(r.dst, r.dst_pitch, r.src, r.src_pitch, r.width_bytes, r.rows, hipMemcpyDeviceToHost,
+                               c->copy_stream));
+    return PSGPU_OK;
+}
+
 int run_cmd(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged = nullptr)
 {
     switch (cmd.type) {
     case CMD_FLUSH: return run_flush(c, cmd, staged);
     case CMD_CLEARS: return run_clears(c, cmd, staged);
     case CMD_GRAPH: return graph_launch_n(c, cmd.graph, cmd.count);
+    case CMD_COPY: return run_copy(c, cmd.copy);
     }
     return PSGPU_OK;
 }
@@ -623,6 +656,7 @@ int do_sync(psgpu_ctx* c)
         HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(PsCounters), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->h_sizes, c->d_sizes, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->copy_stream));
         if (c->profiling) prof_collect(c);
         if (!c->h_cnt->overflow) {
             int64_t sum[PS_ST_N] = {0};
@@ -636,6 +670,11 @@ int do_sync(psgpu_ctx* c)
             c->stats.frags_shaded += sum[PS_ST_SHADED];
             c->stats.segments += sum[PS_ST_SEGS];
             c->log.clear();
+            for (const Fence& fe : c->fences) {
+                c->fence_ev_pool.push_back(fe.main_ev);
+                c->fence_ev_pool.push_back(fe.copy_ev);
+            }
+            c->fences.clear();
             HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
             HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
             return PSGPU_OK;
@@ -766,6 +805,8 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
     c->uniforms.resize(kMaxUniforms);
     memset(&c->stats, 0, sizeof(c->stats));
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->copy_ev, hipEventDisableTiming) == hipSuccess &&
               hipMalloc((void**)&c->display, (size_t)width * height * 4) == hipSuccess &&
               hipMalloc((void**)&c->depth, (size_t)dscan * height * 4) == hipSuccess &&
               hipMalloc((void**)&c->d_cnt, sizeof(PsCounters)) == hipSuccess &&
@@ -791,6 +832,7 @@ int psgpu_destroy(psgpu_ctx* c)
 {
     if (!c) return PSGPU_ERR_INVALID;
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
     auto fr = [](void* p) { if (p) hipFree(p); };
     fr(c->display); fr(c->depth); fr(c->d_cnt); fr(c->d_sizes);
     if (c->h_cnt) hipHostFree(c->h_cnt);
@@ -812,6 +854,10 @@ int psgpu_destroy(psgpu_ctx* c)
     }
     for (auto& r : c->prof_pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : c->ev_pool) hipEventDestroy(e);
+    for (auto& fe : c->fences) { hipEventDestroy(fe.main_ev); hipEventDestroy(fe.copy_ev); }
+    for (auto e : c->fence_ev_pool) hipEventDestroy(e);
+    if (c->copy_ev) hipEventDestroy(c->copy_ev);
+    if (c->copy_stream) hipStreamDestroy(c->copy_stream);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return PSGPU_OK;
@@ -1328,6 +1374,135 @@ int psgpu_read_depth(psgpu_ctx* c, void* host, size_t bytes)
     if (rc) return rc;
     HIPCHK(c, hipMemcpy(host, c->depth, bytes, hipMemcpyDeviceToHost));
     return PSGPU_OK;
+}
+
+// ---- asynchronous readback and fences --------------------------------------
+int psgpu_host_alloc(psgpu_ctx* c, size_t bytes, void** ptr)
+{
+    if (need_ctx(c) || !ptr || !bytes) return PSGPU_ERR_INVALID;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, bytes, 0) != hipSuccess) return fail(c, PSGPU_ERR_NOMEM, "hipHostMalloc of %zu bytes failed", bytes);
+    return PSGPU_OK;
+}
+
+int psgpu_host_free(psgpu_ctx* c, void* ptr)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (!ptr) return PSGPU_OK;
+    int rc = do_sync(c);   // no queued copy may still target it
+    if (rc) return rc;
+    HIPCHK(c, hipHostFree(ptr));
+    return PSGPU_OK;
+}
+
+int psgpu_texture_read_async(psgpu_ctx* c, int tex, int layer, int x, int y, int w, int h, void* host, size_t host_pitch)
+{
+    if (need_ctx(c) || !host) return PSGPU_ERR_INVALID;
+    if (c->capturing) return fail(c, PSGPU_ERR_STATE, "asynchronous reads are not recorded into command lists");
+    const uint8_t* base;
+    int tw, th, elem;
+    size_t pitch;
+    if (tex == PSGPU_DISPLAY) {
+        if (layer != 0) return fail(c, PSGPU_ERR_RANGE, "the display has one layer");
+        base = (const uint8_t*)c->display;
+        tw = c->dev_w; th = c->dev_h; elem = 4; pitch = (size_t)c->dev_w * 4;
+    } else {
+        if (!valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
+        const Tex& t = c->tex[tex];
+        if (layer < 0 || layer >= t.layers) return fail(c, PSGPU_ERR_RANGE, "layer out of range");
+        base = t.d + t.layer_bytes * layer;
+        tw = t.w; th = t.h; elem = t.elem; pitch = (size_t)t.scan;
+    }
+    if (x < 0 || y < 0 || w < 1 || h < 1 || x + w > tw || y + h > th) return fail(c, PSGPU_ERR_RANGE, "region outside the buffer");
+    if (host_pitch < (size_t)w * elem) return fail(c, PSGPU_ERR_INVALID, "host pitch shorter than a row");
+    int rc = queue_to_log(c);   // the draws and clears queued so far come first
+    if (rc) return rc;
+    Cmd cmd;
+    cmd.type = CMD_COPY;
+    cmd.copy.src = base + (size_t)y * pitch + (size_t)x * elem;
+    cmd.copy.src_pitch = pitch;
+    cmd.copy.dst = host;
+    cmd.copy.dst_pitch = host_pitch;
+    cmd.copy.width_bytes = (size_t)w * elem;
+    cmd.copy.rows = (size_t)h;
+    return submit(c, std::move(cmd));
+}
+
+int psgpu_fence(psgpu_ctx* c, uint64_t* out)
+{
+    if (need_ctx(c) || !out) return PSGPU_ERR_INVALID;
+    if (c->capturing) return fail(c, PSGPU_ERR_STATE, "not allowed while recording a command list");
+    int rc = queue_to_log(c);
+    if (rc) return rc;
+    Fence fe;
+    for (hipEvent_t* e : {&fe.main_ev, &fe.copy_ev}) {
+        if (!c->fence_ev_pool.empty()) {
+            *e = c->fence_ev_pool.back();
+            c->fence_ev_pool.pop_back();
+        } else {
+            HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
+    }
+    HIPCHK(c, hipEventRecord(fe.main_ev, c->stream));
+    HIPCHK(c, hipEventRecord(fe.copy_ev, c->copy_stream));
+    fe.log_end = c->log.size();
+    fe.id = c->next_fence++;
+    c->fences.push_back(fe);
+    *out = fe.id;
+    return PSGPU_OK;
+}
+
+static int fence_finish(psgpu_ctx* c, size_t k, int* done)
+{
+    Fence fe = c->fences[k];
+    // a command before the fence that overflowed is replayed (with the
+    // commands after it) by a full sync, which retires every fence
+    for (size_t i = 0; i < fe.log_end && i < c->log.size(); i++)
+        if (c->h_flags[i]) {
+            int rc = do_sync(c);
+            if (rc) return rc;
+            *done = 1;
+            return PSGPU_OK;
+        }
+    c->fence_ev_pool.push_back(fe.main_ev);
+    c->fence_ev_pool.push_back(fe.copy_ev);
+    c->fences.erase(c->fences.begin() + k);
+    *done = 1;
+    return PSGPU_OK;
+}
+
+static int fence_find(psgpu_ctx* c, uint64_t id)
+{
+    for (size_t k = 0; k < c->fences.size(); k++)
+        if (c->fences[k].id == id) return (int)k;
+    return -1;
+}
+
+int psgpu_fence_query(psgpu_ctx* c, uint64_t id, int* done)
+{
+    if (need_ctx(c) || !done) return PSGPU_ERR_INVALID;
+    if (id == 0 || id >= c->next_fence) return fail(c, PSGPU_ERR_RANGE, "unknown fence");
+    *done = 0;
+    int k = fence_find(c, id);
+    if (k < 0) { *done = 1; return PSGPU_OK; }   // retired by a sync or an earlier query
+    for (hipEvent_t e : {c->fences[k].main_ev, c->fences[k].copy_ev}) {
+        hipError_t q = hipEventQuery(e);
+        if (q == hipErrorNotReady) return PSGPU_OK;
+        if (q != hipSuccess) return fail(c, PSGPU_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
+    }
+    return fence_finish(c, (size_t)k, done);
+}
+
+int psgpu_fence_wait(psgpu_ctx* c, uint64_t id)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (id == 0 || id >= c->next_fence) return fail(c, PSGPU_ERR_RANGE, "unknown fence");
+    int k = fence_find(c, id);
+    if (k < 0) return PSGPU_OK;
+    HIPCHK(c, hipEventSynchronize(c->fences[k].main_ev));
+    HIPCHK(c, hipEventSynchronize(c->fences[k].copy_ev));
+    int done = 0;
+    return fence_finish(c, (size_t)k, &done);
 }
 
 int psgpu_display_device_ptr(psgpu_ctx* c, void** ptr)

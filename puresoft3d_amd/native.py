@@ -111,6 +111,13 @@ _SIGS = {
     "psgpu_graph_launch": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "psgpu_graph_destroy": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_selftest_math": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "psgpu_host_alloc": ([C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "psgpu_host_free": ([C.c_void_p, C.c_void_p], C.c_int),
+    "psgpu_texture_read_async": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                  C.c_size_t], C.c_int),
+    "psgpu_fence": ([C.c_void_p, C.POINTER(C.c_uint64)], C.c_int),
+    "psgpu_fence_query": ([C.c_void_p, C.c_uint64, C.POINTER(C.c_int)], C.c_int),
+    "psgpu_fence_wait": ([C.c_void_p, C.c_uint64], C.c_int),
 }
 
 
@@ -377,6 +384,39 @@ class Context:
 
     def graph_destroy(self, graph):
         self._chk(self.L.psgpu_graph_destroy(self.h, graph))
+
+    # ---- asynchronous readback (serving)
+    def host_alloc(self, nbytes):
+        """pinned host buffer as a uint8 numpy array; free with host_free"""
+        p = C.c_void_p()
+        self._chk(self.L.psgpu_host_alloc(self.h, nbytes, C.byref(p)))
+        return np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(p.value))
+
+    def host_free(self, arr):
+        self._chk(self.L.psgpu_host_free(self.h, arr.ctypes.data))
+
+    def read_async(self, tex, out, x=0, y=0, w=None, h=None, layer=0):
+        """queue a copy of rows y..y+h-1, columns x..x+w-1 (memory order) into
+        `out` (uint8, h rows of at least w*4 bytes)"""
+        if w is None or h is None:
+            raise ValueError("region size required")
+        pitch = out.strides[0] if out.ndim > 1 else w * 4
+        if out.nbytes < pitch * (h - 1) + w * 4:
+            raise ValueError("host buffer too small")
+        self._chk(self.L.psgpu_texture_read_async(self.h, tex, layer, x, y, w, h, out.ctypes.data, pitch))
+
+    def fence(self):
+        f = C.c_uint64()
+        self._chk(self.L.psgpu_fence(self.h, C.byref(f)))
+        return f.value
+
+    def fence_done(self, f):
+        d = C.c_int()
+        self._chk(self.L.psgpu_fence_query(self.h, f, C.byref(d)))
+        return bool(d.value)
+
+    def fence_wait(self, f):
+        self._chk(self.L.psgpu_fence_wait(self.h, f))
 
     def selftest_math(self, op, a, b=None):
         a = np.ascontiguousarray(a, np.float32)
