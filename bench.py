@@ -124,6 +124,74 @@ def batch_section(args, N, S, log):
     return out, assets, views
 
 
+def serve_section(args, N, S, log):
+    """2,000 Poisson view requests (area log-uniform 128^2..2048^2) over 32
+    resident scenes (1k-1M triangles, 512^2-4096^2 mipmapped textures),
+    continuous batching with HBM-budget admission, against the plain
+    one-view-per-launch path on the same GPU."""
+    from puresoft3d_amd import serving as SV
+    t0 = time.perf_counter()
+    assets = S.serving_assets(n_scenes=32, seed=21)
+    n = args.serve_requests
+    views, _ = S.poisson_requests(assets, n=n, rate=1.0, seed=23)
+    log("[bench] serve: %d scenes (%.2f M tris), %d requests, generated in %.1f s" % (
+        len(assets), sum(a.triangles() for a in assets) / 1e6, n, time.perf_counter() - t0))
+    ctx = N.Context(64, 64)
+    srv = SV.RenderServer(ctx, assets)
+    cfg = dict(max_batch_views=args.serve_batch_views, max_batch_pixels=args.serve_batch_pixels << 20,
+               max_inflight=2, hbm_budget=args.serve_budget_gib << 30)
+    zeros = np.zeros(n)
+    srv.serve(views, zeros, **cfg)                       # warm-up: target pools and scratch sizes
+    srv.serve(views, zeros, max_batch_views=1, max_inflight=1)
+    # plain path: one view per launch, back to back
+    n_plain = min(n, 600)
+    res_p, st_p, _ = srv.serve(views[:n_plain], zeros[:n_plain], max_batch_views=1, max_inflight=1)
+    single = res_p["done_s"] - res_p["launched_s"]
+    plain_vps = n_plain / st_p["wall_s"]
+    out = {"requests": n, "scenes": len(assets),
+           "plain": {"views_per_s": round(plain_vps, 1),
+                     "host_submit_ms_per_view": round(st_p["host_submit_s"] * 1e3 / n_plain, 4),
+                     "host_flush_ms_per_view": round(st_p["host_flush_s"] * 1e3 / n_plain, 4),
+                     "single_view_ms_p50": round(float(np.percentile(single, 50)) * 1e3, 3),
+                     "single_view_ms_p99": round(float(np.percentile(single, 99)) * 1e3, 3)},
+           "config": {k: int(v) for k, v in cfg.items()}, "runs": []}
+    keep = list(range(0, n, max(1, n // 8)))
+    for load in args.serve_loads:
+        rate = load * plain_vps
+        _, arr = S.poisson_requests(assets, n=n, rate=rate, seed=23)
+        res, st, imgs = srv.serve(views, arr, keep=keep, **cfg)
+        summ = SV.latency_summary(res)
+        lat = res["done_s"] - res["arrival_s"]
+        run = {"offered_x_plain": load, "offered_views_per_s": round(rate, 1),
+               "views_per_s": round(summ["views_per_s"], 1),
+               "speedup_vs_plain": round(summ["views_per_s"] / plain_vps, 2),
+               "latency_ms_p50": round(summ["p50_ms"], 3), "latency_ms_p99": round(summ["p99_ms"], 3),
+               "p99_vs_single_view_p99": round(float(np.percentile(lat, 99)) / float(np.percentile(single, 99)), 2),
+               "batches": st["batches"], "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1),
+               "host_submit_ms_per_batch": round(st["host_submit_s"] * 1e3 / max(1, st["batches"]), 4),
+               "peak_target_bytes": st["peak_bytes"]}
+        same = sum(int(np.array_equal(imgs[i], srv.render_alone(views[i]))) for i in keep)
+        run["images_identical_to_alone"] = "%d/%d" % (same, len(keep))
+        out["runs"].append(run)
+        log("[bench] serve load %.1fx: %s" % (load, json.dumps(run)))
+    ctx.close()
+    return out
+
+
+def adjoint_section(args, log):
+    """optimisation steps of the adjoint workload (512^2, 50k-tri displaced
+    sphere, 1024^2 FP32 RGB texture), in a process of its own: it uses torch
+    on the GPU, and torch must load its HIP runtime before libpsgpu does"""
+    import subprocess
+    r = subprocess.run([sys.executable, "-u", "-m", "puresoft3d_amd.adjoint", "--bench", "--steps",
+                        str(args.adjoint_steps)], capture_output=True, text=True, timeout=600, cwd=ROOT)
+    if r.returncode != 0:
+        raise RuntimeError("adjoint bench failed: " + r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    log("[bench] adjoint: %s" % json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +205,16 @@ def main():
     ap.add_argument("--batch", action="store_true", help="also run the 256-view batch workload")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--tune", action="store_true", help="autotune the raster tiling on the batch workload")
+    ap.add_argument("--serve", action="store_true", help="also run the serving workload")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[3.0],
+                    help="offered request rates as multiples of the plain path's views/s")
+    ap.add_argument("--serve-batch-views", type=int, default=64)
+    ap.add_argument("--serve-batch-pixels", type=int, default=48, metavar="MPIX")
+    ap.add_argument("--serve-budget-gib", type=int, default=8)
+    ap.add_argument("--adjoint", action="store_true", help="also time the adjoint workload")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run one plain copy kernel of MIB MiB (k_selftest op 7) to calibrate HBM counters")
     args = ap.parse_args()
@@ -145,8 +223,16 @@ def main():
               "across GPUs with scenes replicated; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    # the adjoint runs in a child process, started before this one touches the GPU
+    adj = adjoint_section(args, lambda m: print(m, flush=True)) if args.adjoint else None
+
     from puresoft3d_amd import native as N
     from puresoft3d_amd import scenes as S
+
+    if args.serve_only:
+        log = lambda m: print(m, flush=True)  # noqa: E731
+        print(json.dumps({"served": serve_section(args, N, S, log)}), flush=True)
+        return
 
     if args.small:
         scene = S.terrain_scene(width=960, height=540, cells=150, n_spheres=16, tex_size=1024)
@@ -234,6 +320,11 @@ def main():
     if args.batch:
         log = lambda m: print(m, flush=True)  # noqa: E731
         out["batched"], _, _ = batch_section(args, N, S, log)
+    if args.serve:
+        log = lambda m: print(m, flush=True)  # noqa: E731
+        out["served"] = serve_section(args, N, S, log)
+    if adj is not None:
+        out["adjoint"] = adj
     if not args.no_cpu:
         thr = args.cpu_threads or min(os.cpu_count() or 2, 16)
         print("[bench] cpu baseline: oracle with %d threads for <= %.0f s" % (thr, args.cpu_seconds), flush=True)

@@ -117,6 +117,13 @@ int psgpu_behavior_get(psgpu_ctx* ctx, int* bits);
 int psgpu_viewport(psgpu_ctx* ctx, int width, int height);
 int psgpu_target_set(psgpu_ctx* ctx, int fbo_slot, int tex_or_display);   /* setFBO, slot 0 only */
 int psgpu_depth_set(psgpu_ctx* ctx, int tex_or_default);                   /* setDepth */
+/* geometry buffer of the visible fragments (no counterpart in the reference):
+ * while set, Blinn-Phong draws also write, for every pixel whose colour they
+ * produce, 12 floats at device_ptr + (y*vpW + x)*48 bytes (rows bottom-up):
+ * interpolated normal xyz, 1 | world position xyz, 0 | u, v, 0, 0.  Pixels
+ * no draw colours are left as they were.  NULL stops it.  The adjoint entry
+ * points below read this buffer: it fixes visibility for the backward pass. */
+int psgpu_target_gbuffer(psgpu_ctx* ctx, void* device_ptr, size_t bytes);
 int psgpu_clear_depth(psgpu_ctx* ctx, float furthest);                     /* clear16 */
 int psgpu_clear_colour(psgpu_ctx* ctx, uint32_t bgra);                     /* clear4 of the display */
 /* clears every row of the current colour target (FBO slot 0); the reference
@@ -139,6 +146,29 @@ int psgpu_depth_device_ptr(psgpu_ctx* ctx, void** ptr, int* scanline_floats);
 int psgpu_get_stats(psgpu_ctx* ctx, psgpu_stats* out);
 int psgpu_reset_stats(psgpu_ctx* ctx);
 int psgpu_swap_buffers(psgpu_ctx* ctx);           /* swapBuffers: flush; display keeps its buffer */
+
+/* ---- adjoint of texture sampling and shading (no counterpart in the reference) ----
+ * A differentiable shading model evaluated on a geometry buffer (above):
+ *   c = albedo_scale * T(u,v) * max(N.L, 0) + ks * max(N.H, 0)^spec_exp   (RGB)
+ * with N the normalised interpolated normal, L and E the normalised
+ * directions to light and camera, H = normalize(L + E), and T a bilinear
+ * sample of an FP32 RGB texture (th rows of tw texels, row 0 at v = 0,
+ * texel j at u = j/tw, wrapping).  Uncovered pixels give 0.
+ * All buffers are device memory; params = {albedo_scale, spec_exp}.
+ * backward adds dLoss/dT into grad_tex (atomics) and dLoss/dparams into
+ * grad_params, given grad_out = dLoss/dc; visibility stays that of the
+ * geometry buffer.  stream: a hipStream_t to run on, or NULL for the
+ * context's stream. */
+typedef struct {
+    int width, height;          /* geometry buffer / image size */
+    int tex_w, tex_h;           /* learnable texture size */
+    float light[4], camera[4];  /* world positions */
+    float ks;                   /* specular weight */
+} psgpu_diff_desc;
+int psgpu_diff_forward(psgpu_ctx* ctx, const psgpu_diff_desc* desc, const float* gbuf, const float* tex,
+                       const float* params, float* out_rgb, void* stream);
+int psgpu_diff_backward(psgpu_ctx* ctx, const psgpu_diff_desc* desc, const float* gbuf, const float* tex,
+                        const float* params, const float* grad_out, float* grad_tex, float* grad_params, void* stream);
 
 /* ---- asynchronous readback and fences (serving) ----
  * No counterpart in the reference, whose drawVAO returns with the frame done

@@ -72,6 +72,8 @@ typedef struct {
     int64_t peak_bytes;          /* render targets held at once */
     int max_batch_views_seen;
     double wall_s;
+    double host_submit_s;        /* host time spent queueing batches (draws, reads, fence) */
+    double host_flush_s;         /* part of it spent in the flush the first read triggers */
 } psserve_stats;
 
 /* Runs the request list in real time (arrivals are waited for) and returns

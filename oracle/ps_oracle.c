@@ -1292,3 +1292,95 @@ void or_clear_colour(or_target* t, uint32_t bgra)
         for (int x = 0; x < t->width; x++) memcpy(row + (size_t)x * 4, &bgra, 4);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* differentiable shading on a geometry buffer, FP64                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int covered;
+    double lam, s, spec, u, v;
+} or_geo;
+
+static void norm3d(double* a)
+{
+    double r = 1.0 / sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    a[0] *= r; a[1] *= r; a[2] *= r;
+}
+
+static or_geo diff_geometry(const float* g, double e, double ks, const float* light, const float* camera)
+{
+    or_geo o;
+    memset(&o, 0, sizeof(o));
+    o.covered = g[3] != 0.0f;
+    if (!o.covered) return o;
+    double n[3] = {g[0], g[1], g[2]}, L[3], E[3], Hh[3];
+    norm3d(n);
+    for (int k = 0; k < 3; k++) { L[k] = (double)light[k] - g[4 + k]; E[k] = (double)camera[k] - g[4 + k]; }
+    norm3d(L);
+    norm3d(E);
+    for (int k = 0; k < 3; k++) Hh[k] = L[k] + E[k];
+    norm3d(Hh);
+    double lam = n[0] * L[0] + n[1] * L[1] + n[2] * L[2];
+    double s = n[0] * Hh[0] + n[1] * Hh[1] + n[2] * Hh[2];
+    o.lam = lam > 0 ? lam : 0;
+    o.s = s > 0 ? s : 0;
+    o.spec = o.s > 0 ? ks * pow(o.s, e) : 0;
+    o.u = g[8];
+    o.v = g[9];
+    return o;
+}
+
+static void diff_footprint(int tw, int th, double u, double v, int64_t* idx, double* w)
+{
+    /* the texel footprint follows the FP32 coordinate the kernel forms */
+    float fx = (float)u * (float)tw, fy = (float)v * (float)th;
+    if (!(fabsf(fx) < 1073741824.0f)) fx = 0.0f;
+    if (!(fabsf(fy) < 1073741824.0f)) fy = 0.0f;
+    double x0 = floor((double)fx), y0 = floor((double)fy);
+    double ax = (double)fx - x0, ay = (double)fy - y0;
+    long ix = (long)x0 % tw, iy = (long)y0 % th;
+    if (ix < 0) ix += tw;
+    if (iy < 0) iy += th;
+    long ix1 = (ix + 1) % tw, iy1 = (iy + 1) % th;
+    idx[0] = iy * tw + ix; idx[1] = iy * tw + ix1; idx[2] = iy1 * tw + ix; idx[3] = iy1 * tw + ix1;
+    w[0] = (1 - ax) * (1 - ay); w[1] = ax * (1 - ay); w[2] = (1 - ax) * ay; w[3] = ax * ay;
+}
+
+void or_diff_forward_f64(int W, int H, const float* gbuf, int tw, int th, const float* tex, double albedo_scale,
+                         double spec_exp, double ks, const float* light, const float* camera, double* out)
+{
+    for (int64_t p = 0; p < (int64_t)W * H; p++) {
+        or_geo g = diff_geometry(gbuf + p * 12, spec_exp, ks, light, camera);
+        for (int c = 0; c < 3; c++) out[p * 3 + c] = 0;
+        if (!g.covered) continue;
+        int64_t idx[4];
+        double w[4];
+        diff_footprint(tw, th, g.u, g.v, idx, w);
+        for (int c = 0; c < 3; c++) {
+            double t = 0;
+            for (int k = 0; k < 4; k++) t += w[k] * tex[idx[k] * 3 + c];
+            out[p * 3 + c] = albedo_scale * t * g.lam + g.spec;
+        }
+    }
+}
+
+void or_diff_backward_f64(int W, int H, const float* gbuf, int tw, int th, const float* tex, double albedo_scale,
+                          double spec_exp, double ks, const float* light, const float* camera, const double* gout,
+                          double* gtex, double* gparams)
+{
+    for (int64_t p = 0; p < (int64_t)W * H; p++) {
+        or_geo g = diff_geometry(gbuf + p * 12, spec_exp, ks, light, camera);
+        if (!g.covered) continue;
+        int64_t idx[4];
+        double w[4];
+        diff_footprint(tw, th, g.u, g.v, idx, w);
+        const double* go = gout + p * 3;
+        for (int c = 0; c < 3; c++) {
+            double t = 0;
+            for (int k = 0; k < 4; k++) t += w[k] * tex[idx[k] * 3 + c];
+            gparams[0] += go[c] * t * g.lam;
+            for (int k = 0; k < 4; k++) gtex[idx[k] * 3 + c] += go[c] * albedo_scale * g.lam * w[k];
+        }
+        if (g.s > 0) gparams[1] += (go[0] + go[1] + go[2]) * g.spec * log(g.s);
+    }
+}

@@ -139,6 +139,18 @@ float or_log2_approx(float x);
 /* Hash used by the debug planes. */
 uint32_t or_tri_hash(uint32_t tri);
 
+/* Differentiable shading on a geometry buffer (12 floats per pixel: normal
+ * xyz, covered flag | world position xyz, 0 | u, v, 0, 0), in FP64: the
+ * reference of psgpu_diff_forward / psgpu_diff_backward (no counterpart in
+ * Puresoft3D).  tex: th rows of tw RGB float texels, bilinear with texel j at
+ * u = j/tw, wrapping.  out: W*H*3.  gtex: tw*th*3 and gparams[2] are
+ * accumulated (dLoss/dtex, dLoss/d{albedo_scale, spec_exp}) from gout. */
+void or_diff_forward_f64(int W, int H, const float* gbuf, int tw, int th, const float* tex, double albedo_scale,
+                         double spec_exp, double ks, const float* light, const float* camera, double* out);
+void or_diff_backward_f64(int W, int H, const float* gbuf, int tw, int th, const float* tex, double albedo_scale,
+                          double spec_exp, double ks, const float* light, const float* camera, const double* gout,
+                          double* gtex, double* gparams);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,6 +88,13 @@ def lib():
         L.or_tri_hash.restype = C.c_uint32
         L.or_pool_shutdown.argtypes = []
         L.or_pool_shutdown.restype = None
+        vp = C.c_void_p
+        L.or_diff_forward_f64.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, vp, C.c_double, C.c_double,
+                                          C.c_double, vp, vp, vp]
+        L.or_diff_forward_f64.restype = None
+        L.or_diff_backward_f64.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, vp, C.c_double, C.c_double,
+                                           C.c_double, vp, vp, vp, vp, vp]
+        L.or_diff_backward_f64.restype = None
         _lib = L
     return _lib
 
@@ -274,3 +281,34 @@ def render_scene(scene, n_threads=1, debug=False, fp64=False, frame=None, clear=
 def pool_shutdown():
     """stop the oracle's worker threads (they spin while idle, like the reference's)"""
     lib().or_pool_shutdown()
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def diff_forward_f64(gbuf, tex, albedo_scale, spec_exp, ks, light, camera):
+    """FP64 differentiable shading of a geometry buffer (H, W, 12) with an RGB
+    texture (th, tw, 3); returns (H, W, 3) float64"""
+    g, t = _f32(gbuf), _f32(tex)
+    H, W = g.shape[:2]
+    th, tw = t.shape[:2]
+    out = np.zeros((H, W, 3), np.float64)
+    lt, cm = _f32(light), _f32(camera)
+    lib().or_diff_forward_f64(W, H, g.ctypes.data, tw, th, t.ctypes.data, albedo_scale, spec_exp, ks,
+                              lt.ctypes.data, cm.ctypes.data, out.ctypes.data)
+    return out
+
+
+def diff_backward_f64(gbuf, tex, albedo_scale, spec_exp, ks, light, camera, gout):
+    """FP64 adjoint: (dLoss/dtex (th, tw, 3), dLoss/d[albedo_scale, spec_exp]) for gout = dLoss/dout"""
+    g, t = _f32(gbuf), _f32(tex)
+    H, W = g.shape[:2]
+    th, tw = t.shape[:2]
+    go = np.ascontiguousarray(gout, np.float64)
+    gt = np.zeros((th, tw, 3), np.float64)
+    gp = np.zeros(2, np.float64)
+    lt, cm = _f32(light), _f32(camera)
+    lib().or_diff_backward_f64(W, H, g.ctypes.data, tw, th, t.ctypes.data, albedo_scale, spec_exp, ks,
+                               lt.ctypes.data, cm.ctypes.data, go.ctypes.data, gt.ctypes.data, gp.ctypes.data)
+    return gt, gp

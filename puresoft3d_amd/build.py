@@ -17,7 +17,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", 
              "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-value", "-Wno-unused-result", "-fPIC"]
 
-SOURCES = ["ps_kernels.hip", "psgpu_api.hip"]
+SOURCES = ["ps_kernels.hip", "psgpu_api.hip", "ps_adjoint.hip"]
+BUILD_DIR = os.path.join(HERE, "_build")
 
 
 def run(cmd, cwd=None):
@@ -31,10 +32,18 @@ def run(cmd, cwd=None):
 
 
 def build_psgpu():
+    """libpsgpu.so; it links ROCm's HIP runtime by soname (libamdhip64.so.7),
+    so in a process that imported torch first it binds to the runtime torch
+    already loaded, and the two share device memory and streams."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        o = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
+        run([HIPCC] + HIP_FLAGS + ["-c", "-o", o, os.path.join(CSRC, src)])
+        objs.append(o)
     out = os.path.join(HERE, "libpsgpu.so")
-    tmp = out + ".tmp"
-    run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES])
-    os.replace(tmp, out)
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
+    os.replace(out + ".tmp", out)
     return out
 
 
@@ -43,11 +52,11 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 
 
 def build_facade():
-    """C++ PuresoftPipeline facade (host code only) over libpsgpu, and the
-    psrender program that drives it."""
+    """C++ host library over libpsgpu (the PuresoftPipeline facade and the
+    serving loop), and the psrender program that drives the facade."""
     lib = os.path.join(HERE, "libpuresoft3d.so")
     run(["g++"] + CXX_FLAGS + ["-shared", "-o", lib + ".tmp", os.path.join(HOST, "puresoft3d.cpp"),
-                               "-L" + HERE, "-lpsgpu", "-Wl,-rpath,$ORIGIN"])
+                               os.path.join(HOST, "psserve.cpp"), "-L" + HERE, "-lpsgpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(lib + ".tmp", lib)
     exe = os.path.join(HOST, "psrender")
     run(["g++"] + CXX_FLAGS + ["-o", exe + ".tmp", os.path.join(HOST, "psrender.cpp"),

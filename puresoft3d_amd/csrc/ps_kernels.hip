@@ -742,6 +742,14 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
     float in[P::NV > 0 ? P::NV : 1];
 #pragma unroll
     for (int i = 0; i < P::NV; i++) in[i] = vs[i] * r;
+    if constexpr (PROG == PSGPU_PROC_BLINN) {
+        if (V.gbuf) {   // geometry buffer of the visible fragment (rows bottom-up)
+            float4* gp = (float4*)(V.gbuf + ((int64_t)y * V.W + x) * 12);
+            gp[0] = make_float4(in[6], in[7], in[8], 1.0f);
+            gp[1] = make_float4(in[9], in[10], in[11], 0.0f);
+            gp[2] = make_float4(in[12], in[13], 0.0f, 0.0f);
+        }
+    }
     float grad[6] = {0, 0, 0, 0, 0, 0};
     if (P::GRAD) ps_tri_grad(tr, u0, u0 + nvs, u0 + 2 * nvs, 12, grad);
     uint32_t c;

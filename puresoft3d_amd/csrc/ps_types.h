@@ -74,6 +74,7 @@ struct PsView {
     uint32_t* colour; int colour_scan; int colour_top_down; int colour_h;  // scan in uint32
     float* depth; int depth_scan;   // scan in floats
     int* winner;                    // W*H: span index of the last colour-writing pass
+    float* gbuf;                    // optional W*H*12 floats: N.xyz 1 | wp.xyz 0 | u v 0 0 of the winner
 };
 
 // Flush-wide parameters passed by value to kernels.

@@ -57,10 +57,11 @@ struct Targets {
     uint32_t* colour; int colour_scan; int colour_top_down; int colour_w; int colour_h;
     float* depth; int depth_scan; int depth_w; int depth_h;
     int vpw, vph;
+    float* gbuf;
     bool same(const Targets& o) const
     {
         return colour == o.colour && depth == o.depth && vpw == o.vpw && vph == o.vph && colour_scan == o.colour_scan &&
-               depth_scan == o.depth_scan;
+               depth_scan == o.depth_scan && gbuf == o.gbuf;
     }
 };
 
@@ -148,6 +149,8 @@ struct psgpu_ctx {
     int cur_prog = -1;
     int fbo0 = PSGPU_DISPLAY;
     int depth_sel = PSGPU_DEFAULT_DEPTH;
+    float* gbuf = nullptr;         // caller's geometry buffer (device memory), or none
+    size_t gbuf_bytes = 0;
     std::vector<Segment> queue;        // pending views with draws
     std::vector<PsFill> pending_fills; // pending clears (run before the queued draws)
     int64_t queue_tris = 0;
@@ -305,6 +308,7 @@ int current_targets(psgpu_ctx* c, Targets& t)
 {
     t.vpw = c->vp_w;
     t.vph = c->vp_h;
+    t.gbuf = c->gbuf;
     if (c->fbo0 == PSGPU_DISPLAY) {
         t.colour = c->display; t.colour_scan = c->dev_w; t.colour_top_down = 1; t.colour_w = c->dev_w; t.colour_h = c->dev_h;
     } else {
@@ -374,6 +378,7 @@ int build_blob(psgpu_ctx* c, const Cmd& cmd, Blob& b, PsFrame& f, int64_t& T, in
         pv.colour = sg.tg.colour; pv.colour_scan = sg.tg.colour_scan; pv.colour_top_down = sg.tg.colour_top_down;
         pv.colour_h = sg.tg.colour_h;
         pv.depth = sg.tg.depth; pv.depth_scan = sg.tg.depth_scan;
+        pv.gbuf = sg.tg.gbuf;
         pv.winner = (int*)(intptr_t)n_pix;   // offset; rebased once the winner buffer is known
         n_pix += (int64_t)pv.W * pv.H;
         views.push_back(pv);
@@ -1167,6 +1172,15 @@ int psgpu_target_set(psgpu_ctx* c, int fbo_slot, int tex)
     return PSGPU_OK;
 }
 
+int psgpu_target_gbuffer(psgpu_ctx* c, void* device_ptr, size_t bytes)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (device_ptr && ((uintptr_t)device_ptr % 16)) return fail(c, PSGPU_ERR_INVALID, "geometry buffer must be 16-byte aligned");
+    c->gbuf = (float*)device_ptr;
+    c->gbuf_bytes = device_ptr ? bytes : 0;
+    return PSGPU_OK;
+}
+
 int psgpu_depth_set(psgpu_ctx* c, int tex)
 {
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
@@ -1318,6 +1332,12 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     if ((rc = current_targets(c, tg))) return rc;
     if (tg.colour_w < c->vp_w || tg.colour_h < c->vp_h || tg.depth_w < c->vp_w || tg.depth_h < c->vp_h)
         return fail(c, PSGPU_ERR_STATE, "render targets are smaller than the viewport");
+    if (tg.gbuf) {
+        if (d.prog != PSGPU_PROC_BLINN)
+            return fail(c, PSGPU_ERR_UNSUPPORTED, "geometry buffers are written by the Blinn-Phong programme only");
+        if (c->gbuf_bytes < (size_t)c->vp_w * c->vp_h * 48)
+            return fail(c, PSGPU_ERR_STATE, "geometry buffer smaller than the viewport (48 bytes per pixel)");
+    }
     if (d.tri_count == 0) return PSGPU_OK;
     // draws with the targets and viewport of the last pending view join it;
     // otherwise they open a new view of the same flush, unless a pending view

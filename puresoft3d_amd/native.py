@@ -8,6 +8,7 @@ std::bad_alloc -> MemoryError.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -112,6 +113,11 @@ _SIGS = {
     "psgpu_graph_destroy": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_selftest_math": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "psgpu_host_alloc": ([C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "psgpu_target_gbuffer": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "psgpu_diff_forward": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+                           C.c_int),
+    "psgpu_diff_backward": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p], C.c_int),
     "psgpu_host_free": ([C.c_void_p, C.c_void_p], C.c_int),
     "psgpu_texture_read_async": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                   C.c_size_t], C.c_int),
@@ -134,11 +140,23 @@ def declared_symbols():
     return names
 
 
+_runtime = None
+
+
+def runtime():
+    """which HIP runtime libpsgpu bound to: "torch" (the one torch loaded
+    before it), "rocm" (the system one), or None before the first load"""
+    return _runtime
+
+
 def lib():
-    global _lib
+    global _lib, _runtime
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libpsgpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # libpsgpu binds by soname to a HIP runtime torch has already loaded;
+        # importing torch after this point would load a second runtime
+        _runtime = "torch" if "torch" in sys.modules else "rocm"
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             fn = getattr(L, name)
@@ -268,6 +286,9 @@ class Context:
                                               C.byref(out)))
         return out.value
 
+    def program_destroy(self, prog):
+        self._chk(self.L.psgpu_program_destroy(self.h, prog))
+
     def program_use(self, prog):
         self._chk(self.L.psgpu_program_use(self.h, prog))
 
@@ -289,6 +310,11 @@ class Context:
 
     def target(self, tex, slot=0):
         self._chk(self.L.psgpu_target_set(self.h, slot, tex))
+
+    def gbuffer_target(self, device_ptr, nbytes):
+        """geometry buffer written by Blinn-Phong draws (device memory of the
+        same HIP runtime, e.g. a torch tensor's data_ptr()); 0 stops it"""
+        self._chk(self.L.psgpu_target_gbuffer(self.h, C.c_void_p(device_ptr or None), nbytes))
 
     def depth_target(self, tex):
         self._chk(self.L.psgpu_depth_set(self.h, tex))

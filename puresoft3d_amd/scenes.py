@@ -207,6 +207,49 @@ def uv_sphere(stacks, slices, radius=1.0, center=(0, 0, 0), seed=None, tangents=
     return mesh
 
 
+def displaced_sphere(stacks=125, slices=200, amp=0.08, seed=41):
+    """Lat-long sphere (2*stacks*slices triangles) displaced along the radius
+    by seeded periodic value noise; normals, tangents and binormals from the
+    displaced grid.  Slots 0-4 as the Blinn-Phong programme reads them."""
+    rng = np.random.default_rng(seed)
+    hgt = value_noise(stacks + 1, slices + 1, 8, rng, channels=1, octaves=3)[..., 0]
+    i = np.arange(stacks + 1)[:, None]
+    j = np.arange(slices + 1)[None, :]
+    th = np.pi * i / stacks
+    ph = 2 * np.pi * j / slices
+    rad = np.stack(np.broadcast_arrays(np.sin(th) * np.cos(ph), np.cos(th) + 0 * ph, np.sin(th) * np.sin(ph)), -1)
+    pos = rad * (1.0 + amp * (hgt[..., None] - 0.5))
+    dth = np.gradient(pos, axis=0)
+    dph = np.gradient(pos, axis=1)
+    nrm = np.cross(dph, dth)
+    ln = np.linalg.norm(nrm, axis=-1, keepdims=True)
+    pole = ln[..., 0] < 1e-9
+    nrm = np.where(pole[..., None], rad, nrm / np.maximum(ln, 1e-30))
+    nrm *= np.sign((nrm * rad).sum(-1, keepdims=True) + 1e-30)
+    tan = np.stack(np.broadcast_arrays(-np.sin(ph) + 0 * th, 0 * th + 0 * ph, np.cos(ph) + 0 * th), -1)
+    uv = np.stack(np.broadcast_arrays(j / slices + 0 * i, i / stacks + 0 * j), -1)
+    q = []
+    for a in range(stacks):
+        for b in range(slices):
+            q.append(((a, b), (a + 1, b), (a + 1, b + 1)))
+            q.append(((a, b), (a + 1, b + 1), (a, b + 1)))
+    idx = np.array(q)
+    P = pos[idx[..., 0], idx[..., 1]]
+    flip = _orient_ccw(P, P.mean(1))
+    idx[flip] = idx[flip][:, [0, 2, 1]]
+    nv = idx.shape[0] * 3
+    out = {}
+    for slot, arr, w in ((0, pos, 1.0), (1, tan, 0.0), (3, nrm, 0.0)):
+        a4 = np.full((nv, 4), w, np.float32)
+        a4[:, :3] = arr[idx[..., 0], idx[..., 1]].reshape(-1, 3)
+        out[slot] = a4
+    B = np.zeros((nv, 4), np.float32)
+    B[:, :3] = np.cross(out[3][:, :3], out[1][:, :3])
+    out[2] = B
+    out[4] = uv[idx[..., 0], idx[..., 1]].reshape(-1, 2).astype(np.float32)
+    return out
+
+
 def terrain(cells, size, seed, height=8.0, uv_repeat=8.0):
     rng = np.random.default_rng(seed)
     n = cells + 1
@@ -383,6 +426,37 @@ def batch_assets(n_scenes=16, seed=11, tri_lo=10_000, tri_hi=500_000, tex_size=1
     return assets
 
 
+def tiled_texture(make, size, seed, tile=512):
+    """a size x size texture that repeats a seeded periodic tile of side
+    min(size, tile): full-size memory footprint, bounded generation time"""
+    t = min(size, tile)
+    img = make(t, seed)
+    reps = size // t
+    return np.ascontiguousarray(np.tile(img, (reps, reps, 1)))
+
+
+def serving_assets(n_scenes=32, seed=21, tri_lo=1_000, tri_hi=1_000_000, tex_lo=512, tex_hi=4096):
+    """resident scenes of the serving workload: terrain patches with
+    log-uniform triangle counts and power-of-two textures with log-uniform
+    side in [tex_lo, tex_hi], mipmapped"""
+    rng = np.random.default_rng(seed)
+    assets = []
+    lo, hi = int(np.log2(tex_lo)), int(np.log2(tex_hi))
+    for k in range(n_scenes):
+        T = float(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        cells = max(2, int(round(np.sqrt(T / 2.0))))
+        ts = 1 << int(rng.integers(lo, hi + 1))
+        lv = mip_levels(ts, ts)
+        mesh, _, _, _ = terrain(cells, 40.0, seed * 100 + k, height=4.0, uv_repeat=4.0)
+        tex = [TextureSpec([tiled_texture(lambda t, sd: noise_texture(t, sd, cells=8, octaves=3), ts, seed * 100 + k + 1)],
+                           wrap=N.WRAP, levels=lv),
+               TextureSpec([tiled_texture(lambda t, sd: normal_map(t, sd, cells=16), ts, seed * 100 + k + 2)],
+                           wrap=N.WRAP, levels=lv)]
+        assets.append(SceneAsset(mesh, tex, np.array([0.0, 0.0, 0.0]), 28.0,
+                                 "srv%d_%dk_tex%d" % (k, cells * cells * 2 // 1000, ts)))
+    return assets
+
+
 class ViewSpec:
     def __init__(self, scene, width, height, eye, target, light, spec_exp=24):
         self.scene, self.width, self.height = scene, width, height
@@ -414,6 +488,17 @@ def batch_views(assets, n_views=256, seed=12, side_lo=256, side_hi=1024, aspect_
         light = assets[s].center + np.array([20.0, 50.0, 10.0])
         views.append(ViewSpec(s, w, h, eye, assets[s].center, light))
     return views
+
+
+def poisson_requests(assets, n=2000, rate=2000.0, seed=23, side_lo=128, side_hi=2048):
+    """n view requests with exponential inter-arrival times (rate per second),
+    area log-uniform in [side_lo^2, side_hi^2], a uniformly chosen scene and a
+    seeded camera on the upper sphere around it.  Returns (views, arrivals)."""
+    rng = np.random.default_rng(seed)
+    views = batch_views(assets, n_views=n, seed=seed + 1, side_lo=side_lo, side_hi=side_hi)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    arrivals -= arrivals[0]
+    return views, arrivals
 
 
 class GpuBatch:
@@ -474,15 +559,20 @@ class GpuBatch:
 
     def view_scene(self, i):
         """the view as a stand-alone Scene (for the oracle or a one-view render)"""
-        vw = self.views[i]
-        a = self.assets[vw.scene]
-        sc = Scene(vw.width, vw.height)
-        sc.name = "view%d_%s_%dx%d" % (i, a.name, vw.width, vw.height)
-        sc.meshes["m"] = a.mesh
-        sc.textures = list(a.textures)
-        sc.uniforms = vw.uniforms()
-        sc.draws.append(DrawSpec("m", N.PROC_BLINN))
-        return sc
+        return view_as_scene(self.assets, self.views[i], "view%d" % i)
+
+
+def view_as_scene(assets, vw, tag="view"):
+    """a view of a resident asset as a stand-alone Scene (textures 0 and 1 are
+    the asset's albedo and normal map)"""
+    a = assets[vw.scene]
+    sc = Scene(vw.width, vw.height)
+    sc.name = "%s_%s_%dx%d" % (tag, a.name, vw.width, vw.height)
+    sc.meshes["m"] = a.mesh
+    sc.textures = list(a.textures)
+    sc.uniforms = vw.uniforms()
+    sc.draws.append(DrawSpec("m", N.PROC_BLINN))
+    return sc
 
 
 # ---------------------------------------------------------------- scene files

@@ -1,0 +1,147 @@
+"""Render serving over libpsgpu: resident scenes, a Poisson stream of view
+requests, continuous batching with HBM-budget admission (the loop itself is
+native: include/psserve.h, puresoft3d_amd/host/psserve.cpp)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import native as N
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(HERE, "libpuresoft3d.so")
+
+
+class Scene(C.Structure):
+    _fields_ = [("vao", C.c_int), ("albedo_tex", C.c_int), ("normal_tex", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int), ("width", C.c_int), ("height", C.c_int), ("pv", C.c_float * 16),
+                ("eye", C.c_float * 4), ("light", C.c_float * 4), ("spec_exp", C.c_int), ("arrival_s", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("program", C.c_int), ("hbm_budget", C.c_size_t), ("scratch_bytes_per_px", C.c_int),
+                ("max_batch_views", C.c_int), ("max_batch_pixels", C.c_int64), ("max_inflight", C.c_int),
+                ("clear_bgra", C.c_uint32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("admitted_s", C.c_double), ("launched_s", C.c_double),
+                ("done_s", C.c_double), ("batch", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("batches", C.c_int64), ("views", C.c_int64), ("pixels", C.c_int64), ("target_allocs", C.c_int64),
+                ("target_evictions", C.c_int64), ("peak_bytes", C.c_int64), ("max_batch_views_seen", C.c_int),
+                ("wall_s", C.c_double), ("host_submit_s", C.c_double), ("host_flush_s", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        N.lib()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError("%s is not built (python -m puresoft3d_amd.build)" % HOST_LIB)
+        L = C.CDLL(HOST_LIB)
+        L.psserve_run.argtypes = [C.c_void_p, C.POINTER(Scene), C.c_int, C.POINTER(Request), C.c_int,
+                                  C.POINTER(Config), C.POINTER(Result), C.POINTER(C.c_void_p), C.POINTER(Stats)]
+        L.psserve_run.restype = C.c_int
+        L.psserve_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class RenderServer:
+    """Scenes resident on one context; serve() runs a request stream."""
+
+    def __init__(self, ctx, assets, gen_mips=True):
+        self.ctx, self.assets = ctx, assets
+        self.scenes = (Scene * len(assets))()
+        for k, a in enumerate(assets):
+            ids = []
+            for t in a.textures:
+                idx = ctx.texture_create(t.width, t.height, 4, wrap=t.wrap, mip_levels=t.levels, pixels=t.images[0])
+                if t.levels > 1 and gen_mips:
+                    ctx.texture_generate_mips(idx)
+                ids.append(idx)
+            v = ctx.vao_create()
+            for s, arr in a.mesh.items():
+                ctx.vao_attach(v, s, ctx.vbo_create(arr))
+            self.scenes[k] = Scene(v, ids[0], ids[1])
+        self.prog = ctx.program_create(N.PROC_BLINN)
+
+    def requests(self, views, arrivals):
+        reqs = (Request * len(views))()
+        for i, (vw, t) in enumerate(zip(views, arrivals)):
+            u = vw.uniforms()
+            r = reqs[i]
+            r.scene, r.width, r.height = vw.scene, vw.width, vw.height
+            r.pv[:] = [float(x) for x in u[3]]
+            r.eye[:] = [float(x) for x in u[8]]
+            r.light[:] = [float(x) for x in u[7]]
+            r.spec_exp = int(u[11][0])
+            r.arrival_s = float(t)
+        return reqs
+
+    def serve(self, views, arrivals, hbm_budget=8 << 30, max_batch_views=64, max_batch_pixels=48 << 20,
+              max_inflight=2, scratch_bytes_per_px=48, clear_bgra=0x00302010, keep=None):
+        """Runs the stream in real time.  keep: indices whose images to return.
+        Returns (per-request result dict of arrays, stats dict, images dict)."""
+        L = lib()
+        n = len(views)
+        reqs = self.requests(views, arrivals)
+        cfg = Config(self.prog, hbm_budget, scratch_bytes_per_px, max_batch_views, max_batch_pixels, max_inflight,
+                     clear_bgra)
+        res = (Result * n)()
+        st = Stats()
+        imgs, ptrs = {}, None
+        if keep is not None:
+            ptrs = (C.c_void_p * n)()
+            for i in keep:
+                imgs[i] = np.zeros((views[i].height, views[i].width, 4), np.uint8)
+                ptrs[i] = imgs[i].ctypes.data
+        rc = L.psserve_run(self.ctx.h, self.scenes, len(self.assets), reqs, n, C.byref(cfg), res, ptrs, C.byref(st))
+        if rc:
+            raise RuntimeError("psserve_run failed (%d): %s" % (rc, L.psserve_last_error().decode()))
+        out = {k: np.array([getattr(r, k) for r in res]) for k, _ in Result._fields_}
+        stats = {k: getattr(st, k) for k, _ in Stats._fields_}
+        return out, stats, imgs
+
+    def render_alone(self, view, clear_bgra=0x00302010):
+        """one view through the plain path (targets of exactly its size), synchronously"""
+        ctx = self.ctx
+        c = ctx.texture_create(view.width, view.height, 4, wrap=N.CLAMP)
+        d = ctx.texture_create(view.width, view.height, 4, wrap=N.CLAMP)
+        try:
+            sc = self.scenes[view.scene]
+            ctx.program_use(self.prog)
+            ctx.target(c)
+            ctx.depth_target(d)
+            ctx.viewport(view.width, view.height)
+            ctx.clear_depth(1.0)
+            ctx.clear_target(clear_bgra)
+            u = view.uniforms()
+            u[9] = np.array([sc.albedo_tex], np.int32)
+            u[10] = np.array([sc.normal_tex], np.int32)
+            for k, val in u.items():
+                ctx.uniform(k, val)
+            ctx.draw(sc.vao)
+            ctx.target(N.DISPLAY)
+            ctx.depth_target(N.DEFAULT_DEPTH)
+            ctx.viewport(ctx.width, ctx.height)
+            return ctx.texture_download(c).reshape(view.height, view.width, 4)
+        finally:
+            ctx.texture_destroy(c)
+            ctx.texture_destroy(d)
+
+
+def latency_summary(res):
+    lat = res["done_s"] - res["arrival_s"]
+    span = res["done_s"].max() - res["arrival_s"].min()
+    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "max_ms": float(lat.max() * 1e3), "views_per_s": float(len(lat) / span) if span > 0 else None}

@@ -23,3 +23,13 @@ def oracle():
 def announce(msg):
     """name the case on stdout before a GPU step (a fault ends the process)"""
     print("\n[gpu-case] " + msg, flush=True)
+
+
+def oracle_view(oracle, sc, clear_bgra):
+    """oracle render of one view into texture-layout targets (rows bottom-up,
+    tight scanlines, every row cleared), as psgpu_clear_target leaves them"""
+    import numpy as np
+    f = oracle.Frame(sc.width, sc.height, depth_scanline=sc.width, top_down=False)
+    f.colour.view(np.uint32)[:] = clear_bgra
+    f.clear_depth(1.0)
+    return oracle.render_scene(sc, frame=f, clear=False)

@@ -7,7 +7,7 @@ depth-test passes, and xor-hashes of the rasterised and passing triangle ids.
 import numpy as np
 import pytest
 
-from conftest import announce
+from conftest import announce, oracle_view
 import scenegen
 
 pytestmark = pytest.mark.gpu
@@ -190,16 +190,6 @@ def test_capacity_growth_replays_log(oracle):
         ctx.close()
 
 
-def _oracle_view(oracle, sc, clear_bgra):
-    """oracle render of one batched view into texture-layout targets (rows
-    bottom-up, tight scanlines, clear_target fills every row)"""
-    f = oracle.Frame(sc.width, sc.height, depth_scanline=sc.width, top_down=False)
-    f.colour.view(np.uint32)[:] = clear_bgra
-    f.clear_depth(1.0)
-    f2, st = oracle.render_scene(sc, frame=f, clear=False)
-    return f2, st
-
-
 def test_batched_views_match_oracle_and_single(oracle):
     """views of several sizes over two scenes, rendered in one flush, equal
     the oracle's per-view render and the GPU's one-view-per-flush render"""
@@ -224,7 +214,7 @@ def test_batched_views_match_oracle_and_single(oracle):
         ctx.close()
     for i in range(len(views)):
         sc = b.view_scene(i)
-        f, _ = _oracle_view(oracle, sc, 0x00302010)
+        f, _ = oracle_view(oracle, sc, 0x00302010)
         np.testing.assert_array_equal(got[i][0], f.colour)
         np.testing.assert_array_equal(got[i][1].view(np.uint32), f.depth.view(np.uint32))
 
@@ -266,7 +256,7 @@ def test_batched_views_graph_replay(oracle):
     finally:
         ctx.close()
     for i in (0, len(views) - 1):
-        f, _ = _oracle_view(oracle, b.view_scene(i), 0x00302010)
+        f, _ = oracle_view(oracle, b.view_scene(i), 0x00302010)
         np.testing.assert_array_equal(ref[i][0], f.colour)
 
 
