@@ -12,6 +12,13 @@ Prints one JSON line (see the keys below).  The CPU baseline is the oracle
 (a C restatement of the reference pipeline with the reference's threading:
 one producer thread plus row-interleaved workers), run on a bounded number
 of frames of the same workload on the host cores.
+
+Further sections of the same line (each can be skipped):
+  batched  256 views of 16 resident scenes in one graph-replayed launch,
+           batch latency p50/p99 and views/s against one view per launch
+  served   2,000 Poisson view requests over 32 resident scenes, continuous
+           batching: latency p50/p99 and views/s against the plain path
+  adjoint  optimisation steps/s of the differentiable texture workload
 """
 import argparse
 import json
@@ -85,11 +92,12 @@ def batch_section(args, N, S, log):
     g = ctx.capture_end()
     run = lambda: ctx.graph_launch(g, 1)  # noqa: E731
     tuned = None
-    if args.tune:
-        best, table = AT.autotune(ctx, run, reps=3, log=log)
+    if not args.no_tune:
+        best, table = AT.autotune(ctx, run, reps=3)
         tuned = {"best": {"chunk_width": best[0], "tile_rows": best[1], "waves_per_group": best[2]},
                  "searched": len(table),
-                 "table_ms": {"%dx%d/c%d/w%d" % (r[1], r[2], r[0][0], r[0][2]): round(r[3], 3) for r in table}}
+                 "top_ms": {"%dx%d/c%d/w%d" % (r[1], r[2], r[0][0], r[0][2]): round(r[3], 3)
+                            for r in sorted(table, key=lambda r: r[3])[:5]}}
     for _ in range(3):
         run()
     ctx.sync()
@@ -202,10 +210,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced scene for quick checks")
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
-    ap.add_argument("--batch", action="store_true", help="also run the 256-view batch workload")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default raster tiling (frame and batch)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-view batch workload")
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--tune", action="store_true", help="autotune the raster tiling on the batch workload")
-    ap.add_argument("--serve", action="store_true", help="also run the serving workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[3.0],
@@ -213,7 +221,8 @@ def main():
     ap.add_argument("--serve-batch-views", type=int, default=64)
     ap.add_argument("--serve-batch-pixels", type=int, default=48, metavar="MPIX")
     ap.add_argument("--serve-budget-gib", type=int, default=8)
-    ap.add_argument("--adjoint", action="store_true", help="also time the adjoint workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--frame-only", action="store_true", help="only the 1080p frame (and the CPU baseline)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run one plain copy kernel of MIB MiB (k_selftest op 7) to calibrate HBM counters")
@@ -224,7 +233,9 @@ def main():
         sys.exit(2)
 
     # the adjoint runs in a child process, started before this one touches the GPU
-    adj = adjoint_section(args, lambda m: print(m, flush=True)) if args.adjoint else None
+    if args.frame_only:
+        args.no_batch = args.no_serve = args.no_adjoint = True
+    adj = adjoint_section(args, lambda m: print(m, flush=True)) if not args.no_adjoint else None
 
     from puresoft3d_amd import native as N
     from puresoft3d_amd import scenes as S
@@ -256,6 +267,15 @@ def main():
         def frame():  # noqa: F811
             ctx.graph_launch(graph, 1)
 
+    tuned = None
+    if not args.no_tune:
+        from puresoft3d_amd import autotune as AT
+        best, table = AT.autotune(ctx, frame, reps=5)
+        top = sorted(table, key=lambda r: r[3])[:5]
+        tuned = {"best": {"chunk_width": best[0], "tile_rows": best[1], "waves_per_group": best[2]},
+                 "searched": len(table),
+                 "top_ms": {"%dx%d/c%d/w%d" % (r[1], r[2], r[0][0], r[0][2]): round(r[3], 4) for r in top}}
+        print("[bench] tiling autotuned: %s" % json.dumps(tuned), flush=True)
     for _ in range(args.warmup):
         frame()
     ctx.sync()
@@ -310,6 +330,7 @@ def main():
         "frame_alg_bytes": B,
         "frame_alg_GBs": round(B * fps / 1e9, 1),
         "kernel_ms": {k: round(v, 4) for k, v in sorted(per_kernel.items())},
+        "tiling": tuned,
         "roofline": roof,
     }
     if args.calib_copy:
@@ -317,10 +338,10 @@ def main():
         ctx.selftest_math(7, np.ones(n, np.float32))
         out["calib_copy_bytes"] = {"read": n * 4, "written": n * 4}
     ctx.close()
-    if args.batch:
+    if not args.no_batch:
         log = lambda m: print(m, flush=True)  # noqa: E731
         out["batched"], _, _ = batch_section(args, N, S, log)
-    if args.serve:
+    if not args.no_serve:
         log = lambda m: print(m, flush=True)  # noqa: E731
         out["served"] = serve_section(args, N, S, log)
     if adj is not None:

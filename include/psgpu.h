@@ -157,8 +157,8 @@ int psgpu_swap_buffers(psgpu_ctx* ctx);           /* swapBuffers: flush; display
  * All buffers are device memory; params = {albedo_scale, spec_exp}.
  * backward adds dLoss/dT into grad_tex (atomics) and dLoss/dparams into
  * grad_params, given grad_out = dLoss/dc; visibility stays that of the
- * geometry buffer.  stream: a hipStream_t to run on, or NULL for the
- * context's stream. */
+ * geometry buffer.  stream: the hipStream_t to run on (NULL is the null
+ * stream; pass psgpu_stream(ctx) for the context's own). */
 typedef struct {
     int width, height;          /* geometry buffer / image size */
     int tex_w, tex_h;           /* learnable texture size */

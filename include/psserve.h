@@ -10,7 +10,8 @@
  * Policy (one GPU, one context, one host thread):
  *  - arrivals join a FIFO; the head is admitted when its render targets
  *    (colour + depth, rounded up to a size class) plus a per-pixel scratch
- *    estimate fit in the HBM budget;
+ *    estimate fit in the HBM budget, and fewer than max_batch_views requests
+ *    are admitted but not yet launched;
  *  - whenever fewer than max_inflight batches are on the GPU, every admitted
  *    request not yet launched (up to max_batch_views / max_batch_pixels) is
  *    queued as one multi-view flush, followed by asynchronous copies of each

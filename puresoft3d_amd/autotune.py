@@ -14,7 +14,9 @@ LDS_LIMIT = 65536
 
 
 def lds_bytes(chunk_width, waves):
-    return waves * (chunk_width * 64 * 2 + 1024 + 128) * 4
+    """LDS of the raster kernel: per wave, depth and winner columns plus 128
+    staged pieces of 7 words and 128 words of counts (ps_kernels.hip RA_WORDS)"""
+    return waves * (chunk_width * 64 * 2 + 128 * 7 + 128) * 4
 
 
 def candidates(min_side=8, max_side=64, waves=(1, 2, 4, 8)):
@@ -36,7 +38,10 @@ def autotune(ctx, run, reps=5, cands=None, log=None):
     cands = cands or candidates()
     table = []
     for cw, tr, w in cands:
-        ctx.tiling(cw, tr, w)
+        try:
+            ctx.tiling(cw, tr, w)
+        except ValueError:    # refused by the library (LDS)
+            continue
         run()                 # capture / warm up
         ctx.sync()
         ts = []

@@ -164,14 +164,14 @@ int make_args(const psgpu_diff_desc* d, DiffArgs& A)
 
 }  // namespace
 
-// The context only supplies the default stream: these kernels keep no state
-// of their own and touch only the buffers they are given.
+// These kernels keep no state of their own and touch only the buffers they
+// are given; they run on the caller's stream (NULL is HIP's null stream).
 extern "C" int psgpu_diff_forward(psgpu_ctx* ctx, const psgpu_diff_desc* desc, const float* gbuf, const float* tex,
                                   const float* params, float* out_rgb, void* stream)
 {
     DiffArgs A;
     if (!ctx || !gbuf || !tex || !params || !out_rgb || make_args(desc, A)) return PSGPU_ERR_INVALID;
-    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)psgpu_stream(ctx);
+    hipStream_t s = (hipStream_t)stream;
     int64_t n = (int64_t)A.W * A.H;
     hipLaunchKernelGGL(k_diff_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A, gbuf, tex, params, out_rgb);
     return hipGetLastError() == hipSuccess ? PSGPU_OK : PSGPU_ERR_DEVICE;
@@ -184,7 +184,7 @@ extern "C" int psgpu_diff_backward(psgpu_ctx* ctx, const psgpu_diff_desc* desc, 
     DiffArgs A;
     if (!ctx || !gbuf || !tex || !params || !grad_out || !grad_tex || !grad_params || make_args(desc, A))
         return PSGPU_ERR_INVALID;
-    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)psgpu_stream(ctx);
+    hipStream_t s = (hipStream_t)stream;
     int64_t n = (int64_t)A.W * A.H;
     hipLaunchKernelGGL(k_diff_backward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A, gbuf, tex, params,
                        grad_out, grad_tex, grad_params);

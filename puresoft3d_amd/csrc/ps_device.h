@@ -221,6 +221,29 @@ PS_DEV void ps_span_varyings(const PsSpanGeom& g, const float* u0, const float* 
     }
 }
 
+// the same for a run-time number of varying floats (nv <= PS_MAX_NV)
+PS_DEV void ps_span_varyings_n(const PsSpanGeom& g, const float* u0, const float* u1, const float* u2, float* start,
+                               float* step, int nv)
+{
+    float rsc = g.step_count == 0 ? 1.0f : 1.0f / (float)g.step_count;
+    for (int i = 0; i < nv; i++) {
+        float a0 = u0[i] * g.ccL[0], a1 = u1[i] * g.ccL[1], a2 = u2[i] * g.ccL[2];
+        float s = (a0 + a1) + a2;
+        float b0 = u0[i] * g.ccR[0], b1 = u1[i] * g.ccR[1], b2 = u2[i] * g.ccR[2];
+        float e = (b0 + b1) + b2;
+        start[i] = s;
+        step[i] = (e - s) * rsc;
+    }
+    if (g.skip > 0) {
+        if (g.skip == 1) {
+            for (int i = 0; i < nv; i++) start[i] = start[i] + step[i];
+        } else {
+            float n = (float)g.skip;
+            for (int i = 0; i < nv; i++) start[i] = start[i] + step[i] * n;
+        }
+    }
+}
+
 // ---- textures (fbo.cpp:552-594, samplr2d.cpp, samplrcube.cpp) ----------
 PS_DEV int ps_lvl_w(const PsTex& t, int l) { int w = t.width >> l; return w < 1 ? 1 : w; }
 PS_DEV int ps_lvl_h(const PsTex& t, int l) { int h = t.height >> l; return h < 1 ? 1 : h; }
@@ -231,8 +254,14 @@ PS_DEV void ps_clamp_coord(int wrap, int max_row, int max_col, int& row, int& co
         if (row > max_row) row = max_row; else if (row < 0) row = 0;
         if (col > max_col) col = max_col; else if (col < 0) col = 0;
     } else {
-        if (max_row > 0) { row = row % max_row; if (row < 0) row += max_row; } else row = 0;
-        if (max_col > 0) { col = col % max_col; if (col < 0) col += max_col; } else col = 0;
+        // row % max_row is row itself for 0 <= row < max_row: the division
+        // only runs for coordinates outside that range
+        if (max_row > 0) {
+            if ((unsigned)row >= (unsigned)max_row) { row = row % max_row; if (row < 0) row += max_row; }
+        } else row = 0;
+        if (max_col > 0) {
+            if ((unsigned)col >= (unsigned)max_col) { col = col % max_col; if (col < 0) col += max_col; }
+        } else col = 0;
     }
 }
 

@@ -16,8 +16,8 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
               int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt);
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
-                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, uint32_t* keys, uint32_t* vals,
-                  int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
+                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, uint32_t* keys,
+                  uint32_t* vals, PsSeg* segs, float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
 int psk_rs_blocks(int64_t cap);
 void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
                     int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
@@ -25,9 +25,10 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
 void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
                 const PsCounters* c);
 size_t psk_raster_lds(int chunk_w, int waves);
-void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* kstart,
-                const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt);
-void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,
-               const PsDraw* draws, const PsFrame& f, const PsCounters* cnt);
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
+                const uint32_t* kstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
+                PsCounters* cnt);
+void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
+               const float* vary, int nvs, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt);
 void psk_mip(hipStream_t s, const uint8_t* src, int ws, int hs, int ss, uint8_t* dst, int wd, int hd, int elem);
 void psk_selftest(hipStream_t s, int op, const float* a, const float* b, float* out, int n);

@@ -262,54 +262,37 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_partials(uint32_t* partial, int nb, uint32_t* out, const uint32_t* n_ptr,
-                                                        int64_t n_host, uint32_t* total_out, int64_t cap, const PsCounters* c)
+// sum of partial[0, end) over a 256-thread block
+__device__ __forceinline__ uint32_t ps_block_sum_prefix(const uint32_t* partial, int end, uint32_t* lds)
 {
-    // single block: nb <= 1024 * 16
-    __shared__ uint32_t lds[16];
-    int per = (nb + 1023) / 1024;
-    uint32_t loc[16];
     uint32_t s = 0;
-    for (int i = 0; i < per; i++) {
-        int idx = threadIdx.x * per + i;
-        loc[i] = idx < nb ? partial[idx] : 0;
-        s += loc[i];
-    }
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = s;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) lds[wave] = x;
-    __syncthreads();
-    uint32_t wbase = 0, tot = 0;
-    for (int w = 0; w < 16; w++) { if (w < wave) wbase += lds[w]; tot += lds[w]; }
-    uint32_t run = wbase + x - s;
-    for (int i = 0; i < per; i++) {
-        int idx = threadIdx.x * per + i;
-        if (idx < nb) partial[idx] = run;
-        run += loc[i];
-    }
-    if (threadIdx.x == 0) {
-        int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-        bool ok = !(c && ps_poisoned(c)) && n <= cap;
-        if (ok) out[n] = tot;
-        if (total_out && ok) {
-            *total_out = tot;
-            atomicMax(total_out + 2, tot);   // largest total since the counters were reset
-        }
-    }
+    for (int i = threadIdx.x; i < end; i += 256) s += partial[i];
+    uint32_t tot;
+    ps_block_excl_scan(s, lds, tot);
+    return tot;
 }
 
+// block b: prefix of the block sums before it, then the block's own scan.
+// Block 0 also writes out[n] = total (and the running maximum of totals).
 __global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
-                                                    const uint32_t* __restrict__ partial, uint32_t* __restrict__ out,
-                                                    const PsCounters* c)
+                                                    const uint32_t* __restrict__ partial, int nb, uint32_t* __restrict__ out,
+                                                    uint32_t* total_out, int64_t cap, const PsCounters* c)
 {
     __shared__ uint32_t lds[4];
     if (c && ps_poisoned(c)) return;
     int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    if (blockIdx.x == 0) {
+        uint32_t tot = ps_block_sum_prefix(partial, nb, lds);
+        if (threadIdx.x == 0 && n <= cap) {
+            out[n] = tot;
+            if (total_out) {
+                *total_out = tot;
+                atomicMax(total_out + 2, tot);   // largest total since the counters were reset
+            }
+        }
+    }
     if (blockIdx.x * (int64_t)SCAN_BLOCK >= n) return;
+    uint32_t before = blockIdx.x ? ps_block_sum_prefix(partial, blockIdx.x, lds) : 0u;
     int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     ps_load_items(in, base, n, v);
@@ -317,7 +300,7 @@ __global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     uint32_t tot;
-    uint32_t run = partial[blockIdx.x] + ps_block_excl_scan(s, lds, tot);
+    uint32_t run = before + ps_block_excl_scan(s, lds, tot);
     if (base + SCAN_ITEMS <= n) {
         uint4* p = (uint4*)(out + base);
 #pragma unroll
@@ -413,6 +396,12 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
     }
 }
 
+// Spans at least this long keep varying checkpoints at each later piece, so
+// that shading never replays more than about this many pixels; shorter spans
+// are replayed from x1.
+#define PS_CK_MIN 48
+__device__ __forceinline__ bool ps_span_has_ck(const PsSpan& sp) { return sp.x2 - sp.x1 + 1 >= PS_CK_MIN; }
+
 // ---------------------------------------------------------------------------
 // segments: (bin key, span) pairs
 // ---------------------------------------------------------------------------
@@ -426,11 +415,17 @@ __device__ __forceinline__ uint32_t ps_bin_lane(const PsFrame& f, int y, int cx)
     return (uint32_t)((y % f.tile_rows) * f.tile_cols + cx % f.tile_cols);
 }
 
+// One thread per span: the span's pieces per bin chunk, in order.  The walk
+// from x1 applies the serial `+= step` of the pixel loop (interp.cpp:82-92)
+// pixel by pixel, so each piece starts with exactly the accumulators the
+// reference holds at its first pixel; spans of colour-writing draws that
+// cross chunks also keep the varying accumulators at each later piece.
 __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ seg_off,
                                                   const uint32_t* __restrict__ row_off, int64_t T, PsFrame f,
                                                   const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
-                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int64_t cap,
-                                                  PsCounters* cnt)
+                                                  const float* __restrict__ vary, int nvs, uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals, PsSeg* __restrict__ segs,
+                                                  float* __restrict__ ck, int64_t cap, PsCounters* cnt)
 {
     if (ps_poisoned(cnt)) return;
     int64_t R = row_off[T];
@@ -439,6 +434,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->overflow, 1);
         return;
     }
+    const int CW = f.chunk_w;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = base + threadIdx.x;
         unsigned long long nseg = 0;
@@ -446,12 +442,51 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
             uint32_t o = seg_off[i], e = seg_off[i + 1];
             if (o != e) {
                 const PsSpan sp = spans[i];
-                const PsView& v = f.views[draws[tris[sp.tri].code >> 8].view];
+                const PsTri tr = tris[sp.tri];
+                const PsDraw& d = draws[tr.code >> 8];
+                const PsView& v = f.views[d.view];
                 int y = sp.y & 0xffffff;
-                int cx = sp.x1 / f.chunk_w;
+                uint32_t flags = ((uint32_t)(sp.y >> 24) & 3u) << 26 | ((uint32_t)(sp.y >> 28) & 1u) << 28;
+                bool need_ck = (e - o > 1) && ((sp.y >> 28) & 1) && ps_span_has_ck(sp);
+                int nv = d.nv;
+                float vs[PS_MAX_NV], vt[PS_MAX_NV];
+                if (need_ck) {
+                    PsEdge L, Rr;
+                    ps_row_edges(tr, y, L, Rr);
+                    int left = ps_cvt_i32(ps_edge_eval(L, (float)y) + 0.5f);
+                    int right = ps_cvt_i32(ps_edge_eval(Rr, (float)y) + 0.5f);
+                    PsSpanGeom g;
+                    ps_span_geom(tr, y, left, right, L, Rr, g);
+                    const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
+                    ps_span_varyings_n(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt, nv);
+                }
+                float cf2 = sp.cf2, pz = sp.pz;
+                int x = sp.x1;
+                int cx = sp.x1 / CW;
                 for (uint32_t s = o; s < e; s++, cx++) {
+                    int xa = max(sp.x1, cx * CW), xb = min(sp.x2, cx * CW + CW - 1);
+                    for (; x < xa; x++) {
+                        cf2 += sp.cf2_step;
+                        pz += sp.pz_step;
+                        if (need_ck)
+                            for (int k = 0; k < nv; k++) vs[k] = vs[k] + vt[k];
+                    }
+                    PsSeg sg;
+                    sg.si = (uint32_t)i | flags;
+                    sg.xab = (uint32_t)xa | ((uint32_t)xb << 16);
+                    sg.cf2 = cf2;
+                    sg.pz = pz;
+                    sg.cst = sp.cf2_step;
+                    sg.pst = sp.pz_step;
+                    sg.y = y;
+                    sg.pad = 0;
+                    segs[s] = sg;
+                    if (need_ck && s > o) {
+                        float* c = ck + (int64_t)s * f.ck_stride;
+                        for (int k = 0; k < nv; k++) c[k] = vs[k];
+                    }
                     keys[s] = ps_bin_tile(f, v, y, cx);
-                    vals[s] = ((uint32_t)i << 6) | ps_bin_lane(f, y, cx);
+                    vals[s] = s;
                 }
                 nseg = e - o;
             }
@@ -467,44 +502,103 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
 #define RS_BLOCK (256 * RS_ITEMS)
 #define RS_MAXB 2048
 
+// per-block digit counts; blocks past the used part of the key array write nothing
 __global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int shift, int bits,
                                                  uint32_t* __restrict__ hist, int nb_cap, const PsCounters* c)
 {
     __shared__ uint32_t h[RS_MAXB];
+    if (ps_poisoned(c)) return;
+    int64_t n = *n_ptr;
+    int64_t base = blockIdx.x * (int64_t)RS_BLOCK;
+    if (base >= n) return;
     const int nbk = 1 << bits;
     const uint32_t mask = (uint32_t)nbk - 1u;
     for (int d = threadIdx.x; d < nbk; d += 256) h[d] = 0;
     __syncthreads();
-    int64_t n = *n_ptr;
-    int64_t base = blockIdx.x * (int64_t)RS_BLOCK;
-    if (!ps_poisoned(c) && base < n) {
-        for (int i = 0; i < RS_ITEMS; i++) {
-            int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
-            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
-        }
+    for (int i = 0; i < RS_ITEMS; i++) {
+        int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < nbk; d += 256) hist[(int64_t)d * nb_cap + blockIdx.x] = h[d];
 }
 
+// one wave per digit: exclusive prefix of the digit's counts over the used
+// blocks (into offs) and the digit's total (into totals[d])
+__global__ void __launch_bounds__(256) k_rs_digits(const uint32_t* __restrict__ hist, const uint32_t* n_ptr, int bits,
+                                                   int nb_cap, uint32_t* __restrict__ offs, uint32_t* __restrict__ totals,
+                                                   const PsCounters* c)
+{
+    if (ps_poisoned(c)) return;
+    int lane = threadIdx.x & 63;
+    int d = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (d >= (1 << bits)) return;
+    int64_t n = *n_ptr;
+    int nb = (int)((n + RS_BLOCK - 1) / RS_BLOCK);
+    const uint32_t* hrow = hist + (int64_t)d * nb_cap;
+    uint32_t* orow = offs + (int64_t)d * nb_cap;
+    uint32_t run = 0;
+    for (int b0 = 0; b0 < nb; b0 += 64) {
+        int b = b0 + lane;
+        uint32_t v = b < nb ? hrow[b] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (b < nb) orow[b] = run + x - v;
+        run += __shfl(x, 63, 64);
+    }
+    if (lane == 0) totals[d] = run;
+}
+
 template <int BITS>
 __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     const uint32_t* n_ptr, int shift, const uint32_t* __restrict__ offs,
-                                                    int nb_cap, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                    const uint32_t* __restrict__ totals, int nb_cap,
+                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                     const PsCounters* c)
 {
     constexpr int NBK = 1 << BITS;
+    constexpr int PER = (NBK + 255) / 256;   // digits per thread in the digit prefix
     __shared__ uint32_t cntl[4][NBK];
     __shared__ uint32_t base[NBK];
+    __shared__ uint32_t wsum[4];
     if (ps_poisoned(c)) return;
     int64_t n = *n_ptr;
     int64_t b0 = blockIdx.x * (int64_t)RS_BLOCK;
     if (b0 >= n) return;
     int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (int d = tid; d < NBK; d += 256) {
-        cntl[0][d] = cntl[1][d] = cntl[2][d] = cntl[3][d] = 0;
-        base[d] = offs[(int64_t)d * nb_cap + blockIdx.x];
+    // exclusive prefix of the digit totals: thread t owns digits [t*PER, t*PER+PER)
+    {
+        uint32_t tv[PER];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            int d = tid * PER + k;
+            tv[k] = d < NBK ? totals[d] : 0u;
+            sum += tv[k];
+        }
+        uint32_t x = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (int w = 0; w < wave; w++) wbase += wsum[w];
+        uint32_t run = wbase + x - sum;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            int d = tid * PER + k;
+            if (d < NBK) base[d] = run + offs[(int64_t)d * nb_cap + blockIdx.x];
+            run += tv[k];
+        }
     }
+    for (int d = tid; d < NBK; d += 256) cntl[0][d] = cntl[1][d] = cntl[2][d] = cntl[3][d] = 0;
     __syncthreads();
     uint32_t k[RS_ITEMS], v[RS_ITEMS], rank[RS_ITEMS];
     uint64_t lt = ps_lt_mask();
@@ -571,12 +665,19 @@ __global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ key
 // ---------------------------------------------------------------------------
 // raster: ordered depth test per bin
 // ---------------------------------------------------------------------------
-#define RA_CHUNK 1024   // tile-list entries split per pass
+#define RA_CHUNK 128    // tile-list entries staged in LDS per pass
+#define RA_PER (RA_CHUNK / 64)
+
+// LDS of one wave: depth and winner columns [c][lane] (bank = lane), then the
+// staged pieces of the current chunk (SoA, in per-lane order), then per-lane
+// counts and offsets
+#define RA_WORDS(cw) ((cw) * 64 * 2 + RA_CHUNK * 7 + 128)
 
 template <bool DEBUG>
-__global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ vals,
-                         const uint32_t* __restrict__ tstart, const PsTri* __restrict__ tris,
-                         const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles, PsCounters* cnt)
+__global__ void __launch_bounds__(512) k_raster(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
+                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
+                         const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles,
+                         PsCounters* cnt)
 {
     extern __shared__ uint32_t lds[];
     if (ps_poisoned(cnt)) return;
@@ -585,15 +686,20 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
     int waves = blockDim.x >> 6;
     int64_t tile = blockIdx.x * (int64_t)waves + wave;
     if (tile >= n_tiles) return;
-    // LDS per wave: depth and winner columns [c][lane] (bank = lane), the
-    // per-lane span lists of the current chunk, per-lane counts and offsets
-    uint32_t* wl = lds + (size_t)wave * (CW * 64 * 2 + RA_CHUNK + 128);
+    uint32_t* wl = lds + (size_t)wave * RA_WORDS(CW);
     float* dep = (float*)wl;
     int* win = (int*)(dep + CW * 64);
-    uint32_t* list = (uint32_t*)(win + CW * 64);
-    uint32_t* lcnt = list + RA_CHUNK;
+    uint32_t* r_s = (uint32_t*)(win + CW * 64);   // segment index
+    uint32_t* r_fl = r_s + RA_CHUNK;               // behaviour | writes colour (PsSeg.si high bits)
+    uint32_t* r_x = r_fl + RA_CHUNK;               // xa | xb << 16
+    float* r_cf2 = (float*)(r_x + RA_CHUNK);
+    float* r_pz = r_cf2 + RA_CHUNK;
+    float* r_cst = r_pz + RA_CHUNK;
+    float* r_pst = r_cst + RA_CHUNK;
+    uint32_t* lcnt = (uint32_t*)(r_pst + RA_CHUNK);
     uint32_t* loff = lcnt + 64;
 
+    const int cw_shift = __builtin_ctz((unsigned)CW), tc_shift = __builtin_ctz((unsigned)f.tile_cols);
     const PsView& V = f.views[ps_view_of_tile(f, tile)];
     int64_t ltile = tile - V.tile_base;
     int ty = (int)(ltile / V.n_tiles_x), tx = (int)(ltile - (int64_t)ty * V.n_tiles_x);
@@ -612,11 +718,33 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
     unsigned long long passes = 0;
     bool dirty = false;
     const uint64_t lt = ps_lt_mask();
-    const int X1 = X0 + ncol - 1;
     uint32_t s0 = tstart[tile], s1 = tstart[tile + 1];
     for (uint32_t cb = s0; cb < s1; cb += RA_CHUNK) {
         uint32_t ce = min(s1, cb + RA_CHUNK);
-        // stable split of [cb, ce) by lane: count, offsets, place
+        // gather the chunk's pieces: RA_PER entries per lane, all loads in flight
+        uint32_t v[RA_PER], key[RA_PER];
+        uint4 ra[RA_PER], rb[RA_PER];
+#pragma unroll
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + k * 64 + lane;
+            v[k] = e < ce ? vals[e] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + k * 64 + lane;
+            if (e < ce) {
+                const uint4* p = (const uint4*)(segs + v[k]);
+                ra[k] = p[0];
+                rb[k] = p[1];
+            } else {
+                ra[k] = make_uint4(0, 0, 0, 0);
+                rb[k] = make_uint4(0, 0, 0, 0);
+            }
+            // the piece's lane: its row inside the tile and its chunk inside the row
+            int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
+            key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
+        }
+        // stable split by lane key: count, offsets, place
         lcnt[lane] = 0;
         __builtin_amdgcn_wave_barrier();
         for (int pass = 0; pass < 2; pass++) {
@@ -632,62 +760,74 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
                 lcnt[lane] = 0;
                 __builtin_amdgcn_wave_barrier();
             }
-            for (uint32_t rb = cb; rb < ce; rb += 64) {
-                uint32_t e = rb + lane;
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                uint32_t e = cb + k * 64 + lane;
                 bool valid = e < ce;
-                uint32_t v = valid ? vals[e] : 0u;
-                uint32_t key = v & 63u;
+                uint32_t kk = key[k];
                 uint64_t same = __ballot(valid);
 #pragma unroll
                 for (int bit = 0; bit < 6; bit++) {
-                    uint64_t m = __ballot((key >> bit) & 1u);
-                    same &= ((key >> bit) & 1u) ? m : ~m;
+                    uint64_t m = __ballot((kk >> bit) & 1u);
+                    same &= ((kk >> bit) & 1u) ? m : ~m;
                 }
                 uint32_t before = (uint32_t)__popcll(same & lt);
-                uint32_t cur = lcnt[key];
-                if (pass == 1 && valid) list[loff[key] + cur + before] = v >> 6;
+                uint32_t cur = lcnt[kk];
+                if (pass == 1 && valid) {
+                    uint32_t q = loff[kk] + cur + before;
+                    r_s[q] = v[k];
+                    r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
+                    r_x[q] = ra[k].y;
+                    r_cf2[q] = __uint_as_float(ra[k].z);
+                    r_pz[q] = __uint_as_float(ra[k].w);
+                    r_cst[q] = __uint_as_float(rb[k].x);
+                    r_pst[q] = __uint_as_float(rb[k].y);
+                }
                 __builtin_amdgcn_wave_barrier();
-                if (valid && before == 0) lcnt[key] = cur + (uint32_t)__popcll(same);
+                if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
                 __builtin_amdgcn_wave_barrier();
             }
         }
-        if (!inside) continue;
-        uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
-        for (uint32_t q = l0; q < l1; q++) {
-            uint32_t si = list[q];
-            const PsSpan sp = spans[si];
-            int beh = (sp.y >> 24) & 3;
-            bool wc = (sp.y >> 28) & 1;
-            int a = max(sp.x1, X0), b = min(sp.x2, X1);
-            float cf2 = sp.cf2, pz = sp.pz;
-            const float cst = sp.cf2_step, pst = sp.pz_step;
-            for (int x = sp.x1; x < a; x++) { cf2 += cst; pz += pst; }
-            uint32_t th = 0;
-            int tid = 0;
-            if (DEBUG) {
-                const PsTri tr = tris[sp.tri];
-                const PsDraw& d = draws[tr.code >> 8];
-                tid = (int)(d.id_base + (sp.tri - d.tri_start));
-                uint32_t h = (uint32_t)tid;
-                h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
-                th = h;
-            }
-            for (int x = a; x <= b; x++) {
-                // interpolateNextStep (interp.cpp:82-92)
-                float r = 1.0f / cf2;
-                cf2 += cst;
-                float z = pz * r;
-                pz += pst;
-                int c = x - X0;
-                float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
-                int64_t pix = (int64_t)y * VW + x;
-                if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
-                // fragthrd.cpp:227
-                if (-1.0f < z && (z - cur < -0.0001f)) {
-                    passes++;
-                    if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
-                    if (wc) win[c * 64 + lane] = (int)si;
-                    if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+        if (inside) {
+            uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
+            for (uint32_t q = l0; q < l1; q++) {
+                int sidx = (int)r_s[q];
+                uint32_t fl = r_fl[q];
+                int beh = (fl >> 26) & 3;
+                bool wc = (fl >> 28) & 1;
+                uint32_t xab = r_x[q];
+                int a = (int)(xab & 0xffffu), b = (int)(xab >> 16);
+                float cf2 = r_cf2[q], pz = r_pz[q];
+                const float cst = r_cst[q], pst = r_pst[q];
+                uint32_t th = 0;
+                int tid = 0;
+                if (DEBUG) {
+                    int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
+                    int tri = spans[si].tri;
+                    const PsTri tr = tris[tri];
+                    const PsDraw& d = draws[tr.code >> 8];
+                    tid = (int)(d.id_base + (tri - d.tri_start));
+                    uint32_t h = (uint32_t)tid;
+                    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+                    th = h;
+                }
+                for (int x = a; x <= b; x++) {
+                    // interpolateNextStep (interp.cpp:82-92)
+                    float r = 1.0f / cf2;
+                    cf2 += cst;
+                    float z = pz * r;
+                    pz += pst;
+                    int c = x - X0;
+                    float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
+                    int64_t pix = (int64_t)y * VW + x;
+                    if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                    // fragthrd.cpp:227
+                    if (-1.0f < z && (z - cur < -0.0001f)) {
+                        passes++;
+                        if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
+                        if (wc) win[c * 64 + lane] = sidx;
+                        if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                    }
                 }
             }
         }
@@ -705,7 +845,8 @@ __global__ void k_raster(const PsSpan* __restrict__ spans, const uint32_t* __res
 // shade: deferred FP of the last colour-writing pass per pixel
 // ---------------------------------------------------------------------------
 template <int PROG>
-__global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans, const PsTri* __restrict__ tris,
+__global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
+                                               const float* __restrict__ ck, const PsTri* __restrict__ tris,
                                                const float* __restrict__ vary, int nvs,
                                                const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt)
 {
@@ -719,7 +860,8 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
     if (x >= V.W || y >= V.H) return;
     int w = V.winner[(int64_t)y * V.W + x];
     if (w < 0) return;
-    const PsSpan sp = spans[w];
+    const PsSeg sg = segs[w];
+    const PsSpan sp = spans[sg.si & PS_SEG_SPAN_MASK];
     const PsTri tr = tris[sp.tri];
     const PsDraw& d = draws[tr.code >> 8];
     if (d.prog != PROG) return;
@@ -732,8 +874,19 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
     const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
     float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
     ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
+    // the piece's first pixel carries the accumulators as the serial walk from
+    // x1 leaves them (checkpoint); walk on from there to the pixel
+    int xs = sp.x1;
     float cf2 = sp.cf2;
-    for (int k = sp.x1; k < x; k++) {
+    int xa = (int)(sg.xab & 0xffffu);
+    if (xa > sp.x1 && ps_span_has_ck(sp)) {
+        const float* c = ck + (int64_t)w * f.ck_stride;
+#pragma unroll
+        for (int i = 0; i < P::NV; i++) vs[i] = c[i];
+        xs = xa;
+        cf2 = sg.cf2;
+    }
+    for (int k = xs; k < x; k++) {
         cf2 += sp.cf2_step;
 #pragma unroll
         for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
@@ -836,8 +989,7 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
 {
     int nb = psk_scan_blocks(cap);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb, out, n_dev, n_host, total_out, cap, c);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, out, c);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, out, total_out, cap, c);
 }
 
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
@@ -854,11 +1006,12 @@ void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* 
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
-                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, uint32_t* keys, uint32_t* vals,
-                  int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
+                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, uint32_t* keys,
+                  uint32_t* vals, PsSeg* segs, float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
 {
     int g = gsz(span_cap, 256); if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, tris, draws, keys, vals, seg_cap, cnt);
+    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, tris, draws, vary, nvs, keys,
+                       vals, segs, ck, seg_cap, cnt);
 }
 
 int psk_rs_blocks(int64_t cap) { return gsz(cap, RS_BLOCK); }
@@ -872,11 +1025,13 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
     if (passes < 1) passes = 1;
     int bits = (key_bits + passes - 1) / passes;
     uint32_t *ki = keys, *vi = vals, *ko = keys2, *vo = vals2;
+    uint32_t* totals = hist_off + ((int64_t)nb << 11);
     for (int p = 0, shift = 0; p < passes; p++, shift += bits) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(256), 0, s, ki, n_dev, shift, bits, hist, nb, c);
-        psk_scan(s, hist, nullptr, (int64_t)nb << bits, (int64_t)nb << bits, hist_off, partial, nullptr, c);
+        hipLaunchKernelGGL(k_rs_digits, dim3(((1 << bits) + 3) / 4), dim3(256), 0, s, hist, n_dev, bits, nb, hist_off,
+                           totals, c);
         switch (bits) {
-#define RS_CASE(B) case B: hipLaunchKernelGGL(k_rs_scatter<B>, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, nb, ko, vo, c); break;
+#define RS_CASE(B) case B: hipLaunchKernelGGL(k_rs_scatter<B>, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, totals, nb, ko, vo, c); break;
         RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4) RS_CASE(5) RS_CASE(6) RS_CASE(7) RS_CASE(8) RS_CASE(9) RS_CASE(10) RS_CASE(11)
 #undef RS_CASE
         }
@@ -894,31 +1049,30 @@ void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int6
     hipLaunchKernelGGL(k_bounds, dim3(g), dim3(256), 0, s, keys, n_dev, n_keys, start, c);
 }
 
-size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * (chunk_w * 64 * 2 + RA_CHUNK + 128) * 4; }
+size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * RA_WORDS(chunk_w) * 4; }
 
-void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const uint32_t* vals, const uint32_t* tstart,
-                const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt)
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
+                const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
+                PsCounters* cnt)
 {
     int64_t n_tiles = f.n_tiles;
     size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
     dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
-    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, vals, tstart, tris, draws, f, n_tiles, cnt);
-    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, vals, tstart, tris, draws, f, n_tiles, cnt);
+    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, segs, vals, tstart, tris, draws, f, n_tiles, cnt);
+    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, segs, vals, tstart, tris, draws, f, n_tiles, cnt);
 }
 
-void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsTri* tris, const float* vary, int nvs,
-               const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
+void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
+               const float* vary, int nvs, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
 {
     dim3 g((unsigned)f.n_stiles), b(256);
+#define SH_CASE(K) case K: hipLaunchKernelGGL(k_shade<K>, g, b, 0, s, spans, segs, ck, tris, vary, nvs, draws, f, cnt); break;
     switch (prog) {
-    case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF01>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
-    case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF02>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
-    case PSGPU_PROC_DEF03: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF03>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
-    case PSGPU_PROC_DEF04: hipLaunchKernelGGL(k_shade<PSGPU_PROC_DEF04>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
-    case PSGPU_PROC_LAMBERT: hipLaunchKernelGGL(k_shade<PSGPU_PROC_LAMBERT>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
-    case PSGPU_PROC_BLINN: hipLaunchKernelGGL(k_shade<PSGPU_PROC_BLINN>, g, b, 0, s, spans, tris, vary, nvs, draws, f, cnt); break;
+    SH_CASE(PSGPU_PROC_DEF01) SH_CASE(PSGPU_PROC_DEF02) SH_CASE(PSGPU_PROC_DEF03) SH_CASE(PSGPU_PROC_DEF04)
+    SH_CASE(PSGPU_PROC_LAMBERT) SH_CASE(PSGPU_PROC_BLINN)
     default: break;
     }
+#undef SH_CASE
 }
 
 void psk_mip(hipStream_t s, const uint8_t* src, int ws, int hs, int ss, uint8_t* dst, int wd, int hd, int elem)

@@ -62,6 +62,19 @@ struct PsSpan {
     float cf2, cf2_step, pz, pz_step;
 };
 
+// The piece of a span inside one bin chunk, with the serial accumulators
+// (correction factor, depth numerator) as they stand at its first pixel xa.
+// si packs the span index (bits 0-25), the depth behaviour bits (26-27) and
+// whether the draw writes colour (28).
+#define PS_SEG_SPAN_MASK 0x3ffffffu
+struct PsSeg {
+    uint32_t si;
+    uint32_t xab;   // xa | xb << 16 (first and last pixel of the piece)
+    float cf2, pz, cst, pst;
+    int y;          // row
+    uint32_t pad;
+};
+
 // One view of a flush: a viewport and its colour/depth targets.  A flush
 // renders any number of views with the same kernels; tiles and shading
 // blocks of all views are numbered consecutively.
@@ -73,13 +86,14 @@ struct PsView {
     int64_t stile_base;             // first shading block of the view
     uint32_t* colour; int colour_scan; int colour_top_down; int colour_h;  // scan in uint32
     float* depth; int depth_scan;   // scan in floats
-    int* winner;                    // W*H: span index of the last colour-writing pass
+    int* winner;                    // W*H: segment index of the last colour-writing pass
     float* gbuf;                    // optional W*H*12 floats: N.xyz 1 | wp.xyz 0 | u v 0 0 of the winner
 };
 
 // Flush-wide parameters passed by value to kernels.
 struct PsFrame {
     int chunk_w, tile_rows, tile_cols;   // raster tiling
+    int ck_stride;                       // floats per varying checkpoint (the flush's widest NV)
     int n_views;
     int64_t n_tiles, n_stiles;
     const PsView* views;

@@ -171,6 +171,8 @@ struct psgpu_ctx {
     DevBuf<uint32_t> d_segcount, d_segoff;
     DevBuf<uint32_t> d_keys, d_vals, d_keys2, d_vals2, d_hist, d_histoff, d_kstart;
     DevBuf<int> d_winner;
+    DevBuf<PsSeg> d_segs;
+    DevBuf<float> d_ck;
     int64_t span_cap = 0, seg_cap = 0;
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
@@ -461,16 +463,19 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         c->span_cap = std::max<int64_t>(1 << 20, 6 * T);
         c->seg_cap = c->span_cap * 3 / 2;
     }
-    if (c->span_cap > ((int64_t)1 << 26)) return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans in one flush");
-    int64_t scan_cap = std::max<int64_t>(std::max<int64_t>(T, c->span_cap), (int64_t)psk_rs_blocks(c->seg_cap) * 2048);
+    if (c->span_cap > ((int64_t)1 << 26) || c->seg_cap > ((int64_t)1 << 31))
+        return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans or 2^31 pieces in one flush");
+    int64_t scan_cap = std::max<int64_t>(T, c->span_cap);
     if ((rc = ensure(c, c->d_partial, psk_scan_blocks(scan_cap) + 1)) || (rc = ensure(c, c->d_spantri, c->span_cap)) ||
         (rc = ensure(c, c->d_spans, c->span_cap)) || (rc = ensure(c, c->d_segcount, c->span_cap)) ||
         (rc = ensure(c, c->d_segoff, c->span_cap + 1)) || (rc = ensure(c, c->d_keys, c->seg_cap)) ||
         (rc = ensure(c, c->d_vals, c->seg_cap)) || (rc = ensure(c, c->d_keys2, c->seg_cap)) ||
-        (rc = ensure(c, c->d_vals2, c->seg_cap)))
+        (rc = ensure(c, c->d_vals2, c->seg_cap)) || (rc = ensure(c, c->d_segs, c->seg_cap)) ||
+        (rc = ensure(c, c->d_ck, (size_t)c->seg_cap * nvs)))
         return rc;
+    f.ck_stride = nvs;
     int nbr = psk_rs_blocks(c->seg_cap);
-    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 1)) ||
+    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 2048)) ||
         (rc = ensure(c, c->d_kstart, f.n_tiles + 1)))
         return rc;
     if (c->debug) {
@@ -509,8 +514,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     {
         ProfScope p(c, "segments");
-        psk_segments(s, c->d_spans.p, c->d_segoff.p, c->d_rowoff.p, T, f, c->d_tris.p, dd, c->d_keys.p, c->d_vals.p,
-                     c->span_cap, c->seg_cap, c->d_cnt);
+        psk_segments(s, c->d_spans.p, c->d_segoff.p, c->d_rowoff.p, T, f, c->d_tris.p, dd, c->d_vary.p, nvs,
+                     c->d_keys.p, c->d_vals.p, c->d_segs.p, c->d_ck.p, c->span_cap, c->seg_cap, c->d_cnt);
     }
     bool second = false;
     {
@@ -526,12 +531,14 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     {
         ProfScope p(c, "raster");
-        psk_raster(s, c->debug, c->d_spans.p, svals, c->d_kstart.p, c->d_tris.p, dd, f, c->tiling.waves_per_group, c->d_cnt);
+        psk_raster(s, c->debug, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
+                   c->tiling.waves_per_group, c->d_cnt);
     }
     {
         ProfScope p(c, "shade");
         for (int k = 1; k < 8; k++)
-            if (shade_progs[k]) psk_shade(s, k, c->d_spans.p, c->d_tris.p, c->d_vary.p, nvs, dd, f, c->d_cnt);
+            if (shade_progs[k])
+                psk_shade(s, k, c->d_spans.p, c->d_segs.p, c->d_ck.p, c->d_tris.p, c->d_vary.p, nvs, dd, f, c->d_cnt);
     }
     HIPCHK(c, hipGetLastError());
     c->stats.flushes++;
@@ -848,6 +855,7 @@ int psgpu_destroy(psgpu_ctx* c)
     fr(c->d_blob.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
     fr(c->d_spantri.p); fr(c->d_spans.p); fr(c->d_segcount.p); fr(c->d_segoff.p); fr(c->d_keys.p); fr(c->d_vals.p);
     fr(c->d_keys2.p); fr(c->d_vals2.p); fr(c->d_hist.p); fr(c->d_histoff.p); fr(c->d_kstart.p); fr(c->d_winner.p);
+    fr(c->d_segs.p); fr(c->d_ck.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
     for (int i = 0; i < psgpu_ctx::kStage; i++) {
         if (c->h_stage[i]) hipHostFree(c->h_stage[i]);

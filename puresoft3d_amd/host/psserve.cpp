@@ -224,7 +224,8 @@ struct Server {
                 res[next].arrival_s = reqs[next].arrival_s;
                 waiting.push_back(next++);
             }
-            while (!waiting.empty()) {
+            // targets are held only by requests the next batch can take
+            while (!waiting.empty() && (int)admitted.size() < cfg.max_batch_views) {
                 int ok = 0;
                 if ((rc = acquire(waiting.front(), &ok))) return rc;
                 if (!ok) break;

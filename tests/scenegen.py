@@ -131,3 +131,25 @@ def behavior_scene(seed, width=96, height=64, n=60):
             sc.draws.append(S.DrawSpec(name, prog, behavior=beh))
     sc.draws.insert(0, S.DrawSpec("m", N.PROC_BLINN))
     return sc
+
+
+def big_triangle_scene(prog, seed, width=512, height=384, n=12):
+    """a few triangles spanning most of the screen, overlapping at varying
+    depths: spans hundreds of pixels long cross many bin chunks, so every
+    chunk after the first starts from the serial accumulators carried
+    across the span"""
+    rng = np.random.default_rng(seed)
+    sc = micro_scene(prog, seed, width, height, n=12)
+    sc.name = "big_p%d_s%d_%dx%d" % (prog, seed, width, height)
+    P = np.zeros((n, 3, 4), np.float32)
+    for t in range(n):
+        xy = rng.uniform(-1.6, 1.6, (3, 2))
+        z = rng.uniform(-0.9, 0.9, 3)
+        w = rng.uniform(0.6, 1.8, 3)
+        P[t, :, 0] = xy[:, 0] * w
+        P[t, :, 1] = xy[:, 1] * w
+        P[t, :, 2] = z * w
+        P[t, :, 3] = w
+    sc.meshes["m"] = _slots_for(prog, P.reshape(-1, 4), rng)
+    sc.draws[0].behavior = N.UPDATE_DEPTH | N.TEST_DEPTH   # both windings drawn
+    return sc

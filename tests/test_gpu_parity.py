@@ -263,3 +263,13 @@ def test_batched_views_graph_replay(oracle):
 @pytest.mark.parametrize("seed", [1, 2])
 def test_behaviors_bitexact(oracle, seed):
     assert_same(scenegen.behavior_scene(seed), oracle)
+
+
+@pytest.mark.parametrize("prog", [N.PROC_DEF02, N.PROC_DEF03, N.PROC_BLINN])
+@pytest.mark.parametrize("tiling", [None, (4, 16, 2), (64, 64, 1)])
+def test_long_spans_bitexact(oracle, prog, tiling):
+    """spans of hundreds of pixels: the per-chunk pieces start from the
+    accumulators of the serial walk, for depth and for every varying"""
+    sc = scenegen.big_triangle_scene(prog, 5)
+    st = assert_same(sc, oracle, tiling=tiling)
+    assert st["segments"] > 4 * st["spans"] // 3
