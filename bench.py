@@ -216,10 +216,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-loads", type=float, nargs="+", default=[3.0],
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[1.5, 2.0, 3.0],
                     help="offered request rates as multiples of the plain path's views/s")
     ap.add_argument("--serve-batch-views", type=int, default=64)
-    ap.add_argument("--serve-batch-pixels", type=int, default=48, metavar="MPIX")
+    ap.add_argument("--serve-batch-pixels", type=int, default=16, metavar="MPIX")
     ap.add_argument("--serve-budget-gib", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--frame-only", action="store_true", help="only the 1080p frame (and the CPU baseline)")

@@ -103,16 +103,20 @@ PS_DEV bool ps_back_face(const float* v0, const float* v1, const float* v2)
     return ps_dot4(test, c) < 0.0f;
 }
 
+// element i (0..2) of a 3-array by selects: a run-time index into a small
+// local array would otherwise send the array to LDS or scratch memory
+PS_DEV float ps_sel3(const float* a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
 // ---- rasterizer edges (rasterizer.cpp:95-120) --------------------------
 struct PsEdge { float dx, dy, x0, y0; int i0, i1; };
 
 PS_DEV PsEdge ps_edge(const float* vx, const float* vy, int i0, int i1)
 {
     PsEdge e;
-    e.x0 = vx[i0];
-    e.y0 = vy[i0];
-    e.dx = vx[i1] - e.x0;
-    e.dy = vy[i1] - e.y0;
+    e.x0 = ps_sel3(vx, i0);
+    e.y0 = ps_sel3(vy, i0);
+    e.dx = ps_sel3(vx, i1) - e.x0;
+    e.dy = ps_sel3(vy, i1) - e.y0;
     e.i0 = i0;
     e.i1 = i1;
     if (fabsf(e.dy) < 0.000001f) e.dy = FLT_MAX;
@@ -152,8 +156,9 @@ struct PsSpanGeom {
 // lineSegmentlinearInterpolate, interp.cpp:151-160
 PS_DEV void ps_line_contrib(const float* vx, const float* vy, int v1, int v2, float x, float y, float* c)
 {
-    float dx = vx[v1] - vx[v2], dy = vy[v1] - vy[v2];
-    float cv = fabsf(dx) > fabsf(dy) ? ((x - vx[v2]) / dx) : ((y - vy[v2]) / dy);
+    float x1 = ps_sel3(vx, v1), x2 = ps_sel3(vx, v2), y1 = ps_sel3(vy, v1), y2 = ps_sel3(vy, v2);
+    float dx = x1 - x2, dy = y1 - y2;
+    float cv = fabsf(dx) > fabsf(dy) ? ((x - x2) / dx) : ((y - y2) / dy);
     float o = 1.0f - cv;
     // c[v1] = cv, c[v2] = 1 - cv, c[other] = 0
     c[0] = (v1 == 0) ? cv : ((v2 == 0) ? o : 0.0f);
@@ -226,7 +231,9 @@ PS_DEV void ps_span_varyings_n(const PsSpanGeom& g, const float* u0, const float
                                float* step, int nv)
 {
     float rsc = g.step_count == 0 ? 1.0f : 1.0f / (float)g.step_count;
-    for (int i = 0; i < nv; i++) {
+#pragma unroll
+    for (int i = 0; i < PS_MAX_NV; i++) {
+        if (i >= nv) break;
         float a0 = u0[i] * g.ccL[0], a1 = u1[i] * g.ccL[1], a2 = u2[i] * g.ccL[2];
         float s = (a0 + a1) + a2;
         float b0 = u0[i] * g.ccR[0], b1 = u1[i] * g.ccR[1], b2 = u2[i] * g.ccR[2];
@@ -236,10 +243,14 @@ PS_DEV void ps_span_varyings_n(const PsSpanGeom& g, const float* u0, const float
     }
     if (g.skip > 0) {
         if (g.skip == 1) {
-            for (int i = 0; i < nv; i++) start[i] = start[i] + step[i];
+#pragma unroll
+            for (int i = 0; i < PS_MAX_NV; i++)
+                if (i < nv) start[i] = start[i] + step[i];
         } else {
             float n = (float)g.skip;
-            for (int i = 0; i < nv; i++) start[i] = start[i] + step[i] * n;
+#pragma unroll
+            for (int i = 0; i < PS_MAX_NV; i++)
+                if (i < nv) start[i] = start[i] + step[i] * n;
         }
     }
 }
@@ -304,7 +315,20 @@ PS_DEV uint32_t ps_sample_cube(const PsTex& t, const float* d)
     return ps_fetch(t, face, 0, row, col);
 }
 
-// bilinear filter: texel j covers u = j/W (the nearest sampler's rounding rule)
+// one coordinate of clampCoord (fbo.cpp:552-594)
+PS_DEV int ps_clamp_1(int wrap, int maxv, int v)
+{
+    if (!wrap) return v > maxv ? maxv : (v < 0 ? 0 : v);
+    if (maxv <= 0) return 0;
+    if ((unsigned)v >= (unsigned)maxv) {   // v % maxv is v itself inside [0, maxv)
+        v = v % maxv;
+        if (v < 0) v += maxv;
+    }
+    return v;
+}
+
+// bilinear filter: texel j covers u = j/W (the nearest sampler's rounding rule).
+// Rows and columns are addressed once each; the four texels share them.
 PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, float* out)
 {
     int w = ps_lvl_w(t, level), h = ps_lvl_h(t, level);
@@ -313,10 +337,16 @@ PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, floa
     float ax = fx - x0, ay = fy - y0;
     int ix = ps_cvt_i32(x0), iy = ps_cvt_i32(y0);
     int ix1 = (int)((unsigned)ix + 1u), iy1 = (int)((unsigned)iy + 1u);
-    uint32_t a = ps_fetch(t, 0, level, iy, ix);
-    uint32_t b = ps_fetch(t, 0, level, iy, ix1);
-    uint32_t c = ps_fetch(t, 0, level, iy1, ix);
-    uint32_t d = ps_fetch(t, 0, level, iy1, ix1);
+    int r0 = ps_clamp_1(t.wrap, h - 1, iy), r1 = ps_clamp_1(t.wrap, h - 1, iy1);
+    int c0 = ps_clamp_1(t.wrap, w - 1, ix), c1 = ps_clamp_1(t.wrap, w - 1, ix1);
+    int scan = level == 0 ? t.scanline : w * t.elem_len;
+    const uint8_t* base = t.data + t.level_off[level];
+    const uint8_t* row0 = base + (int64_t)r0 * scan;
+    const uint8_t* row1 = base + (int64_t)r1 * scan;
+    uint32_t a = *(const uint32_t*)(row0 + c0 * 4);
+    uint32_t b = *(const uint32_t*)(row0 + c1 * 4);
+    uint32_t c = *(const uint32_t*)(row1 + c0 * 4);
+    uint32_t d = *(const uint32_t*)(row1 + c1 * 4);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         float fa = (float)((a >> (8 * k)) & 255u), fb = (float)((b >> (8 * k)) & 255u);

@@ -62,6 +62,17 @@ __device__ __forceinline__ int ps_view_of_stile(const PsFrame& f, int64_t t)
     return lo;
 }
 
+// f(draw record): when every active lane of the wave uses the same draw, the
+// record is addressed through a wave-uniform index, so its fields come in
+// with scalar loads instead of one vector load per lane
+template <typename F>
+__device__ __forceinline__ void ps_with_draw(const PsDraw* __restrict__ draws, int di, F&& f)
+{
+    int d0 = __builtin_amdgcn_readfirstlane(di);
+    if (__all(di == d0)) f(draws[d0]);
+    else f(draws[di]);
+}
+
 // ---------------------------------------------------------------------------
 // fills (graph friendly: no hipMemset nodes)
 // ---------------------------------------------------------------------------
@@ -108,10 +119,9 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool live = t < T;
     int di = live ? ps_find_draw(draws, nd, t) : 0;
-    const PsDraw& d = draws[di];
-    live = live && d.prog == PROG;
     unsigned culled = 0, zrej = 0, dropped = 0;
-    if (live) {
+    ps_with_draw(draws, di, [&](const PsDraw& d) {
+    if (live && d.prog == PROG) {
         const int W = views[d.view].W, H = views[d.view].H;
         int64_t lt = t - d.tri_start;
         float pos[3][4];
@@ -172,13 +182,14 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
                 } else {
                     int top, bottom, third = 2;
                     if (vy[0] > vy[1]) { top = 0; bottom = 1; } else { top = 1; bottom = 0; }
-                    if (vy[top] < vy[third]) { int s = top; top = third; third = s; }
-                    else if (vy[bottom] > vy[third]) { int s = bottom; bottom = third; third = s; }
+                    if (ps_sel3(vy, top) < ps_sel3(vy, third)) { int s = top; top = third; third = s; }
+                    else if (ps_sel3(vy, bottom) > ps_sel3(vy, third)) { int s = bottom; bottom = third; third = s; }
                     PsEdge tt = ps_edge(vx, vy, top, third), tb = ps_edge(vx, vy, top, bottom);
-                    bool left = ps_edge_eval(tb, vy[third]) > ps_edge_eval(tt, vy[third]);
+                    float y3 = ps_sel3(vy, third);
+                    bool left = ps_edge_eval(tb, y3) > ps_edge_eval(tt, y3);
                     code = (uint32_t)top | ((uint32_t)bottom << 2) | ((uint32_t)third << 4) | 64u | (left ? 128u : 0u);
                     // last row of the lower half: processTriangle's iyMax for yMax = third.y
-                    split = vy[third] > (float)(H - 1) ? H - 1 : ps_cvt_i32(vy[third]);
+                    split = y3 > (float)(H - 1) ? H - 1 : ps_cvt_i32(y3);
                 }
 #pragma unroll
                 for (int v = 0; v < 3; v++) { tr.vx[v] = vx[v]; tr.vy[v] = vy[v]; tr.rw[v] = rw[v]; tr.z[v] = z[v]; }
@@ -197,6 +208,7 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
         }
         nrows[t] = n;
     }
+    });
     ps_wave_add(cnt, PS_ST_CULLED, culled);
     ps_wave_add(cnt, PS_ST_ZREJ, zrej);
     ps_wave_add(cnt, PS_ST_DROPPED, dropped);
@@ -468,8 +480,11 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                     for (; x < xa; x++) {
                         cf2 += sp.cf2_step;
                         pz += sp.pz_step;
-                        if (need_ck)
-                            for (int k = 0; k < nv; k++) vs[k] = vs[k] + vt[k];
+                        if (need_ck) {
+#pragma unroll
+                            for (int k = 0; k < PS_MAX_NV; k++)
+                                if (k < nv) vs[k] = vs[k] + vt[k];
+                        }
                     }
                     PsSeg sg;
                     sg.si = (uint32_t)i | flags;
@@ -483,7 +498,9 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                     segs[s] = sg;
                     if (need_ck && s > o) {
                         float* c = ck + (int64_t)s * f.ck_stride;
-                        for (int k = 0; k < nv; k++) c[k] = vs[k];
+#pragma unroll
+                        for (int k = 0; k < PS_MAX_NV; k++)
+                            if (k < nv) c[k] = vs[k];
                     }
                     keys[s] = ps_bin_tile(f, v, y, cx);
                     vals[s] = s;
@@ -863,7 +880,7 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
     const PsSeg sg = segs[w];
     const PsSpan sp = spans[sg.si & PS_SEG_SPAN_MASK];
     const PsTri tr = tris[sp.tri];
-    const PsDraw& d = draws[tr.code >> 8];
+    ps_with_draw(draws, (int)(tr.code >> 8), [&](const PsDraw& d) {
     if (d.prog != PROG) return;
     PsEdge L, R;
     ps_row_edges(tr, y, L, R);
@@ -910,6 +927,7 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
         int64_t row = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
         V.colour[row * V.colour_scan + x] = c;
     }
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -160,7 +160,7 @@ struct psgpu_ctx {
     std::vector<Cmd> capture_list;
     std::vector<Graph> graphs;
     uint64_t buf_gen = 1;          // bumped whenever a scratch buffer moves
-    psgpu_tiling tiling = {16, 16, 1};
+    psgpu_tiling tiling = {4, 8, 1};      // 8 x 32-pixel tiles: short per-tile lists for dense and sparse scenes alike
     // scratch
     DevBuf<uint8_t> d_blob;
     DevBuf<PsTri> d_tris;
