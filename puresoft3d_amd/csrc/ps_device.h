@@ -328,7 +328,9 @@ PS_DEV int ps_clamp_1(int wrap, int maxv, int v)
 }
 
 // bilinear filter: texel j covers u = j/W (the nearest sampler's rounding rule).
-// Rows and columns are addressed once each; the four texels share them.
+// Rows and columns are addressed once each; the four texels share them.  The
+// built-in programmes use the B, G, R channels only (NCH = 3).
+template <int NCH = 3>
 PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, float* out)
 {
     int w = ps_lvl_w(t, level), h = ps_lvl_h(t, level);
@@ -348,7 +350,7 @@ PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, floa
     uint32_t c = *(const uint32_t*)(row1 + c0 * 4);
     uint32_t d = *(const uint32_t*)(row1 + c1 * 4);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NCH; k++) {
         float fa = (float)((a >> (8 * k)) & 255u), fb = (float)((b >> (8 * k)) & 255u);
         float fc = (float)((c >> (8 * k)) & 255u), fd = (float)((d >> (8 * k)) & 255u);
         float top = fa + (fb - fa) * ax;
@@ -400,18 +402,19 @@ PS_DEV float ps_lod(const float* g, float u, float v, float r, int w0, int h0)
     return 0.5f * ps_log2(rho2);
 }
 
+template <int NCH = 3>
 PS_DEV void ps_sample_trilinear(const PsTex& t, float u, float v, float lod, float* out)
 {
     int maxl = t.levels - 1;
-    if (!(lod > 0.0f) || maxl == 0) { ps_sample_bilinear(t, 0, u, v, out); return; }
-    if (!(lod < (float)maxl)) { ps_sample_bilinear(t, maxl, u, v, out); return; }
+    if (!(lod > 0.0f) || maxl == 0) { ps_sample_bilinear<NCH>(t, 0, u, v, out); return; }
+    if (!(lod < (float)maxl)) { ps_sample_bilinear<NCH>(t, maxl, u, v, out); return; }
     int l0 = (int)lod;
     float f = lod - (float)l0;
     float a[4], b[4];
-    ps_sample_bilinear(t, l0, u, v, a);
-    ps_sample_bilinear(t, l0 + 1, u, v, b);
+    ps_sample_bilinear<NCH>(t, l0, u, v, a);
+    ps_sample_bilinear<NCH>(t, l0 + 1, u, v, b);
 #pragma unroll
-    for (int k = 0; k < 4; k++) out[k] = a[k] + (b[k] - a[k]) * f;
+    for (int k = 0; k < NCH; k++) out[k] = a[k] + (b[k] - a[k]) * f;
 }
 
 // screen gradients of rw, u*rw, v*rw over a triangle (mip selection)

@@ -6,7 +6,7 @@
 void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCounters* c);
 void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_elems, const PsCounters* c);
 void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
-               uint32_t* nrows, PsCounters* cnt, const PsView* views);
+               float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views);
 int psk_scan_blocks(int64_t cap);
 // exclusive scan; out[n] and *total_out (if given) receive the total
 void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
@@ -19,9 +19,12 @@ void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, c
                   const PsFrame& f, const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, uint32_t* keys,
                   uint32_t* vals, PsSeg* segs, float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
 int psk_rs_blocks(int64_t cap);
+int psk_radix_passes(int key_bits);
+// stable sort by key; with a single pass, kstart[k] receives the first position
+// of key k for every k < 2^key_bits (the lower bounds), else it is not written
 void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
                     int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
-                    bool& result_in_second);
+                    bool& result_in_second, uint32_t* kstart);
 void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
                 const PsCounters* c);
 size_t psk_raster_lds(int chunk_w, int waves);
@@ -29,6 +32,7 @@ void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* seg
                 const uint32_t* kstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
                 PsCounters* cnt);
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
-               const float* vary, int nvs, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt);
+               const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f,
+               const PsCounters* cnt);
 void psk_mip(hipStream_t s, const uint8_t* src, int ws, int hs, int ss, uint8_t* dst, int wd, int hd, int elem);
 void psk_selftest(hipStream_t s, int op, const float* a, const float* b, float* out, int n);

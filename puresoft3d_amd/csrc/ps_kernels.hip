@@ -112,8 +112,9 @@ __device__ __forceinline__ int ps_find_draw(const PsDraw* draws, int nd, int64_t
 
 template <int PROG>
 __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
-                                               float* __restrict__ vary, int nvs, uint32_t* __restrict__ nrows,
-                                               PsCounters* cnt, const PsView* __restrict__ views)
+                                               float* __restrict__ vary, int nvs, float* __restrict__ grads,
+                                               uint32_t* __restrict__ nrows, PsCounters* cnt,
+                                               const PsView* __restrict__ views)
 {
     typedef PsProc<PROG> P;
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -204,6 +205,13 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
                 for (int v = 0; v < 3; v++)
 #pragma unroll
                     for (int k = 0; k < P::NV; k++) vo[v * nvs + k] = u[v][k];
+                if constexpr (P::GRAD) {   // screen gradients for the mip level, once per triangle
+                    float g[6];
+                    ps_tri_grad(tr, u[0], u[1], u[2], P::IU, g);
+                    float4* go = (float4*)(grads + t * 8);
+                    go[0] = make_float4(g[0], g[1], g[2], g[3]);
+                    go[1] = make_float4(g[4], g[5], 0.0f, 0.0f);
+                }
             }
         }
         nrows[t] = n;
@@ -515,7 +523,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
 // ---------------------------------------------------------------------------
 // stable LSD radix sort, digits of up to 11 bits, 4096 pairs per block
 // ---------------------------------------------------------------------------
-#define RS_ITEMS 16
+#define RS_ITEMS 8
 #define RS_BLOCK (256 * RS_ITEMS)
 #define RS_MAXB 2048
 
@@ -575,7 +583,7 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
                                                     const uint32_t* n_ptr, int shift, const uint32_t* __restrict__ offs,
                                                     const uint32_t* __restrict__ totals, int nb_cap,
                                                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                    const PsCounters* c)
+                                                    uint32_t* __restrict__ kstart, const PsCounters* c)
 {
     constexpr int NBK = 1 << BITS;
     constexpr int PER = (NBK + 255) / 256;   // digits per thread in the digit prefix
@@ -585,7 +593,8 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
     if (ps_poisoned(c)) return;
     int64_t n = *n_ptr;
     int64_t b0 = blockIdx.x * (int64_t)RS_BLOCK;
-    if (b0 >= n) return;
+    // block 0 always runs: on a single-pass sort it also writes the key starts
+    if (b0 >= n && !(kstart && blockIdx.x == 0)) return;
     int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // exclusive prefix of the digit totals: thread t owns digits [t*PER, t*PER+PER)
     {
@@ -611,10 +620,15 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             int d = tid * PER + k;
-            if (d < NBK) base[d] = run + offs[(int64_t)d * nb_cap + blockIdx.x];
+            if (d < NBK) {
+                // digit = whole key: the prefix is where each key's run starts
+                if (kstart && blockIdx.x == 0) kstart[d] = run;
+                if (b0 < n) base[d] = run + offs[(int64_t)d * nb_cap + blockIdx.x];
+            }
             run += tv[k];
         }
     }
+    if (b0 >= n) return;
     for (int d = tid; d < NBK; d += 256) cntl[0][d] = cntl[1][d] = cntl[2][d] = cntl[3][d] = 0;
     __syncthreads();
     uint32_t k[RS_ITEMS], v[RS_ITEMS], rank[RS_ITEMS];
@@ -864,7 +878,7 @@ __global__ void __launch_bounds__(512) k_raster(const PsSpan* __restrict__ spans
 template <int PROG>
 __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
                                                const float* __restrict__ ck, const PsTri* __restrict__ tris,
-                                               const float* __restrict__ vary, int nvs,
+                                               const float* __restrict__ vary, int nvs, const float* __restrict__ grads,
                                                const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt)
 {
     typedef PsProc<PROG> P;
@@ -921,7 +935,11 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
         }
     }
     float grad[6] = {0, 0, 0, 0, 0, 0};
-    if (P::GRAD) ps_tri_grad(tr, u0, u0 + nvs, u0 + 2 * nvs, 12, grad);
+    if constexpr (P::GRAD) {
+        const float4* gi = (const float4*)(grads + (int64_t)sp.tri * 8);
+        float4 g0 = gi[0], g1 = gi[1];
+        grad[0] = g0.x; grad[1] = g0.y; grad[2] = g0.z; grad[3] = g0.w; grad[4] = g1.x; grad[5] = g1.y;
+    }
     uint32_t c;
     if (P::fp(d, in, r, grad, c)) {
         int64_t row = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
@@ -986,18 +1004,15 @@ void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_elems, con
 
 
 void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
-               uint32_t* nrows, PsCounters* cnt, const PsView* views)
+               float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
     dim3 g(gsz(T, 256)), b(256);
+#define SU_CASE(K) case K: hipLaunchKernelGGL(k_setup<K>, g, b, 0, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); break;
     switch (prog) {
-    case PSGPU_PROC_DEF01: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF01>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
-    case PSGPU_PROC_DEF02: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF02>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
-    case PSGPU_PROC_DEF03: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF03>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
-    case PSGPU_PROC_DEF04: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF04>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
-    case PSGPU_PROC_DEF05: hipLaunchKernelGGL(k_setup<PSGPU_PROC_DEF05>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
-    case PSGPU_PROC_LAMBERT: hipLaunchKernelGGL(k_setup<PSGPU_PROC_LAMBERT>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
-    case PSGPU_PROC_BLINN: hipLaunchKernelGGL(k_setup<PSGPU_PROC_BLINN>, g, b, 0, s, draws, nd, T, tris, vary, nvs, nrows, cnt, views); break;
+    SU_CASE(PSGPU_PROC_DEF01) SU_CASE(PSGPU_PROC_DEF02) SU_CASE(PSGPU_PROC_DEF03) SU_CASE(PSGPU_PROC_DEF04)
+    SU_CASE(PSGPU_PROC_DEF05) SU_CASE(PSGPU_PROC_LAMBERT) SU_CASE(PSGPU_PROC_BLINN)
     }
+#undef SU_CASE
 }
 
 int psk_scan_blocks(int64_t cap) { return gsz(cap, SCAN_BLOCK); }
@@ -1034,14 +1049,20 @@ void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, c
 
 int psk_rs_blocks(int64_t cap) { return gsz(cap, RS_BLOCK); }
 
+int psk_radix_passes(int key_bits)
+{
+    int passes = (key_bits + 10) / 11;
+    return passes < 1 ? 1 : passes;
+}
+
 void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
                     int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
-                    bool& result_in_second)
+                    bool& result_in_second, uint32_t* kstart)
 {
     int nb = psk_rs_blocks(cap);
-    int passes = (key_bits + 10) / 11;
-    if (passes < 1) passes = 1;
+    int passes = psk_radix_passes(key_bits);
     int bits = (key_bits + passes - 1) / passes;
+    if (passes > 1) kstart = nullptr;
     uint32_t *ki = keys, *vi = vals, *ko = keys2, *vo = vals2;
     uint32_t* totals = hist_off + ((int64_t)nb << 11);
     for (int p = 0, shift = 0; p < passes; p++, shift += bits) {
@@ -1049,7 +1070,7 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
         hipLaunchKernelGGL(k_rs_digits, dim3(((1 << bits) + 3) / 4), dim3(256), 0, s, hist, n_dev, bits, nb, hist_off,
                            totals, c);
         switch (bits) {
-#define RS_CASE(B) case B: hipLaunchKernelGGL(k_rs_scatter<B>, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, totals, nb, ko, vo, c); break;
+#define RS_CASE(B) case B: hipLaunchKernelGGL(k_rs_scatter<B>, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, totals, nb, ko, vo, kstart, c); break;
         RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4) RS_CASE(5) RS_CASE(6) RS_CASE(7) RS_CASE(8) RS_CASE(9) RS_CASE(10) RS_CASE(11)
 #undef RS_CASE
         }
@@ -1081,10 +1102,10 @@ void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* seg
 }
 
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
-               const float* vary, int nvs, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
+               const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
 {
     dim3 g((unsigned)f.n_stiles), b(256);
-#define SH_CASE(K) case K: hipLaunchKernelGGL(k_shade<K>, g, b, 0, s, spans, segs, ck, tris, vary, nvs, draws, f, cnt); break;
+#define SH_CASE(K) case K: hipLaunchKernelGGL(k_shade<K>, g, b, 0, s, spans, segs, ck, tris, vary, nvs, grads, draws, f, cnt); break;
     switch (prog) {
     SH_CASE(PSGPU_PROC_DEF01) SH_CASE(PSGPU_PROC_DEF02) SH_CASE(PSGPU_PROC_DEF03) SH_CASE(PSGPU_PROC_DEF04)
     SH_CASE(PSGPU_PROC_LAMBERT) SH_CASE(PSGPU_PROC_BLINN)

@@ -173,6 +173,7 @@ struct psgpu_ctx {
     DevBuf<int> d_winner;
     DevBuf<PsSeg> d_segs;
     DevBuf<float> d_ck;
+    DevBuf<float> d_grad;   // per triangle: screen gradients for the mip level (8 floats)
     int64_t span_cap = 0, seg_cap = 0;
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
@@ -457,6 +458,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     PsDraw* dd = (PsDraw*)dev;
     f.views = (const PsView*)(dev + blob.off_views);
     if ((rc = ensure(c, c->d_tris, T)) || (rc = ensure(c, c->d_vary, (size_t)T * 3 * nvs)) ||
+        (rc = ensure(c, c->d_grad, (size_t)T * 8)) ||
         (rc = ensure(c, c->d_nrows, T)) || (rc = ensure(c, c->d_rowoff, T + 1)))
         return rc;
     if (c->span_cap == 0) {
@@ -476,7 +478,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     f.ck_stride = nvs;
     int nbr = psk_rs_blocks(c->seg_cap);
     if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 2048)) ||
-        (rc = ensure(c, c->d_kstart, f.n_tiles + 1)))
+        (rc = ensure(c, c->d_kstart, std::max<int64_t>(f.n_tiles + 1, 2048))))
         return rc;
     if (c->debug) {
         if (f.n_views != 1 || c->dbg_w != cmd.views[0].tg.vpw || c->dbg_h != cmd.views[0].tg.vph)
@@ -491,7 +493,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     {
         ProfScope p(c, "setup");
         for (int k = 1; k < 8; k++)
-            if (progs[k]) psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_nrows.p, c->d_cnt, dv);
+            if (progs[k])
+                psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, c->d_nrows.p, c->d_cnt, dv);
     }
     {
         ProfScope p(c, "scan_rows");
@@ -521,11 +524,11 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     {
         ProfScope p(c, "radix_sort");
         psk_radix_sort(s, c->d_keys.p, c->d_vals.p, c->d_keys2.p, c->d_vals2.p, c->d_sizes + 1, c->seg_cap,
-                       key_bits(f.n_tiles), c->d_hist.p, c->d_histoff.p, c->d_partial.p, c->d_cnt, second);
+                       key_bits(f.n_tiles), c->d_hist.p, c->d_histoff.p, c->d_partial.p, c->d_cnt, second, c->d_kstart.p);
     }
     uint32_t* skeys = second ? c->d_keys2.p : c->d_keys.p;
     uint32_t* svals = second ? c->d_vals2.p : c->d_vals.p;
-    {
+    if (psk_radix_passes(key_bits(f.n_tiles)) > 1) {   // one pass leaves the tile starts already
         ProfScope p(c, "bounds");
         psk_bounds(s, skeys, c->d_sizes + 1, f.n_tiles, c->d_kstart.p, c->d_cnt);
     }
@@ -538,7 +541,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         ProfScope p(c, "shade");
         for (int k = 1; k < 8; k++)
             if (shade_progs[k])
-                psk_shade(s, k, c->d_spans.p, c->d_segs.p, c->d_ck.p, c->d_tris.p, c->d_vary.p, nvs, dd, f, c->d_cnt);
+                psk_shade(s, k, c->d_spans.p, c->d_segs.p, c->d_ck.p, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, dd, f,
+                          c->d_cnt);
     }
     HIPCHK(c, hipGetLastError());
     c->stats.flushes++;
@@ -855,7 +859,7 @@ int psgpu_destroy(psgpu_ctx* c)
     fr(c->d_blob.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
     fr(c->d_spantri.p); fr(c->d_spans.p); fr(c->d_segcount.p); fr(c->d_segoff.p); fr(c->d_keys.p); fr(c->d_vals.p);
     fr(c->d_keys2.p); fr(c->d_vals2.p); fr(c->d_hist.p); fr(c->d_histoff.p); fr(c->d_kstart.p); fr(c->d_winner.p);
-    fr(c->d_segs.p); fr(c->d_ck.p);
+    fr(c->d_segs.p); fr(c->d_ck.p); fr(c->d_grad.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
     for (int i = 0; i < psgpu_ctx::kStage; i++) {
         if (c->h_stage[i]) hipHostFree(c->h_stage[i]);
