@@ -151,37 +151,49 @@ def serve_section(args, N, S, log):
     zeros = np.zeros(n)
     srv.serve(views, zeros, **cfg)                       # warm-up: target pools and scratch sizes
     srv.serve(views, zeros, max_batch_views=1, max_inflight=1)
-    # plain path: one view per launch, back to back
-    n_plain = min(n, 600)
-    res_p, st_p, _ = srv.serve(views[:n_plain], zeros[:n_plain], max_batch_views=1, max_inflight=1)
-    single = res_p["done_s"] - res_p["launched_s"]
-    plain_vps = n_plain / st_p["wall_s"]
-    out = {"requests": n, "scenes": len(assets),
-           "plain": {"views_per_s": round(plain_vps, 1),
-                     "host_submit_ms_per_view": round(st_p["host_submit_s"] * 1e3 / n_plain, 4),
-                     "host_flush_ms_per_view": round(st_p["host_flush_s"] * 1e3 / n_plain, 4),
-                     "single_view_ms_p50": round(float(np.percentile(single, 50)) * 1e3, 3),
-                     "single_view_ms_p99": round(float(np.percentile(single, 99)) * 1e3, 3)},
-           "config": {k: int(v) for k, v in cfg.items()}, "runs": []}
+    out = {"requests": n, "scenes": len(assets), "config": {k: int(v) for k, v in cfg.items()}}
+    if not args.no_tune:   # raster tiling for this mix of views, on a sample of 64 requests
+        from puresoft3d_amd import autotune as AT
+        sample = views[:64]
+        best, _ = AT.autotune(ctx, lambda: srv.serve(sample, zeros[:64], to_host=False, **cfg), reps=2,
+                              cands=AT.candidates(waves=(1, 2)))
+        out["tiling"] = {"chunk_width": best[0], "tile_rows": best[1], "waves_per_group": best[2]}
     keep = list(range(0, n, max(1, n // 8)))
-    for load in args.serve_loads:
-        rate = load * plain_vps
-        _, arr = S.poisson_requests(assets, n=n, rate=rate, seed=23)
-        res, st, imgs = srv.serve(views, arr, keep=keep, **cfg)
-        summ = SV.latency_summary(res)
-        lat = res["done_s"] - res["arrival_s"]
-        run = {"offered_x_plain": load, "offered_views_per_s": round(rate, 1),
-               "views_per_s": round(summ["views_per_s"], 1),
-               "speedup_vs_plain": round(summ["views_per_s"] / plain_vps, 2),
-               "latency_ms_p50": round(summ["p50_ms"], 3), "latency_ms_p99": round(summ["p99_ms"], 3),
-               "p99_vs_single_view_p99": round(float(np.percentile(lat, 99)) / float(np.percentile(single, 99)), 2),
-               "batches": st["batches"], "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1),
-               "host_submit_ms_per_batch": round(st["host_submit_s"] * 1e3 / max(1, st["batches"]), 4),
-               "peak_target_bytes": st["peak_bytes"]}
-        same = sum(int(np.array_equal(imgs[i], srv.render_alone(views[i]))) for i in keep)
-        run["images_identical_to_alone"] = "%d/%d" % (same, len(keep))
-        out["runs"].append(run)
-        log("[bench] serve load %.1fx: %s" % (load, json.dumps(run)))
+    for mode in ("host", "device"):
+        to_host = mode == "host"
+        # plain path: one view per launch, back to back
+        n_plain = min(n, 600)
+        res_p, st_p, _ = srv.serve(views[:n_plain], zeros[:n_plain], max_batch_views=1, max_inflight=1,
+                                   to_host=to_host)
+        single = res_p["done_s"] - res_p["launched_s"]
+        plain_vps = n_plain / st_p["wall_s"]
+        sec = {"plain": {"views_per_s": round(plain_vps, 1),
+                         "host_submit_ms_per_view": round(st_p["host_submit_s"] * 1e3 / n_plain, 4),
+                         "single_view_ms_p50": round(float(np.percentile(single, 50)) * 1e3, 3),
+                         "single_view_ms_p99": round(float(np.percentile(single, 99)) * 1e3, 3)},
+               "runs": []}
+        for load in args.serve_loads:
+            rate = load * plain_vps
+            _, arr = S.poisson_requests(assets, n=n, rate=rate, seed=23)
+            res, st, imgs = srv.serve(views, arr, keep=keep if to_host else None, to_host=to_host, **cfg)
+            summ = SV.latency_summary(res)
+            lat = res["done_s"] - res["arrival_s"]
+            run = {"offered_x_plain": load, "offered_views_per_s": round(rate, 1),
+                   "views_per_s": round(summ["views_per_s"], 1),
+                   "speedup_vs_plain": round(summ["views_per_s"] / plain_vps, 2),
+                   "latency_ms_p50": round(summ["p50_ms"], 3), "latency_ms_p99": round(summ["p99_ms"], 3),
+                   "p99_vs_single_view_p99": round(float(np.percentile(lat, 99)) /
+                                                   float(np.percentile(single, 99)), 2),
+                   "batches": st["batches"], "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1),
+                   "host_submit_ms_per_batch": round(st["host_submit_s"] * 1e3 / max(1, st["batches"]), 4),
+                   "peak_target_bytes": st["peak_bytes"]}
+            if to_host:
+                same = sum(int(np.array_equal(imgs[i], srv.render_alone(views[i]))) for i in keep)
+                run["images_identical_to_alone"] = "%d/%d" % (same, len(keep))
+            sec["runs"].append(run)
+            log("[bench] serve (%s) load %.1fx: %s" % (mode, load, json.dumps(run)))
+        out["deliver_" + mode] = sec
+    srv.close()
     ctx.close()
     return out
 

@@ -60,6 +60,8 @@ typedef struct {
     int64_t max_batch_pixels;
     int max_inflight;           /* batches queued on the GPU at once */
     uint32_t clear_bgra;
+    int deliver_to_host;        /* 1: each image is copied to pinned host memory before the
+                                 * request completes (PCIe-inclusive); 0: images stay in HBM */
 } psserve_config;
 
 typedef struct {
@@ -70,21 +72,31 @@ typedef struct {
 typedef struct {
     int64_t batches, views, pixels;
     int64_t target_allocs, target_evictions;
-    int64_t peak_bytes;          /* render targets held at once */
+    int64_t peak_bytes;          /* render targets allocated at once (pooled ones included) */
     int max_batch_views_seen;
     double wall_s;
     double host_submit_s;        /* host time spent queueing batches (draws, reads, fence) */
     double host_flush_s;         /* part of it spent in the flush the first read triggers */
 } psserve_stats;
 
+/* A server: the scene list and configuration, plus the pooled render
+ * targets and pinned buffers it keeps between runs. */
+typedef struct psserve_server psserve_server;
+
+int psserve_create(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_config* cfg,
+                   psserve_server** out);
+int psserve_set_config(psserve_server* server, const psserve_config* cfg);
+/* frees the pooled targets (syncs the context) */
+int psserve_destroy(psserve_server* server);
+
 /* Runs the request list in real time (arrivals are waited for) and returns
  * per-request timestamps.  images: NULL, or one host pointer per request
  * (NULL entries allowed) that receives height rows of width*4 bytes (the
- * colour target's memory rows). */
-int psserve_run(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_request* reqs, int n,
-                const psserve_config* cfg, psserve_result* results, uint8_t* const* images, psserve_stats* stats);
+ * colour target's memory rows); only filled with deliver_to_host. */
+int psserve_run(psserve_server* server, const psserve_request* reqs, int n, psserve_result* results,
+                uint8_t* const* images, psserve_stats* stats);
 
-/* message of the last failed psserve_run on this thread */
+/* message of the last failed psserve_* call on this thread */
 const char* psserve_last_error(void);
 
 #ifdef __cplusplus

@@ -750,27 +750,49 @@ __global__ void __launch_bounds__(512) k_raster(const PsSpan* __restrict__ spans
     bool dirty = false;
     const uint64_t lt = ps_lt_mask();
     uint32_t s0 = tstart[tile], s1 = tstart[tile + 1];
+    // software pipeline over chunks of the tile list: while chunk i is split and
+    // walked, the records of chunk i+1 and the indices of chunk i+2 are in flight
+    uint32_t vn[RA_PER], vnn[RA_PER];
+    uint4 ran[RA_PER], rbn[RA_PER];
+#pragma unroll
+    for (int k = 0; k < RA_PER; k++) {
+        uint32_t e = s0 + k * 64 + lane;
+        vn[k] = e < s1 ? vals[e] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < RA_PER; k++) {
+        uint32_t e = s0 + k * 64 + lane;
+        const uint4* p = (const uint4*)(segs + vn[k]);
+        ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
+        rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < RA_PER; k++) {
+        uint32_t e = s0 + RA_CHUNK + k * 64 + lane;
+        vnn[k] = e < s1 ? vals[e] : 0u;
+    }
     for (uint32_t cb = s0; cb < s1; cb += RA_CHUNK) {
         uint32_t ce = min(s1, cb + RA_CHUNK);
-        // gather the chunk's pieces: RA_PER entries per lane, all loads in flight
         uint32_t v[RA_PER], key[RA_PER];
         uint4 ra[RA_PER], rb[RA_PER];
 #pragma unroll
         for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + k * 64 + lane;
-            v[k] = e < ce ? vals[e] : 0u;
+            v[k] = vn[k];
+            ra[k] = ran[k];
+            rb[k] = rbn[k];
+            vn[k] = vnn[k];
         }
 #pragma unroll
         for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + k * 64 + lane;
-            if (e < ce) {
-                const uint4* p = (const uint4*)(segs + v[k]);
-                ra[k] = p[0];
-                rb[k] = p[1];
-            } else {
-                ra[k] = make_uint4(0, 0, 0, 0);
-                rb[k] = make_uint4(0, 0, 0, 0);
-            }
+            uint32_t e = cb + RA_CHUNK + k * 64 + lane;
+            const uint4* p = (const uint4*)(segs + vn[k]);
+            ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
+            rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
+            uint32_t e2 = cb + 2 * RA_CHUNK + k * 64 + lane;
+            vnn[k] = e2 < s1 ? vals[e2] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < RA_PER; k++) {
             // the piece's lane: its row inside the tile and its chunk inside the row
             int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
             key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));

@@ -55,8 +55,11 @@ struct Batch {
     std::vector<int> reqs;
 };
 
-struct Server {
+}  // namespace
+
+struct psserve_server {
     psgpu_ctx* ctx;
+    std::vector<psserve_scene> scene_list;
     const psserve_scene* scenes;
     int n_scenes;
     const psserve_request* reqs;
@@ -179,11 +182,14 @@ struct Server {
         double t1 = secs();
         if ((rc = psgpu(psgpu_flush(ctx)))) return rc;
         st.host_flush_s += secs() - t1;
-        for (int i : ids) {
-            const psserve_request& r = reqs[i];
-            const Slot& s = held[i];
-            if ((rc = psgpu(psgpu_texture_read_async(ctx, s.colour, 0, 0, 0, r.width, r.height, s.host, (size_t)r.width * 4))))
-                return rc;
+        if (cfg.deliver_to_host) {
+            for (int i : ids) {
+                const psserve_request& r = reqs[i];
+                const Slot& s = held[i];
+                if ((rc = psgpu(psgpu_texture_read_async(ctx, s.colour, 0, 0, 0, r.width, r.height, s.host,
+                                                         (size_t)r.width * 4))))
+                    return rc;
+            }
         }
         rc = psgpu(psgpu_fence(ctx, fence));
         st.host_submit_s += secs() - t0;
@@ -212,7 +218,8 @@ struct Server {
                 for (int i : inflight.front().reqs) {
                     const psserve_request& r = reqs[i];
                     res[i].done_s = t;
-                    if (images && images[i]) memcpy(images[i], held[i].host, (size_t)r.width * r.height * 4);
+                    if (images && images[i] && cfg.deliver_to_host)
+                        memcpy(images[i], held[i].host, (size_t)r.width * r.height * 4);
                     release(i);
                     done++;
                 }
@@ -271,13 +278,21 @@ struct Server {
         return PSGPU_OK;
     }
 
-    // frees every target, idle or still held after a failed run
-    int cleanup()
+    // after a run: targets still held (a failed run) go back to the pools
+    int settle()
     {
         int rc = psgpu(psgpu_sync(ctx));
         for (Slot& s : held)
-            if (s.colour >= 0 && !rc) rc = free_slot(s);
+            if (s.colour >= 0) idle[std::make_pair(s.cw, s.ch)].push_back(s);
         held.clear();
+        inflight_scratch = 0;
+        return rc;
+    }
+
+    // frees every pooled target
+    int release_all()
+    {
+        int rc = settle();
         for (auto& kv : idle)
             for (Slot& s : kv.second)
                 if (!rc) rc = free_slot(s);
@@ -286,40 +301,82 @@ struct Server {
     }
 };
 
+namespace {
+
+int check_config(const psserve_config* cfg)
+{
+    if (!cfg) return fail(PSGPU_ERR_INVALID, "null configuration");
+    if (cfg->max_batch_views < 1 || cfg->max_inflight < 1 || cfg->max_batch_pixels < 1 || cfg->scratch_bytes_per_px < 0)
+        return fail(PSGPU_ERR_INVALID, "bad configuration");
+    return PSGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int psserve_run(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_request* reqs, int n,
-                const psserve_config* cfg, psserve_result* results, uint8_t* const* images, psserve_stats* stats)
+int psserve_create(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_config* cfg,
+                   psserve_server** out)
 {
-    if (!ctx || !scenes || !reqs || !cfg || !results || n < 0 || n_scenes < 1)
-        return fail(PSGPU_ERR_INVALID, "null argument");
-    if (cfg->max_batch_views < 1 || cfg->max_inflight < 1 || cfg->max_batch_pixels < 1 || cfg->scratch_bytes_per_px < 0)
-        return fail(PSGPU_ERR_INVALID, "bad configuration");
+    if (!out) return fail(PSGPU_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (!ctx || !scenes || n_scenes < 1) return fail(PSGPU_ERR_INVALID, "null argument");
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    psserve_server* s = new psserve_server();
+    s->ctx = ctx;
+    s->scene_list.assign(scenes, scenes + n_scenes);
+    s->scenes = s->scene_list.data();
+    s->n_scenes = n_scenes;
+    s->cfg = *cfg;
+    *out = s;
+    return PSGPU_OK;
+}
+
+int psserve_set_config(psserve_server* s, const psserve_config* cfg)
+{
+    if (!s) return fail(PSGPU_ERR_INVALID, "null server");
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    s->cfg = *cfg;
+    return PSGPU_OK;
+}
+
+int psserve_run(psserve_server* s, const psserve_request* reqs, int n, psserve_result* results, uint8_t* const* images,
+                psserve_stats* stats)
+{
+    if (!s || !reqs || !results || n < 0) return fail(PSGPU_ERR_INVALID, "null argument");
     for (int i = 0; i < n; i++) {
         const psserve_request& r = reqs[i];
-        if (r.scene < 0 || r.scene >= n_scenes) return fail(PSGPU_ERR_RANGE, "request %d: scene %d out of range", i, r.scene);
+        if (r.scene < 0 || r.scene >= s->n_scenes)
+            return fail(PSGPU_ERR_RANGE, "request %d: scene %d out of range", i, r.scene);
         if (r.width < 1 || r.height < 1 || r.width > 16384 || r.height > 16384)
             return fail(PSGPU_ERR_INVALID, "request %d: bad size %dx%d", i, r.width, r.height);
         if (i && r.arrival_s < reqs[i - 1].arrival_s) return fail(PSGPU_ERR_INVALID, "requests must be in arrival order");
     }
-    Server s;
-    s.ctx = ctx;
-    s.scenes = scenes;
-    s.n_scenes = n_scenes;
-    s.reqs = reqs;
-    s.n = n;
-    s.cfg = *cfg;
-    s.res = results;
-    s.images = images;
-    memset(&s.st, 0, sizeof(s.st));
+    s->reqs = reqs;
+    s->n = n;
+    s->res = results;
+    s->images = images;
+    memset(&s->st, 0, sizeof(s->st));
+    s->st.peak_bytes = (int64_t)s->target_bytes;   // pooled targets count from the start
     memset(results, 0, sizeof(psserve_result) * (size_t)n);
-    int rc = s.run();
-    int rc2 = s.cleanup();
-    if (stats) *stats = s.st;
+    int rc = s->run();
+    int rc2 = s->settle();
+    if (stats) *stats = s->st;
+    s->reqs = nullptr;
+    s->res = nullptr;
+    s->images = nullptr;
     if (rc) return rc;
     return rc2;
+}
+
+int psserve_destroy(psserve_server* s)
+{
+    if (!s) return PSGPU_OK;
+    int rc = s->release_all();
+    delete s;
+    return rc;
 }
 
 const char* psserve_last_error(void) { return g_err.c_str(); }

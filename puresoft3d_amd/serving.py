@@ -24,7 +24,7 @@ class Request(C.Structure):
 class Config(C.Structure):
     _fields_ = [("program", C.c_int), ("hbm_budget", C.c_size_t), ("scratch_bytes_per_px", C.c_int),
                 ("max_batch_views", C.c_int), ("max_batch_pixels", C.c_int64), ("max_inflight", C.c_int),
-                ("clear_bgra", C.c_uint32)]
+                ("clear_bgra", C.c_uint32), ("deliver_to_host", C.c_int)]
 
 
 class Result(C.Structure):
@@ -48,8 +48,14 @@ def lib():
         if not os.path.exists(HOST_LIB):
             raise RuntimeError("%s is not built (python -m puresoft3d_amd.build)" % HOST_LIB)
         L = C.CDLL(HOST_LIB)
-        L.psserve_run.argtypes = [C.c_void_p, C.POINTER(Scene), C.c_int, C.POINTER(Request), C.c_int,
-                                  C.POINTER(Config), C.POINTER(Result), C.POINTER(C.c_void_p), C.POINTER(Stats)]
+        L.psserve_create.argtypes = [C.c_void_p, C.POINTER(Scene), C.c_int, C.POINTER(Config), C.POINTER(C.c_void_p)]
+        L.psserve_create.restype = C.c_int
+        L.psserve_set_config.argtypes = [C.c_void_p, C.POINTER(Config)]
+        L.psserve_set_config.restype = C.c_int
+        L.psserve_destroy.argtypes = [C.c_void_p]
+        L.psserve_destroy.restype = C.c_int
+        L.psserve_run.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.POINTER(Result), C.POINTER(C.c_void_p),
+                                  C.POINTER(Stats)]
         L.psserve_run.restype = C.c_int
         L.psserve_last_error.restype = C.c_char_p
         _lib = L
@@ -57,7 +63,9 @@ def lib():
 
 
 class RenderServer:
-    """Scenes resident on one context; serve() runs a request stream."""
+    """Scenes resident on one context; serve() runs a request stream.  The
+    native server keeps its pooled render targets between runs; close() it
+    before closing the context."""
 
     def __init__(self, ctx, assets, gen_mips=True):
         self.ctx, self.assets = ctx, assets
@@ -74,6 +82,26 @@ class RenderServer:
                 ctx.vao_attach(v, s, ctx.vbo_create(arr))
             self.scenes[k] = Scene(v, ids[0], ids[1])
         self.prog = ctx.program_create(N.PROC_BLINN)
+        L = lib()
+        h = C.c_void_p()
+        self._check(L.psserve_create(ctx.h, self.scenes, len(assets), C.byref(self._config()), C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def _check(rc):
+        if rc:
+            raise RuntimeError("psserve failed (%d): %s" % (rc, lib().psserve_last_error().decode()))
+
+    def _config(self, hbm_budget=8 << 30, max_batch_views=64, max_batch_pixels=16 << 20, max_inflight=2,
+                scratch_bytes_per_px=48, clear_bgra=0x00302010, to_host=True):
+        return Config(self.prog, hbm_budget, scratch_bytes_per_px, max_batch_views, max_batch_pixels, max_inflight,
+                      clear_bgra, 1 if to_host else 0)
+
+    def close(self):
+        """frees the server's pooled targets (the scenes stay on the context)"""
+        if getattr(self, "h", None):
+            self._check(lib().psserve_destroy(self.h))
+            self.h = None
 
     def requests(self, views, arrivals):
         reqs = (Request * len(views))()
@@ -88,15 +116,20 @@ class RenderServer:
             r.arrival_s = float(t)
         return reqs
 
-    def serve(self, views, arrivals, hbm_budget=8 << 30, max_batch_views=64, max_batch_pixels=48 << 20,
-              max_inflight=2, scratch_bytes_per_px=48, clear_bgra=0x00302010, keep=None):
-        """Runs the stream in real time.  keep: indices whose images to return.
-        Returns (per-request result dict of arrays, stats dict, images dict)."""
+    def serve(self, views, arrivals, hbm_budget=8 << 30, max_batch_views=64, max_batch_pixels=16 << 20,
+              max_inflight=2, scratch_bytes_per_px=48, clear_bgra=0x00302010, keep=None, to_host=True):
+        """Runs the stream in real time.  keep: indices whose images to return
+        (needs to_host).  to_host=False leaves images in HBM: a request is done
+        when its view is rendered.  Returns (per-request result dict of arrays,
+        stats dict, images dict)."""
+        if keep is not None and not to_host:
+            raise ValueError("images are returned only when delivered to the host")
         L = lib()
         n = len(views)
         reqs = self.requests(views, arrivals)
-        cfg = Config(self.prog, hbm_budget, scratch_bytes_per_px, max_batch_views, max_batch_pixels, max_inflight,
-                     clear_bgra)
+        cfg = self._config(hbm_budget, max_batch_views, max_batch_pixels, max_inflight, scratch_bytes_per_px,
+                           clear_bgra, to_host)
+        self._check(L.psserve_set_config(self.h, C.byref(cfg)))
         res = (Result * n)()
         st = Stats()
         imgs, ptrs = {}, None
@@ -105,9 +138,7 @@ class RenderServer:
             for i in keep:
                 imgs[i] = np.zeros((views[i].height, views[i].width, 4), np.uint8)
                 ptrs[i] = imgs[i].ctypes.data
-        rc = L.psserve_run(self.ctx.h, self.scenes, len(self.assets), reqs, n, C.byref(cfg), res, ptrs, C.byref(st))
-        if rc:
-            raise RuntimeError("psserve_run failed (%d): %s" % (rc, L.psserve_last_error().decode()))
+        self._check(L.psserve_run(self.h, reqs, n, res, ptrs, C.byref(st)))
         out = {k: np.array([getattr(r, k) for r in res]) for k, _ in Result._fields_}
         stats = {k: getattr(st, k) for k, _ in Stats._fields_}
         return out, stats, imgs

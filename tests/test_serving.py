@@ -30,9 +30,13 @@ def test_serve_rejects_bad_arguments():
     from puresoft3d_amd import serving
     import ctypes as C
     L = serving.lib()
-    cfg = serving.Config(0, 1 << 30, 16, 8, 1 << 20, 2, 0)
-    rc = L.psserve_run(None, None, 0, None, 0, C.byref(cfg), None, None, None)
-    assert rc == N.ERR_INVALID
+    cfg = serving.Config(0, 1 << 30, 16, 8, 1 << 20, 2, 0, 1)
+    h = C.c_void_p()
+    assert L.psserve_create(None, None, 0, C.byref(cfg), C.byref(h)) == N.ERR_INVALID and not h.value
+    assert L.psserve_run(None, None, 0, None, None, None) == N.ERR_INVALID
+    bad = serving.Config(0, 1 << 30, 16, 0, 1 << 20, 2, 0, 1)      # no views per batch
+    assert L.psserve_set_config(None, C.byref(bad)) == N.ERR_INVALID
+    assert L.psserve_destroy(None) == 0
 
 
 def _small_server(ctx):
@@ -57,6 +61,9 @@ def test_served_images_equal_alone_and_oracle(oracle):
             np.testing.assert_array_equal(imgs[i], alone[i], err_msg="view %d" % i)
         summ = serving.latency_summary(res)
         assert summ["p99_ms"] >= summ["p50_ms"] > 0
+        res_d, st_d, _ = srv.serve(views, arr, max_batch_views=16, to_host=False)
+        assert st_d["views"] == len(views) and (res_d["done_s"] > 0).all()
+        srv.close()
     finally:
         ctx.close()
     order = np.argsort([v.width * v.height for v in views])
@@ -82,6 +89,7 @@ def test_budget_admission_and_plain_mode():
         assert st1["batches"] == 30 and st1["max_batch_views_seen"] == 1
         for i in range(len(views)):
             np.testing.assert_array_equal(imgs[i], imgs1[i])
+        srv.close()
     finally:
         ctx.close()
 
@@ -96,6 +104,7 @@ def test_async_read_after_overflow_is_replayed():
         assets, srv = _small_server(ctx)
         views, _ = S.poisson_requests(assets, n=4, rate=1000.0, seed=11, side_lo=96, side_hi=200)
         ref = [srv.render_alone(v) for v in views]
+        srv.close()
         ctx2 = N.Context(64, 64, flags=1)
         try:
             from puresoft3d_amd import serving
@@ -133,6 +142,7 @@ def test_async_read_after_overflow_is_replayed():
                 np.testing.assert_array_equal(buf.reshape(v.height, v.width, 4), r)
             for buf in bufs:
                 ctx2.host_free(buf)
+            srv2.close()
         finally:
             ctx2.close()
     finally:
