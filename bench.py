@@ -111,6 +111,13 @@ def batch_section(args, N, S, log):
     st = ctx.stats()
     tot = sum(lat)
     px = sum(v.width * v.height for v in views)
+    # per-kernel time of one batch (eager, HIP events on the context stream)
+    ctx.profile(True)
+    ctx.profile_reset()
+    b.submit()
+    ctx.sync()
+    kprof = {k: round(v[0], 4) for k, v in ctx.profile_read().items()}
+    ctx.profile(False)
     # one view per launch, same views: the plain path's per-view latency
     single = []
     for i in range(0, len(views), 8):
@@ -126,7 +133,8 @@ def batch_section(args, N, S, log):
            "views_per_s": round(len(views) * args.batch_steps / tot, 1),
            "fragments_shaded_per_s": round(st["frags_shaded"] / tot, 1),
            "single_view_ms_p50": round(float(np.median(single)) * 1e3, 3),
-           "single_view_views_per_s": round(1.0 / float(np.mean(single)), 1)}
+           "single_view_views_per_s": round(1.0 / float(np.mean(single)), 1),
+           "kernel_ms_per_batch": kprof}
     if tuned:
         out["autotune"] = tuned
     return out, assets, views
@@ -193,6 +201,19 @@ def serve_section(args, N, S, log):
             sec["runs"].append(run)
             log("[bench] serve (%s) load %.1fx: %s" % (mode, load, json.dumps(run)))
         out["deliver_" + mode] = sec
+    # latency against the batch size cap, device delivery at twice the plain rate
+    sweep = []
+    plain_dev = out["deliver_device"]["plain"]["views_per_s"]
+    _, arr = S.poisson_requests(assets, n=n, rate=2.0 * plain_dev, seed=23)
+    for cap in args.serve_caps:
+        c2 = dict(cfg, max_batch_pixels=cap << 20)
+        res, st, _ = srv.serve(views, arr, to_host=False, **c2)
+        summ = SV.latency_summary(res)
+        sweep.append({"max_batch_mpix": cap, "views_per_s": round(summ["views_per_s"], 1),
+                      "latency_ms_p50": round(summ["p50_ms"], 3), "latency_ms_p99": round(summ["p99_ms"], 3),
+                      "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1)})
+        log("[bench] serve (device, 2x) cap %d MPix: %s" % (cap, json.dumps(sweep[-1])))
+    out["batch_cap_sweep_device_2x"] = sweep
     srv.close()
     ctx.close()
     return out
@@ -231,6 +252,8 @@ def main():
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[1.5, 2.0, 3.0],
                     help="offered request rates as multiples of the plain path's views/s")
     ap.add_argument("--serve-batch-views", type=int, default=64)
+    ap.add_argument("--serve-caps", type=int, nargs="*", default=[2, 4, 8], metavar="MPIX",
+                    help="batch pixel caps swept at twice the plain rate (device delivery)")
     ap.add_argument("--serve-batch-pixels", type=int, default=16, metavar="MPIX")
     ap.add_argument("--serve-budget-gib", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")

@@ -199,13 +199,19 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
                 tr.split = split;
                 tr.code = code | ((uint32_t)di << 8);
                 n = (uint32_t)(last - first + 1);
-                tris[t] = tr;
+                // a triangle without rows has no spans: nothing reads its records
+                // (rows can lie outside the vertices' y range, where edges
+                // extrapolate, so no x range test is safe here)
+                bool pieces = n > 0;
+                if (pieces) tris[t] = tr;
                 float* vo = vary + t * 3 * (int64_t)nvs;
+                if (pieces) {
 #pragma unroll
-                for (int v = 0; v < 3; v++)
+                    for (int v = 0; v < 3; v++)
 #pragma unroll
-                    for (int k = 0; k < P::NV; k++) vo[v * nvs + k] = u[v][k];
-                if constexpr (P::GRAD) {   // screen gradients for the mip level, once per triangle
+                        for (int k = 0; k < P::NV; k++) vo[v * nvs + k] = u[v][k];
+                }
+                if constexpr (P::GRAD) if (pieces) {   // screen gradients for the mip level, once per triangle
                     float g[6];
                     ps_tri_grad(tr, u[0], u[1], u[2], P::IU, g);
                     float4* go = (float4*)(grads + t * 8);
@@ -678,18 +684,33 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // bounds: first pair of each bin key (lower_bound)
 // ---------------------------------------------------------------------------
+// One wave per key k: the first position i with keys[i] >= k, by a 64-way
+// search (each step loads 64 pivots at once), so a lookup costs about four
+// dependent loads instead of twenty.
 __global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int64_t n_keys,
                                                 uint32_t* __restrict__ start, const PsCounters* c)
 {
     if (ps_poisoned(c)) return;
+    int lane = threadIdx.x & 63;
     int64_t n = *n_ptr;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k <= n_keys; k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t k = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); k <= n_keys; k += waves) {
         int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if (keys[mid] < (uint32_t)k) lo = mid + 1; else hi = mid;
+        while (hi - lo > 64) {
+            int64_t step = (hi - lo + 63) / 64;
+            int64_t p = lo + lane * step;
+            bool below = p < hi && keys[p] < (uint32_t)k;
+            int cnt = __popcll(__ballot(below));
+            if (cnt == 0) { hi = lo; break; }
+            int64_t nlo = lo + (int64_t)(cnt - 1) * step + 1;
+            int64_t nhi = lo + (int64_t)cnt * step;
+            lo = nlo;
+            hi = nhi < hi ? nhi : hi;
         }
-        start[k] = (uint32_t)lo;
+        int64_t q = lo + lane;
+        bool below = q < hi && keys[q] < (uint32_t)k;
+        int cnt = __popcll(__ballot(below));
+        if (lane == 0) start[k] = (uint32_t)(lo + cnt);
     }
 }
 
@@ -1106,7 +1127,7 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
 void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
                 const PsCounters* c)
 {
-    int g = gsz(n_keys + 1, 256); if (g > 8192) g = 8192;
+    int g = gsz(n_keys + 1, 4); if (g > 16384) g = 16384;   // four keys (waves) per block
     hipLaunchKernelGGL(k_bounds, dim3(g), dim3(256), 0, s, keys, n_dev, n_keys, start, c);
 }
 

@@ -153,3 +153,33 @@ def big_triangle_scene(prog, seed, width=512, height=384, n=12):
     sc.meshes["m"] = _slots_for(prog, P.reshape(-1, 4), rng)
     sc.draws[0].behavior = N.UPDATE_DEPTH | N.TEST_DEPTH   # both windings drawn
     return sc
+
+
+def sliver_scene(prog, seed, width=160, height=96, n=1000):
+    """triangles lying wholly left or right of the screen whose first row is
+    below their lowest vertex: the row range truncates y toward zero, so that
+    row evaluates the edges by extrapolation (rasterizer.cpp:65-68,149-163),
+    and a long, nearly flat edge carries the span back onto the screen"""
+    rng = np.random.default_rng(seed)
+    sc = micro_scene(prog, seed, width, height, n=12)
+    sc.name = "sliver_p%d_s%d_%dx%d" % (prog, seed, width, height)
+    hw, hh = width // 2, height // 2
+    P = np.zeros((n, 3, 4), np.float32)
+    for t in range(n):
+        side = -1.0 if rng.random() < 0.5 else 1.0
+        yb = float(rng.integers(2, height - 4)) + rng.uniform(0.5, 0.98)
+        xb = -rng.uniform(3, 60) if side < 0 else width + rng.uniform(3, 60)
+        dy = rng.uniform(0.05, 0.6)
+        far = (xb + side * rng.uniform(100, 400), yb + dy)     # long, flat edge away from the screen
+        third = (xb + side * rng.uniform(0, 20), yb + rng.uniform(1.0, 3.0))
+        pts = [(xb, yb), far, third]
+        if rng.random() < 0.5:
+            pts = [pts[0], pts[2], pts[1]]
+        for v, (X, Y) in enumerate(pts):
+            P[t, v, 0] = (X - hw) / hw
+            P[t, v, 1] = (Y - hh) / hh
+        P[t, :, 2] = rng.uniform(-0.5, 0.5)
+        P[t, :, 3] = 1.0
+    sc.meshes["m"] = _slots_for(prog, P.reshape(-1, 4), rng)
+    sc.draws[0].behavior = N.UPDATE_DEPTH | N.TEST_DEPTH
+    return sc

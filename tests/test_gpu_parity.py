@@ -273,3 +273,9 @@ def test_long_spans_bitexact(oracle, prog, tiling):
     sc = scenegen.big_triangle_scene(prog, 5)
     st = assert_same(sc, oracle, tiling=tiling)
     assert st["segments"] > 4 * st["spans"] // 3
+
+
+@pytest.mark.parametrize("prog", [N.PROC_DEF02, N.PROC_BLINN])
+def test_sliver_extrapolated_spans_bitexact(oracle, prog):
+    sc = scenegen.sliver_scene(prog, 9)
+    assert_same(sc, oracle)
