@@ -52,6 +52,27 @@ def kernel_alg_bytes(name, scene):
     return {"setup": vb, "raster": px * 4, "shade": px * 4 + tb}.get(name)
 
 
+PROFILE_NAMES = {"setup": "k_setup<7>", "raster": "k_raster<false>", "shade": "k_shade<7>", "span": "k_span",
+                 "segments": "k_segments", "expand": "k_expand"}
+
+
+def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "traffic_terrain1080p.md")):
+    """HBM bytes per launch of a kernel of this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration it states), or None"""
+    name = PROFILE_NAMES.get(kernel)
+    if not name or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 7 and cells[0] == name:
+            try:
+                return int(round(float(cells[6]) * 1e6))
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(scene, seconds, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -339,7 +360,9 @@ def main():
     vb, fb, tb = algorithmic_bytes(scene)
     B = vb + fb + tb
     roof = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-            "traffic": None}
+            "traffic": measured_traffic(dom)}
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = "profiles/traffic_terrain1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
     if alg:
         achieved = alg / (per_kernel[dom] * 1e-3) / 1e9
         roof.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), alg_bytes=alg,
