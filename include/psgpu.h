@@ -226,7 +226,8 @@ int psgpu_graph_destroy(psgpu_ctx* ctx, int graph);
 
 /* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a),
  *      5:(int)(unsigned)x86(a), 6:log2 approximation of the trilinear LOD, 7:a (a plain copy kernel,
- *      the calibration point for HBM traffic counters)} ---- */
+ *      the calibration point for HBM traffic counters), 8 and 9: the wrapped texel coordinates of v and
+ *      v + 1 modulo m (v, m and the result are int32 bit patterns in the float arrays a, b, out)} ---- */
 int psgpu_selftest_math(psgpu_ctx* ctx, int op, const float* a, const float* b, float* out, int n);
 
 #ifdef __cplusplus

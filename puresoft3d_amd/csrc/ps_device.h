@@ -259,20 +259,34 @@ PS_DEV void ps_span_varyings_n(const PsSpanGeom& g, const float* u0, const float
 PS_DEV int ps_lvl_w(const PsTex& t, int l) { int w = t.width >> l; return w < 1 ? 1 : w; }
 PS_DEV int ps_lvl_h(const PsTex& t, int l) { int h = t.height >> l; return h < 1 ? 1 : h; }
 
+// floor modulo v - m*floor(v/m), in [0, m), for m >= 1: what clampCoord's
+// "v % m, plus m when negative" yields (fbo.cpp:552-594).  For |v| < 2^21 the
+// quotient comes from the 1-ulp hardware reciprocal: |v * rcp(m) - v/m| <
+// 2^21 * 1.5 * 2^-23 < 1, so the floored estimate is within one of the true
+// quotient and a single correction each way makes the remainder exact.
+// Larger |v| (and INT_MIN, the conversion of NaN) take the integer remainder.
+PS_DEV int ps_floor_mod(int v, int m, float rm)
+{
+    if ((unsigned)v + (1u << 21) < (2u << 21)) {
+        int q = (int)floorf((float)v * rm);
+        int r = v - q * m;
+        r = r < 0 ? r + m : r;
+        return r >= m ? r - m : r;
+    }
+    int r = v % m;
+    return r < 0 ? r + m : r;
+}
+
+PS_DEV float ps_rcp_approx(int m) { return __builtin_amdgcn_rcpf((float)m); }
+
 PS_DEV void ps_clamp_coord(int wrap, int max_row, int max_col, int& row, int& col)
 {
     if (!wrap) {
         if (row > max_row) row = max_row; else if (row < 0) row = 0;
         if (col > max_col) col = max_col; else if (col < 0) col = 0;
     } else {
-        // row % max_row is row itself for 0 <= row < max_row: the division
-        // only runs for coordinates outside that range
-        if (max_row > 0) {
-            if ((unsigned)row >= (unsigned)max_row) { row = row % max_row; if (row < 0) row += max_row; }
-        } else row = 0;
-        if (max_col > 0) {
-            if ((unsigned)col >= (unsigned)max_col) { col = col % max_col; if (col < 0) col += max_col; }
-        } else col = 0;
+        row = max_row > 0 ? ps_floor_mod(row, max_row, ps_rcp_approx(max_row)) : 0;
+        col = max_col > 0 ? ps_floor_mod(col, max_col, ps_rcp_approx(max_col)) : 0;
     }
 }
 
@@ -315,16 +329,22 @@ PS_DEV uint32_t ps_sample_cube(const PsTex& t, const float* d)
     return ps_fetch(t, face, 0, row, col);
 }
 
-// one coordinate of clampCoord (fbo.cpp:552-594)
-PS_DEV int ps_clamp_1(int wrap, int maxv, int v)
+// clampCoord of the two neighbouring coordinates v and v + 1 (the second
+// wraps around at INT_MAX like the reference's int arithmetic).  With wrap, the
+// second is the first's successor modulo maxv unless v + 1 overflows.
+PS_DEV void ps_clamp_pair(int wrap, int maxv, int v, int& a, int& b)
 {
-    if (!wrap) return v > maxv ? maxv : (v < 0 ? 0 : v);
-    if (maxv <= 0) return 0;
-    if ((unsigned)v >= (unsigned)maxv) {   // v % maxv is v itself inside [0, maxv)
-        v = v % maxv;
-        if (v < 0) v += maxv;
+    int v1 = (int)((unsigned)v + 1u);
+    if (!wrap) {
+        a = v > maxv ? maxv : (v < 0 ? 0 : v);
+        b = v1 > maxv ? maxv : (v1 < 0 ? 0 : v1);
+        return;
     }
-    return v;
+    if (maxv <= 0) { a = b = 0; return; }
+    float rm = ps_rcp_approx(maxv);
+    a = ps_floor_mod(v, maxv, rm);
+    if (v != 0x7fffffff) b = a + 1 == maxv ? 0 : a + 1;
+    else b = ps_floor_mod(v1, maxv, rm);
 }
 
 // bilinear filter: texel j covers u = j/W (the nearest sampler's rounding rule).
@@ -338,9 +358,9 @@ PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, floa
     float x0 = floorf(fx), y0 = floorf(fy);
     float ax = fx - x0, ay = fy - y0;
     int ix = ps_cvt_i32(x0), iy = ps_cvt_i32(y0);
-    int ix1 = (int)((unsigned)ix + 1u), iy1 = (int)((unsigned)iy + 1u);
-    int r0 = ps_clamp_1(t.wrap, h - 1, iy), r1 = ps_clamp_1(t.wrap, h - 1, iy1);
-    int c0 = ps_clamp_1(t.wrap, w - 1, ix), c1 = ps_clamp_1(t.wrap, w - 1, ix1);
+    int r0, r1, c0, c1;
+    ps_clamp_pair(t.wrap, h - 1, iy, r0, r1);
+    ps_clamp_pair(t.wrap, w - 1, ix, c0, c1);
     int scan = level == 0 ? t.scanline : w * t.elem_len;
     const uint8_t* base = t.data + t.level_off[level];
     const uint8_t* row0 = base + (int64_t)r0 * scan;

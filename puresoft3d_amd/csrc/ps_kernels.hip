@@ -1022,6 +1022,12 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 5: r = (float)(int)ps_cvt_u32(a[i]); break;
     case 6: r = ps_log2(a[i]); break;
     case 7: r = a[i]; break;   // plain copy: 4 bytes read + 4 written per element
+    case 8: case 9: {           // wrapped texel pair (v, v + 1) modulo m; int bits in and out
+        int v = __float_as_int(a[i]), m = __float_as_int(b[i]), c0, c1;
+        ps_clamp_pair(1, m, v, c0, c1);
+        r = __int_as_float(op == 8 ? c0 : c1);
+        break;
+    }
     }
     out[i] = r;
 }

@@ -78,6 +78,37 @@ def test_math_selftest():
         ctx.close()
 
 
+def test_wrapped_texel_pairs():
+    """clampCoord's wrap (fbo.cpp:552-594) of a texel pair (v, v + 1): the
+    kernels' reciprocal-based floor modulo equals the integer remainder, plus m
+    when negative, for every divisor up to 8192 and dividends around the fast
+    path's edges (+-2^21), the multiples of m and the int32 extremes"""
+    announce("selftest wrapped texel pairs")
+    rng = np.random.default_rng(3)
+    ms, vs = [], []
+    for m in list(range(1, 8193)) + [16383, 32767, 65535]:
+        q = rng.integers(-(1 << 21) // m - 2, (1 << 21) // m + 2, 8)
+        v = np.concatenate([q * m, q * m - 1, q * m + 1, rng.integers(-(1 << 22), 1 << 22, 8),
+                            [0, -1, (1 << 21) - 1, -(1 << 21), 1 << 21, -(1 << 21) - 1,
+                             2**31 - 1, -2**31, 2**31 - 2]])
+        ms.append(np.full(v.size, m)); vs.append(v)
+    m = np.concatenate(ms).astype(np.int64)
+    v = np.clip(np.concatenate(vs), -2**31, 2**31 - 1).astype(np.int64)
+    v1 = ((v + 1 + 2**31) % 2**32) - 2**31
+    want0 = np.mod(v, m).astype(np.int32)      # Python's mod is the floor modulo
+    want1 = np.mod(v1, m).astype(np.int32)
+    a = v.astype(np.int32).view(np.float32)
+    b = m.astype(np.int32).view(np.float32)
+    ctx = N.Context(8, 8)
+    try:
+        got0 = ctx.selftest_math(8, a, b).view(np.int32)
+        got1 = ctx.selftest_math(9, a, b).view(np.int32)
+    finally:
+        ctx.close()
+    np.testing.assert_array_equal(got0, want0)
+    np.testing.assert_array_equal(got1, want1)
+
+
 def test_log2_matches_oracle(oracle):
     announce("selftest log2")
     rng = np.random.default_rng(1)
