@@ -100,14 +100,34 @@ __global__ void k_fills(const PsFill* __restrict__ fills, const PsCounters* c)
 // ---------------------------------------------------------------------------
 // setup
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int ps_find_draw(const PsDraw* draws, int nd, int64_t t)
+// Draw of triangle t (the last draw with tri_start <= t; draws are sorted by
+// tri_start and every t is covered).  The wave first finds the draw of its
+// first triangle with a 64-way search (64 pivots per step, so a couple of
+// dependent loads for thousands of draws); each lane then steps over the draw
+// boundaries between that triangle and its own, usually none.
+__device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draws, int nd, int64_t t)
 {
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (draws[mid].tri_start <= t) lo = mid; else hi = mid - 1;
+    int lane = threadIdx.x & 63;
+    int64_t t0 = (int64_t)__builtin_amdgcn_readfirstlane((int)(t >> 32)) << 32 |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t);
+    // last index with tri_start <= t0 = (first index with tri_start > t0) - 1
+    int lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        int step = (hi - lo + 63) / 64;
+        int p = lo + lane * step;
+        bool le = p < hi && draws[p].tri_start <= t0;
+        int cnt = __popcll(__ballot(le));
+        // pivots lo, lo+step, ... below cnt are <= t0: the answer is in
+        // [lo + (cnt-1)*step, lo + cnt*step)
+        int nlo = lo + (cnt - 1) * step;
+        int nhi = lo + cnt * step;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
+        if (step == 1) break;
     }
-    return lo;
+    int d = lo;
+    while (d + 1 < nd && draws[d + 1].tri_start <= t) d++;
+    return d;
 }
 
 template <int PROG>
@@ -117,10 +137,14 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
                                                const PsView* __restrict__ views)
 {
     typedef PsProc<PROG> P;
+    extern __shared__ float su[];   // per wave: the varyings of its 64 triangles, 3*nvs floats each
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool live = t < T;
-    int di = live ? ps_find_draw(draws, nd, t) : 0;
+    int di = ps_find_draw_wave(draws, nd, live ? t : T - 1);
     unsigned culled = 0, zrej = 0, dropped = 0;
+    const int lane = threadIdx.x & 63, n3 = 3 * nvs;
+    float* wl = su + (size_t)(threadIdx.x & ~63) * n3;
+    bool pieces = false;
     ps_with_draw(draws, di, [&](const PsDraw& d) {
     if (live && d.prog == PROG) {
         const int W = views[d.view].W, H = views[d.view].H;
@@ -202,14 +226,13 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
                 // a triangle without rows has no spans: nothing reads its records
                 // (rows can lie outside the vertices' y range, where edges
                 // extrapolate, so no x range test is safe here)
-                bool pieces = n > 0;
+                pieces = n > 0;
                 if (pieces) tris[t] = tr;
-                float* vo = vary + t * 3 * (int64_t)nvs;
-                if (pieces) {
+                if (pieces) {   // staged: the wave stores whole triangles below
 #pragma unroll
                     for (int v = 0; v < 3; v++)
 #pragma unroll
-                        for (int k = 0; k < P::NV; k++) vo[v * nvs + k] = u[v][k];
+                        for (int k = 0; k < P::NV; k++) wl[lane * n3 + v * nvs + k] = u[v][k];
                 }
                 if constexpr (P::GRAD) if (pieces) {   // screen gradients for the mip level, once per triangle
                     float g[6];
@@ -223,6 +246,23 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
         nrows[t] = n;
     }
     });
+    // varyings of the wave's triangles with rows, one triangle per store
+    // instruction (3*nvs consecutive floats) instead of one 3*nvs-float stride
+    // per lane; floats past NV of each vertex are left as they are
+    uint64_t m = __ballot(pieces);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {
+        int vv = lane / (nvs > 0 ? nvs : 1), kk = lane - vv * nvs;
+        bool mine = vv < 3 && kk < P::NV;
+        int64_t tw = t - lane;
+        while (m) {
+            int tl = __builtin_ctzll(m);
+            m &= m - 1;
+            if (mine) vary[(tw + tl) * n3 + lane] = wl[tl * n3 + lane];
+        }
+    }
     ps_wave_add(cnt, PS_ST_CULLED, culled);
     ps_wave_add(cnt, PS_ST_ZREJ, zrej);
     ps_wave_add(cnt, PS_ST_DROPPED, dropped);
@@ -1056,7 +1096,8 @@ void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, 
                float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
     dim3 g(gsz(T, 256)), b(256);
-#define SU_CASE(K) case K: hipLaunchKernelGGL(k_setup<K>, g, b, 0, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); break;
+    size_t lds = (size_t)256 * 3 * nvs * sizeof(float);
+#define SU_CASE(K) case K: hipLaunchKernelGGL(k_setup<K>, g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); break;
     switch (prog) {
     SU_CASE(PSGPU_PROC_DEF01) SU_CASE(PSGPU_PROC_DEF02) SU_CASE(PSGPU_PROC_DEF03) SU_CASE(PSGPU_PROC_DEF04)
     SU_CASE(PSGPU_PROC_DEF05) SU_CASE(PSGPU_PROC_LAMBERT) SU_CASE(PSGPU_PROC_BLINN)
