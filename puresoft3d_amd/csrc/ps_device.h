@@ -437,6 +437,69 @@ PS_DEV void ps_sample_trilinear(const PsTex& t, float u, float v, float lod, flo
     for (int k = 0; k < NCH; k++) out[k] = a[k] + (b[k] - a[k]) * f;
 }
 
+// Two textures of one geometry (size, row pitch, texel size, levels, wrap):
+// texel coordinates, wrap and addresses are computed once and both are
+// fetched and filtered with them; each result is exactly what
+// ps_sample_trilinear gives for that texture alone.
+PS_DEV bool ps_same_geometry(const PsTex& a, const PsTex& b)
+{
+    return a.width == b.width && a.height == b.height && a.scanline == b.scanline && a.elem_len == b.elem_len &&
+           a.levels == b.levels && a.wrap == b.wrap && a.level_off[1] == b.level_off[1];
+}
+
+template <int NCH = 3>
+PS_DEV void ps_sample_bilinear_pair(const PsTex& t, const uint8_t* data1, int level, float u, float v, float* out0,
+                                    float* out1)
+{
+    int w = ps_lvl_w(t, level), h = ps_lvl_h(t, level);
+    float fx = (float)w * u, fy = (float)h * v;
+    float x0 = floorf(fx), y0 = floorf(fy);
+    float ax = fx - x0, ay = fy - y0;
+    int ix = ps_cvt_i32(x0), iy = ps_cvt_i32(y0);
+    int r0, r1, c0, c1;
+    ps_clamp_pair(t.wrap, h - 1, iy, r0, r1);
+    ps_clamp_pair(t.wrap, w - 1, ix, c0, c1);
+    int scan = level == 0 ? t.scanline : w * t.elem_len;
+    int64_t o00 = t.level_off[level] + (int64_t)r0 * scan + c0 * 4, o01 = t.level_off[level] + (int64_t)r0 * scan + c1 * 4;
+    int64_t o10 = t.level_off[level] + (int64_t)r1 * scan + c0 * 4, o11 = t.level_off[level] + (int64_t)r1 * scan + c1 * 4;
+    const uint8_t* ds[2] = {t.data, data1};
+    float* outs[2] = {out0, out1};
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        uint32_t a = *(const uint32_t*)(ds[q] + o00);
+        uint32_t b = *(const uint32_t*)(ds[q] + o01);
+        uint32_t c = *(const uint32_t*)(ds[q] + o10);
+        uint32_t d = *(const uint32_t*)(ds[q] + o11);
+#pragma unroll
+        for (int k = 0; k < NCH; k++) {
+            float fa = (float)((a >> (8 * k)) & 255u), fb = (float)((b >> (8 * k)) & 255u);
+            float fc = (float)((c >> (8 * k)) & 255u), fd = (float)((d >> (8 * k)) & 255u);
+            float top = fa + (fb - fa) * ax;
+            float bot = fc + (fd - fc) * ax;
+            outs[q][k] = top + (bot - top) * ay;
+        }
+    }
+}
+
+template <int NCH = 3>
+PS_DEV void ps_sample_trilinear_pair(const PsTex& t, const uint8_t* data1, float u, float v, float lod, float* out0,
+                                     float* out1)
+{
+    int maxl = t.levels - 1;
+    if (!(lod > 0.0f) || maxl == 0) { ps_sample_bilinear_pair<NCH>(t, data1, 0, u, v, out0, out1); return; }
+    if (!(lod < (float)maxl)) { ps_sample_bilinear_pair<NCH>(t, data1, maxl, u, v, out0, out1); return; }
+    int l0 = (int)lod;
+    float f = lod - (float)l0;
+    float a0[4], b0[4], a1[4], b1[4];
+    ps_sample_bilinear_pair<NCH>(t, data1, l0, u, v, a0, a1);
+    ps_sample_bilinear_pair<NCH>(t, data1, l0 + 1, u, v, b0, b1);
+#pragma unroll
+    for (int k = 0; k < NCH; k++) {
+        out0[k] = a0[k] + (b0[k] - a0[k]) * f;
+        out1[k] = a1[k] + (b1[k] - a1[k]) * f;
+    }
+}
+
 // screen gradients of rw, u*rw, v*rw over a triangle (mip selection)
 PS_DEV void ps_tri_grad(const PsTri& t, const float* u0, const float* u1, const float* u2, int iu, float* g)
 {

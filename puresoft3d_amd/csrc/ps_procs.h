@@ -259,8 +259,12 @@ template <> struct PsProc<PSGPU_PROC_BLINN> {
         float u = in[12], v = in[13];
         float lod = ps_lod(g, u, v, r, d.tex0.width, d.tex0.height);
         float alb[4], nm[4];
-        ps_sample_trilinear(d.tex0, u, v, lod, alb);
-        ps_sample_trilinear(d.tex1, u, v, lod, nm);
+        if (ps_same_geometry(d.tex0, d.tex1)) {
+            ps_sample_trilinear_pair(d.tex0, d.tex1.data, u, v, lod, alb, nm);
+        } else {
+            ps_sample_trilinear(d.tex0, u, v, lod, alb);
+            ps_sample_trilinear(d.tex1, u, v, lod, nm);
+        }
         const float k2 = 2.0f / 255.0f;
         float bx = nm[2] * k2 - 1.0f, by = nm[1] * k2 - 1.0f, bz = nm[0] * k2 - 1.0f;
         float n[3];
