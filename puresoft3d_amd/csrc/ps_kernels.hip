@@ -84,15 +84,29 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v, const PsCounters*
 }
 
 // a clear command: fill f = blockIdx.y; clear4 (fbo.cpp:332-346) is a fill of
-// rows 0 .. height-2, clear16 (fbo.cpp:348-371) one of the whole buffer
-__global__ void k_fills(const PsFill* __restrict__ fills, const PsCounters* c)
+// rows 0 .. height-2, clear16 (fbo.cpp:348-371) one of the whole buffer.
+// Work items are pieces of 1024 values of one row (a 4 KiB stride keeps the
+// row's alignment); inside a piece the 16-byte aligned middle is stored four
+// values per thread.
+#define FILL_PIECE 1024
+__global__ void __launch_bounds__(256) k_fills(const PsFill* __restrict__ fills, const PsCounters* c)
 {
     if (c && ps_poisoned(c)) return;
     const PsFill fl = fills[blockIdx.y];
-    int64_t n = fl.rows * fl.width;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t y = i / fl.width, x = i - y * fl.width;
-        fl.ptr[y * fl.scan + x] = fl.value;
+    const uint4 v4 = make_uint4(fl.value, fl.value, fl.value, fl.value);
+    const int64_t npc = (fl.width + FILL_PIECE - 1) / FILL_PIECE;
+    const int64_t total = fl.rows * npc;
+    for (int64_t k = blockIdx.x; k < total; k += gridDim.x) {
+        int64_t y = k / npc, x0 = (k - y * npc) * FILL_PIECE;
+        int64_t n = fl.width - x0 < FILL_PIECE ? fl.width - x0 : FILL_PIECE;
+        uint32_t* p = fl.ptr + y * fl.scan + x0;
+        int head = (int)(((16u - ((uintptr_t)p & 15u)) & 15u) / 4);
+        if (head > n) head = (int)n;
+        int64_t nvec = (n - head) / 4;
+        int64_t tail0 = head + nvec * 4;
+        if ((int)threadIdx.x < head) p[threadIdx.x] = fl.value;
+        if ((int64_t)threadIdx.x < nvec) ((uint4*)(p + head))[threadIdx.x] = v4;
+        if (tail0 + threadIdx.x < n) p[tail0 + threadIdx.x] = fl.value;
     }
 }
 
@@ -1084,11 +1098,13 @@ void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCou
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, p, n, v, c);
 }
 
-void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_elems, const PsCounters* c)
+int64_t psk_fill_pieces(int64_t rows, int64_t width) { return rows * ((width + FILL_PIECE - 1) / FILL_PIECE); }
+
+void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, const PsCounters* c)
 {
     if (n <= 0) return;
-    int g = gsz(max_elems, 256); if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_fills, dim3(g, n), dim3(256), 0, s, fills, c);
+    int64_t g = max_pieces < 1 ? 1 : max_pieces; if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fills, dim3((unsigned)g, n), dim3(256), 0, s, fills, c);
 }
 
 

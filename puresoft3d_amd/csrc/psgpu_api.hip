@@ -561,7 +561,7 @@ int run_clears(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     int rc;
     if (!dev && (rc = stage_blob(c, blob, &dev))) return rc;
     int64_t mx = 1;
-    for (const PsFill& fl : cmd.fills) mx = std::max<int64_t>(mx, fl.rows * fl.width);
+    for (const PsFill& fl : cmd.fills) mx = std::max<int64_t>(mx, psk_fill_pieces(fl.rows, fl.width));
     ProfScope p(c, "clear");
     for (size_t i = 0; i < cmd.fills.size(); i += 65535)
         psk_fills(c->stream, (const PsFill*)dev + i, (int)std::min<size_t>(65535, cmd.fills.size() - i), mx, c->d_cnt);
