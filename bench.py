@@ -73,6 +73,23 @@ def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "traffic_terrai
     return None
 
 
+def measured_valu_share(kernel, path=os.path.join(ROOT, "profiles", "valu_terrain1080p.md")):
+    """VALU issue share of a kernel of this workload (SQ_INSTS_VALU x 4 cycles
+    over the SIMD-cycles of its duration at peak clock), from the committed
+    rocprofv3 SQ-counter summary, or None"""
+    name = PROFILE_NAMES.get(kernel)
+    if not name or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 9 and cells[0] == name and cells[8].endswith("%"):
+            try:
+                return round(float(cells[8][:-1]) / 100.0, 3)
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(scene, seconds, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -363,6 +380,10 @@ def main():
             "traffic": measured_traffic(dom)}
     if roof["traffic"] is not None:
         roof["traffic_source"] = "profiles/traffic_terrain1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
+    valu = measured_valu_share(dom)
+    if valu is not None:   # the exact-FP32 shading is bound by vector ALU issue, not HBM
+        roof["valu_issue_share"] = valu
+        roof["valu_source"] = "profiles/valu_terrain1080p.md (rocprofv3 SQ_INSTS_VALU per wave, peak clock)"
     if alg:
         achieved = alg / (per_kernel[dom] * 1e-3) / 1e9
         roof.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), alg_bytes=alg,
