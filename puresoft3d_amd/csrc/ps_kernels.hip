@@ -1140,7 +1140,7 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
               int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt)
 {
-    int g = gsz(cap, 256); if (g > 8192) g = 8192;
+    int g = gsz(cap, 256); if (g > 2048) g = 2048;
     hipLaunchKernelGGL(k_span, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, spans, seg_count, views, chunk_w, cnt);
 }
 
@@ -1148,7 +1148,7 @@ void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, c
                   const PsFrame& f, const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, uint32_t* keys,
                   uint32_t* vals, PsSeg* segs, float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
 {
-    int g = gsz(span_cap, 256); if (g > 8192) g = 8192;
+    int g = gsz(span_cap, 256); if (g > 2048) g = 2048;
     hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, tris, draws, vary, nvs, keys,
                        vals, segs, ck, seg_cap, cnt);
 }
