@@ -147,6 +147,15 @@ def test_terrain_reduced_bitexact(oracle):
     assert_same(sc, oracle)
 
 
+def test_terrain_1080p_bitexact(oracle):
+    """the benchmark workload itself: 1920x1080, 200,480 triangles, 2048^2
+    trilinear albedo and normal maps, default 8x32 tiles (8,100 tiles: 13-bit
+    keys, two sort passes and the tile-start search)"""
+    sc = S.terrain_scene()
+    st = assert_same(sc, oracle)
+    assert st["frags_shaded"] > 1_000_000
+
+
 def test_two_frames_accumulate(oracle):
     sc = S.sphere_scene(width=200, height=150)
     assert_same(sc, oracle, frames=2)
