@@ -183,3 +183,22 @@ def sliver_scene(prog, seed, width=160, height=96, n=1000):
     sc.meshes["m"] = _slots_for(prog, P.reshape(-1, 4), rng)
     sc.draws[0].behavior = N.UPDATE_DEPTH | N.TEST_DEPTH
     return sc
+
+
+def objx_scene(which, width=640, height=480):
+    """the reference's own assets (src/test/sphere.objx, src/test2/plane.objx,
+    kept gzipped in tests/golden) through the OBJX loader.  sphere.objx's one
+    mesh has no 'object/' prefix, which the demo loader skips; it is drawn here
+    as 'ball/sphere' with a diffuse file, so that it takes the textured
+    tangent-space programme."""
+    import gzip
+    import os
+    from puresoft3d_amd import objx as X
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "%s.objx.gz" % which)
+    with gzip.open(path, "rb") as f:
+        sc = X.read_objx(f.read())
+    if which == "sphere":
+        m = sc.meshes[0]
+        sc.meshes = [X.ObjxMesh("ball/" + m.name, m.vertices, m.normals, m.tangents, m.texcoords, m.indices, m.ambient,
+                                m.diffuse, m.specular, m.specular_exponent, "sphere.jpg")]
+    return X.load_scene(sc, width, height, name="objx_%s_%dx%d" % (which, width, height))

@@ -156,6 +156,16 @@ def test_terrain_1080p_bitexact(oracle):
     assert st["frags_shaded"] > 1_000_000
 
 
+@pytest.mark.parametrize("which", ["sphere", "plane"])
+def test_objx_assets_bitexact(oracle, which):
+    """the reference's own meshes (src/test/sphere.objx: 528 triangles;
+    src/test2/plane.objx: 13 meshes, 10,414 triangles, a third of the jar's
+    tangents NaN as exported), loaded as the demo loader prepares them"""
+    sc = scenegen.objx_scene(which)
+    st = assert_same(sc, oracle)
+    assert st["frags_shaded"] > 10000
+
+
 def test_two_frames_accumulate(oracle):
     sc = S.sphere_scene(width=200, height=150)
     assert_same(sc, oracle, frames=2)
