@@ -47,6 +47,10 @@ extern "C" {
 #define PSGPU_PROC_DEF05    5   /* shadow.cpp          */
 #define PSGPU_PROC_LAMBERT  6   /* Lambert x bilinear texture, directional light (slot 7 = light dir, 9 = tex) */
 #define PSGPU_PROC_BLINN    7   /* Blinn-Phong + normal map, trilinear (7 light, 8 camera, 9 albedo, 10 normal map, 11 exponent) */
+#define PSGPU_PROC_SHADOWED 8   /* PSGPU_PROC_BLINN lit through a shadow map: the Lambert term times
+                                   PuresoftSamplerProjection::get (samplrproj.cpp:20-37) of the world position
+                                   projected by uniform 12 (16 floats, the light's PV) into the FP32 depth
+                                   texture named by uniform 13 (rendered by a PSGPU_PROC_DEF05 pass) */
 
 /* ---- wrap modes (fbo.h:18) ---- */
 #define PSGPU_CLAMP 0

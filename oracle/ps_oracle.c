@@ -216,6 +216,30 @@ static inline void fetch4(const or_texture* t, int layer, int level, int row, in
     memcpy(out, p, 4);
 }
 
+/* PuresoftSamplerProjection::get, samplrproj.cpp:3-37.  div_3_4 multiplies by
+   the reciprocal of w (vector.cpp:158-169, rcpps as 1/x); the row offsets are
+   POISSON_DISK_X[i]/500 and the column offsets POISSON_DISK_Y[i]/500, as there;
+   the taps convert with (unsigned int) and read through directRead4
+   (clampCoord, fbo.cpp:286-291), the four bytes taken as a float. */
+float or_sample_projection(const or_texture* t, const float* proj)
+{
+    static const float PDX[4] = {-0.94201624f, 0.94558609f, -0.094184101f, 0.34495938f};
+    static const float PDY[4] = {-0.39906216f, -0.76890725f, -0.92938870f, 0.29387760f};
+    float r = 1.0f / proj[3];
+    float tx = proj[0] * r, ty = proj[1] * r, tz = proj[2] * r;
+    float f = 1.0f;
+    for (int i = 0; i < 4; i++) {
+        int row = (int)cvt_u32((float)t->height * (ty + PDX[i] / 500.0f));
+        int col = (int)cvt_u32((float)t->width * (tx + PDY[i] / 500.0f));
+        uint8_t b[4];
+        fetch4(t, 0, 0, row, col, b);
+        float d;
+        memcpy(&d, b, 4);
+        if (d < tz) f -= 0.2f;
+    }
+    return f;
+}
+
 /* PuresoftSampler2D::get4, samplr2d.cpp:19-25 -> directRead4, fbo.cpp:286-291 */
 void or_sample_nearest(const or_texture* t, int layer, float u, float v, uint8_t* out4)
 {
@@ -637,7 +661,8 @@ int or_is_back_face(const float* v0, const float* v1, const float* v2)
 typedef struct {
     int prog, nv, needs_grad;
     const float *PV, *M, *Mrot, *P, *V, *lightPos, *cameraPos, *lightDir;
-    const or_texture *tex0, *tex1;
+    const or_texture *tex0, *tex1, *tex2;
+    const float* shadowPV;  /* SHADOWED */
     unsigned spec_exp;
     float invView[16];  /* DEF04 */
     float PVM[16];      /* DEF05 */
@@ -653,6 +678,7 @@ static int prog_nv(int prog)
     case OR_PROG_DEF05: return 0;
     case OR_PROG_LAMBERT: return 5;
     case OR_PROG_BLINN: return 14;
+    case OR_PROG_SHADOWED: return 18;
     default: return -1;
     }
 }
@@ -706,7 +732,7 @@ static int prog_prepare(const or_draw* d, or_prog* p)
         p->tex0 = tex_from_uniform(d, 9);
         if (!p->tex0) return -4;
         break;
-    case OR_PROG_BLINN:
+    case OR_PROG_BLINN: case OR_PROG_SHADOWED:
         p->PV = u[3]; p->M = u[4]; p->Mrot = u[5]; p->lightPos = u[7]; p->cameraPos = u[8];
         if (!p->PV || !p->M || !p->Mrot || !p->lightPos || !p->cameraPos) return -3;
         p->tex0 = tex_from_uniform(d, 9);
@@ -714,6 +740,12 @@ static int prog_prepare(const or_draw* d, or_prog* p)
         if (!p->tex0 || !p->tex1) return -4;
         p->spec_exp = u[11] ? (unsigned)*(const int*)u[11] : 32u;
         p->needs_grad = 1;
+        if (d->prog == OR_PROG_SHADOWED) {
+            p->shadowPV = u[12];
+            p->tex2 = tex_from_uniform(d, 13);
+            if (!p->shadowPV) return -3;
+            if (!p->tex2) return -4;
+        }
         break;
     }
     return 0;
@@ -791,9 +823,14 @@ static void run_vp(const or_draw* d, const or_prog* p, int64_t i, float* pos, fl
         user[0] = n[0]; user[1] = n[1]; user[2] = n[2]; user[3] = uv[0]; user[4] = uv[1];
         break;
     }
-    case OR_PROG_BLINN: {
+    case OR_PROG_BLINN: case OR_PROG_SHADOWED: {
         m4v4(wp, p->M, slot_elem(d, 0, i));
         m4v4(pos, p->PV, wp);
+        if (p->prog == OR_PROG_SHADOWED) {   /* world position in shadow-map clip space */
+            float sc[4];
+            m4v4(sc, p->shadowPV, wp);
+            for (int k = 0; k < 4; k++) user[14 + k] = sc[k];
+        }
         float t[4], b[4], n[4];
         m4v4(t, p->Mrot, slot_elem(d, 1, i));
         m4v4(b, p->Mrot, slot_elem(d, 2, i));
@@ -927,9 +964,11 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
         out[3] = 0;
         return 1;
     }
-    case OR_PROG_BLINN: {
+    case OR_PROG_BLINN: case OR_PROG_SHADOWED: {
         const float *T = in, *B = in + 3, *N = in + 6, *wp = in + 9;
         float u = in[12], v = in[13];
+        /* the shadow factor is a discrete lookup: both variants take it in FP32 */
+        float sf = p->prog == OR_PROG_SHADOWED ? or_sample_projection(p->tex2, in + 14) : 1.0f;
         if (!fp64) {
             float lod = lod_from_grad(grad, u, v, r, p->tex0->width, p->tex0->height);
             float alb[4], nm[4];
@@ -953,6 +992,7 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
             for (int k = 0; k < 3; k++) H[k] = E[k] + L[k];
             norm3(H);
             float lam = dot3(L, n);
+            lam = lam * sf;
             float spec = dot3(H, n);
             spec = spec < 0 ? 0 : spec;
             spec = opt_powf(spec, p->spec_exp);
@@ -977,7 +1017,7 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
             for (int k = 0; k < 3; k++) H[k] = E[k] + L[k];
             s = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
             for (int k = 0; k < 3; k++) H[k] /= s;
-            double lam = L[0] * n[0] + L[1] * n[1] + L[2] * n[2];
+            double lam = (L[0] * n[0] + L[1] * n[1] + L[2] * n[2]) * sf;
             double spec = H[0] * n[0] + H[1] * n[1] + H[2] * n[2];
             spec = spec < 0 ? 0 : spec;
             spec = opt_powd(spec, p->spec_exp);

@@ -47,6 +47,8 @@ extern "C" {
 #define OR_PROG_DEF05 5   /* shadow.cpp          */
 #define OR_PROG_LAMBERT 6 /* Lambert x bilinear texture, directional light  */
 #define OR_PROG_BLINN   7 /* Blinn-Phong + normal map, trilinear, point light */
+#define OR_PROG_SHADOWED 8 /* OR_PROG_BLINN with the Lambert term times the projective shadow sampler
+                              (uniform 12: the light's PV, 16 floats; uniform 13: FP32 shadow-map texture) */
 
 typedef struct {
     int width, height;     /* level-0 texel dims                         */
@@ -135,6 +137,9 @@ void or_sample_nearest(const or_texture* tex, int layer, float u, float v, uint8
 void or_sample_cube(const or_texture* tex, const float* dir, uint8_t* out4);
 void or_sample_bilinear(const or_texture* tex, int layer, int level, float u, float v, float* out4);
 float or_log2_approx(float x);
+/* PuresoftSamplerProjection::get (samplrproj.cpp:20-37) over an FP32 texture:
+ * 1 minus 0.2 per Poisson tap whose stored depth is below the projected z */
+float or_sample_projection(const or_texture* tex, const float* proj4);
 
 /* Hash used by the debug planes. */
 uint32_t or_tri_hash(uint32_t tri);

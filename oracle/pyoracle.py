@@ -84,6 +84,8 @@ def lib():
         L.or_sample_bilinear.argtypes = [C.POINTER(Texture), C.c_int, C.c_int, C.c_float, C.c_float, f]
         L.or_log2_approx.argtypes = [C.c_float]
         L.or_log2_approx.restype = C.c_float
+        L.or_sample_projection.argtypes = [C.POINTER(Texture), f]
+        L.or_sample_projection.restype = C.c_float
         L.or_tri_hash.argtypes = [C.c_uint32]
         L.or_tri_hash.restype = C.c_uint32
         L.or_pool_shutdown.argtypes = []
@@ -254,6 +256,22 @@ class OracleTexture:
                               width=self.w, height=self.h)
 
 
+class _TexSpec:
+    """one-layer, one-level, clamped texture of raw 4-byte texels"""
+
+    def __init__(self, img):
+        self.images = [img]
+        self.height, self.width = img.shape[:2]
+        self.layers, self.levels, self.wrap = 1, 1, 0
+
+
+def sample_projection(tex, proj):
+    """PuresoftSamplerProjection::get of a projected coordinate (4 floats)"""
+    p = np.ascontiguousarray(proj, np.float32)
+    s = tex.struct()
+    return float(lib().or_sample_projection(C.byref(s), _fp(p)))
+
+
 def mesh_triangles(mesh):
     return min(a.shape[0] for a in mesh.values()) // 3
 
@@ -262,12 +280,25 @@ def render_scene(scene, n_threads=1, debug=False, fp64=False, frame=None, clear=
     """Run a puresoft3d_amd.scenes.Scene through the oracle: clearDepth +
     clearColour + the draws in order.  Returns (Frame, Stats)."""
     f = frame if frame is not None else Frame(scene.width, scene.height, debug=debug)
+    st = Stats()
+    texs = [OracleTexture(t) for t in scene.textures]
+    sp = getattr(scene, "shadow", None)
+    base = 0   # triangle ids count every draw in submission order
+    if sp is not None:
+        # the shadow map first: DEF05 into a size x size FP32 depth texture
+        sf = Frame(sp.size, sp.size, depth_scanline=sp.size, top_down=False)
+        sf.clear_depth(1.0)
+        for d in sp.draws:
+            u = {3: sp.light_pv}
+            u.update(d.uniforms)
+            DrawCall(d.prog, scene.meshes[d.mesh], u, [], behavior=DEFAULT_BEHAVIOR, tri_base=base).run(
+                sf, n_threads, st)
+            base += mesh_triangles(scene.meshes[d.mesh])
+        img = np.ascontiguousarray(sf.depth[:, : sp.size]).view(np.uint8).reshape(sp.size, sp.size, 4)
+        texs.append(OracleTexture(_TexSpec(img)))
     if clear:
         f.clear_depth(scene.clear_depth)
         f.clear_colour(scene.clear_colour)
-    texs = [OracleTexture(t) for t in scene.textures]
-    st = Stats()
-    base = 0
     for d in scene.draws:
         u = dict(scene.uniforms)
         u.update(d.uniforms)

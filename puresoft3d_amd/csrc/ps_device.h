@@ -329,6 +329,31 @@ PS_DEV uint32_t ps_sample_cube(const PsTex& t, const float* d)
     return ps_fetch(t, face, 0, row, col);
 }
 
+// PuresoftSamplerProjection::get (samplrproj.cpp:20-37): the projected
+// coordinate divided by w (div_3_4: times the reciprocal), then four taps
+// offset by the Poisson disk / 500, each darkening by 0.2 where the stored
+// depth is below the coordinate's z.  The row offsets come from
+// POISSON_DISK_X and the column offsets from POISSON_DISK_Y, as there; the
+// taps convert with (unsigned int) and read through clampCoord (directRead4).
+PS_DEV float ps_sample_projection(const PsTex& t, const float* proj)
+{
+    const float DX0 = -0.94201624f, DX1 = 0.94558609f, DX2 = -0.094184101f, DX3 = 0.34495938f;
+    const float DY0 = -0.39906216f, DY1 = -0.76890725f, DY2 = -0.92938870f, DY3 = 0.29387760f;
+    float r = 1.0f / proj[3];
+    float tx = proj[0] * r, ty = proj[1] * r, tz = proj[2] * r;
+    float f = 1.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float dx = i == 0 ? DX0 : i == 1 ? DX1 : i == 2 ? DX2 : DX3;
+        float dy = i == 0 ? DY0 : i == 1 ? DY1 : i == 2 ? DY2 : DY3;
+        int row = (int)ps_cvt_u32((float)t.height * (ty + dx / 500.0f));
+        int col = (int)ps_cvt_u32((float)t.width * (tx + dy / 500.0f));
+        float d = __uint_as_float(ps_fetch(t, 0, 0, row, col));
+        if (d < tz) f -= 0.2f;
+    }
+    return f;
+}
+
 // clampCoord of the two neighbouring coordinates v and v + 1 (the second
 // wraps around at INT_MAX like the reference's int arithmetic).  With wrap, the
 // second is the first's successor modulo maxv unless v + 1 overflows.

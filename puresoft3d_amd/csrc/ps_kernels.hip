@@ -1116,7 +1116,7 @@ void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, 
 #define SU_CASE(K) case K: hipLaunchKernelGGL(k_setup<K>, g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); break;
     switch (prog) {
     SU_CASE(PSGPU_PROC_DEF01) SU_CASE(PSGPU_PROC_DEF02) SU_CASE(PSGPU_PROC_DEF03) SU_CASE(PSGPU_PROC_DEF04)
-    SU_CASE(PSGPU_PROC_DEF05) SU_CASE(PSGPU_PROC_LAMBERT) SU_CASE(PSGPU_PROC_BLINN)
+    SU_CASE(PSGPU_PROC_DEF05) SU_CASE(PSGPU_PROC_LAMBERT) SU_CASE(PSGPU_PROC_BLINN) SU_CASE(PSGPU_PROC_SHADOWED)
     }
 #undef SU_CASE
 }
@@ -1214,7 +1214,7 @@ void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, 
 #define SH_CASE(K) case K: hipLaunchKernelGGL(k_shade<K>, g, b, 0, s, spans, segs, ck, tris, vary, nvs, grads, draws, f, cnt); break;
     switch (prog) {
     SH_CASE(PSGPU_PROC_DEF01) SH_CASE(PSGPU_PROC_DEF02) SH_CASE(PSGPU_PROC_DEF03) SH_CASE(PSGPU_PROC_DEF04)
-    SH_CASE(PSGPU_PROC_LAMBERT) SH_CASE(PSGPU_PROC_BLINN)
+    SH_CASE(PSGPU_PROC_LAMBERT) SH_CASE(PSGPU_PROC_BLINN) SH_CASE(PSGPU_PROC_SHADOWED)
     default: break;
     }
 #undef SH_CASE

@@ -230,70 +230,102 @@ template <> struct PsProc<PSGPU_PROC_LAMBERT> {
 
 // ---- BLINN: Blinn-Phong + tangent-space normal map, trilinear ----------
 // varyings: tangent[3], binormal[3], normal[3], worldPos[3], texcoord[2]
+// (SHADOWED adds the world position in shadow-map clip space[4])
+PS_DEV void ps_blinn_vp(const PsDraw& d, int64_t i, float* pos, float* u, float* wp)
+{
+    float p[4], a[4], r[4];
+    ps_load4(p, ps_slot(d, 0, i));
+    ps_m4v4(wp, d.M, p);
+    ps_m4v4(pos, d.PV, wp);
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        ps_load4(a, ps_slot(d, 1 + s, i));
+        ps_m4v4(r, d.Mrot, a);
+        u[3 * s] = r[0]; u[3 * s + 1] = r[1]; u[3 * s + 2] = r[2];
+    }
+    u[9] = wp[0]; u[10] = wp[1]; u[11] = wp[2];
+    const float* tc = ps_slot(d, 4, i);
+    u[12] = tc[0]; u[13] = tc[1];
+}
+
+// the Blinn-Phong colour, the Lambert term scaled by `shadow`
+PS_DEV bool ps_blinn_fp(const PsDraw& d, const float* in, float r, const float* g, float shadow, uint32_t& out)
+{
+    const float* T = in;
+    const float* B = in + 3;
+    const float* N = in + 6;
+    const float* wp = in + 9;
+    float u = in[12], v = in[13];
+    float lod = ps_lod(g, u, v, r, d.tex0.width, d.tex0.height);
+    float alb[4], nm[4];
+    if (ps_same_geometry(d.tex0, d.tex1)) {
+        ps_sample_trilinear_pair(d.tex0, d.tex1.data, u, v, lod, alb, nm);
+    } else {
+        ps_sample_trilinear(d.tex0, u, v, lod, alb);
+        ps_sample_trilinear(d.tex1, u, v, lod, nm);
+    }
+    const float k2 = 2.0f / 255.0f;
+    float bx = nm[2] * k2 - 1.0f, by = nm[1] * k2 - 1.0f, bz = nm[0] * k2 - 1.0f;
+    float n[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float a = bx * T[k], b = by * B[k], c = bz * N[k];
+        n[k] = (a + b) + c;
+    }
+    ps_norm3(n);
+    float L[3], E[3], H[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) L[k] = d.lightPos[k] - wp[k];
+    float dist = sqrtf(ps_dot3(L, L));
+    float rd = 1.0f / dist;
+#pragma unroll
+    for (int k = 0; k < 3; k++) L[k] *= rd;
+#pragma unroll
+    for (int k = 0; k < 3; k++) E[k] = d.cameraPos[k] - wp[k];
+    ps_norm3(E);
+#pragma unroll
+    for (int k = 0; k < 3; k++) H[k] = E[k] + L[k];
+    ps_norm3(H);
+    float lam = ps_dot3(L, n);
+    lam = lam * shadow;   // x 1.0f (Blinn-Phong) is exact
+    float spec = ps_dot3(H, n);
+    spec = spec < 0 ? 0 : spec;
+    spec = ps_opt_pow(spec, (unsigned)d.spec_exp);
+    float s255 = 255.0f * spec;
+    out = ps_pack_bgr(ps_clamp((alb[0] + s255) * lam, 0.0f, 255.0f), ps_clamp((alb[1] + s255) * lam, 0.0f, 255.0f),
+                      ps_clamp((alb[2] + s255) * lam, 0.0f, 255.0f));
+    return true;
+}
+
 template <> struct PsProc<PSGPU_PROC_BLINN> {
     static constexpr int NV = 14;
     static constexpr bool GRAD = true;
     static constexpr int IU = 12;
     PS_DEV static void vp(const PsDraw& d, int64_t i, float* pos, float* u)
     {
-        float p[4], wp[4], a[4], r[4];
-        ps_load4(p, ps_slot(d, 0, i));
-        ps_m4v4(wp, d.M, p);
-        ps_m4v4(pos, d.PV, wp);
-#pragma unroll
-        for (int s = 0; s < 3; s++) {
-            ps_load4(a, ps_slot(d, 1 + s, i));
-            ps_m4v4(r, d.Mrot, a);
-            u[3 * s] = r[0]; u[3 * s + 1] = r[1]; u[3 * s + 2] = r[2];
-        }
-        u[9] = wp[0]; u[10] = wp[1]; u[11] = wp[2];
-        const float* tc = ps_slot(d, 4, i);
-        u[12] = tc[0]; u[13] = tc[1];
+        float wp[4];
+        ps_blinn_vp(d, i, pos, u, wp);
     }
     PS_DEV static bool fp(const PsDraw& d, const float* in, float r, const float* g, uint32_t& out)
     {
-        const float* T = in;
-        const float* B = in + 3;
-        const float* N = in + 6;
-        const float* wp = in + 9;
-        float u = in[12], v = in[13];
-        float lod = ps_lod(g, u, v, r, d.tex0.width, d.tex0.height);
-        float alb[4], nm[4];
-        if (ps_same_geometry(d.tex0, d.tex1)) {
-            ps_sample_trilinear_pair(d.tex0, d.tex1.data, u, v, lod, alb, nm);
-        } else {
-            ps_sample_trilinear(d.tex0, u, v, lod, alb);
-            ps_sample_trilinear(d.tex1, u, v, lod, nm);
-        }
-        const float k2 = 2.0f / 255.0f;
-        float bx = nm[2] * k2 - 1.0f, by = nm[1] * k2 - 1.0f, bz = nm[0] * k2 - 1.0f;
-        float n[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float a = bx * T[k], b = by * B[k], c = bz * N[k];
-            n[k] = (a + b) + c;
-        }
-        ps_norm3(n);
-        float L[3], E[3], H[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) L[k] = d.lightPos[k] - wp[k];
-        float dist = sqrtf(ps_dot3(L, L));
-        float rd = 1.0f / dist;
-#pragma unroll
-        for (int k = 0; k < 3; k++) L[k] *= rd;
-#pragma unroll
-        for (int k = 0; k < 3; k++) E[k] = d.cameraPos[k] - wp[k];
-        ps_norm3(E);
-#pragma unroll
-        for (int k = 0; k < 3; k++) H[k] = E[k] + L[k];
-        ps_norm3(H);
-        float lam = ps_dot3(L, n);
-        float spec = ps_dot3(H, n);
-        spec = spec < 0 ? 0 : spec;
-        spec = ps_opt_pow(spec, (unsigned)d.spec_exp);
-        float s255 = 255.0f * spec;
-        out = ps_pack_bgr(ps_clamp((alb[0] + s255) * lam, 0.0f, 255.0f), ps_clamp((alb[1] + s255) * lam, 0.0f, 255.0f),
-                          ps_clamp((alb[2] + s255) * lam, 0.0f, 255.0f));
-        return true;
+        return ps_blinn_fp(d, in, r, g, 1.0f, out);
+    }
+};
+
+// ---- SHADOWED: BLINN lit through a projective shadow map ---------------
+template <> struct PsProc<PSGPU_PROC_SHADOWED> {
+    static constexpr int NV = 18;
+    static constexpr bool GRAD = true;
+    static constexpr int IU = 12;
+    PS_DEV static void vp(const PsDraw& d, int64_t i, float* pos, float* u)
+    {
+        float wp[4], sc[4];
+        ps_blinn_vp(d, i, pos, u, wp);
+        ps_m4v4(sc, d.shadowPV, wp);
+        u[14] = sc[0]; u[15] = sc[1]; u[16] = sc[2]; u[17] = sc[3];
+    }
+    PS_DEV static bool fp(const PsDraw& d, const float* in, float r, const float* g, uint32_t& out)
+    {
+        return ps_blinn_fp(d, in, r, g, ps_sample_projection(d.tex2, in + 14), out);
     }
 };

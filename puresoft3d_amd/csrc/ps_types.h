@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #define PS_MAX_NV      20   // widest varying block (DEF03: 5 float4 slots)
+#define PS_N_PROGS     9    // programme kinds are 1 .. PS_N_PROGS-1 (PSGPU_PROC_*)
 #define PS_MAX_LEVELS  16
 #define PS_MAX_LAYERS  6
 
@@ -37,7 +38,9 @@ struct PsDraw {
     float invView[16];  // DEF04
     float PVM[16];      // DEF05
     float lightPos[4], cameraPos[4], lightDir[4];
+    float shadowPV[16];  // PSGPU_PROC_SHADOWED: world -> shadow-map clip space
     PsTex tex0, tex1;
+    PsTex tex2;          // PSGPU_PROC_SHADOWED: the FP32 shadow map
 };
 
 // Triangle after vertex processing and setup (vertthrd.cpp:14-65,

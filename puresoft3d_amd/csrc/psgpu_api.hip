@@ -291,6 +291,7 @@ int prog_nv(int kind)
     case PSGPU_PROC_DEF05: return 0;
     case PSGPU_PROC_LAMBERT: return 5;
     case PSGPU_PROC_BLINN: return 14;
+    case PSGPU_PROC_SHADOWED: return 18;
     default: return -1;
     }
 }
@@ -449,7 +450,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     PsFrame f;
     int64_t T, n_pix;
     int nvs;
-    bool progs[8] = {false}, shade_progs[8] = {false};
+    bool progs[PS_N_PROGS] = {false}, shade_progs[PS_N_PROGS] = {false};
     int rc = build_blob(c, cmd, blob, f, T, nvs, progs, shade_progs, n_pix);
     if (rc) return rc;
     if (T == 0) return PSGPU_OK;
@@ -480,11 +481,20 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 2048)) ||
         (rc = ensure(c, c->d_kstart, std::max<int64_t>(f.n_tiles + 1, 2048))))
         return rc;
+    // debug planes record the flushes of the display pass: one view of the
+    // debug viewport into the default depth buffer (a shadow-map pass into a
+    // depth texture is not recorded)
+    bool dbg = false;
     if (c->debug) {
-        if (f.n_views != 1 || c->dbg_w != cmd.views[0].tg.vpw || c->dbg_h != cmd.views[0].tg.vph)
+        const Targets& tg0 = cmd.views[0].tg;
+        bool display_pass = tg0.depth == c->depth;
+        if (display_pass && (f.n_views != 1 || c->dbg_w != tg0.vpw || c->dbg_h != tg0.vph))
             return fail(c, PSGPU_ERR_STATE, "debug planes need a single view of the debug viewport");
-        f.dbg_last = c->dbg_last; f.dbg_rcount = c->dbg_rcount; f.dbg_rhash = c->dbg_rhash;
-        f.dbg_pcount = c->dbg_pcount; f.dbg_phash = c->dbg_phash;
+        if (display_pass) {
+            dbg = true;
+            f.dbg_last = c->dbg_last; f.dbg_rcount = c->dbg_rcount; f.dbg_rhash = c->dbg_rhash;
+            f.dbg_pcount = c->dbg_pcount; f.dbg_phash = c->dbg_phash;
+        }
     }
 
     hipStream_t s = c->stream;
@@ -492,7 +502,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     const PsView* dv = f.views;
     {
         ProfScope p(c, "setup");
-        for (int k = 1; k < 8; k++)
+        for (int k = 1; k < PS_N_PROGS; k++)
             if (progs[k])
                 psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, c->d_nrows.p, c->d_cnt, dv);
     }
@@ -534,12 +544,12 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     {
         ProfScope p(c, "raster");
-        psk_raster(s, c->debug, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
+        psk_raster(s, dbg, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
                    c->tiling.waves_per_group, c->d_cnt);
     }
     {
         ProfScope p(c, "shade");
-        for (int k = 1; k < 8; k++)
+        for (int k = 1; k < PS_N_PROGS; k++)
             if (shade_progs[k])
                 psk_shade(s, k, c->d_spans.p, c->d_segs.p, c->d_ck.p, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, dd, f,
                           c->d_cnt);
@@ -750,7 +760,7 @@ int graph_rebuild(psgpu_ctx* c, Graph& g)
         PsFrame f;
         int64_t T, npx;
         int nvs;
-        bool p1[8] = {false}, p2[8] = {false};
+        bool p1[PS_N_PROGS] = {false}, p2[PS_N_PROGS] = {false};
         int rc = build_blob(c, g.cmds[i], b, f, T, nvs, p1, p2, npx);
         if (rc) return rc;
         HIPCHK(c, hipMemcpyAsync(g.blobs[i], b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, c->stream));
@@ -1287,7 +1297,7 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     case PSGPU_PROC_DEF02: need[1] = 16; need[2] = 16; break;
     case PSGPU_PROC_DEF03: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
     case PSGPU_PROC_LAMBERT: need[3] = 16; need[4] = 8; break;
-    case PSGPU_PROC_BLINN: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
+    case PSGPU_PROC_BLINN: case PSGPU_PROC_SHADOWED: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
     default: break;
     }
     for (int s = 0; s < 5; s++) {
@@ -1301,16 +1311,24 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     const uint8_t* p;
     int rc;
     switch (d.prog) {
-    case PSGPU_PROC_DEF01: case PSGPU_PROC_DEF02: case PSGPU_PROC_DEF03: case PSGPU_PROC_BLINN:
+    case PSGPU_PROC_DEF01: case PSGPU_PROC_DEF02: case PSGPU_PROC_DEF03: case PSGPU_PROC_BLINN: case PSGPU_PROC_SHADOWED:
         if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(d.PV, p, 64);
         if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(d.M, p, 64);
         if ((rc = uniform_bytes(c, 5, 64, &p))) return rc; memcpy(d.Mrot, p, 64);
         if ((rc = uniform_bytes(c, 7, 16, &p))) return rc; memcpy(d.lightPos, p, 16);
         if ((rc = uniform_bytes(c, 8, 16, &p))) return rc; memcpy(d.cameraPos, p, 16);
         if (d.prog != PSGPU_PROC_DEF02 && (rc = uniform_tex(c, 9, &d.tex0))) return rc;
-        if ((d.prog == PSGPU_PROC_DEF03 || d.prog == PSGPU_PROC_BLINN) && (rc = uniform_tex(c, 10, &d.tex1))) return rc;
+        if ((d.prog == PSGPU_PROC_DEF03 || d.prog == PSGPU_PROC_BLINN || d.prog == PSGPU_PROC_SHADOWED) &&
+            (rc = uniform_tex(c, 10, &d.tex1)))
+            return rc;
         d.spec_exp = 32;
-        if (d.prog == PSGPU_PROC_BLINN && c->uniforms[11].size() >= 4) memcpy(&d.spec_exp, c->uniforms[11].data(), 4);
+        if ((d.prog == PSGPU_PROC_BLINN || d.prog == PSGPU_PROC_SHADOWED) && c->uniforms[11].size() >= 4)
+            memcpy(&d.spec_exp, c->uniforms[11].data(), 4);
+        if (d.prog == PSGPU_PROC_SHADOWED) {
+            if ((rc = uniform_bytes(c, 12, 64, &p))) return rc;
+            memcpy(d.shadowPV, p, 64);
+            if ((rc = uniform_tex(c, 13, &d.tex2))) return rc;
+        }
         if (d.spec_exp < 0) return fail(c, PSGPU_ERR_INVALID, "specular exponent must be >= 0");
         break;
     case PSGPU_PROC_DEF04: {
@@ -1339,6 +1357,11 @@ int psgpu_draw(psgpu_ctx* c, int vao)
         if ((rc = uniform_tex(c, 9, &d.tex0))) return rc;
         break;
     }
+    // a texture that a pending view renders into (a shadow map, say) is
+    // sampled only after that view's flush: the shading of one flush runs
+    // after all of its rasterisation, but colour is written while shading
+    for (const PsTex* t : {&d.tex0, &d.tex1, &d.tex2})
+        if (t->data && queue_uses(c, t->data) && (rc = queue_to_log(c))) return rc;
     // targets must cover the viewport
     Targets tg;
     if ((rc = current_targets(c, tg))) return rc;
@@ -1716,7 +1739,7 @@ int psgpu_capture_end(psgpu_ctx* c, int* out_graph)
             PsFrame f;
             int64_t T, npx;
             int nvs;
-            bool p1[8] = {false}, p2[8] = {false};
+            bool p1[PS_N_PROGS] = {false}, p2[PS_N_PROGS] = {false};
             if ((rc = build_blob(c, cmd, b, f, T, nvs, p1, p2, npx))) return rc;
             g.flushes++;
         } else if (cmd.type == CMD_CLEARS) {

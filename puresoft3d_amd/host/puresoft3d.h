@@ -94,6 +94,7 @@ PS_DECLARE_PROCS(DEF04, PSGPU_PROC_DEF04)      // skybox.h
 PS_DECLARE_PROCS(DEF05, PSGPU_PROC_DEF05)      // shadow.h
 PS_DECLARE_PROCS(LAMBERT, PSGPU_PROC_LAMBERT)  // Lambert x bilinear, directional light
 PS_DECLARE_PROCS(BLINN, PSGPU_PROC_BLINN)      // Blinn-Phong + normal map, trilinear
+PS_DECLARE_PROCS(SHADOWED, PSGPU_PROC_SHADOWED) // the same, lit through a projective shadow map (samplrproj.cpp)
 #undef PS_DECLARE_PROCS
 
 // ---- rndr.h: a headless renderer owning the host copy of the display ------------

@@ -19,6 +19,7 @@ OK, ERR_INVALID, ERR_RANGE, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_UNSUPPORTED = 
 UPDATE_DEPTH, TEST_DEPTH, FACE_CULLING, ALPHABLEND = 1, 2, 4, 8
 DEFAULT_BEHAVIOR = UPDATE_DEPTH | TEST_DEPTH | FACE_CULLING   # pipeline.cpp:34
 PROC_DEF01, PROC_DEF02, PROC_DEF03, PROC_DEF04, PROC_DEF05, PROC_LAMBERT, PROC_BLINN = 1, 2, 3, 4, 5, 6, 7
+PROC_SHADOWED = 8   # Blinn-Phong lit through a projective shadow map (uniforms 12, 13)
 CLAMP, WRAP = 0, 1
 DISPLAY = -1
 DEFAULT_DEPTH = -1

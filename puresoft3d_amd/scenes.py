@@ -29,6 +29,17 @@ class DrawSpec:
         self.behavior = behavior
 
 
+class ShadowPass:
+    """A depth-only pass (DEF05, shadow.cpp) rendered before the scene's own
+    draws into a size x size FP32 depth texture, which becomes the scene's
+    texture number len(scene.textures) (clamped).  light_pv: the light's PV
+    (column-major); draws: DrawSpecs of DEF05 with their model matrix in
+    uniform 4."""
+
+    def __init__(self, size, light_pv, draws):
+        self.size, self.light_pv, self.draws = size, light_pv, draws
+
+
 class Scene:
     def __init__(self, width, height):
         self.width, self.height = width, height
@@ -36,6 +47,7 @@ class Scene:
         self.textures = []
         self.uniforms = {}
         self.draws = []
+        self.shadow = None
         self.clear_depth = 1.0
         self.clear_colour = 0
         self.name = ""
@@ -343,6 +355,29 @@ def terrain_scene(width=1920, height=1080, seed=7, cells=300, n_spheres=64, tex_
     return sc
 
 
+def shadow_terrain_scene(width=480, height=270, seed=7, cells=60, n_spheres=8, tex_size=256, shadow_size=256,
+                         light=(30.0, 60.0, 40.0)):
+    """The terrain scene lit through a shadow map: a DEF05 pass renders the
+    terrain and spheres from the point light into a shadow_size^2 depth
+    texture, then PSGPU_PROC_SHADOWED shades with the Lambert term times the
+    projective sampler's factor.  Uniform 12 maps world positions to the
+    shadow map: texture x, y in [0, 1] (a 0.5 scale and offset on the light's
+    clip x, y) and the light's clip z, which is what the map stores."""
+    sc = terrain_scene(width, height, seed, cells, n_spheres, tex_size)
+    sc.name = "shadowed_" + sc.name
+    lp = np.asarray(light, np.float64)
+    sc.uniforms[7] = vec4(*lp, 0.0)
+    LPV = perspective(np.radians(75.0), 1.0, 20.0, 200.0) @ look_at(lp, (0.0, 0.0, 0.0))
+    bias = np.array([[0.5, 0, 0, 0.5], [0, 0.5, 0, 0.5], [0, 0, 1, 0], [0, 0, 0, 1]], np.float64)
+    draws = [DrawSpec(d.mesh, N.PROC_DEF05, {4: d.uniforms.get(4, sc.uniforms[4])}) for d in sc.draws]
+    sc.shadow = ShadowPass(shadow_size, colmajor(LPV), draws)
+    for d in sc.draws:
+        d.prog = N.PROC_SHADOWED
+    sc.uniforms[12] = colmajor(bias @ LPV)
+    sc.uniforms[13] = np.array([len(sc.textures)], np.int32)
+    return sc
+
+
 # ---------------------------------------------------------------- GPU upload
 class GpuScene:
     """A Scene resident on one psgpu context (VBOs, VAOs, textures, programmes)."""
@@ -365,13 +400,38 @@ class GpuScene:
                 ctx.vao_attach(v, s, ctx.vbo_create(arr))
             self.vao[name] = v
         self.progs = {}
-        for d in scene.draws:
+        for d in scene.draws + (scene.shadow.draws if scene.shadow else []):
             if d.prog not in self.progs:
                 self.progs[d.prog] = ctx.program_create(d.prog)
+        self.shadow_tex = None
+        if scene.shadow is not None:
+            n = scene.shadow.size
+            self.shadow_tex = ctx.texture_create(n, n, 4, wrap=N.CLAMP)
+
+    def shadow_pass(self):
+        """the shadow map: depth of the DEF05 draws, seen from the light"""
+        ctx, sp = self.ctx, self.scene.shadow
+        ctx.depth_target(self.shadow_tex)
+        ctx.viewport(sp.size, sp.size)
+        ctx.clear_depth(1.0)
+        ctx.enable(N.DEFAULT_BEHAVIOR)
+        ctx.uniform(3, sp.light_pv)
+        cur = None
+        for d in sp.draws:
+            if d.prog != cur:
+                ctx.program_use(self.progs[d.prog])
+                cur = d.prog
+            for k, v in d.uniforms.items():
+                ctx.uniform(k, v)
+            ctx.draw(self.vao[d.mesh])
+        ctx.depth_target(N.DEFAULT_DEPTH)
+        ctx.viewport(self.scene.width, self.scene.height)
 
     def frame(self, clear=True):
         """clearDepth + clearColour + the scene's drawVAO calls (queued)"""
         ctx, sc = self.ctx, self.scene
+        if sc.shadow is not None:
+            self.shadow_pass()
         if clear:
             ctx.clear_depth(sc.clear_depth)
             ctx.clear_colour(sc.clear_colour)

@@ -20,7 +20,7 @@ def _slots_for(prog, pos, rng):
     if prog == N.PROC_DEF02:
         col = rng.uniform(0, 255, (n, 4)).astype(np.float32)
         return {0: pos, 1: f4(), 2: col}
-    if prog in (N.PROC_DEF03, N.PROC_BLINN):
+    if prog in (N.PROC_DEF03, N.PROC_BLINN, N.PROC_SHADOWED):
         return {0: pos, 1: f4(), 2: f4(), 3: f4(), 4: uv}
     return {0: pos}
 
@@ -104,12 +104,21 @@ def micro_scene(prog, seed, width=96, height=64, n=120, tex=32):
         sc.uniforms[5] = S.colmajor(np.eye(4) + np.diag(rng.uniform(-0.2, 0.2, 4)))
         sc.uniforms[7] = S.vec4(*rng.uniform(-3, 3, 3), 0.0) if prog != N.PROC_LAMBERT else S.vec4(0.48, 0.6, 0.64)
         sc.uniforms[8] = S.vec4(*rng.uniform(-3, 3, 3), 0.0)
-        lv = S.mip_levels(tex, tex) if prog == N.PROC_BLINN else 1
+        lv = S.mip_levels(tex, tex) if prog in (N.PROC_BLINN, N.PROC_SHADOWED) else 1
         sc.textures.append(S.TextureSpec([S.noise_texture(tex, seed + 1, cells=4)], wrap=int(rng.integers(0, 2)), levels=lv))
         sc.textures.append(S.TextureSpec([S.normal_map(tex, seed + 2, cells=4)], wrap=N.WRAP, levels=lv))
         sc.uniforms[9] = np.array([0], np.int32)
         sc.uniforms[10] = np.array([1], np.int32)
         sc.uniforms[11] = np.array([int(rng.integers(1, 64))], np.int32)
+        if prog == N.PROC_SHADOWED:
+            # a random FP32 depth map (some NaN and inf texels) and a random
+            # projection that reaches beyond the map on every side
+            dm = rng.uniform(-1.2, 1.2, (tex, tex + 7)).astype(np.float32)
+            dm.ravel()[rng.integers(0, dm.size, 6)] = [np.nan, np.inf, -np.inf, np.nan, 0.0, -0.0]
+            img = dm.view(np.uint8).reshape(tex, tex + 7, 4)
+            sc.textures.append(S.TextureSpec([img], wrap=int(rng.integers(0, 2))))
+            sc.uniforms[12] = S.colmajor(np.eye(4) * 0.8 + rng.uniform(-0.4, 0.4, (4, 4)))
+            sc.uniforms[13] = np.array([2], np.int32)
     sc.draws.append(S.DrawSpec("m", prog))
     sc.clear_colour = 0x00203040
     return sc

@@ -282,10 +282,12 @@ def test_threads_do_not_change_results(oracle):
     assert sa.as_dict() == sb.as_dict()
 
 
-@pytest.mark.parametrize("which", ["sphere", "terrain"])
+@pytest.mark.parametrize("which", ["sphere", "terrain", "shadowed"])
 def test_fp32_shading_within_one_lsb_of_fp64(oracle, which):
     if which == "sphere":
         sc = S.sphere_scene(width=320, height=240)
+    elif which == "shadowed":   # the shadow factor is a discrete lookup, shared by both variants
+        sc = S.shadow_terrain_scene(width=320, height=180, cells=40, n_spheres=4, tex_size=128, shadow_size=128)
     else:
         sc = S.terrain_scene(width=320, height=180, cells=40, n_spheres=4, tex_size=128)
     a, _ = oracle.render_scene(sc)
@@ -329,3 +331,100 @@ def test_behaviors(oracle, beh):
     np.testing.assert_array_equal(f.pass_count.reshape(48, 64), passes)
     np.testing.assert_allclose(f.depth[:48, :64], d, rtol=0, atol=1e-6)
     assert st.frags_shaded == passes.sum()
+
+
+# ---------------------------------------------------------------- projective shadow sampler
+_PDX = np.float32([-0.94201624, 0.94558609, -0.094184101, 0.34495938])   # samplrproj.cpp:4-10
+_PDY = np.float32([-0.39906216, -0.76890725, -0.92938870, 0.29387760])   # samplrproj.cpp:12-18
+
+
+def _depth_tex(oracle, depth, wrap=0):
+    img = np.ascontiguousarray(depth, np.float32).view(np.uint8).reshape(depth.shape[0], depth.shape[1], 4)
+    return oracle.OracleTexture(oracle._TexSpec(img) if wrap == 0 else _WrapSpec(img))
+
+
+class _WrapSpec:
+    def __init__(self, img):
+        self.images, (self.height, self.width) = [img], img.shape[:2]
+        self.layers, self.levels, self.wrap = 1, 1, 1
+
+
+def _taps(w, h, tx, ty):
+    """(row, col) of the four taps in FP32, written from the reference's
+    formula: row from POISSON_DISK_X and the height, column from
+    POISSON_DISK_Y and the width, (unsigned int) conversion"""
+    f = np.float32
+    rows = [int(np.floor(f(h) * (f(ty) + _PDX[i] / f(500.0)))) for i in range(4)]
+    cols = [int(np.floor(f(w) * (f(tx) + _PDY[i] / f(500.0)))) for i in range(4)]
+    return list(zip(rows, cols))
+
+
+def test_projection_sampler_constant_maps(oracle):
+    """1 minus 0.2 per tap whose stored depth is below the projected z: all or
+    none of the four taps on a constant map; w divides x, y and z"""
+    t = _depth_tex(oracle, np.full((64, 64), 0.5, np.float32))
+    four = np.float32(1.0)
+    for _ in range(4):
+        four = np.float32(four - np.float32(0.2))
+    assert oracle.sample_projection(t, [0.5, 0.5, 0.4, 1.0]) == 1.0
+    assert oracle.sample_projection(t, [0.5, 0.5, 0.5, 1.0]) == 1.0        # equal depth is lit
+    assert oracle.sample_projection(t, [0.5, 0.5, 0.6, 1.0]) == float(four)
+    assert oracle.sample_projection(t, [1.0, 1.0, 1.2, 2.0]) == float(four)   # (0.5, 0.5, 0.6) after / w
+    assert oracle.sample_projection(t, [1.0, 1.0, 0.8, 2.0]) == 1.0
+
+
+@pytest.mark.parametrize("tap", range(4))
+def test_projection_sampler_tap_positions(oracle, tap):
+    """one dark texel under exactly one tap darkens by one step: the taps sit
+    where the reference's formula puts them (rows from POISSON_DISK_X, columns
+    from POISSON_DISK_Y, scaled by 1/500), which a transposed or mirrored disk
+    would miss"""
+    W, H = 1000, 600
+    tx, ty = 0.4321, 0.6789
+    taps = _taps(W, H, tx, ty)
+    assert len(set(taps)) == 4
+    depth = np.ones((H, W), np.float32)
+    r, c = taps[tap]
+    depth[r, c] = 0.0
+    t = _depth_tex(oracle, depth)
+    assert oracle.sample_projection(t, [tx, ty, 0.5, 1.0]) == float(np.float32(1.0) - np.float32(0.2))
+    # the transposed disk lands elsewhere (tap 3's two offsets are too close to
+    # tell apart on this map)
+    rows_t = int(np.floor(np.float32(H) * (np.float32(ty) + _PDY[tap] / np.float32(500.0))))
+    cols_t = int(np.floor(np.float32(W) * (np.float32(tx) + _PDX[tap] / np.float32(500.0))))
+    if tap != 3:
+        assert (rows_t, cols_t) != (r, c)
+    if (rows_t, cols_t) not in taps:
+        d2 = np.ones((H, W), np.float32)
+        d2[rows_t, cols_t] = 0.0
+        assert oracle.sample_projection(_depth_tex(oracle, d2), [tx, ty, 0.5, 1.0]) == 1.0
+
+
+def test_projection_sampler_outside_the_map(oracle):
+    """coordinates left of and below the map convert through (unsigned int) to
+    huge values, which the int of directRead4 makes negative: clamped to
+    texel 0 on a CLAMP map, wrapped by clampCoord's modulo on a WRAP map"""
+    depth = np.ones((16, 16), np.float32)
+    depth[0, 0] = 0.0
+    t = _depth_tex(oracle, depth)
+    f = oracle.sample_projection(t, [-3.0, -3.0, 0.5, 1.0])
+    four = np.float32(1.0)
+    for _ in range(4):
+        four = np.float32(four - np.float32(0.2))
+    assert f == float(four)             # every tap clamps to texel (0, 0)
+    # beyond the top-right corner every tap clamps to texel (15, 15)
+    depth2 = np.ones((16, 16), np.float32)
+    depth2[15, 15] = 0.0
+    assert oracle.sample_projection(_depth_tex(oracle, depth2), [7.0, 7.0, 0.5, 1.0]) == float(four)
+    # WRAP: -3.0 * 16 and its tap offsets wrap modulo 15 (the reference's size - 1)
+    tw = _depth_tex(oracle, depth, wrap=1)
+    got = oracle.sample_projection(tw, [-3.0, -3.0, 0.5, 1.0])
+    want = np.float32(1.0)
+    for r0, c0 in _taps(16, 16, -3.0, -3.0):
+        r = (r0 + 2**32) % 2**32
+        c = (c0 + 2**32) % 2**32
+        r = r - 2**32 if r >= 2**31 else r
+        c = c - 2**32 if c >= 2**31 else c
+        if depth[r % 15, c % 15] < 0.5:
+            want = np.float32(want - np.float32(0.2))
+    assert got == float(want)
