@@ -151,6 +151,27 @@ int psgpu_get_stats(psgpu_ctx* ctx, psgpu_stats* out);
 int psgpu_reset_stats(psgpu_ctx* ctx);
 int psgpu_swap_buffers(psgpu_ctx* ctx);           /* swapBuffers: flush; display keeps its buffer */
 
+/* ---- post-processing: PuresoftPipeline::postProcess (post.cpp:3-18, proc.h:89-94) ----
+ * The reference hands its post-processor the colour target of slot 0 and the
+ * depth buffer once the draws before it are done.  Here fn runs in stream
+ * order: queued draws are flushed, then fn is called with the device
+ * addresses of the current colour target and depth buffer and the context
+ * stream, and enqueues its own work (kernels, copies) on that stream; later
+ * draws and reads come after that work.  The call is logged like the draws:
+ * when a flush before it is re-run after its scratch buffers grew, fn runs
+ * again after it, so it must only enqueue stream work.  Inside a recorded
+ * command list (psgpu_capture_begin/end) its work becomes part of the graph.
+ * fn returns 0 or a negative status, which the call reports. */
+typedef struct {
+    void* colour;          /* BGRA8 rows of colour_pitch bytes; display rows are top-down */
+    int colour_pitch, colour_top_down;
+    void* depth;           /* FP32 rows of depth_pitch bytes */
+    int depth_pitch;
+    int width, height;     /* the viewport */
+} psgpu_post_frame;
+typedef int (*psgpu_post_fn)(void* user, const psgpu_post_frame* frame, void* stream);
+int psgpu_post_process(psgpu_ctx* ctx, psgpu_post_fn fn, void* user);
+
 /* ---- adjoint of texture sampling and shading (no counterpart in the reference) ----
  * A differentiable shading model evaluated on a geometry buffer (above):
  *   c = albedo_scale * T(u,v) * max(N.L, 0) + ks * max(N.H, 0)^spec_exp   (RGB)

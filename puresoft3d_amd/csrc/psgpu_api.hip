@@ -71,7 +71,7 @@ struct Segment {
     std::vector<PsDraw> draws;
 };
 
-enum CmdType { CMD_FLUSH = 0, CMD_CLEARS = 1, CMD_GRAPH = 3, CMD_COPY = 4 };
+enum CmdType { CMD_FLUSH = 0, CMD_CLEARS = 1, CMD_GRAPH = 3, CMD_COPY = 4, CMD_POST = 5 };
 
 // a device-to-host copy of a 2D region, run on the copy stream after the
 // commands before it
@@ -88,6 +88,9 @@ struct Cmd {
     std::vector<PsFill> fills;    // CMD_CLEARS
     int graph = -1, count = 0;    // CMD_GRAPH
     CopyRegion copy;              // CMD_COPY
+    psgpu_post_fn post = nullptr; // CMD_POST: callback, its argument and the frame it sees
+    void* post_user = nullptr;
+    psgpu_post_frame post_frame = {};
 };
 
 // a point in the command stream: done when both streams passed it and no
@@ -597,6 +600,12 @@ int run_cmd(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged = nullptr)
     case CMD_CLEARS: return run_clears(c, cmd, staged);
     case CMD_GRAPH: return graph_launch_n(c, cmd.graph, cmd.count);
     case CMD_COPY: return run_copy(c, cmd.copy);
+    case CMD_POST: {
+        int rc = cmd.post(cmd.post_user, &cmd.post_frame, (void*)c->stream);
+        if (rc) return fail(c, rc < 0 ? rc : PSGPU_ERR_DEVICE, "post-processor returned %d", rc);
+        HIPCHK(c, hipGetLastError());
+        return PSGPU_OK;
+    }
     }
     return PSGPU_OK;
 }
@@ -1410,6 +1419,28 @@ int psgpu_sync(psgpu_ctx* c)
 }
 
 int psgpu_swap_buffers(psgpu_ctx* c) { return psgpu_flush(c); }
+
+int psgpu_post_process(psgpu_ctx* c, psgpu_post_fn fn, void* user)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (!fn) return fail(c, PSGPU_ERR_INVALID, "no post-processor");
+    Targets tg;
+    int rc = current_targets(c, tg);
+    if (rc) return rc;
+    if ((rc = queue_to_log(c))) return rc;   // the draws before it (post.cpp: after the queued rows)
+    Cmd cmd;
+    cmd.type = CMD_POST;
+    cmd.post = fn;
+    cmd.post_user = user;
+    cmd.post_frame.colour = tg.colour;
+    cmd.post_frame.colour_pitch = tg.colour_scan * 4;
+    cmd.post_frame.colour_top_down = tg.colour_top_down;
+    cmd.post_frame.depth = tg.depth;
+    cmd.post_frame.depth_pitch = tg.depth_scan * 4;
+    cmd.post_frame.width = tg.vpw;
+    cmd.post_frame.height = tg.vph;
+    return submit(c, std::move(cmd));
+}
 
 int psgpu_read_colour(psgpu_ctx* c, void* host, size_t bytes)
 {

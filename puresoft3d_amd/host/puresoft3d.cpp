@@ -188,6 +188,18 @@ void PuresoftPipeline::finish(void) { check(psgpu_sync(m_ctx)); }
 
 void PuresoftPipeline::readDisplay(void* out) { check(psgpu_read_colour(m_ctx, out, (size_t)m_width * m_height * 4)); }
 
+static int post_trampoline(void* user, const psgpu_post_frame* frame, void* stream)
+{
+    return static_cast<PuresoftPostProcessor*>(user)->process(*frame, stream);
+}
+
+void PuresoftPipeline::postProcess(PuresoftPostProcessor* processor)
+{
+    // post.cpp:3-18: after the draws so far, over the slot-0 target and the depth buffer
+    if (!processor) throw std::invalid_argument("postProcess: no processor");
+    check(psgpu_post_process(m_ctx, post_trampoline, processor));
+}
+
 void PuresoftPipeline::swapBuffers(void)
 {
     // the display becomes visible through the renderer's buffer (pipeline.cpp:314-322)

@@ -97,6 +97,18 @@ PS_DECLARE_PROCS(BLINN, PSGPU_PROC_BLINN)      // Blinn-Phong + normal map, tril
 PS_DECLARE_PROCS(SHADOWED, PSGPU_PROC_SHADOWED) // the same, lit through a projective shadow map (samplrproj.cpp)
 #undef PS_DECLARE_PROCS
 
+// ---- proc.h:89-94: the post-processor, run on the device ---------------------------
+// The reference calls process(threadIndex, threadCount, frame, depth) from its
+// fragment threads over host FBOs (post.cpp:3-18).  Here the frame lives in
+// HBM: process() receives the device addresses of the colour target and depth
+// buffer and the context stream, and enqueues its own kernels or copies there.
+// It may be called again for the same frame when earlier draws are re-run.
+class PuresoftPostProcessor {
+public:
+    virtual ~PuresoftPostProcessor() {}
+    virtual int process(const psgpu_post_frame& frame, void* stream) = 0;   // 0 or a negative status
+};
+
 // ---- rndr.h: a headless renderer owning the host copy of the display ------------
 class PuresoftHeadlessRenderer {
 public:
@@ -146,6 +158,7 @@ public:
     void setUniform(int idx, const void* data, size_t len);
     void drawVAO(int vao, bool callerThrdForFragProc = false);
     void swapBuffers(void);                            // display -> renderer buffer
+    void postProcess(PuresoftPostProcessor* processor); // not owned; must outlive the frame
     void enable(int behavior);
     void disable(int behavior);
     void clearDepth(float furthest = 1.0f);

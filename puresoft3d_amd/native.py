@@ -49,6 +49,15 @@ class Stats(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class PostFrame(C.Structure):
+    """psgpu_post_frame: device addresses of the colour target and depth buffer"""
+    _fields_ = [("colour", C.c_void_p), ("colour_pitch", C.c_int), ("colour_top_down", C.c_int),
+                ("depth", C.c_void_p), ("depth_pitch", C.c_int), ("width", C.c_int), ("height", C.c_int)]
+
+
+POST_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(PostFrame), C.c_void_p)
+
+
 class Tiling(C.Structure):
     _fields_ = [("chunk_width", C.c_int), ("tile_rows", C.c_int), ("waves_per_group", C.c_int)]
 
@@ -99,6 +108,7 @@ _SIGS = {
     "psgpu_get_stats": ([C.c_void_p, C.POINTER(Stats)], C.c_int),
     "psgpu_reset_stats": ([C.c_void_p], C.c_int),
     "psgpu_swap_buffers": ([C.c_void_p], C.c_int),
+    "psgpu_post_process": ([C.c_void_p, POST_FN, C.c_void_p], C.c_int),
     "psgpu_debug_enable": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_debug_reset": ([C.c_void_p], C.c_int),
     "psgpu_debug_read": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
@@ -444,6 +454,15 @@ class Context:
 
     def fence_wait(self, f):
         self._chk(self.L.psgpu_fence_wait(self.h, f))
+
+    def post_process(self, fn):
+        """PuresoftPipeline::postProcess: fn(frame: PostFrame, stream: int) -> int
+        runs in stream order after the draws queued so far and enqueues its own
+        device work on `stream` (it may run again when an earlier flush is
+        re-run).  The callback object is kept alive with the context."""
+        cb = POST_FN(lambda user, frame, stream: int(fn(frame.contents, stream or 0)))
+        self._post_cbs = getattr(self, "_post_cbs", []) + [cb]
+        self._chk(self.L.psgpu_post_process(self.h, cb, None))
 
     def selftest_math(self, op, a, b=None):
         a = np.ascontiguousarray(a, np.float32)

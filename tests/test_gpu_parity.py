@@ -340,3 +340,50 @@ def test_long_spans_bitexact(oracle, prog, tiling):
 def test_sliver_extrapolated_spans_bitexact(oracle, prog):
     sc = scenegen.sliver_scene(prog, 9)
     assert_same(sc, oracle)
+
+
+def _hip_runtime():
+    """the HIP runtime libpsgpu is linked against, for a post-processor's own work"""
+    import ctypes as C
+    L = C.CDLL("libamdhip64.so.7")
+    L.hipMemset2DAsync.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_size_t, C.c_size_t, C.c_void_p]
+    L.hipMemset2DAsync.restype = C.c_int
+    return L
+
+
+@pytest.mark.parametrize("tiny", [False, True])
+def test_post_process_runs_in_stream_order(oracle, tiny):
+    """PuresoftPipeline::postProcess (post.cpp:3-18): the post-processor sees
+    the frame after the draws before it, its work lands before later commands,
+    and with scratch buffers too small at first (tiny) the replayed flush is
+    followed by the post-processor again"""
+    sc = S.terrain_scene(width=320, height=180, cells=60, n_spheres=4, tex_size=128) if tiny else \
+        S.sphere_scene(width=200, height=150)
+    announce("post-process %s tiny=%s" % (sc.name, tiny))
+    hip = _hip_runtime()
+    calls = []
+
+    def band(frame, stream):    # rows 40..59, columns 0..99 of the colour target := 0x7f bytes
+        calls.append((frame.width, frame.height, frame.colour_top_down))
+        return hip.hipMemset2DAsync(frame.colour + 40 * frame.colour_pitch, frame.colour_pitch, 0x7F, 100 * 4, 20,
+                                    stream)
+
+    f, _ = oracle.render_scene(sc)
+    want = f.colour.reshape(sc.height, sc.width, 4).copy()
+    want[40:60, :100] = 0x7F
+    ctx = N.Context(sc.width, sc.height, flags=1 if tiny else 0)
+    try:
+        gs = S.GpuScene(ctx, sc)
+        gs.frame()
+        ctx.post_process(band)
+        col = ctx.read_colour().reshape(sc.height, sc.width, 4)
+        np.testing.assert_array_equal(col, want)
+        assert calls[0] == (sc.width, sc.height, 1)
+        if tiny:
+            assert ctx.stats()["retries"] >= 1 and len(calls) >= 2
+        # a clear after it overwrites the band: later commands come after its work
+        ctx.clear_colour(0x00112233)
+        col = ctx.read_colour().reshape(sc.height, sc.width, 4)
+        assert np.all(col[40:60, :100].view(np.uint32) == 0x00112233)
+    finally:
+        ctx.close()
