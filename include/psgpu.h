@@ -47,6 +47,13 @@ extern "C" {
 #define PSGPU_PROC_DEF05    5   /* shadow.cpp          */
 #define PSGPU_PROC_LAMBERT  6   /* Lambert x bilinear texture, directional light (slot 7 = light dir, 9 = tex) */
 #define PSGPU_PROC_BLINN    7   /* Blinn-Phong + normal map, trilinear (7 light, 8 camera, 9 albedo, 10 normal map, 11 exponent) */
+#define PSGPU_PROC_CLOUD    9   /* VP_Cloud + FP_Cloud (test/testproc.cpp:384-420, 526-574): white with the
+                                   nearest texel's red as alpha, times 2 (L . n) and the projective shadow
+                                   factor, blended by blend4 (fbo.cpp:208-229) under PSGPU_ALPHABLEND
+                                   (3 PV, 4 M, 5 Mrot, 7 light, 9 texture, 15 shadow map, 16 the light's PV) */
+#define PSGPU_PROC_CLOUD_SHADOW 10 /* VP_CloudShadow + FP_CloudShadow (test/testproc.cpp:589-609, 672-686):
+                                   depth only, the position shrunk by 0.95, the fragment discarded (no depth
+                                   write) where the nearest texel's red is below 150 (3 PV, 4 M, 9 texture) */
 #define PSGPU_PROC_SHADOWED 8   /* PSGPU_PROC_BLINN lit through a shadow map: the Lambert term times
                                    PuresoftSamplerProjection::get (samplrproj.cpp:20-37) of the world position
                                    projected by uniform 12 (16 floats, the light's PV) into the FP32 depth
@@ -252,7 +259,8 @@ int psgpu_graph_destroy(psgpu_ctx* ctx, int graph);
 /* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a),
  *      5:(int)(unsigned)x86(a), 6:log2 approximation of the trilinear LOD, 7:a (a plain copy kernel,
  *      the calibration point for HBM traffic counters), 8 and 9: the wrapped texel coordinates of v and
- *      v + 1 modulo m (v, m and the result are int32 bit patterns in the float arrays a, b, out)} ---- */
+ *      v + 1 modulo m (v, m and the result are int32 bit patterns in the float arrays a, b, out),
+ *      10: one lane of FP_Cloud's cvtps2dq/packusdw/packuswb tail (testproc.cpp:562-568) as a float} ---- */
 int psgpu_selftest_math(psgpu_ctx* ctx, int op, const float* a, const float* b, float* out, int n);
 
 #ifdef __cplusplus

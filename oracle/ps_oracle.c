@@ -33,6 +33,7 @@
 #define _GNU_SOURCE
 #include "ps_oracle.h"
 
+#include <emmintrin.h>
 #include <float.h>
 #include <limits.h>
 #include <math.h>
@@ -238,6 +239,20 @@ float or_sample_projection(const or_texture* t, const float* proj)
         if (d < tz) f -= 0.2f;
     }
     return f;
+}
+
+/* PuresoftFBO::blend4, fbo.cpp:208-229: per byte, the saturating difference
+   src - dst (subs_pu16) times the source alpha, low 16 bits (mullo_pi16),
+   shifted right by 8, plus dst, packed with unsigned saturation */
+void or_blend4(uint8_t* dst, const uint8_t* src)
+{
+    unsigned a = src[3];
+    for (int k = 0; k < 4; k++) {
+        unsigned s = src[k], d = dst[k];
+        unsigned t = s > d ? s - d : 0u;
+        unsigned v = (((t * a) & 0xffffu) >> 8) + d;
+        dst[k] = (uint8_t)(v > 255u ? 255u : v);
+    }
 }
 
 /* PuresoftSampler2D::get4, samplr2d.cpp:19-25 -> directRead4, fbo.cpp:286-291 */
@@ -662,7 +677,7 @@ typedef struct {
     int prog, nv, needs_grad;
     const float *PV, *M, *Mrot, *P, *V, *lightPos, *cameraPos, *lightDir;
     const or_texture *tex0, *tex1, *tex2;
-    const float* shadowPV;  /* SHADOWED */
+    const float* shadowPV;  /* SHADOWED, CLOUD */
     unsigned spec_exp;
     float invView[16];  /* DEF04 */
     float PVM[16];      /* DEF05 */
@@ -679,6 +694,8 @@ static int prog_nv(int prog)
     case OR_PROG_LAMBERT: return 5;
     case OR_PROG_BLINN: return 14;
     case OR_PROG_SHADOWED: return 18;
+    case OR_PROG_CLOUD: return 14;
+    case OR_PROG_CLOUD_SHADOW: return 2;
     default: return -1;
     }
 }
@@ -720,11 +737,19 @@ static int prog_prepare(const or_draw* d, or_prog* p)
         p->invView[12] = p->invView[13] = p->invView[14] = p->invView[15] = 0.0f;
         break;
     }
-    case OR_PROG_DEF05:
-        /* shadow.cpp:15-29 */
+    case OR_PROG_DEF05: case OR_PROG_CLOUD_SHADOW:
+        /* shadow.cpp:15-29; VP_Shadow, test2/testproc.cpp:505-520 */
         p->PV = u[3]; p->M = u[4];
         if (!p->PV || !p->M) return -3;
         m4m4(p->PVM, p->PV, p->M);
+        if (d->prog == OR_PROG_CLOUD_SHADOW) { p->tex0 = tex_from_uniform(d, 9); if (!p->tex0) return -4; }
+        break;
+    case OR_PROG_CLOUD:   /* VP_Cloud / FP_Cloud::preprocess, test/testproc.cpp:384-390, 526-532 */
+        p->PV = u[3]; p->M = u[4]; p->Mrot = u[5]; p->lightPos = u[7]; p->shadowPV = u[16];
+        if (!p->PV || !p->M || !p->Mrot || !p->lightPos || !p->shadowPV) return -3;
+        p->tex0 = tex_from_uniform(d, 9);
+        p->tex2 = tex_from_uniform(d, 15);
+        if (!p->tex0 || !p->tex2) return -4;
         break;
     case OR_PROG_LAMBERT:
         p->PV = u[3]; p->M = u[4]; p->Mrot = u[5]; p->lightDir = u[7];
@@ -807,11 +832,29 @@ static void run_vp(const or_draw* d, const or_prog* p, int64_t i, float* pos, fl
         memcpy(pos, src, 16);
         break;
     }
-    case OR_PROG_DEF05: {   /* shadow.cpp:21-29 */
-        float pa[4];
+    case OR_PROG_DEF05: case OR_PROG_CLOUD_SHADOW: {
+        /* shadow.cpp:21-29 (0.9); VP_CloudShadow, test/testproc.cpp:595-609 (0.95) */
+        float pa[4], f = p->prog == OR_PROG_CLOUD_SHADOW ? 0.95f : 0.9f;
         memcpy(pa, slot_elem(d, 0, i), 16);
-        pa[0] *= 0.9f; pa[1] *= 0.9f; pa[2] *= 0.9f;
+        pa[0] *= f; pa[1] *= f; pa[2] *= f;
         m4v4(pos, p->PVM, pa);
+        if (p->prog == OR_PROG_CLOUD_SHADOW) {
+            const float* uv = slot_elem(d, 4, i);
+            user[0] = uv[0]; user[1] = uv[1];
+        }
+        break;
+    }
+    case OR_PROG_CLOUD: {   /* VP_Cloud::process, test/testproc.cpp:392-420 */
+        float sc[4], n[4];
+        m4v4(wp, p->M, slot_elem(d, 0, i));
+        m4v4(sc, p->shadowPV, wp);
+        m4v4(pos, p->PV, wp);
+        m4v4(n, p->Mrot, slot_elem(d, 3, i));
+        const float* uv = slot_elem(d, 4, i);
+        /* PROCDATA_CLOUD: normal, worldPos (w = 0), texcoord, shadowcoord */
+        for (int k = 0; k < 4; k++) { user[k] = n[k]; user[10 + k] = sc[k]; }
+        user[4] = wp[0]; user[5] = wp[1]; user[6] = wp[2]; user[7] = 0.0f;
+        user[8] = uv[0]; user[9] = uv[1];
         break;
     }
     case OR_PROG_LAMBERT: {
@@ -844,12 +887,44 @@ static void run_vp(const or_draw* d, const or_prog* p, int64_t i, float* pos, fl
 }
 
 static inline uint8_t u8_of(float v) { return (uint8_t)(int)v; }
+
+/* cvtps2dq, packusdw, packuswb of one lane (testproc.cpp:562-568): cvtps2dq
+   itself (MXCSR round-to-nearest-even, INT_MIN for NaN and out of range),
+   then the two saturating packs, the second reading its words as signed */
+uint8_t or_sse_pack_byte(float f)
+{
+    int v = _mm_cvtsi128_si32(_mm_cvtps_epi32(_mm_set1_ps(f)));
+    unsigned w = v < 0 ? 0u : (v > 65535 ? 65535u : (unsigned)v);
+    return (uint8_t)((int16_t)(uint16_t)w < 0 ? 0u : (w > 255u ? 255u : w));
+}
 static inline uint8_t u8_of_d(double v) { return (uint8_t)(int)v; }
 
-/* returns 1 when a colour is written to *out */
+/* returns 1 when a colour is written to *out, 2 when the fragment is
+   discarded (FBOBridge::discard, fragthrd.cpp:21-24: no depth update) */
 static int run_fp(const or_prog* p, const float* in, float r, const float* grad, int fp64, uint8_t* out)
 {
     switch (p->prog) {
+    case OR_PROG_CLOUD: {   /* FP_Cloud::process, test/testproc.cpp:534-574 */
+        uint8_t t[4];
+        or_sample_nearest(p->tex0, 0, in[8], in[9], t);
+        float sf = or_sample_projection(p->tex2, in + 10);
+        float L[4];
+        for (int k = 0; k < 4; k++) L[k] = p->lightPos[k] - in[4 + k];
+        float dist = len4(L);
+        div4(L, dist);
+        /* E and H (testproc.cpp:553-558) feed nothing */
+        float lambert = 2.0f * dot4(L, in);
+        float cc = 255.0f * (lambert * sf);
+        uint8_t b = or_sse_pack_byte(cc);
+        out[0] = out[1] = out[2] = b;
+        out[3] = or_sse_pack_byte((float)t[2]);
+        return 1;
+    }
+    case OR_PROG_CLOUD_SHADOW: {   /* FP_CloudShadow, test/testproc.cpp:677-686 */
+        uint8_t c[4];
+        or_sample_nearest(p->tex0, 0, in[0], in[1], c);
+        return c[2] < 150 ? 2 : 0;
+    }
     case OR_PROG_DEF01: case OR_PROG_DEF02: case OR_PROG_DEF03: {
         /* tex1light1.cpp:152-193, colr1light1.cpp:144-190, tex1bump1light1.cpp:180-239 */
         const float* wp; const float* nrm;
@@ -1076,8 +1151,12 @@ static void run_task(const or_draw* d, const or_prog* p, or_target* t, const or_
         if (t->dbg_raster_count) { t->dbg_raster_count[pi]++; t->dbg_raster_hash[pi] ^= th; }
         if (-1.0f < z && (z - cur < -0.0001f)) {
             uint8_t c[4];
-            if (run_fp(p, in, r, k->grad, d->shade_fp64, c)) memcpy(crow + (size_t)x * 4, c, 4);
-            if (beh & OR_UPDATE_DEPTH) drow[x] = z;
+            int fr = run_fp(p, in, r, k->grad, d->shade_fp64, c);
+            if (fr == 1) {   /* FBOBridge::write4: blend4 under ALPHABLEND (fragthrd.cpp:70-82) */
+                if (beh & OR_ALPHABLEND) or_blend4(crow + (size_t)x * 4, c);
+                else memcpy(crow + (size_t)x * 4, c, 4);
+            }
+            if (fr != 2 && (beh & OR_UPDATE_DEPTH)) drow[x] = z;
             ws->frags_shaded++;
             if (t->dbg_pass_count) {
                 t->dbg_pass_count[pi]++; t->dbg_pass_hash[pi] ^= th; t->dbg_last_pass[pi] = k->tri;

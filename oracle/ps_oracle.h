@@ -47,6 +47,11 @@ extern "C" {
 #define OR_PROG_DEF05 5   /* shadow.cpp          */
 #define OR_PROG_LAMBERT 6 /* Lambert x bilinear texture, directional light  */
 #define OR_PROG_BLINN   7 /* Blinn-Phong + normal map, trilinear, point light */
+#define OR_PROG_CLOUD   9 /* VP_Cloud + FP_Cloud (test/testproc.cpp:384-420,526-574): white, alpha = the nearest
+                             texel's red, times 2 (L . n) and the projective shadow factor (uniforms 15, 16);
+                             blend4 under OR_ALPHABLEND */
+#define OR_PROG_CLOUD_SHADOW 10 /* VP_CloudShadow + FP_CloudShadow (test/testproc.cpp:589-609,672-686):
+                                  depth only (0.95 shrink), discarded where the nearest texel's red < 150 */
 #define OR_PROG_SHADOWED 8 /* OR_PROG_BLINN with the Lambert term times the projective shadow sampler
                               (uniform 12: the light's PV, 16 floats; uniform 13: FP32 shadow-map texture) */
 
@@ -140,6 +145,10 @@ float or_log2_approx(float x);
 /* PuresoftSamplerProjection::get (samplrproj.cpp:20-37) over an FP32 texture:
  * 1 minus 0.2 per Poisson tap whose stored depth is below the projected z */
 float or_sample_projection(const or_texture* tex, const float* proj4);
+/* PuresoftFBO::blend4 (fbo.cpp:208-229) of a BGRA texel onto dst, in place */
+void or_blend4(uint8_t* dst, const uint8_t* src);
+/* one lane of FP_Cloud's cvtps2dq/packusdw/packuswb tail (testproc.cpp:562-568) */
+uint8_t or_sse_pack_byte(float f);
 
 /* Hash used by the debug planes. */
 uint32_t or_tri_hash(uint32_t tri);

@@ -86,6 +86,10 @@ def lib():
         L.or_log2_approx.restype = C.c_float
         L.or_sample_projection.argtypes = [C.POINTER(Texture), f]
         L.or_sample_projection.restype = C.c_float
+        L.or_blend4.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_blend4.restype = None
+        L.or_sse_pack_byte.argtypes = [C.c_float]
+        L.or_sse_pack_byte.restype = C.c_uint8
         L.or_tri_hash.argtypes = [C.c_uint32]
         L.or_tri_hash.restype = C.c_uint32
         L.or_pool_shutdown.argtypes = []
@@ -272,6 +276,19 @@ def sample_projection(tex, proj):
     return float(lib().or_sample_projection(C.byref(s), _fp(p)))
 
 
+def blend4(dst, src):
+    """PuresoftFBO::blend4 of BGRA bytes src onto dst; returns the new dst"""
+    d = np.array(dst, np.uint8).copy()
+    sv = np.ascontiguousarray(src, np.uint8)
+    lib().or_blend4(d.ctypes.data, sv.ctypes.data)
+    return d
+
+
+def sse_pack_byte(f):
+    """one lane of FP_Cloud's cvtps2dq/packusdw/packuswb tail"""
+    return int(lib().or_sse_pack_byte(float(np.float32(f))))
+
+
 def mesh_triangles(mesh):
     return min(a.shape[0] for a in mesh.values()) // 3
 
@@ -291,7 +308,7 @@ def render_scene(scene, n_threads=1, debug=False, fp64=False, frame=None, clear=
         for d in sp.draws:
             u = {3: sp.light_pv}
             u.update(d.uniforms)
-            DrawCall(d.prog, scene.meshes[d.mesh], u, [], behavior=DEFAULT_BEHAVIOR, tri_base=base).run(
+            DrawCall(d.prog, scene.meshes[d.mesh], u, texs, behavior=DEFAULT_BEHAVIOR, tri_base=base).run(
                 sf, n_threads, st)
             base += mesh_triangles(scene.meshes[d.mesh])
         img = np.ascontiguousarray(sf.depth[:, : sp.size]).view(np.uint8).reshape(sp.size, sp.size, 4)

@@ -354,6 +354,23 @@ PS_DEV float ps_sample_projection(const PsTex& t, const float* proj)
     return f;
 }
 
+// PuresoftFBO::blend4 (fbo.cpp:208-229) of src over dst: per byte, the
+// saturating difference src - dst (subs_pu16) times the source alpha (low 16
+// bits, mullo_pi16), shifted right by 8, plus dst, packed with unsigned
+// saturation (packs_pu16)
+PS_DEV uint32_t ps_blend4(uint32_t dst, uint32_t src)
+{
+    uint32_t a = src >> 24, out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t s = (src >> (8 * k)) & 255u, d = (dst >> (8 * k)) & 255u;
+        uint32_t t = s > d ? s - d : 0u;
+        uint32_t v = (((t * a) & 0xffffu) >> 8) + d;
+        out |= (v > 255u ? 255u : v) << (8 * k);
+    }
+    return out;
+}
+
 // clampCoord of the two neighbouring coordinates v and v + 1 (the second
 // wraps around at INT_MAX like the reference's int arithmetic).  With wrap, the
 // second is the first's successor modulo maxv unless v + 1 overflows.
@@ -546,6 +563,17 @@ PS_DEV void ps_tri_grad(const PsTri& t, const float* u0, const float* u1, const 
         g[2 * a] = (d1 * e2y - d2 * e1y) * inv;
         g[2 * a + 1] = (d2 * e1x - d1 * e2x) * inv;
     }
+}
+
+// one lane of cvtps2dq, packusdw, packuswb (testproc.cpp:562-568): round to
+// nearest even (out of range or NaN -> INT_MIN), saturate to an unsigned word,
+// then pack that word *as a signed one* to an unsigned byte, so words of
+// 0x8000 and above come out 0.
+PS_DEV uint32_t ps_sse_pack_byte(float f)
+{
+    int v = ps_cvt_i32(rintf(f));
+    uint32_t w = v < 0 ? 0u : (v > 65535 ? 65535u : (uint32_t)v);
+    return w >= 0x8000u ? 0u : (w > 255u ? 255u : w);
 }
 
 PS_DEV uint32_t ps_pack_bgr(float b, float g, float r)

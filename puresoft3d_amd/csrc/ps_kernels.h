@@ -33,6 +33,11 @@ size_t psk_raster_lds(int chunk_w, int waves);
 void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
                 const uint32_t* kstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
                 PsCounters* cnt);
+// the raster walk with the FP of `prog` on every pass (blending / discarding
+// programmes), writing colour at once; no winner buffer, no shading pass
+void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
+                    const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
+                    PsCounters* cnt, const float* vary, int nvs, const float* ck);
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
                const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f,
                const PsCounters* cnt);

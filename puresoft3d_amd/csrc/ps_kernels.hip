@@ -970,6 +970,243 @@ __global__ void __launch_bounds__(512) k_raster(const PsSpan* __restrict__ spans
 }
 
 // ---------------------------------------------------------------------------
+// raster with the FP in the ordered walk: programmes that blend or discard
+// ---------------------------------------------------------------------------
+// A discarded fragment leaves the depth unchanged and a blended one depends
+// on the colour beneath it, so neither can be deferred to one shading pass
+// per pixel.  The same tile walk runs the programme's FP on every depth-test
+// pass, in submission order, and writes (or blends) the colour at once.
+template <int PROG, bool DEBUG>
+__global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
+                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
+                         const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles,
+                         PsCounters* cnt, const float* __restrict__ vary, int nvs, const float* __restrict__ ck)
+{
+    typedef PsProc<PROG> P;
+    extern __shared__ uint32_t lds[];
+    if (ps_poisoned(cnt)) return;
+    const int CW = f.chunk_w;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int waves = blockDim.x >> 6;
+    int64_t tile = blockIdx.x * (int64_t)waves + wave;
+    if (tile >= n_tiles) return;
+    uint32_t* wl = lds + (size_t)wave * RA_WORDS(CW);
+    float* dep = (float*)wl;
+    int* win = (int*)(dep + CW * 64);
+    uint32_t* r_s = (uint32_t*)(win + CW * 64);   // segment index
+    uint32_t* r_fl = r_s + RA_CHUNK;               // behaviour | writes colour (PsSeg.si high bits)
+    uint32_t* r_x = r_fl + RA_CHUNK;               // xa | xb << 16
+    float* r_cf2 = (float*)(r_x + RA_CHUNK);
+    float* r_pz = r_cf2 + RA_CHUNK;
+    float* r_cst = r_pz + RA_CHUNK;
+    float* r_pst = r_cst + RA_CHUNK;
+    uint32_t* lcnt = (uint32_t*)(r_pst + RA_CHUNK);
+    uint32_t* loff = lcnt + 64;
+
+    const int cw_shift = __builtin_ctz((unsigned)CW), tc_shift = __builtin_ctz((unsigned)f.tile_cols);
+    const PsView& V = f.views[ps_view_of_tile(f, tile)];
+    int64_t ltile = tile - V.tile_base;
+    int ty = (int)(ltile / V.n_tiles_x), tx = (int)(ltile - (int64_t)ty * V.n_tiles_x);
+    int ry = lane / f.tile_cols, rx = lane - ry * f.tile_cols;
+    int y = ty * f.tile_rows + ry;
+    int cx = tx * f.tile_cols + rx;
+    int X0 = cx * CW;
+    const int VW = V.W;
+    bool inside = y < V.H && X0 < VW;
+    int ncol = inside ? min(CW, VW - X0) : 0;
+    float* drow = V.depth + (int64_t)y * V.depth_scan;
+    for (int c = 0; c < ncol; c++) {
+        dep[c * 64 + lane] = drow[X0 + c];
+        win[c * 64 + lane] = -1;
+    }
+    unsigned long long passes = 0;
+    bool dirty = false;
+    const uint64_t lt = ps_lt_mask();
+    uint32_t s0 = tstart[tile], s1 = tstart[tile + 1];
+    // software pipeline over chunks of the tile list: while chunk i is split and
+    // walked, the records of chunk i+1 and the indices of chunk i+2 are in flight
+    uint32_t vn[RA_PER], vnn[RA_PER];
+    uint4 ran[RA_PER], rbn[RA_PER];
+#pragma unroll
+    for (int k = 0; k < RA_PER; k++) {
+        uint32_t e = s0 + k * 64 + lane;
+        vn[k] = e < s1 ? vals[e] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < RA_PER; k++) {
+        uint32_t e = s0 + k * 64 + lane;
+        const uint4* p = (const uint4*)(segs + vn[k]);
+        ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
+        rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < RA_PER; k++) {
+        uint32_t e = s0 + RA_CHUNK + k * 64 + lane;
+        vnn[k] = e < s1 ? vals[e] : 0u;
+    }
+    for (uint32_t cb = s0; cb < s1; cb += RA_CHUNK) {
+        uint32_t ce = min(s1, cb + RA_CHUNK);
+        uint32_t v[RA_PER], key[RA_PER];
+        uint4 ra[RA_PER], rb[RA_PER];
+#pragma unroll
+        for (int k = 0; k < RA_PER; k++) {
+            v[k] = vn[k];
+            ra[k] = ran[k];
+            rb[k] = rbn[k];
+            vn[k] = vnn[k];
+        }
+#pragma unroll
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + RA_CHUNK + k * 64 + lane;
+            const uint4* p = (const uint4*)(segs + vn[k]);
+            ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
+            rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
+            uint32_t e2 = cb + 2 * RA_CHUNK + k * 64 + lane;
+            vnn[k] = e2 < s1 ? vals[e2] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < RA_PER; k++) {
+            // the piece's lane: its row inside the tile and its chunk inside the row
+            int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
+            key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
+        }
+        // stable split by lane key: count, offsets, place
+        lcnt[lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int pass = 0; pass < 2; pass++) {
+            if (pass == 1) {
+                uint32_t cntv = lcnt[lane];
+                uint32_t x = cntv;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint32_t yv = __shfl_up(x, o, 64);
+                    if (lane >= o) x += yv;
+                }
+                loff[lane] = x - cntv;
+                lcnt[lane] = 0;
+                __builtin_amdgcn_wave_barrier();
+            }
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                uint32_t e = cb + k * 64 + lane;
+                bool valid = e < ce;
+                uint32_t kk = key[k];
+                uint64_t same = __ballot(valid);
+#pragma unroll
+                for (int bit = 0; bit < 6; bit++) {
+                    uint64_t m = __ballot((kk >> bit) & 1u);
+                    same &= ((kk >> bit) & 1u) ? m : ~m;
+                }
+                uint32_t before = (uint32_t)__popcll(same & lt);
+                uint32_t cur = lcnt[kk];
+                if (pass == 1 && valid) {
+                    uint32_t q = loff[kk] + cur + before;
+                    r_s[q] = v[k];
+                    r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
+                    r_x[q] = ra[k].y;
+                    r_cf2[q] = __uint_as_float(ra[k].z);
+                    r_pz[q] = __uint_as_float(ra[k].w);
+                    r_cst[q] = __uint_as_float(rb[k].x);
+                    r_pst[q] = __uint_as_float(rb[k].y);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (inside) {
+            uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
+            for (uint32_t q = l0; q < l1; q++) {
+                int sidx = (int)r_s[q];
+                uint32_t fl = r_fl[q];
+                int beh = (fl >> 26) & 3;
+                bool wc = (fl >> 28) & 1;
+                uint32_t xab = r_x[q];
+                int a = (int)(xab & 0xffffu), b = (int)(xab >> 16);
+                float cf2 = r_cf2[q], pz = r_pz[q];
+                const float cst = r_cst[q], pst = r_pst[q];
+                uint32_t th = 0;
+                int tid = 0;
+                if (DEBUG) {
+                    int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
+                    int tri = spans[si].tri;
+                    const PsTri tr = tris[tri];
+                    const PsDraw& d = draws[tr.code >> 8];
+                    tid = (int)(d.id_base + (tri - d.tri_start));
+                    uint32_t h = (uint32_t)tid;
+                    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+                    th = h;
+                }
+                // the span's varyings at pixel a: from the piece's checkpoint or
+                // by the serial walk from x1 (interp.cpp:82-92), as the deferred
+                // shading does
+                const int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
+                const PsSpan sp = spans[si];
+                const PsTri tr = tris[sp.tri];
+                const PsDraw& d = draws[tr.code >> 8];
+                float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
+                {
+                    PsEdge Le, Re;
+                    ps_row_edges(tr, y, Le, Re);
+                    int left = ps_cvt_i32(ps_edge_eval(Le, (float)y) + 0.5f);
+                    int right = ps_cvt_i32(ps_edge_eval(Re, (float)y) + 0.5f);
+                    PsSpanGeom g;
+                    ps_span_geom(tr, y, left, right, Le, Re, g);
+                    const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
+                    ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
+                    int xs = sp.x1;
+                    if (a > sp.x1 && ps_span_has_ck(sp) && wc) {
+                        const float* cp = ck + (int64_t)sidx * f.ck_stride;
+#pragma unroll
+                        for (int i = 0; i < P::NV; i++) vs[i] = cp[i];
+                        xs = a;
+                    }
+                    for (int xx = xs; xx < a; xx++) {
+#pragma unroll
+                        for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
+                    }
+                }
+                const bool blend = (d.behavior & PSGPU_ALPHABLEND) != 0;
+                const int64_t crow = (V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y) * V.colour_scan;
+                for (int x = a; x <= b; x++) {
+                    // interpolateNextStep (interp.cpp:82-92)
+                    float r = 1.0f / cf2;
+                    cf2 += cst;
+                    float z = pz * r;
+                    pz += pst;
+                    int c = x - X0;
+                    float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
+                    int64_t pix = (int64_t)y * VW + x;
+                    if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                    // fragthrd.cpp:227-236: the FP runs on a pass; a discarded
+                    // fragment leaves depth as it was
+                    if (-1.0f < z && (z - cur < -0.0001f)) {
+                        passes++;
+                        float in[P::NV > 0 ? P::NV : 1];
+#pragma unroll
+                        for (int i = 0; i < P::NV; i++) in[i] = vs[i] * r;
+                        uint32_t col = 0;
+                        int fr = P::fpi(d, in, col);
+                        if (fr != 2 && (beh & PSGPU_UPDATE_DEPTH)) { dep[c * 64 + lane] = z; dirty = true; }
+                        if (fr == 1 && wc) {   // FBOBridge::write4 (fragthrd.cpp:70-82)
+                            uint32_t* cp = V.colour + crow + x;
+                            *cp = blend ? ps_blend4(*cp, col) : col;
+                        }
+                        if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                    }
+#pragma unroll
+                    for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int c = 0; c < ncol; c++)
+        if (dirty) drow[X0 + c] = dep[c * 64 + lane];
+    ps_wave_add(cnt, PS_ST_SHADED, passes);
+}
+
+// ---------------------------------------------------------------------------
 // shade: deferred FP of the last colour-writing pass per pixel
 // ---------------------------------------------------------------------------
 template <int PROG>
@@ -1082,6 +1319,7 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
         r = __int_as_float(op == 8 ? c0 : c1);
         break;
     }
+    case 10: r = (float)ps_sse_pack_byte(a[i]); break;
     }
     out[i] = r;
 }
@@ -1117,6 +1355,7 @@ void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, 
     switch (prog) {
     SU_CASE(PSGPU_PROC_DEF01) SU_CASE(PSGPU_PROC_DEF02) SU_CASE(PSGPU_PROC_DEF03) SU_CASE(PSGPU_PROC_DEF04)
     SU_CASE(PSGPU_PROC_DEF05) SU_CASE(PSGPU_PROC_LAMBERT) SU_CASE(PSGPU_PROC_BLINN) SU_CASE(PSGPU_PROC_SHADOWED)
+    SU_CASE(PSGPU_PROC_CLOUD) SU_CASE(PSGPU_PROC_CLOUD_SHADOW)
     }
 #undef SU_CASE
 }
@@ -1205,6 +1444,29 @@ void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* seg
     dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
     if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, segs, vals, tstart, tris, draws, f, n_tiles, cnt);
     else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, segs, vals, tstart, tris, draws, f, n_tiles, cnt);
+}
+
+void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
+                    const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
+                    PsCounters* cnt, const float* vary, int nvs, const float* ck)
+{
+    int64_t n_tiles = f.n_tiles;
+    size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
+    dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
+#define RI_CASE(K)                                                                                                   \
+    case K:                                                                                                          \
+        if (debug)                                                                                                   \
+            hipLaunchKernelGGL((k_raster_imm<K, true>), g, b, lds, s, spans, segs, vals, tstart, tris, draws, f,     \
+                               n_tiles, cnt, vary, nvs, ck);                                                         \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_raster_imm<K, false>), g, b, lds, s, spans, segs, vals, tstart, tris, draws, f,    \
+                               n_tiles, cnt, vary, nvs, ck);                                                         \
+        break;
+    switch (prog) {
+    RI_CASE(PSGPU_PROC_CLOUD) RI_CASE(PSGPU_PROC_CLOUD_SHADOW)
+    default: break;
+    }
+#undef RI_CASE
 }
 
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,

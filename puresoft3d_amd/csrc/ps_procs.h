@@ -329,3 +329,66 @@ template <> struct PsProc<PSGPU_PROC_SHADOWED> {
         return ps_blinn_fp(d, in, r, g, ps_sample_projection(d.tex2, in + 14), out);
     }
 };
+
+// ---- programmes shaded inside the raster walk (blending, discard) ------
+// fpi() returns 1 with a colour to write, 2 to discard the fragment (no depth
+// update, FBOBridge::discard), 0 for neither.
+
+// CLOUD: VP_Cloud + FP_Cloud (test/testproc.cpp:384-420, 534-574): white
+// with the texel's red as alpha, times 2 (L . n) and the shadow factor,
+// packed the way the reference's SSE tail packs it.
+// varyings: normal[4], worldPos[4] (w = 0), texcoord[2], shadowcoord[4]
+template <> struct PsProc<PSGPU_PROC_CLOUD> {
+    static constexpr int NV = 14;
+    static constexpr bool GRAD = false;
+    PS_DEV static void vp(const PsDraw& d, int64_t i, float* pos, float* u)
+    {
+        float p[4], wp[4], sc[4], n[4];
+        ps_load4(p, ps_slot(d, 0, i));
+        ps_m4v4(wp, d.M, p);
+        ps_m4v4(sc, d.shadowPV, wp);
+        ps_m4v4(pos, d.PV, wp);
+        ps_load4(p, ps_slot(d, 3, i));
+        ps_m4v4(n, d.Mrot, p);
+        const float* tc = ps_slot(d, 4, i);
+#pragma unroll
+        for (int k = 0; k < 4; k++) { u[k] = n[k]; u[10 + k] = sc[k]; }
+        u[4] = wp[0]; u[5] = wp[1]; u[6] = wp[2]; u[7] = 0.0f;
+        u[8] = tc[0]; u[9] = tc[1];
+    }
+    PS_DEV static int fpi(const PsDraw& d, const float* in, uint32_t& out)
+    {
+        uint32_t t = ps_sample_nearest(d.tex0, in[8], in[9]);
+        float shadow = ps_sample_projection(d.tex2, in + 10);
+        float L[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) L[k] = d.lightPos[k] - in[4 + k];
+        float rd = 1.0f / sqrtf(ps_dot4(L, L));
+#pragma unroll
+        for (int k = 0; k < 4; k++) L[k] *= rd;
+        float lambert = 2.0f * ps_dot4(L, in);
+        uint32_t c = ps_sse_pack_byte(255.0f * (lambert * shadow));
+        out = c | (c << 8) | (c << 16) | (((t >> 16) & 255u) << 24);
+        return 1;
+    }
+};
+
+// CLOUD_SHADOW: VP_CloudShadow + FP_CloudShadow (test/testproc.cpp:595-609, 677-686)
+template <> struct PsProc<PSGPU_PROC_CLOUD_SHADOW> {
+    static constexpr int NV = 2;
+    static constexpr bool GRAD = false;
+    PS_DEV static void vp(const PsDraw& d, int64_t i, float* pos, float* u)
+    {
+        float p[4];
+        ps_load4(p, ps_slot(d, 0, i));
+        p[0] *= 0.95f; p[1] *= 0.95f; p[2] *= 0.95f;
+        ps_m4v4(pos, d.PVM, p);
+        const float* tc = ps_slot(d, 4, i);
+        u[0] = tc[0]; u[1] = tc[1];
+    }
+    PS_DEV static int fpi(const PsDraw& d, const float* in, uint32_t& out)
+    {
+        uint32_t t = ps_sample_nearest(d.tex0, in[0], in[1]);
+        return ((t >> 16) & 255u) < 150u ? 2 : 0;
+    }
+};

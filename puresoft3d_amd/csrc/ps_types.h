@@ -5,7 +5,7 @@
 #include <stdint.h>
 
 #define PS_MAX_NV      20   // widest varying block (DEF03: 5 float4 slots)
-#define PS_N_PROGS     9    // programme kinds are 1 .. PS_N_PROGS-1 (PSGPU_PROC_*)
+#define PS_N_PROGS     11   // programme kinds are 1 .. PS_N_PROGS-1 (PSGPU_PROC_*)
 #define PS_MAX_LEVELS  16
 #define PS_MAX_LAYERS  6
 

@@ -157,6 +157,7 @@ struct psgpu_ctx {
     std::vector<Segment> queue;        // pending views with draws
     std::vector<PsFill> pending_fills; // pending clears (run before the queued draws)
     int64_t queue_tris = 0;
+    int queue_imm = 0;        // programme of the queued draws when they shade in the raster walk, else 0
     std::vector<Cmd> log;
     bool capturing = false;        // between psgpu_capture_begin and _end
     bool in_stream_capture = false;
@@ -295,6 +296,8 @@ int prog_nv(int kind)
     case PSGPU_PROC_LAMBERT: return 5;
     case PSGPU_PROC_BLINN: return 14;
     case PSGPU_PROC_SHADOWED: return 18;
+    case PSGPU_PROC_CLOUD: return 14;
+    case PSGPU_PROC_CLOUD_SHADOW: return 2;
     default: return -1;
     }
 }
@@ -545,6 +548,19 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         ProfScope p(c, "bounds");
         psk_bounds(s, skeys, c->d_sizes + 1, f.n_tiles, c->d_kstart.p, c->d_cnt);
     }
+    int imm = 0;   // a flush of one programme that shades in the raster walk
+    {
+        const PsDraw* hd = (const PsDraw*)blob.bytes.data();
+        if (nd > 0 && (hd[0].prog == PSGPU_PROC_CLOUD || hd[0].prog == PSGPU_PROC_CLOUD_SHADOW)) imm = hd[0].prog;
+    }
+    if (imm) {
+        ProfScope p(c, "raster");
+        psk_raster_imm(s, imm, dbg, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
+                       c->tiling.waves_per_group, c->d_cnt, c->d_vary.p, nvs, c->d_ck.p);
+        HIPCHK(c, hipGetLastError());
+        c->stats.flushes++;
+        return PSGPU_OK;
+    }
     {
         ProfScope p(c, "raster");
         psk_raster(s, dbg, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
@@ -652,6 +668,7 @@ int queue_to_log(psgpu_ctx* c)
         cmd.type = CMD_FLUSH;
         cmd.views.swap(c->queue);
         c->queue_tris = 0;
+        c->queue_imm = 0;
         if ((rc = submit(c, std::move(cmd)))) return rc;
     }
     return PSGPU_OK;
@@ -1280,16 +1297,19 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
     // drawvao.cpp:12-15: silently nothing without a programme or with a bad VAO
     if (c->cur_prog < 0 || vao < 0 || vao >= (int)c->vao.size() || !c->vao[vao].alive) return PSGPU_OK;
-    if (c->behavior & PSGPU_ALPHABLEND)
-        return fail(c, PSGPU_ERR_UNSUPPORTED, "alpha blending is not supported by the GPU path");
     const Prog& pg = c->prog[c->cur_prog];
+    // blending changes nothing for the depth-only programmes (the demo leaves it
+    // on for its cloud's shadow pass, testobjs.cpp:93-112)
+    if ((c->behavior & PSGPU_ALPHABLEND) && pg.vp != PSGPU_PROC_CLOUD && pg.vp != PSGPU_PROC_CLOUD_SHADOW &&
+        pg.vp != PSGPU_PROC_DEF05)
+        return fail(c, PSGPU_ERR_UNSUPPORTED, "alpha blending is supported for PSGPU_PROC_CLOUD only");
     const Vao& a = c->vao[vao];
     PsDraw d;
     memset(&d, 0, sizeof(d));
     d.prog = pg.vp;
     d.behavior = c->behavior;
     d.nv = prog_nv(pg.vp);
-    d.writes_colour = pg.vp != PSGPU_PROC_DEF05;
+    d.writes_colour = pg.vp != PSGPU_PROC_DEF05 && pg.vp != PSGPU_PROC_CLOUD_SHADOW;
     // processVertices stops at the first empty VBO (vertthrd.cpp:14-51)
     int64_t nverts = -1;
     for (int s = 0; s < kMaxSlots; s++)
@@ -1307,6 +1327,8 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     case PSGPU_PROC_DEF03: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
     case PSGPU_PROC_LAMBERT: need[3] = 16; need[4] = 8; break;
     case PSGPU_PROC_BLINN: case PSGPU_PROC_SHADOWED: need[1] = need[2] = need[3] = 16; need[4] = 8; break;
+    case PSGPU_PROC_CLOUD: need[3] = 16; need[4] = 8; break;
+    case PSGPU_PROC_CLOUD_SHADOW: need[4] = 8; break;
     default: break;
     }
     for (int s = 0; s < 5; s++) {
@@ -1351,13 +1373,23 @@ int psgpu_draw(psgpu_ctx* c, int vao)
         if ((rc = uniform_tex(c, 2, &d.tex0))) return rc;
         break;
     }
-    case PSGPU_PROC_DEF05: {
+    case PSGPU_PROC_DEF05: case PSGPU_PROC_CLOUD_SHADOW: {
         float PV[16], M[16];
         if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(PV, p, 64);
         if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(M, p, 64);
         host_m4m4(d.PVM, PV, M);
+        if (d.prog == PSGPU_PROC_CLOUD_SHADOW && (rc = uniform_tex(c, 9, &d.tex0))) return rc;
         break;
     }
+    case PSGPU_PROC_CLOUD:   // VP_Cloud / FP_Cloud::preprocess, test/testproc.cpp:384-390, 526-532
+        if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(d.PV, p, 64);
+        if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(d.M, p, 64);
+        if ((rc = uniform_bytes(c, 5, 64, &p))) return rc; memcpy(d.Mrot, p, 64);
+        if ((rc = uniform_bytes(c, 7, 16, &p))) return rc; memcpy(d.lightPos, p, 16);
+        if ((rc = uniform_bytes(c, 16, 64, &p))) return rc; memcpy(d.shadowPV, p, 64);
+        if ((rc = uniform_tex(c, 9, &d.tex0))) return rc;
+        if ((rc = uniform_tex(c, 15, &d.tex2))) return rc;
+        break;
     case PSGPU_PROC_LAMBERT:
         if ((rc = uniform_bytes(c, 3, 64, &p))) return rc; memcpy(d.PV, p, 64);
         if ((rc = uniform_bytes(c, 4, 64, &p))) return rc; memcpy(d.M, p, 64);
@@ -1389,6 +1421,11 @@ int psgpu_draw(psgpu_ctx* c, int vao)
     if (c->queue_tris + d.tri_count > kMaxFlushTris && !c->queue.empty()) {
         if ((rc = queue_to_log(c))) return rc;     // bound the scratch of one flush
     }
+    // programmes that shade in the raster walk (blending, discard) get flushes
+    // of their own, one programme each
+    const int imm = (d.prog == PSGPU_PROC_CLOUD || d.prog == PSGPU_PROC_CLOUD_SHADOW) ? d.prog : 0;
+    if (!c->queue.empty() && c->queue_imm != imm && (rc = queue_to_log(c))) return rc;
+    c->queue_imm = imm;
     if (c->queue.empty() || !c->queue.back().tg.same(tg)) {
         if (queue_uses(c, tg.colour) || queue_uses(c, tg.depth)) {
             if ((rc = queue_to_log(c))) return rc;

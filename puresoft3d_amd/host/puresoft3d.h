@@ -95,6 +95,8 @@ PS_DECLARE_PROCS(DEF05, PSGPU_PROC_DEF05)      // shadow.h
 PS_DECLARE_PROCS(LAMBERT, PSGPU_PROC_LAMBERT)  // Lambert x bilinear, directional light
 PS_DECLARE_PROCS(BLINN, PSGPU_PROC_BLINN)      // Blinn-Phong + normal map, trilinear
 PS_DECLARE_PROCS(SHADOWED, PSGPU_PROC_SHADOWED) // the same, lit through a projective shadow map (samplrproj.cpp)
+PS_DECLARE_PROCS(CLOUD, PSGPU_PROC_CLOUD)      // the demo's cloud, blended (test/testproc.cpp:384-574)
+PS_DECLARE_PROCS(CLOUD_SHADOW, PSGPU_PROC_CLOUD_SHADOW) // its shadow pass, discarding thin cloud (:589-686)
 #undef PS_DECLARE_PROCS
 
 // ---- proc.h:89-94: the post-processor, run on the device ---------------------------

@@ -20,6 +20,9 @@ UPDATE_DEPTH, TEST_DEPTH, FACE_CULLING, ALPHABLEND = 1, 2, 4, 8
 DEFAULT_BEHAVIOR = UPDATE_DEPTH | TEST_DEPTH | FACE_CULLING   # pipeline.cpp:34
 PROC_DEF01, PROC_DEF02, PROC_DEF03, PROC_DEF04, PROC_DEF05, PROC_LAMBERT, PROC_BLINN = 1, 2, 3, 4, 5, 6, 7
 PROC_SHADOWED = 8   # Blinn-Phong lit through a projective shadow map (uniforms 12, 13)
+PROC_CLOUD = 9      # the demo's cloud layer: lit, shadowed, alpha from the texel's red; blended under
+                    # ALPHABLEND (uniforms 15, 16: shadow map, light PV; shaded in the raster walk)
+PROC_CLOUD_SHADOW = 10   # depth only, discarded where the texel's red < 150 (shaded in the raster walk)
 CLAMP, WRAP = 0, 1
 DISPLAY = -1
 DEFAULT_DEPTH = -1

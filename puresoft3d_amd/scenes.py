@@ -378,6 +378,40 @@ def shadow_terrain_scene(width=480, height=270, seed=7, cells=60, n_spheres=8, t
     return sc
 
 
+def cloud_texture(size, seed):
+    """cloud density in the red channel (what PSGPU_PROC_CLOUD takes as alpha
+    and PSGPU_PROC_CLOUD_SHADOW tests against 150); the other channels 255"""
+    rng = np.random.default_rng(seed)
+    n = value_noise(size, size, 8, rng, channels=1, octaves=4)[..., 0]
+    img = np.full((size, size, 4), 255, np.uint8)
+    img[..., 2] = (np.clip((n - 0.3) * 2.5, 0.0, 1.0) * 255.0).astype(np.uint8)
+    return img
+
+
+def cloud_terrain_scene(width=480, height=270, seed=7, cells=60, n_spheres=8, tex_size=256, shadow_size=256,
+                        cloud_cells=24, cloud_y=14.0, cloud_tex=128):
+    """The shadowed terrain under a cloud layer, drawn the way the reference's
+    demo draws its clouds (test/puresoft.cpp:179-205, testobjs.cpp:93-112): a
+    rolling sheet of cloud_cells^2 cells at height cloud_y casts shadows
+    through PSGPU_PROC_CLOUD_SHADOW in the shadow pass (texels with red below
+    150 discarded), then is drawn last with PSGPU_PROC_CLOUD and blending on:
+    white lit by 2 (L . n), times the shadow factor, alpha = the texel's red."""
+    sc = shadow_terrain_scene(width, height, seed, cells, n_spheres, tex_size, shadow_size)
+    sc.name = "clouded_" + sc.name
+    sheet = terrain(cloud_cells, 70.0, seed + 5, height=1.5, uv_repeat=1.0)[0]
+    sc.meshes["cloud"] = {k: sheet[k] for k in (0, 3, 4)}
+    ci = np.array([len(sc.textures)], np.int32)
+    sc.textures.append(TextureSpec([cloud_texture(cloud_tex, seed + 6)], wrap=N.CLAMP))
+    shadow_idx = np.array([len(sc.textures)], np.int32)   # the map follows the scene's textures
+    sc.uniforms[13] = shadow_idx
+    sc.uniforms[15] = shadow_idx
+    sc.uniforms[16] = sc.uniforms[12]
+    M = colmajor(translate(0.0, cloud_y, 0.0))
+    sc.shadow.draws.append(DrawSpec("cloud", N.PROC_CLOUD_SHADOW, {4: M, 9: ci}))
+    sc.draws.append(DrawSpec("cloud", N.PROC_CLOUD, {4: M, 9: ci}, behavior=N.DEFAULT_BEHAVIOR | N.ALPHABLEND))
+    return sc
+
+
 # ---------------------------------------------------------------- GPU upload
 class GpuScene:
     """A Scene resident on one psgpu context (VBOs, VAOs, textures, programmes)."""
@@ -445,7 +479,7 @@ class GpuScene:
             want = N.DEFAULT_BEHAVIOR if d.behavior is None else d.behavior
             if want != beh:
                 ctx.enable(want)
-                ctx.disable(~want & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING))
+                ctx.disable(~want & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING | N.ALPHABLEND))
                 beh = want
             for k, v in d.uniforms.items():
                 ctx.uniform(k, v)
@@ -455,7 +489,7 @@ class GpuScene:
                     ctx.uniform(k, sc.uniforms[k])
         if beh != N.DEFAULT_BEHAVIOR:
             ctx.enable(N.DEFAULT_BEHAVIOR)
-            ctx.disable(~N.DEFAULT_BEHAVIOR & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING))
+            ctx.disable(~N.DEFAULT_BEHAVIOR & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING | N.ALPHABLEND))
         ctx.flush()
 
 

@@ -22,6 +22,12 @@ def _slots_for(prog, pos, rng):
         return {0: pos, 1: f4(), 2: col}
     if prog in (N.PROC_DEF03, N.PROC_BLINN, N.PROC_SHADOWED):
         return {0: pos, 1: f4(), 2: f4(), 3: f4(), 4: uv}
+    if prog == N.PROC_CLOUD:
+        nrm = f4()
+        nrm[::7] *= 150.0   # 2 (L . n) 255 beyond 32767: the signed byte pack gives 0
+        return {0: pos, 3: nrm, 4: uv}
+    if prog == N.PROC_CLOUD_SHADOW:
+        return {0: pos, 4: uv}
     return {0: pos}
 
 
@@ -80,6 +86,18 @@ def coplanar_pairs(rng, n):
     return np.concatenate(out, 0)
 
 
+def add_depth_map(sc, rng, tex, tex_slot, pv_slot):
+    """a random FP32 depth map (some NaN and inf texels) as the scene's next
+    texture, named by uniform tex_slot, and in uniform pv_slot a random
+    projection that reaches beyond the map on every side"""
+    dm = rng.uniform(-1.2, 1.2, (tex, tex + 7)).astype(np.float32)
+    dm.ravel()[rng.integers(0, dm.size, 6)] = [np.nan, np.inf, -np.inf, np.nan, 0.0, -0.0]
+    img = dm.view(np.uint8).reshape(tex, tex + 7, 4)
+    sc.uniforms[tex_slot] = np.array([len(sc.textures)], np.int32)
+    sc.textures.append(S.TextureSpec([img], wrap=int(rng.integers(0, 2))))
+    sc.uniforms[pv_slot] = S.colmajor(np.eye(4) * 0.8 + rng.uniform(-0.4, 0.4, (4, 4)))
+
+
 def micro_scene(prog, seed, width=96, height=64, n=120, tex=32):
     rng = np.random.default_rng(seed)
     sc = S.Scene(width, height)
@@ -111,15 +129,13 @@ def micro_scene(prog, seed, width=96, height=64, n=120, tex=32):
         sc.uniforms[10] = np.array([1], np.int32)
         sc.uniforms[11] = np.array([int(rng.integers(1, 64))], np.int32)
         if prog == N.PROC_SHADOWED:
-            # a random FP32 depth map (some NaN and inf texels) and a random
-            # projection that reaches beyond the map on every side
-            dm = rng.uniform(-1.2, 1.2, (tex, tex + 7)).astype(np.float32)
-            dm.ravel()[rng.integers(0, dm.size, 6)] = [np.nan, np.inf, -np.inf, np.nan, 0.0, -0.0]
-            img = dm.view(np.uint8).reshape(tex, tex + 7, 4)
-            sc.textures.append(S.TextureSpec([img], wrap=int(rng.integers(0, 2))))
-            sc.uniforms[12] = S.colmajor(np.eye(4) * 0.8 + rng.uniform(-0.4, 0.4, (4, 4)))
-            sc.uniforms[13] = np.array([2], np.int32)
-    sc.draws.append(S.DrawSpec("m", prog))
+            add_depth_map(sc, rng, tex, 13, 12)
+        if prog == N.PROC_CLOUD:
+            add_depth_map(sc, rng, tex, 15, 16)
+    beh = None
+    if prog == N.PROC_CLOUD and seed % 2:
+        beh = N.DEFAULT_BEHAVIOR | N.ALPHABLEND
+    sc.draws.append(S.DrawSpec("m", prog, behavior=beh))
     sc.clear_colour = 0x00203040
     return sc
 
@@ -139,6 +155,26 @@ def behavior_scene(seed, width=96, height=64, n=60):
             sc.meshes[name] = _slots_for(prog, pos, rng)
             sc.draws.append(S.DrawSpec(name, prog, behavior=beh))
     sc.draws.insert(0, S.DrawSpec("m", N.PROC_BLINN))
+    return sc
+
+
+def blend_scene(seed, width=96, height=64, n=60):
+    """the two programmes shaded in the raster walk (PSGPU_PROC_CLOUD with and
+    without blending, PSGPU_PROC_CLOUD_SHADOW's discard) between Blinn-Phong
+    draws, under every combination of the depth behaviours, so blended and
+    discarded fragments land on one another's depth in submission order"""
+    rng = np.random.default_rng(seed)
+    sc = micro_scene(N.PROC_BLINN, seed, width, height, n)
+    add_depth_map(sc, rng, 32, 15, 16)
+    sc.name = "blends_s%d_%dx%d" % (seed, width, height)
+    sc.draws = [S.DrawSpec("m", N.PROC_BLINN)]
+    for beh in range(8):
+        for prog, extra in ((N.PROC_CLOUD, N.ALPHABLEND), (N.PROC_BLINN, 0), (N.PROC_CLOUD_SHADOW, N.ALPHABLEND * (beh & 1)),
+                            (N.PROC_CLOUD, 0)):
+            name = "m%d_%d_%d" % (beh, prog, extra)
+            pos = random_clip_triangles(rng, n // 4)
+            sc.meshes[name] = _slots_for(prog, pos, rng)
+            sc.draws.append(S.DrawSpec(name, prog, behavior=beh | extra))
     return sc
 
 
@@ -211,3 +247,41 @@ def objx_scene(which, width=640, height=480):
         sc.meshes = [X.ObjxMesh("ball/" + m.name, m.vertices, m.normals, m.tangents, m.texcoords, m.indices, m.ambient,
                                 m.diffuse, m.specular, m.specular_exponent, "sphere.jpg")]
     return X.load_scene(sc, width, height, name="objx_%s_%dx%d" % (which, width, height))
+
+
+def cloud_quad_scene(shadow_depth, normal_scale, red, bg=0x00406080, w=64, h=48):
+    """one clip-space quad of PSGPU_PROC_CLOUD under blending: normal (0, 0, 1)
+    times normal_scale, a light far along +z (L . n = normal_scale to FP32
+    precision), a constant texel with the given red, a constant shadow map"""
+    sc = S.Scene(w, h)
+    q = np.array([[-0.8, -0.7, 0.0, 1.0], [0.8, -0.7, 0.0, 1.0], [0.8, 0.7, 0.0, 1.0],
+                  [-0.8, -0.7, 0.0, 1.0], [0.8, 0.7, 0.0, 1.0], [-0.8, 0.7, 0.0, 1.0]], np.float32)
+    nrm = np.tile(np.float32([0.0, 0.0, normal_scale, 0.0]), (6, 1))
+    uv = np.clip(q[:, :2] * 0.5 + 0.5, 0, 1).astype(np.float32)
+    sc.meshes["q"] = {0: q, 3: nrm, 4: uv}
+    tex = np.zeros((4, 4, 4), np.uint8)
+    tex[..., 2] = red
+    sc.textures.append(S.TextureSpec([tex], wrap=N.CLAMP))
+    dm = np.full((16, 16), shadow_depth, np.float32)
+    sc.textures.append(S.TextureSpec([dm.view(np.uint8).reshape(16, 16, 4)], wrap=N.CLAMP))
+    bias = np.array([[0.5, 0, 0, 0.5], [0, 0.5, 0, 0.5], [0, 0, 1, 0], [0, 0, 0, 1]], np.float64)
+    sc.uniforms.update({3: S.colmajor(np.eye(4)), 4: S.colmajor(np.eye(4)), 5: S.colmajor(np.eye(4)), 7: S.vec4(0.0, 0.0, 1e6, 0.0), 9: np.array([0], np.int32),
+                        15: np.array([1], np.int32), 16: S.colmajor(bias)})
+    sc.draws.append(S.DrawSpec("q", N.PROC_CLOUD, behavior=N.UPDATE_DEPTH | N.TEST_DEPTH | N.ALPHABLEND))
+    sc.clear_colour = bg
+    return sc
+
+
+def cloud_shadow_quad(red, w=64, h=48):
+    """one clip-space quad of PSGPU_PROC_CLOUD_SHADOW at z = 0.25 over a constant texel"""
+    sc = S.Scene(w, h)
+    q = np.array([[-0.9, -0.9, 0.25, 1.0], [0.9, -0.9, 0.25, 1.0], [0.9, 0.9, 0.25, 1.0],
+                  [-0.9, -0.9, 0.25, 1.0], [0.9, 0.9, 0.25, 1.0], [-0.9, 0.9, 0.25, 1.0]], np.float32)
+    sc.meshes["q"] = {0: q, 4: np.clip(q[:, :2] * 0.5 + 0.5, 0, 1).astype(np.float32)}
+    tex = np.zeros((4, 4, 4), np.uint8)
+    tex[..., 2] = red
+    sc.textures.append(S.TextureSpec([tex], wrap=N.CLAMP))
+    sc.uniforms.update({3: S.colmajor(np.eye(4)), 4: S.colmajor(np.eye(4)), 9: np.array([0], np.int32)})
+    sc.draws.append(S.DrawSpec("q", N.PROC_CLOUD_SHADOW, behavior=N.UPDATE_DEPTH | N.TEST_DEPTH))
+    sc.clear_colour = 0x00112233
+    return sc

@@ -136,10 +136,55 @@ def test_sphere_tilings_identical(oracle, tiling):
 
 
 @pytest.mark.parametrize("prog", [N.PROC_DEF01, N.PROC_DEF02, N.PROC_DEF03, N.PROC_DEF04, N.PROC_DEF05,
-                                  N.PROC_LAMBERT, N.PROC_BLINN, N.PROC_SHADOWED])
+                                  N.PROC_LAMBERT, N.PROC_BLINN, N.PROC_SHADOWED, N.PROC_CLOUD,
+                                  N.PROC_CLOUD_SHADOW])
 @pytest.mark.parametrize("seed", [1, 2])
 def test_micro_scenes(oracle, prog, seed):
     sc = scenegen.micro_scene(prog, seed)
+    assert_same(sc, oracle)
+
+
+def test_sse_pack_selftest(oracle):
+    """FP_Cloud's cvtps2dq/packusdw/packuswb tail on the device against the
+    oracle's (which runs cvtps2dq itself): ties, the word and byte
+    saturations, NaN and out-of-range values"""
+    announce("selftest sse pack")
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-300, 600, 20000), rng.uniform(-1e5, 1e5, 5000),
+                        np.arange(-4, 70000, 0.5), [np.nan, np.inf, -np.inf, 3e9, -3e9, 2147483520.0, -2147483648.0,
+                                                   32767.5, 32768.5, 65535.5, 255.5, 254.5]]).astype(np.float32)
+    ctx = N.Context(8, 8)
+    try:
+        got = ctx.selftest_math(10, x)
+    finally:
+        ctx.close()
+    want = np.array([oracle.sse_pack_byte(v) for v in x], np.float32)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_blend_scenes_bitexact(oracle, seed):
+    """blended and discarding programmes between deferred Blinn-Phong draws,
+    under every depth behaviour: each programme change starts a new flush and
+    the raster walk keeps submission order per pixel"""
+    assert_same(scenegen.blend_scene(seed), oracle)
+
+
+@pytest.mark.parametrize("case", [(1e30, 1.0), (-1e30, 1.0), (1e30, 100.0)])
+def test_cloud_quads_bitexact(oracle, case):
+    assert_same(scenegen.cloud_quad_scene(case[0], case[1], red=170), oracle)
+
+
+@pytest.mark.parametrize("red", [149, 150])
+def test_cloud_shadow_quads_bitexact(oracle, red):
+    assert_same(scenegen.cloud_shadow_quad(red), oracle)
+
+
+def test_cloud_layer_bitexact(oracle):
+    """the demo's cloud: its shadow programme in the shadow pass (discarding
+    thin cloud), then the blended, lit and shadowed cloud over the shadowed
+    terrain"""
+    sc = S.cloud_terrain_scene()
     assert_same(sc, oracle)
 
 

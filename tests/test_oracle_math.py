@@ -10,6 +10,7 @@ import pytest
 
 from puresoft3d_amd import native as N
 from puresoft3d_amd import scenes as S
+import scenegen
 
 I4 = S.colmajor(np.eye(4))
 
@@ -428,3 +429,115 @@ def test_projection_sampler_outside_the_map(oracle):
         if depth[r % 15, c % 15] < 0.5:
             want = np.float32(want - np.float32(0.2))
     assert got == float(want)
+
+
+# ---------------------------------------------------------------- blending, the cloud programme's tail
+def _mmx_blend4(d, s):
+    """blend4 (fbo.cpp:208-229) instruction by instruction on 16-bit words:
+    unpack bytes to words, psubusw src - dst, pmullw by the source alpha (low
+    word), psrlw 8, paddw dst, packuswb (signed words to unsigned bytes)"""
+    d = np.asarray(d, np.uint16)
+    s = np.asarray(s, np.uint16)
+    a = s[..., 3:4]
+    t = np.where(s > d, s - d, 0).astype(np.uint16)
+    m = (t.astype(np.uint32) * a.astype(np.uint32) & 0xFFFF).astype(np.uint16)
+    v = ((m >> 8) + d).astype(np.uint16).view(np.int16)
+    return np.clip(v, 0, 255).astype(np.uint8)
+
+
+def test_blend4_instruction_sequence(oracle):
+    rng = np.random.default_rng(5)
+    n = 20000
+    d = rng.integers(0, 256, (n, 4)).astype(np.uint8)
+    s = rng.integers(0, 256, (n, 4)).astype(np.uint8)
+    # corners: alpha 0 and 255, equal bytes, extremes
+    d[:8] = [[0] * 4, [255] * 4, [0] * 4, [255] * 4, [10, 20, 30, 40], [200, 0, 7, 9], [0] * 4, [128] * 4]
+    s[:8] = [[255] * 4, [0] * 4, [255, 255, 255, 0], [255] * 4, [10, 20, 30, 40], [100, 255, 7, 255],
+             [1, 1, 1, 255], [129, 255, 0, 128]]
+    got = np.stack([oracle.blend4(d[i], s[i]) for i in range(n)])
+    np.testing.assert_array_equal(got, _mmx_blend4(d, s))
+    # what the sequence fixes regardless of its arithmetic
+    assert (got[s[:, 3] == 0] == d[s[:, 3] == 0]).all()                 # alpha 0 keeps dst
+    le = s <= d
+    assert (got[le] == d[le]).all()                                      # never darkens
+    assert (got >= d).all() and (got[~le] <= s[~le]).all()               # ends between dst and src
+    assert list(got[0]) == [254, 254, 254, 254]                          # 255 * 255 >> 8 = 254
+    assert list(got[2]) == [0, 0, 0, 0]
+
+
+def test_blend4_monotone_in_alpha(oracle):
+    d = np.array([30, 60, 90, 0], np.uint8)
+    prev = d
+    for a in range(256):
+        cur = oracle.blend4(d, np.array([200, 250, 255, a], np.uint8))
+        assert (cur >= prev).all()
+        prev = cur
+    assert list(prev[:3]) == [30 + (170 * 255 >> 8), 60 + (190 * 255 >> 8), 90 + (165 * 255 >> 8)]
+
+
+def test_sse_pack_byte_known_values(oracle):
+    """cvtps2dq rounds half to even and gives 0x80000000 for NaN and out-of-range
+    values; packusdw saturates to 0..65535; packuswb reads those words as
+    signed, so 32768..65535 pack to 0 and 256..32767 to 255"""
+    cases = {0.0: 0, -0.0: 0, 0.5: 0, 1.5: 2, 2.5: 2, 3.49: 3, 127.5: 128, 254.5: 254, 254.51: 255, 255.0: 255,
+             255.5: 255, 256.0: 255, 1000.0: 255, 32767.0: 255, 32767.5: 0, 32768.0: 0, 51000.0: 0,
+             65535.0: 0, 70000.0: 0, 1e9: 0, 3e9: 0, -0.5: 0, -0.51: 0, -3.0: 0, -1e9: 0,
+             float("nan"): 0, float("inf"): 0, float("-inf"): 0, 101.99999: 102, 102.5: 102, 103.5: 104}
+    for x, want in cases.items():
+        assert oracle.sse_pack_byte(x) == want, x
+
+
+def _cloud_expect(oracle, sc, grey, red):
+    """the closed form: covered pixels (depth 0 written) = blend4 of
+    (grey, grey, grey, red) onto the clear colour, the rest untouched"""
+    bgf = oracle.Frame(sc.width, sc.height)
+    bgf.clear_depth(1.0)
+    bgf.clear_colour(sc.clear_colour)
+    exp = bgf.colour.copy()
+    f, _ = oracle.render_scene(sc)
+    cov = f.depth_image()[::-1] == 0.0   # depth rows are bottom-up, colour rows top-down
+    assert cov.sum() > 0.5 * sc.width * sc.height
+    src = np.array([grey, grey, grey, red], np.uint8)
+    for yx in zip(*np.nonzero(cov)):
+        exp[yx] = _mmx_blend4(exp[yx], src)
+    return exp, f
+
+
+@pytest.mark.parametrize("case", ["lit", "shadowed", "word_overflow"])
+def test_cloud_closed_form(oracle, case):
+    """FP_Cloud (testproc.cpp:534-574) on a flat quad: lit, 2 (L . n) 255 = 510
+    saturates to 255; under a map whose four taps all occlude, the factor is
+    1 - 4 x 0.2 and 510 x 0.2 rounds to 102; a normal of length 100 makes the
+    word 51000, which the signed byte pack turns into 0 (blend4 then keeps the
+    background)"""
+    depth, scale, grey = {"lit": (1e30, 1.0, 255), "shadowed": (-1e30, 1.0, 102),
+                          "word_overflow": (1e30, 100.0, 0)}[case]
+    sc = scenegen.cloud_quad_scene(depth, scale, red=170)
+    exp, f = _cloud_expect(oracle, sc, grey, 170)
+    np.testing.assert_array_equal(f.colour, exp)
+
+
+@pytest.mark.parametrize("red", [149, 150])
+def test_cloud_shadow_discard_and_shrink(oracle, red):
+    """FP_CloudShadow (testproc.cpp:677-686): red below 150 discards, and a
+    discarded fragment writes no depth (fragthrd.cpp:227-236); at 150 the
+    quad's depth lands where VP_CloudShadow's 0.95 shrink puts it (0.9 would
+    leave the outer columns untouched); colour is never written"""
+    W, H = 64, 48
+    sc = scenegen.cloud_shadow_quad(red, W, H)
+    f, st = oracle.render_scene(sc)
+    d = f.depth_image()
+    if red < 150:
+        assert (d == 1.0).all()
+        assert st.frags_shaded > 0          # fragments passed the test, then were discarded
+    else:
+        cov = d < 1.0
+        cols = np.nonzero(cov.any(0))[0]
+        rows = np.nonzero(cov.any(1))[0]
+        # the shrunk quad spans NDC +-0.855: pixel centres x + 0.5 - 32 within 32 * 0.855 = 27.36
+        assert cols.min() == 32 - 28 and cols.max() == 32 + 27
+        assert rows.min() == 24 - 21 and rows.max() == 24 + 20
+        np.testing.assert_allclose(d[cov], 0.25 * 0.95, rtol=1e-6)   # z shrinks with x and y
+    ref = oracle.Frame(W, H)
+    ref.clear_colour(sc.clear_colour)
+    np.testing.assert_array_equal(f.colour, ref.colour)
