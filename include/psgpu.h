@@ -126,7 +126,10 @@ int psgpu_behavior_enable(psgpu_ctx* ctx, int bits);
 int psgpu_behavior_disable(psgpu_ctx* ctx, int bits);
 int psgpu_behavior_get(psgpu_ctx* ctx, int* bits);
 int psgpu_viewport(psgpu_ctx* ctx, int width, int height);
-int psgpu_target_set(psgpu_ctx* ctx, int fbo_slot, int tex_or_display);   /* setFBO, slot 0 only */
+/* setFBO (pipeline.cpp:260-275): slot 0 takes a texture or PSGPU_DISPLAY; slots
+ * 1..15 take a texture or PSGPU_DISPLAY for none, and are only recorded: every
+ * processor (here and in the reference) writes slot 0 */
+int psgpu_target_set(psgpu_ctx* ctx, int fbo_slot, int tex_or_display);
 int psgpu_depth_set(psgpu_ctx* ctx, int tex_or_default);                   /* setDepth */
 /* geometry buffer of the visible fragments (no counterpart in the reference):
  * while set, Blinn-Phong draws also write, for every pixel whose colour they

@@ -151,6 +151,7 @@ struct psgpu_ctx {
     int behavior = PSGPU_UPDATE_DEPTH | PSGPU_TEST_DEPTH | PSGPU_FACE_CULLING;  // pipeline.cpp:34
     int cur_prog = -1;
     int fbo0 = PSGPU_DISPLAY;
+    int fbo_more[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};   // slots 1..15
     int depth_sel = PSGPU_DEFAULT_DEPTH;
     float* gbuf = nullptr;         // caller's geometry buffer (device memory), or none
     size_t gbuf_bytes = 0;
@@ -1214,9 +1215,12 @@ int psgpu_target_set(psgpu_ctx* c, int fbo_slot, int tex)
 {
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
     if (fbo_slot < 0 || fbo_slot >= 16) return fail(c, PSGPU_ERR_RANGE, "FBO slot out of range");
-    if (fbo_slot != 0) return fail(c, PSGPU_ERR_UNSUPPORTED, "the built-in processors write FBO slot 0 only");
     if (tex != PSGPU_DISPLAY && !valid_tex(c, tex)) return fail(c, PSGPU_ERR_RANGE, "texture index out of range");
-    c->fbo0 = tex;
+    // slots 1..15 are bound as the reference binds them (pipeline.cpp:260-275;
+    // PSGPU_DISPLAY stands for NULL there); no processor twin writes them, as
+    // no processor of the reference does
+    if (fbo_slot != 0) c->fbo_more[fbo_slot] = tex;
+    else c->fbo0 = tex;
     return PSGPU_OK;
 }
 

@@ -74,6 +74,9 @@ PuresoftProcessor* make_proc(int kind, int role)
         PS_CASE(PSGPU_PROC_DEF05, DEF05)
         PS_CASE(PSGPU_PROC_LAMBERT, LAMBERT)
         PS_CASE(PSGPU_PROC_BLINN, BLINN)
+        PS_CASE(PSGPU_PROC_SHADOWED, SHADOWED)
+        PS_CASE(PSGPU_PROC_CLOUD, CLOUD)
+        PS_CASE(PSGPU_PROC_CLOUD_SHADOW, CLOUD_SHADOW)
     }
 #undef PS_CASE
     throw std::invalid_argument("unknown processor kind " + std::to_string(kind));
@@ -84,7 +87,7 @@ PuresoftProcessor* make_proc(int kind, int role)
 int main(int argc, char** argv)
 {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s <scene.pssc> <out.bgra> [frames]\n", argv[0]);
+        fprintf(stderr, "usage: %s <scene.pssc> <out.bgra> [frames] [dump-prefix]\n", argv[0]);
         return 2;
     }
     int frames = argc > 3 ? atoi(argv[3]) : 1;
@@ -147,8 +150,8 @@ int main(int argc, char** argv)
                 programmes[d.prog] = pipe.createProgramme(v, i, f);
             }
         }
-        const int all = BEHAVIOR_UPDATE_DEPTH | BEHAVIOR_TEST_DEPTH | BEHAVIOR_FACE_CULLING;
-        const int deflt = all;
+        const int all = BEHAVIOR_UPDATE_DEPTH | BEHAVIOR_TEST_DEPTH | BEHAVIOR_FACE_CULLING | BEHAVIOR_ALPHABLEND;
+        const int deflt = BEHAVIOR_UPDATE_DEPTH | BEHAVIOR_TEST_DEPTH | BEHAVIOR_FACE_CULLING;
 
         auto frame = [&]() {
             pipe.clearDepth(clear_depth);
@@ -186,6 +189,16 @@ int main(int argc, char** argv)
         FILE* f = fopen(argv[2], "wb");
         if (!f || fwrite(pixels.data(), 1, pixels.size(), f) != pixels.size()) throw std::runtime_error("cannot write output");
         fclose(f);
+        if (argc > 4) {   // the debug api's dumps of the display and the depth buffer
+            std::string pre = argv[4];
+            pipe.saveTextureAsRaw(-2, (pre + ".display.raw").c_str());
+            pipe.saveTextureAsRaw(-1, (pre + ".depth.raw").c_str());
+            pipe.saveTexture(-2, (pre + ".display.bmp").c_str(), false);
+            pipe.saveTexture(-1, (pre + ".depth.bmp").c_str(), true);
+            unsigned q[4] = {1, 1, 1, 1};
+            pipe.getTaskQCounters(q, q + 1, q + 2, q + 3);
+            printf("psrender: task queue counters %u %u %u %u\n", q[0], q[1], q[2], q[3]);
+        }
     } catch (const std::exception& e) {
         fprintf(stderr, "psrender: %s\n", e.what());
         return 1;

@@ -3,6 +3,7 @@
 // rendering goes through the C ABI.
 #include "puresoft3d.h"
 
+#include <stdio.h>
 #include <string.h>
 
 #include <new>
@@ -170,7 +171,114 @@ void PuresoftPipeline::destroyVAO(int vao)
 
 // ---- rendering ------------------------------------------------------------------------
 void PuresoftPipeline::setViewport(int width, int height, uintptr_t) { check(psgpu_viewport(m_ctx, width, height)); }
-void PuresoftPipeline::setDepth(int idx) { check(psgpu_depth_set(m_ctx, idx)); }
+void PuresoftPipeline::setDepth(int idx)
+{
+    check(psgpu_depth_set(m_ctx, idx));
+    m_depthTex = idx;
+}
+
+void PuresoftPipeline::setRenderer(PuresoftHeadlessRenderer* rndr)
+{
+    // pipeline.cpp:241-258: a null renderer makes the pipeline create its own
+    if (rndr) {
+        m_renderer = rndr;
+    } else {
+        m_ownRenderer.reset(new PuresoftHeadlessRenderer(m_width, m_height));
+        m_renderer = m_ownRenderer.get();
+    }
+}
+
+// ---- debug api (dbg.cpp:3-61) ---------------------------------------------------
+void PuresoftPipeline::grab(int idx, std::vector<unsigned char>& bytes, int& w, int& scan, int& h, int& elem)
+{
+    if (idx == -2) {   // the display, rows top-down as it lies in memory
+        w = m_width; h = m_height; elem = 4; scan = w * 4;
+        bytes.resize((size_t)scan * h);
+        check(psgpu_read_colour(m_ctx, bytes.data(), bytes.size()));
+        return;
+    }
+    if (idx == -1 && m_depthTex < 0) {   // the default depth buffer, FP32
+        w = m_width; h = m_height; elem = 4; scan = (int)(w / 4.0f + 0.5f) * 4 * 4;
+        bytes.resize((size_t)scan * h);
+        check(psgpu_read_depth(m_ctx, bytes.data(), bytes.size()));
+        return;
+    }
+    int t = idx == -1 ? m_depthTex : idx;
+    if (t < 0) throw std::out_of_range("PuresoftPipeline::saveTexture");
+    check(psgpu_texture_info(m_ctx, t, 0, &w, &scan, &h, &elem));
+    bytes.resize((size_t)scan * h);
+    check(psgpu_texture_download(m_ctx, t, 0, 0, bytes.data(), bytes.size()));
+}
+
+static std::string narrow(const wchar_t* p)
+{
+    std::string s;
+    for (; p && *p; p++) s.push_back((char)*p);
+    return s;
+}
+
+void PuresoftPipeline::saveTextureAsRaw(int idx, const char* path)
+{
+    // PuresoftFBO::saveAsRawFile (fbo.cpp:542-550): the buffer's bytes as they are
+    std::vector<unsigned char> b;
+    int w, scan, h, e;
+    grab(idx, b, w, scan, h, e);
+    FILE* f = fopen(path, "wb");
+    if (!f) return;   // the reference ignores an unopenable path
+    fwrite(b.data(), 1, b.size(), f);
+    fclose(f);
+}
+
+void PuresoftPipeline::saveTextureAsRaw(int idx, const wchar_t* path) { saveTextureAsRaw(idx, narrow(path).c_str()); }
+
+static void put16(FILE* f, unsigned v) { fputc(v & 255, f); fputc((v >> 8) & 255, f); }
+static void put32(FILE* f, unsigned v) { put16(f, v & 0xffff); put16(f, v >> 16); }
+
+void PuresoftPipeline::saveTexture(int idx, const char* path, bool dataIsFloat)
+{
+    // PuresoftFBO::saveAsBmpFile (fbo.cpp:473-540), written as a well-formed
+    // BMP: byte texels as 32-bit BGRA with memory row h-1 first, FP32 texels
+    // as 8-bit grey of value * 255 under a grey palette, rows in memory order.
+    std::vector<unsigned char> b;
+    int w, scan, h, e;
+    grab(idx, b, w, scan, h, e);
+    FILE* f = fopen(path, "wb");
+    if (!f) return;
+    if (!dataIsFloat) {
+        unsigned rowb = (unsigned)w * e, img = rowb * h;
+        fputc('B', f); fputc('M', f); put32(f, 54 + img); put32(f, 0); put32(f, 54);
+        put32(f, 40); put32(f, w); put32(f, h); put16(f, 1); put16(f, e * 8); put32(f, 0); put32(f, img);
+        put32(f, 2835); put32(f, 2835); put32(f, 0); put32(f, 0);
+        for (int y = h - 1; y >= 0; y--) fwrite(b.data() + (size_t)y * scan, 1, rowb, f);
+    } else {
+        int chans = e / 4, line = (w * chans + 3) / 4 * 4;
+        unsigned img = (unsigned)line * h, off = 54 + 1024;
+        fputc('B', f); fputc('M', f); put32(f, off + img); put32(f, 0); put32(f, off);
+        put32(f, 40); put32(f, w); put32(f, h); put16(f, 1); put16(f, chans * 8); put32(f, 0); put32(f, img);
+        put32(f, 2835); put32(f, 2835); put32(f, 256); put32(f, 0);
+        for (int i = 0; i < 256; i++) { fputc(i, f); fputc(i, f); fputc(i, f); fputc(0, f); }
+        std::vector<unsigned char> row(line, 0);
+        for (int y = 0; y < h; y++) {
+            const float* src = (const float*)(b.data() + (size_t)y * scan);
+            for (int x = 0; x < w * chans; x++) row[x] = (unsigned char)(int)(src[x] * 255.0f);
+            fwrite(row.data(), 1, line, f);
+        }
+    }
+    fclose(f);
+}
+
+void PuresoftPipeline::saveTexture(int idx, const wchar_t* path, bool dataIsFloat)
+{
+    saveTexture(idx, narrow(path).c_str(), dataIsFloat);
+}
+
+void PuresoftPipeline::getTaskQCounters(unsigned int* taskPushCalled, unsigned int* taskPushSpinned,
+                                        unsigned int* taskPopCalled, unsigned int* taskPopSpinned)
+{
+    // the reference reports zeros unless built with PROFILING (dbg.cpp:45-61);
+    // there are no row-task queues here
+    *taskPushCalled = *taskPushSpinned = *taskPopCalled = *taskPopSpinned = 0;
+}
 void PuresoftPipeline::setFBO(int idx, int tex) { check(psgpu_target_set(m_ctx, idx, tex)); }
 
 void PuresoftPipeline::setUniform(int idx, const void* data, size_t len)

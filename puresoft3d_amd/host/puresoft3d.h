@@ -154,6 +154,7 @@ public:
     void destroyVAO(int vao);
 
     // rendering api
+    void setRenderer(PuresoftHeadlessRenderer* rndr);   // not owned; null: a renderer of the device size
     void setViewport(int width, int height, uintptr_t canvasWindow = 0);
     void setDepth(int idx = -1);
     void setFBO(int idx, int textureOrDisplay);      // texture index, or -1 for the display
@@ -166,6 +167,14 @@ public:
     void clearDepth(float furthest = 1.0f);
     void clearColour(PURESOFTBGRA bkgnd = PURESOFTBGRA_BLACK);
 
+    // debug api (dbg.cpp): idx -1 is the current depth buffer, -2 the display
+    void saveTexture(int idx, const char* path, bool dataIsFloat);
+    void saveTexture(int idx, const wchar_t* path, bool dataIsFloat);
+    void saveTextureAsRaw(int idx, const char* path);
+    void saveTextureAsRaw(int idx, const wchar_t* path);
+    void getTaskQCounters(unsigned int* taskPushCalled, unsigned int* taskPushSpinned, unsigned int* taskPopCalled,
+                          unsigned int* taskPopSpinned);
+
     // access for callers that read results
     void finish(void);
     void readDisplay(void* bgraTopDown);
@@ -173,7 +182,10 @@ public:
 
 private:
     void check(int rc) const;
+    // bytes of texture idx, the depth buffer (-1) or the display (-2), level 0, layer 0, in memory row order
+    void grab(int idx, std::vector<unsigned char>& bytes, int& w, int& scan, int& h, int& elem);
     psgpu_ctx* m_ctx = nullptr;
+    int m_depthTex = -1;
     int m_width, m_height;
     std::unique_ptr<PuresoftHeadlessRenderer> m_ownRenderer;
     PuresoftHeadlessRenderer* m_renderer;

@@ -22,7 +22,8 @@ FACADE = os.path.join(PKG, "libpuresoft3d.so")
 FACADE_METHODS = ["createTexture", "destroyTexture", "getTexture", "addProcessor", "destroyProcessor",
                   "createProgramme", "destroyProgramme", "useProgramme", "createVAO", "attachVBO", "detachVBO",
                   "getVBO", "destroyVAO", "setViewport", "setDepth", "setFBO", "setUniform", "drawVAO",
-                  "swapBuffers", "enable", "disable", "clearDepth", "clearColour"]
+                  "swapBuffers", "enable", "disable", "clearDepth", "clearColour", "postProcess", "setRenderer",
+                  "saveTexture", "saveTextureAsRaw", "getTaskQCounters"]
 
 
 def _built():
@@ -62,24 +63,55 @@ def test_scene_file_layout(tmp_path):
     assert len(raw) == 24 + 4 + tex + 4 + mesh + uni + draws
 
 
-def _psrender(scene, tmp_path, frames=1):
+def _psrender(scene, tmp_path, frames=1, dump=None):
     p = str(tmp_path / (scene.name + ".pssc"))
     o = str(tmp_path / (scene.name + ".bgra"))
     S.save_scene(scene, p)
     announce("psrender %s frames=%d" % (scene.name, frames))
-    r = subprocess.run([PSRENDER, p, o, str(frames)], capture_output=True, text=True, timeout=100)
+    r = subprocess.run([PSRENDER, p, o, str(frames)] + ([dump] if dump else []), capture_output=True, text=True,
+                       timeout=100)
     assert r.returncode == 0, r.stderr
+    if dump:
+        assert "task queue counters 0 0 0 0" in r.stdout
     return np.fromfile(o, np.uint8).reshape(scene.height, scene.width, 4)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["sphere", "blinn", "cube", "behaviors"])
+@pytest.mark.parametrize("which", ["sphere", "blinn", "cube", "behaviors", "blends"])
 def test_psrender_matches_oracle(oracle, tmp_path, which):
     _built()
     sc = {"sphere": lambda: S.sphere_scene(width=320, height=240),
           "blinn": lambda: scenegen.micro_scene(N.PROC_BLINN, 3),
           "cube": lambda: scenegen.micro_scene(N.PROC_DEF04, 3),
-          "behaviors": lambda: scenegen.behavior_scene(4)}[which]()
+          "behaviors": lambda: scenegen.behavior_scene(4),
+          "blends": lambda: scenegen.blend_scene(5)}[which]()
     got = _psrender(sc, tmp_path, frames=2)
     f, _ = oracle.render_scene(sc)
     np.testing.assert_array_equal(got, f.colour)
+
+
+@pytest.mark.gpu
+def test_debug_api_dumps(oracle, tmp_path):
+    """saveTextureAsRaw (fbo.cpp:542-550) writes the display and the depth
+    buffer as they lie in memory; saveTexture writes them as BMP (32-bit
+    BGRA bottom row first; FP32 depth as 8-bit grey of value * 255);
+    getTaskQCounters reports zeros, as the reference does without PROFILING"""
+    _built()
+    sc = scenegen.micro_scene(N.PROC_BLINN, 4)
+    pre = str(tmp_path / "dump")
+    got = _psrender(sc, tmp_path, dump=pre)
+    f, _ = oracle.render_scene(sc)
+    W, H = sc.width, sc.height
+    raw = np.fromfile(pre + ".display.raw", np.uint8).reshape(H, W, 4)
+    np.testing.assert_array_equal(raw, got)
+    dep = np.fromfile(pre + ".depth.raw", np.float32).reshape(H, -1)
+    np.testing.assert_array_equal(dep.view(np.uint32), f.depth.view(np.uint32))
+    bmp = np.fromfile(pre + ".display.bmp", np.uint8)
+    assert bytes(bmp[:2]) == b"BM" and bmp.size == 54 + W * H * 4
+    np.testing.assert_array_equal(bmp[54:].reshape(H, W, 4), f.colour[::-1])
+    gb = np.fromfile(pre + ".depth.bmp", np.uint8)
+    line = (W + 3) // 4 * 4
+    assert gb.size == 54 + 1024 + line * H
+    grey = gb[54 + 1024:].reshape(H, line)[:, :W]
+    want = (f.depth[:, :W] * np.float32(255.0)).astype(np.int32).astype(np.uint8)
+    np.testing.assert_array_equal(grey, want)

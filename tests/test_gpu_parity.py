@@ -188,6 +188,32 @@ def test_cloud_layer_bitexact(oracle):
     assert_same(sc, oracle)
 
 
+def test_extra_fbo_slots_are_recorded_only(oracle):
+    """setFBO on slots 1..15 binds as the reference binds (pipeline.cpp:260-275)
+    and nothing writes there: the texture keeps its pixels, the frame is the
+    oracle's; a bad texture index or slot 16 is refused"""
+    sc = scenegen.micro_scene(N.PROC_BLINN, 3)
+    announce("extra FBO slots " + sc.name)
+    ctx = N.Context(sc.width, sc.height)
+    try:
+        gs = S.GpuScene(ctx, sc)
+        px = np.arange(16 * 16 * 4, dtype=np.uint32).astype(np.uint8).reshape(16, 16, 4)
+        t = ctx.texture_create(16, 16, 4, pixels=px)
+        for slot in (1, 7, 15):
+            ctx.target(t, slot)
+        ctx.target(N.DISPLAY, 7)   # NULL: unbound again
+        for bad in ((t, 16), (t, -1), (t + 100, 3)):
+            with pytest.raises(IndexError):   # PSGPU_ERR_RANGE
+                ctx.target(*bad)
+        gs.frame()
+        col = ctx.read_colour()
+        np.testing.assert_array_equal(ctx.texture_download(t), px.reshape(16, 64))
+    finally:
+        ctx.close()
+    f, _ = oracle.render_scene(sc)
+    np.testing.assert_array_equal(col, f.colour)
+
+
 def test_terrain_reduced_bitexact(oracle):
     sc = S.terrain_scene(width=480, height=270, cells=60, n_spheres=8, tex_size=256)
     assert_same(sc, oracle)
