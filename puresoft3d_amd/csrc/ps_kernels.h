@@ -7,7 +7,7 @@ void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCou
 // work pieces of a fill (1024 values of a row); psk_fills takes the largest
 int64_t psk_fill_pieces(int64_t rows, int64_t width);
 void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, const PsCounters* c);
-void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
+void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views);
 int psk_scan_blocks(int64_t cap);
 // exclusive scan; out[n] and *total_out (if given) receive the total

@@ -62,6 +62,16 @@ __device__ __forceinline__ int ps_view_of_stile(const PsFrame& f, int64_t t)
     return lo;
 }
 
+// where triangle tri of draw d finds its three vertices' varyings
+__device__ __forceinline__ int ps_vary_index(const PsDraw& d, int tri) { return tri + d.vary_delta; }
+
+// the first of the two texcoord varyings the mip-level gradients use (-1: none)
+template <class P> __device__ constexpr int ps_grad_iu()
+{
+    if constexpr (P::GRAD) return P::IU;
+    else return -1;
+}
+
 // f(draw record): when every active lane of the wave uses the same draw, the
 // record is addressed through a wave-uniform index, so its fields come in
 // with scalar loads instead of one vector load per lane
@@ -144,7 +154,7 @@ __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draw
     return d;
 }
 
-template <int PROG>
+template <int PROG, bool SHARE>
 __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
                                                float* __restrict__ vary, int nvs, float* __restrict__ grads,
                                                uint32_t* __restrict__ nrows, PsCounters* cnt,
@@ -158,7 +168,7 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
     unsigned culled = 0, zrej = 0, dropped = 0;
     const int lane = threadIdx.x & 63, n3 = 3 * nvs;
     float* wl = su + (size_t)(threadIdx.x & ~63) * n3;
-    bool pieces = false;
+    bool stage = false;
     ps_with_draw(draws, di, [&](const PsDraw& d) {
     if (live && d.prog == PROG) {
         const int W = views[d.view].W, H = views[d.view].H;
@@ -166,9 +176,12 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
         float pos[3][4];
         float u[3][P::NV > 0 ? P::NV : 1];
         float rw[3], z[3];
+        bool pieces = false;
+        const bool skip = SHARE && ps_shares_varyings(PROG) && d.vary_skip;
 #pragma unroll
         for (int v = 0; v < 3; v++) {
-            P::vp(d, lt * 3 + v, pos[v], u[v]);
+            if (skip) ps_vp_pos(d, lt * 3 + v, pos[v], u[v], ps_grad_iu<P>());
+            else P::vp(d, lt * 3 + v, pos[v], u[v]);
             // perspective division (vertthrd.cpp:37-47)
             float r = 1.0f / pos[v][3];
             pos[v][0] *= r; pos[v][1] *= r; pos[v][2] *= r; pos[v][3] *= r;
@@ -242,7 +255,7 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
                 // extrapolate, so no x range test is safe here)
                 pieces = n > 0;
                 if (pieces) tris[t] = tr;
-                if (pieces) {   // staged: the wave stores whole triangles below
+                if (!SHARE && pieces) {   // staged: the wave stores whole triangles below
 #pragma unroll
                     for (int v = 0; v < 3; v++)
 #pragma unroll
@@ -258,12 +271,20 @@ __global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws,
             }
         }
         nrows[t] = n;
+        // varyings of triangles with rows, or of all when other views read them
+        stage = SHARE ? !skip && (pieces || d.vary_all) : pieces;
+        if (SHARE && stage) {
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+#pragma unroll
+                for (int k = 0; k < P::NV; k++) wl[lane * n3 + v * nvs + k] = u[v][k];
+        }
     }
     });
-    // varyings of the wave's triangles with rows, one triangle per store
+    // varyings of the wave's staged triangles, one triangle per store
     // instruction (3*nvs consecutive floats) instead of one 3*nvs-float stride
     // per lane; floats past NV of each vertex are left as they are
-    uint64_t m = __ballot(pieces);
+    uint64_t m = __ballot(stage);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -537,7 +558,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                     int right = ps_cvt_i32(ps_edge_eval(Rr, (float)y) + 0.5f);
                     PsSpanGeom g;
                     ps_span_geom(tr, y, left, right, L, Rr, g);
-                    const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
+                    const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
                     ps_span_varyings_n(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt, nv);
                 }
                 float cf2 = sp.cf2, pz = sp.pz;
@@ -1152,7 +1173,7 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
                     int right = ps_cvt_i32(ps_edge_eval(Re, (float)y) + 0.5f);
                     PsSpanGeom g;
                     ps_span_geom(tr, y, left, right, Le, Re, g);
-                    const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
+                    const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
                     ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
                     int xs = sp.x1;
                     if (a > sp.x1 && ps_span_has_ck(sp) && wc) {
@@ -1236,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
     int right = ps_cvt_i32(ps_edge_eval(R, (float)y) + 0.5f);
     PsSpanGeom g;
     ps_span_geom(tr, y, left, right, L, R, g);
-    const float* u0 = vary + (int64_t)sp.tri * 3 * nvs;
+    const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
     float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
     ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
     // the piece's first pixel carries the accumulators as the serial walk from
@@ -1346,12 +1367,18 @@ void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, co
 }
 
 
-void psk_setup(hipStream_t s, int prog, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
+void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
     dim3 g(gsz(T, 256)), b(256);
     size_t lds = (size_t)256 * 3 * nvs * sizeof(float);
-#define SU_CASE(K) case K: hipLaunchKernelGGL(k_setup<K>, g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); break;
+#define SU_CASE(K)                                                                                                  \
+    case K:                                                                                                         \
+        if (share && ps_shares_varyings(K))                                                                         \
+            hipLaunchKernelGGL((k_setup<K, true>), g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_setup<K, false>), g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); \
+        break;
     switch (prog) {
     SU_CASE(PSGPU_PROC_DEF01) SU_CASE(PSGPU_PROC_DEF02) SU_CASE(PSGPU_PROC_DEF03) SU_CASE(PSGPU_PROC_DEF04)
     SU_CASE(PSGPU_PROC_DEF05) SU_CASE(PSGPU_PROC_LAMBERT) SU_CASE(PSGPU_PROC_BLINN) SU_CASE(PSGPU_PROC_SHADOWED)

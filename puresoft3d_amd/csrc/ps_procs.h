@@ -10,6 +10,14 @@
 
 template <int PROG> struct PsProc;
 
+static_assert(ps_shares_varyings(PSGPU_PROC_DEF01) && ps_shares_varyings(PSGPU_PROC_DEF02) &&
+                  ps_shares_varyings(PSGPU_PROC_DEF03) && ps_shares_varyings(PSGPU_PROC_LAMBERT) &&
+                  ps_shares_varyings(PSGPU_PROC_BLINN) && ps_shares_varyings(PSGPU_PROC_SHADOWED) &&
+                  ps_shares_varyings(PSGPU_PROC_CLOUD) && !ps_shares_varyings(PSGPU_PROC_DEF04) &&
+                  !ps_shares_varyings(PSGPU_PROC_DEF05) && !ps_shares_varyings(PSGPU_PROC_CLOUD_SHADOW),
+              "the programmes that share varyings across views");
+
+
 PS_DEV const float* ps_slot(const PsDraw& d, int s, int64_t i)
 {
     return (const float*)(d.slot[s] + i * (int64_t)d.unit[s]);
@@ -18,6 +26,20 @@ PS_DEV const float* ps_slot(const PsDraw& d, int s, int64_t i)
 PS_DEV void ps_load4(float* r, const float* p)
 {
     r[0] = p[0]; r[1] = p[1]; r[2] = p[2]; r[3] = p[3];
+}
+
+// the clip position of a sharing programme, the same operations as its VP,
+// plus the texcoord its mip-level gradients need (slot 4 into u[iu], u[iu + 1])
+PS_DEV void ps_vp_pos(const PsDraw& d, int64_t i, float* pos, float* u, int iu)
+{
+    float p[4], wp[4];
+    ps_load4(p, ps_slot(d, 0, i));
+    ps_m4v4(wp, d.M, p);
+    ps_m4v4(pos, d.PV, wp);
+    if (iu >= 0) {
+        const float* tc = ps_slot(d, 4, i);
+        u[iu] = tc[0]; u[iu + 1] = tc[1];
+    }
 }
 
 // common lighting tail of DEF01/02/03 (tex1light1.cpp:321-349)

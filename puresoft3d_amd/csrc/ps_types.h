@@ -9,6 +9,14 @@
 #define PS_MAX_LEVELS  16
 #define PS_MAX_LAYERS  6
 
+// programmes (PSGPU_PROC_* values) whose clip position is PV (M p) and whose
+// varyings depend on the vertex streams, M, Mrot and shadowPV only, so that
+// views of one scene can share them
+constexpr bool ps_shares_varyings(int prog)
+{
+    return prog == 1 || prog == 2 || prog == 3 || prog == 6 || prog == 7 || prog == 8 || prog == 9;
+}
+
 // One texture (a PuresoftFBO used as a texture, fbo.h:6-100) in HBM.
 // Layer-major; inside a layer level 0 (rows of `scanline` bytes) then the
 // tightly packed mip levels.
@@ -31,6 +39,11 @@ struct PsDraw {
     int64_t tri_start; // first global triangle id of this draw (flush-local)
     int64_t tri_count;
     int64_t id_base;   // debug triangle-id base
+    // views of one scene in one flush share its varyings (they do not depend
+    // on the view): triangle t of the draw reads those of triangle t + vary_delta;
+    // vary_all: write them for every triangle (another draw reads them);
+    // vary_skip: write none, compute positions only
+    int vary_delta, vary_all, vary_skip, pad0;
     const uint8_t* slot[5];
     int unit[5];
     int spec_exp;

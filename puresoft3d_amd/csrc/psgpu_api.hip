@@ -125,6 +125,7 @@ struct Blob {
     std::vector<uint8_t> bytes;
     size_t off_views = 0, off_fills = 0;
     int nd = 0, nv = 0, nf = 0;
+    bool shares[PS_N_PROGS] = {false};   // some draws of the programme share varyings
 };
 
 }  // namespace
@@ -357,6 +358,47 @@ int key_bits(int64_t n_keys)
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Views of one scene share its varyings: a draw whose programme, vertex
+// streams and model transforms (M, Mrot, shadowPV) equal those of an earlier
+// draw of the same command reads that draw's varyings, which then writes them
+// for all its triangles, and computes clip positions only.
+static bool same_varyings(const PsDraw& a, const PsDraw& b)
+{
+    return a.prog == b.prog && a.tri_count == b.tri_count && !memcmp(a.slot, b.slot, sizeof(a.slot)) &&
+           !memcmp(a.unit, b.unit, sizeof(a.unit)) && !memcmp(a.M, b.M, sizeof(a.M)) &&
+           !memcmp(a.Mrot, b.Mrot, sizeof(a.Mrot)) && !memcmp(a.shadowPV, b.shadowPV, sizeof(a.shadowPV));
+}
+
+static void share_varyings(std::vector<PsDraw>& draws)
+{
+    std::multimap<uint64_t, size_t> seen;
+    for (size_t i = 0; i < draws.size(); i++) {
+        PsDraw& d = draws[i];
+        d.vary_delta = 0;
+        d.vary_all = d.vary_skip = 0;
+        if (!ps_shares_varyings(d.prog)) continue;
+        // FNV-1a over the fields same_varyings compares
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&h](const void* p, size_t n) {
+            for (size_t k = 0; k < n; k++) h = (h ^ ((const uint8_t*)p)[k]) * 1099511628211ull;
+        };
+        mix(&d.prog, sizeof(d.prog)); mix(&d.tri_count, sizeof(d.tri_count)); mix(d.slot, sizeof(d.slot));
+        mix(d.M, sizeof(d.M)); mix(d.Mrot, sizeof(d.Mrot));
+        bool found = false;
+        for (auto r = seen.equal_range(h); r.first != r.second; ++r.first) {
+            PsDraw& s = draws[r.first->second];
+            if (same_varyings(s, d)) {
+                d.vary_delta = (int)(s.tri_start - d.tri_start);
+                d.vary_skip = 1;
+                s.vary_all = 1;
+                found = true;
+                break;
+            }
+        }
+        if (!found) seen.emplace(h, i);
+    }
+}
+
 // lay out a command's device records: draws (view-tagged, in submission
 // order), views (tiling and scratch offsets are filled in here), fills
 int build_blob(psgpu_ctx* c, const Cmd& cmd, Blob& b, PsFrame& f, int64_t& T, int& nvs, bool* progs, bool* shade_progs,
@@ -403,6 +445,9 @@ int build_blob(psgpu_ctx* c, const Cmd& cmd, Blob& b, PsFrame& f, int64_t& T, in
             draws.push_back(d);
         }
     }
+    share_varyings(draws);
+    for (const PsDraw& d : draws)
+        if (d.vary_all) b.shares[d.prog] = true;
     f.n_views = (int)views.size();
     f.n_tiles = tile_base;
     f.n_stiles = stile_base;
@@ -511,7 +556,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         ProfScope p(c, "setup");
         for (int k = 1; k < PS_N_PROGS; k++)
             if (progs[k])
-                psk_setup(s, k, dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, c->d_nrows.p, c->d_cnt, dv);
+                psk_setup(s, k, blob.shares[k], dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, c->d_nrows.p,
+                          c->d_cnt, dv);
     }
     {
         ProfScope p(c, "scan_rows");
