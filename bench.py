@@ -287,7 +287,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-loads", type=float, nargs="+", default=[1.5, 2.0, 3.0],
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[1.5, 2.0, 3.0, 4.0],
                     help="offered request rates as multiples of the plain path's views/s")
     ap.add_argument("--serve-batch-views", type=int, default=64)
     ap.add_argument("--serve-caps", type=int, nargs="*", default=[2, 4, 8], metavar="MPIX",

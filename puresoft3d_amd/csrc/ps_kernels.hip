@@ -644,17 +644,28 @@ __global__ void __launch_bounds__(256) k_rs_digits(const uint32_t* __restrict__ 
     const uint32_t* hrow = hist + (int64_t)d * nb_cap;
     uint32_t* orow = offs + (int64_t)d * nb_cap;
     uint32_t run = 0;
-    for (int b0 = 0; b0 < nb; b0 += 64) {
-        int b = b0 + lane;
-        uint32_t v = b < nb ? hrow[b] : 0u;
-        uint32_t x = v;
+    // the counts of up to 8 x 64 blocks are loaded before any is scanned, so
+    // the loads overlap instead of costing one round trip per 64 blocks
+    constexpr int PF = 8;
+    for (int b0 = 0; b0 < nb; b0 += 64 * PF) {
+        uint32_t v[PF];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
+        for (int k = 0; k < PF; k++) {
+            int b = b0 + k * 64 + lane;
+            v[k] = b < nb ? hrow[b] : 0u;
         }
-        if (b < nb) orow[b] = run + x - v;
-        run += __shfl(x, 63, 64);
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            int b = b0 + k * 64 + lane;
+            uint32_t x = v[k];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (b < nb) orow[b] = run + x - v[k];
+            run += __shfl(x, 63, 64);
+        }
     }
     if (lane == 0) totals[d] = run;
 }
