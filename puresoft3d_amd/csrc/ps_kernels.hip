@@ -95,15 +95,30 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v, const PsCounters*
 
 // a clear command: fill f = blockIdx.y; clear4 (fbo.cpp:332-346) is a fill of
 // rows 0 .. height-2, clear16 (fbo.cpp:348-371) one of the whole buffer.
-// Work items are pieces of 1024 values of one row (a 4 KiB stride keeps the
-// row's alignment); inside a piece the 16-byte aligned middle is stored four
-// values per thread.
+// Rows without padding form one run, stored four values per thread; otherwise
+// work items are pieces of 1024 values of one row (a 4 KiB stride keeps the
+// row's alignment), and inside a piece the 16-byte aligned middle is stored
+// four values per thread.
 #define FILL_PIECE 1024
 __global__ void __launch_bounds__(256) k_fills(const PsFill* __restrict__ fills, const PsCounters* c)
 {
     if (c && ps_poisoned(c)) return;
     const PsFill fl = fills[blockIdx.y];
     const uint4 v4 = make_uint4(fl.value, fl.value, fl.value, fl.value);
+    if (fl.scan == fl.width) {
+        // rows without padding are one run: four values per thread, grid-stride
+        const int64_t n = fl.rows * fl.width;
+        uint32_t* p = fl.ptr;
+        int64_t head = (int64_t)(((16u - ((uintptr_t)p & 15u)) & 15u) / 4);
+        if (head > n) head = n;
+        const int64_t nvec = (n - head) / 4, tail0 = head + nvec * 4;
+        if (blockIdx.x == 0 && (int64_t)threadIdx.x < head) p[threadIdx.x] = fl.value;
+        if (blockIdx.x == 0 && tail0 + (int64_t)threadIdx.x < n) p[tail0 + threadIdx.x] = fl.value;
+        uint4* q = (uint4*)(p + head);
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * blockDim.x)
+            q[i] = v4;
+        return;
+    }
     const int64_t npc = (fl.width + FILL_PIECE - 1) / FILL_PIECE;
     const int64_t total = fl.rows * npc;
     for (int64_t k = blockIdx.x; k < total; k += gridDim.x) {
