@@ -290,6 +290,15 @@ PS_DEV void ps_clamp_coord(int wrap, int max_row, int max_col, int& row, int& co
     }
 }
 
+// one texel of texture memory (device global memory): a global load, not a
+// flat one, addressed as base + 32-bit offset
+typedef __attribute__((address_space(1))) const uint8_t ps_gbyte;
+typedef __attribute__((address_space(1))) const uint32_t ps_gword;
+PS_DEV uint32_t ps_texel(const uint8_t* base, uint32_t off)
+{
+    return *(ps_gword*)((ps_gbyte*)base + off);
+}
+
 PS_DEV uint32_t ps_fetch(const PsTex& t, int layer, int level, int row, int col)
 {
     int w = ps_lvl_w(t, level), h = ps_lvl_h(t, level);
@@ -404,13 +413,13 @@ PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, floa
     ps_clamp_pair(t.wrap, h - 1, iy, r0, r1);
     ps_clamp_pair(t.wrap, w - 1, ix, c0, c1);
     int scan = level == 0 ? t.scanline : w * t.elem_len;
-    const uint8_t* base = t.data + t.level_off[level];
-    const uint8_t* row0 = base + (int64_t)r0 * scan;
-    const uint8_t* row1 = base + (int64_t)r1 * scan;
-    uint32_t a = *(const uint32_t*)(row0 + c0 * 4);
-    uint32_t b = *(const uint32_t*)(row0 + c1 * 4);
-    uint32_t c = *(const uint32_t*)(row1 + c0 * 4);
-    uint32_t d = *(const uint32_t*)(row1 + c1 * 4);
+    // a layer stays below 4 GiB (psgpu_texture_create): 32-bit texel offsets
+    const uint32_t lo = (uint32_t)t.level_off[level];
+    const uint32_t q0 = lo + (uint32_t)r0 * (uint32_t)scan, q1 = lo + (uint32_t)r1 * (uint32_t)scan;
+    uint32_t a = ps_texel(t.data, q0 + (uint32_t)c0 * 4u);
+    uint32_t b = ps_texel(t.data, q0 + (uint32_t)c1 * 4u);
+    uint32_t c = ps_texel(t.data, q1 + (uint32_t)c0 * 4u);
+    uint32_t d = ps_texel(t.data, q1 + (uint32_t)c1 * 4u);
 #pragma unroll
     for (int k = 0; k < NCH; k++) {
         float fa = (float)((a >> (8 * k)) & 255u), fb = (float)((b >> (8 * k)) & 255u);
@@ -502,16 +511,19 @@ PS_DEV void ps_sample_bilinear_pair(const PsTex& t, const uint8_t* data1, int le
     ps_clamp_pair(t.wrap, h - 1, iy, r0, r1);
     ps_clamp_pair(t.wrap, w - 1, ix, c0, c1);
     int scan = level == 0 ? t.scanline : w * t.elem_len;
-    int64_t o00 = t.level_off[level] + (int64_t)r0 * scan + c0 * 4, o01 = t.level_off[level] + (int64_t)r0 * scan + c1 * 4;
-    int64_t o10 = t.level_off[level] + (int64_t)r1 * scan + c0 * 4, o11 = t.level_off[level] + (int64_t)r1 * scan + c1 * 4;
+    // a layer stays below 4 GiB (psgpu_texture_create): 32-bit texel offsets
+    const uint32_t lo = (uint32_t)t.level_off[level];
+    const uint32_t q0 = lo + (uint32_t)r0 * (uint32_t)scan, q1 = lo + (uint32_t)r1 * (uint32_t)scan;
+    const uint32_t o00 = q0 + (uint32_t)c0 * 4u, o01 = q0 + (uint32_t)c1 * 4u;
+    const uint32_t o10 = q1 + (uint32_t)c0 * 4u, o11 = q1 + (uint32_t)c1 * 4u;
     const uint8_t* ds[2] = {t.data, data1};
     float* outs[2] = {out0, out1};
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-        uint32_t a = *(const uint32_t*)(ds[q] + o00);
-        uint32_t b = *(const uint32_t*)(ds[q] + o01);
-        uint32_t c = *(const uint32_t*)(ds[q] + o10);
-        uint32_t d = *(const uint32_t*)(ds[q] + o11);
+        uint32_t a = ps_texel(ds[q], o00);
+        uint32_t b = ps_texel(ds[q], o01);
+        uint32_t c = ps_texel(ds[q], o10);
+        uint32_t d = ps_texel(ds[q], o11);
 #pragma unroll
         for (int k = 0; k < NCH; k++) {
             float fa = (float)((a >> (8 * k)) & 255u), fb = (float)((b >> (8 * k)) & 255u);

@@ -991,6 +991,8 @@ int psgpu_texture_create(psgpu_ctx* c, int width, int scanline, int height, int 
         off += (int64_t)w * h * elem_len;
     }
     t.layer_bytes = (off + 255) / 256 * 256;
+    // the samplers address texels of a layer with 32-bit offsets
+    if (t.layer_bytes > (int64_t)0xffffffffu) return fail(c, PSGPU_ERR_INVALID, "a texture layer must stay below 4 GiB");
     int rc = do_sync(c);
     if (rc) return rc;
     HIPCHK(c, hipMalloc((void**)&t.d, (size_t)t.layer_bytes * t.layers));
