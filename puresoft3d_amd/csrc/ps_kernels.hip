@@ -62,6 +62,21 @@ __device__ __forceinline__ int ps_view_of_stile(const PsFrame& f, int64_t t)
     return lo;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  This maps workgroups to logical
+// indices so that each XCD gets runs of G consecutive indices: neighbouring
+// screen tiles then share texels and triangle records in one L2, while the
+// runs still alternate over the XCDs (contiguous eighths leave XCDs idle
+// where the screen is empty).  A bijection on [0, n) whatever the dispatch:
+// whole groups of 8G are permuted within, the tail keeps its order.
+template <int G>
+__device__ __forceinline__ int64_t ps_xcd_remap(int64_t b, int64_t n)
+{
+    const int64_t s = b / (8 * G), j = b - s * (8 * G);
+    if ((s + 1) * (8 * G) > n) return b;
+    return s * (8 * G) + (j & 7) * G + (j >> 3);
+}
+
 // where triangle tri of draw d finds its three vertices' varyings
 __device__ __forceinline__ int ps_vary_index(const PsDraw& d, int tri) { return tri + d.vary_delta; }
 
@@ -1264,8 +1279,9 @@ __global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans,
 {
     typedef PsProc<PROG> P;
     if (ps_poisoned(cnt)) return;
-    const PsView& V = f.views[ps_view_of_stile(f, blockIdx.x)];
-    int64_t st = blockIdx.x - V.stile_base;
+    const int64_t lb = ps_xcd_remap<4>(blockIdx.x, gridDim.x);
+    const PsView& V = f.views[ps_view_of_stile(f, lb)];
+    int64_t st = lb - V.stile_base;
     int sy = (int)(st / V.n_stiles_x), sx = (int)(st - (int64_t)sy * V.n_stiles_x);
     int x = sx * 16 + (threadIdx.x & 15);
     int y = sy * 16 + (threadIdx.x >> 4);
