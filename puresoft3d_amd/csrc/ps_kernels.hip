@@ -852,7 +852,7 @@ __global__ void __launch_bounds__(512) k_raster(const PsSpan* __restrict__ spans
     const int CW = f.chunk_w;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int waves = blockDim.x >> 6;
-    int64_t tile = blockIdx.x * (int64_t)waves + wave;
+    int64_t tile = ps_xcd_remap<4>(blockIdx.x, gridDim.x) * (int64_t)waves + wave;   // neighbouring tiles share an L2
     if (tile >= n_tiles) return;
     uint32_t* wl = lds + (size_t)wave * RA_WORDS(CW);
     float* dep = (float*)wl;
@@ -1050,7 +1050,7 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     const int CW = f.chunk_w;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int waves = blockDim.x >> 6;
-    int64_t tile = blockIdx.x * (int64_t)waves + wave;
+    int64_t tile = ps_xcd_remap<4>(blockIdx.x, gridDim.x) * (int64_t)waves + wave;   // neighbouring tiles share an L2
     if (tile >= n_tiles) return;
     uint32_t* wl = lds + (size_t)wave * RA_WORDS(CW);
     float* dep = (float*)wl;
