@@ -301,8 +301,8 @@ def main():
                     help="also run one plain copy kernel of MIB MiB (k_selftest op 7) to calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (views would be sharded "
-              "across GPUs with scenes replicated; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (views would be sharded "
+              "across GPUs with scenes replicated). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     # the adjoint runs in a child process, started before this one touches the GPU

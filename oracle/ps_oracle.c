@@ -151,7 +151,7 @@ static inline double opt_powd(double x, unsigned n)
     return pw;
 }
 
-/* 3-lane helpers for the two processors this build adds */
+/* 3-lane helpers for the two processors without a reference counterpart */
 static inline float dot3(const float* a, const float* b)
 {
     float p0 = a[0] * b[0], p1 = a[1] * b[1], p2 = a[2] * b[2];
@@ -295,7 +295,7 @@ void or_sample_cube(const or_texture* t, const float* dir, uint8_t* out4)
     fetch4(t, face, 0, row, col, out4);
 }
 
-/* Bilinear filter (added by this build; the reference has nearest only).
+/* Bilinear filter (an addition: the reference has nearest only).
  * Texel j of a level covers u = j/W (the nearest sampler's rounding rule),
  * addressing follows clampCoord. */
 void or_sample_bilinear(const or_texture* t, int layer, int level, float u, float v, float* out4)

@@ -4,7 +4,10 @@
 // a span (NV floats, lane-wise linear like every built-in IP), and an FP.
 // Processor sources: tex1light1.cpp (DEF01), colr1light1.cpp (DEF02),
 // tex1bump1light1.cpp (DEF03), skybox.cpp (DEF04), shadow.cpp (DEF05); LAMBERT
-// and BLINN are the two processors this build adds for the benchmark scenes.
+// and BLINN are two processors without a reference counterpart, added for the
+// benchmark scenes; SHADOWED is BLINN lit through the projective shadow
+// sampler (samplrproj.cpp); CLOUD and CLOUD_SHADOW are the demo's cloud pair
+// (test/testproc.cpp:384-686), shaded inside the raster walk.
 #pragma once
 #include "ps_device.h"
 
