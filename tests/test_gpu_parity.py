@@ -4,6 +4,8 @@ Bit-exact: colour (BGRA8), depth (FP32 bits), and per pixel the last
 depth-test-passing triangle, the number of rasterised fragments, the number of
 depth-test passes, and xor-hashes of the rasterised and passing triangle ids.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -218,10 +220,10 @@ _SWEEP_PROGS = [N.PROC_DEF01, N.PROC_DEF02, N.PROC_DEF03, N.PROC_DEF04, N.PROC_D
                 N.PROC_SHADOWED, N.PROC_CLOUD, N.PROC_CLOUD_SHADOW]
 
 
-@pytest.mark.parametrize("seed", range(30))
+@pytest.mark.parametrize("seed", range(int(os.environ.get("PS_SWEEP_SEEDS", "30"))))
 def test_random_scene_sweep(oracle, seed):
     """every programme over seeded random viewports (odd sizes included),
-    triangle counts and raster tilings"""
+    triangle counts and raster tilings (PS_SWEEP_SEEDS widens the sweep)"""
     rng = np.random.default_rng(1000 + seed)
     prog = _SWEEP_PROGS[seed % len(_SWEEP_PROGS)]
     w, h = int(rng.integers(17, 230)), int(rng.integers(9, 170))
