@@ -13,6 +13,7 @@ int psk_scan_blocks(int64_t cap);
 // exclusive scan; out[n] and *total_out (if given) receive the total
 void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
               uint32_t* partial, uint32_t* total_out, const PsCounters* c);
+// row item -> triangle table for k_span (large flushes; k_span searches row_off without it)
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
                 PsCounters* cnt);
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,

@@ -454,6 +454,9 @@ __global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // expand: triangle id of every drawn row
 // ---------------------------------------------------------------------------
+// row item -> triangle, written out: one wave per 64 triangles stores each
+// triangle's index once per row (used for flushes of many triangles, where
+// the search in k_span would cross long runs of triangles without rows)
 __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nrows, const uint32_t* __restrict__ row_off,
                                                 int64_t T, int* __restrict__ span_tri, int64_t cap, PsCounters* cnt)
 {
@@ -477,21 +480,72 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nro
     }
 }
 
+// Triangle of row item i: the t with row_off[t] <= i < row_off[t + 1]
+// (row_off is the exclusive scan of the row counts, row_off[T] = R).  The
+// wave finds the triangle of its lowest unresolved item with a 64-way search
+// over row_off (64 pivots per step, a few dependent loads), holds the next 64
+// row starts one per lane, and every lane counts those at or below its own
+// item with a 6-step search over the lanes.  Lanes whose item lies beyond
+// that window go round again from the lowest of them, so every round
+// resolves at least one lane and runs of triangles without rows are jumped.
+// All lanes of the wave call it; i0 is the wave's first item.
+__device__ __forceinline__ int64_t ps_row_tri(const uint32_t* __restrict__ row_off, int64_t T, int64_t R, int64_t i0)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = i0 + lane;
+    int64_t found = -1;
+    uint64_t pend = __ballot(i < R);
+    while (pend) {
+        const int64_t tgt = i0 + __builtin_ctzll(pend);
+        int64_t lo = 0, hi = T;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
+        while (hi - lo > 1) {
+            int64_t step = (hi - lo + 63) / 64;
+            int64_t p = lo + lane * step;
+            bool le = p < hi && (int64_t)row_off[p] <= tgt;
+            int c = __popcll(__ballot(le));
+            int64_t nhi = lo + c * step;
+            lo += (int64_t)(c - 1) * step;
+            hi = nhi < hi ? nhi : hi;
+        }
+        int64_t we = lo + 1 + lane;
+        int64_t e = we <= T ? (int64_t)row_off[we] : INT64_MAX;
+        int pos = 0;   // row starts of the window at or below i: 0..63, or 64 past it
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1) {
+            int64_t v = __shfl(e, pos + st - 1, 64);
+            if (v <= i) pos += st;
+        }
+        if (__shfl(e, 63, 64) <= i) pos = 64;
+        if (found < 0 && i < R && i >= tgt && pos < 64) found = lo + pos;
+        pend = __ballot(found < 0 && i < R);
+    }
+    return found;
+}
+
 // ---------------------------------------------------------------------------
 // span setup
 // ---------------------------------------------------------------------------
+template <bool SEARCH>
 __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
                                               const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
-                                              int64_t T, PsSpan* __restrict__ spans, uint32_t* __restrict__ seg_count,
+                                              int64_t T, int64_t cap,
+                                              PsSpan* __restrict__ spans, uint32_t* __restrict__ seg_count,
                                               const PsView* __restrict__ views, int chunk_w, PsCounters* cnt)
 {
     if (ps_poisoned(cnt)) return;
     int64_t R = row_off[T];
+    if (R > cap) {   // more rows than span records: flag it, write nothing
+        if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&cnt->overflow, 1);
+        return;
+    }
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = base + threadIdx.x;
         unsigned long long nspan = 0, nfrag = 0;
+        // the item's triangle: from k_expand's table, or searched in row_off
+        int64_t tw = -1;
+        if constexpr (SEARCH) tw = ps_row_tri(row_off, T, R, base + (threadIdx.x & ~63));
         if (i < R) {
-            int t = span_tri[i];
+            int t = SEARCH ? (int)tw : span_tri[i];
             const PsTri tr = tris[t];
             const PsDraw& d = draws[tr.code >> 8];
             const int W = views[d.view].W;
@@ -1449,7 +1503,12 @@ void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* 
               int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_span, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, spans, seg_count, views, chunk_w, cnt);
+    if (span_tri)
+        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans,
+                           seg_count, views, chunk_w, cnt);
+    else
+        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans,
+                           seg_count, views, chunk_w, cnt);
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,

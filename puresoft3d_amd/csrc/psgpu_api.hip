@@ -28,6 +28,7 @@ namespace {
 const int kMaxSlots = 16;      // config.h:4
 const int kMaxUniforms = 1024; // config.h:5
 const int64_t kMaxFlushTris = (int64_t)8 << 20;  // triangles queued before a flush is forced
+const int64_t kSpanSearchTris = (int64_t)1 << 20; // larger flushes expand rows to triangles first
 
 struct Tex {
     bool alive = false;
@@ -521,7 +522,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     if (c->span_cap > ((int64_t)1 << 26) || c->seg_cap > ((int64_t)1 << 31))
         return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans or 2^31 pieces in one flush");
     int64_t scan_cap = std::max<int64_t>(T, c->span_cap);
-    if ((rc = ensure(c, c->d_partial, psk_scan_blocks(scan_cap) + 1)) || (rc = ensure(c, c->d_spantri, c->span_cap)) ||
+    if ((rc = ensure(c, c->d_partial, psk_scan_blocks(scan_cap) + 1)) ||
+        (T > kSpanSearchTris && (rc = ensure(c, c->d_spantri, c->span_cap))) ||
         (rc = ensure(c, c->d_spans, c->span_cap)) || (rc = ensure(c, c->d_segcount, c->span_cap)) ||
         (rc = ensure(c, c->d_segoff, c->span_cap + 1)) || (rc = ensure(c, c->d_keys, c->seg_cap)) ||
         (rc = ensure(c, c->d_vals, c->seg_cap)) || (rc = ensure(c, c->d_keys2, c->seg_cap)) ||
@@ -565,14 +567,18 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         // largest row total of the log is known even after an overflow
         psk_scan(s, c->d_nrows.p, nullptr, T, T, c->d_rowoff.p, c->d_partial.p, c->d_sizes + 0, nullptr);
     }
-    {
+    // up to kSpanSearchTris triangles k_span finds each row's triangle by a
+    // search in the row offsets; beyond, where views leave long runs of
+    // triangles without rows, k_expand writes the table out first
+    const bool expand = T > kSpanSearchTris;
+    if (expand) {
         ProfScope p(c, "expand");
         psk_expand(s, c->d_nrows.p, c->d_rowoff.p, T, c->d_spantri.p, c->span_cap, c->d_cnt);
     }
     {
         ProfScope p(c, "span");
-        psk_span(s, c->d_tris.p, dd, c->d_spantri.p, c->d_rowoff.p, T, c->d_spans.p, c->d_segcount.p, dv, f.chunk_w,
-                 c->span_cap, c->d_cnt);
+        psk_span(s, c->d_tris.p, dd, expand ? c->d_spantri.p : nullptr, c->d_rowoff.p, T, c->d_spans.p,
+                 c->d_segcount.p, dv, f.chunk_w, c->span_cap, c->d_cnt);
     }
     {
         ProfScope p(c, "scan_segs");

@@ -236,6 +236,17 @@ def test_random_scene_sweep(oracle, seed):
     assert_same(sc, oracle, tiling=(cw, tr, wv))
 
 
+def test_large_flush_expands_rows(oracle):
+    """a flush of more than 2^20 triangles takes the row table written by
+    k_expand instead of k_span's search (1.1 M triangles of a dense sphere at
+    320x240: most are sub-pixel and have no rows)"""
+    sc = S.sphere_scene(width=320, height=240, tex_size=64)
+    sc.meshes["sphere"] = S.uv_sphere(600, 920, 1.0, seed=1)
+    sc.name = "uv_sphere_1.1M_320x240"
+    assert sc.triangles() > (1 << 20)
+    assert_same(sc, oracle)
+
+
 def test_terrain_reduced_bitexact(oracle):
     sc = S.terrain_scene(width=480, height=270, cells=60, n_spheres=8, tex_size=256)
     assert_same(sc, oracle)
