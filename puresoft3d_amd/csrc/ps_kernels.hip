@@ -185,7 +185,7 @@ __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draw
 }
 
 template <int PROG, bool SHARE>
-__global__ void __launch_bounds__(256) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
+__global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
                                                float* __restrict__ vary, int nvs, float* __restrict__ grads,
                                                uint32_t* __restrict__ nrows, PsCounters* cnt,
                                                const PsView* __restrict__ views)
@@ -1466,8 +1466,8 @@ void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, co
 void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
-    dim3 g(gsz(T, 256)), b(256);
-    size_t lds = (size_t)256 * 3 * nvs * sizeof(float);
+    dim3 g(gsz(T, 128)), b(128);
+    size_t lds = (size_t)128 * 3 * nvs * sizeof(float);
 #define SU_CASE(K)                                                                                                  \
     case K:                                                                                                         \
         if (share && ps_shares_varyings(K))                                                                         \
