@@ -896,7 +896,7 @@ __global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ key
 #define RA_WORDS(cw) ((cw) * 64 * 2 + RA_CHUNK * 7 + 128)
 
 template <bool DEBUG>
-__global__ void __launch_bounds__(512) k_raster(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
+__global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
                          const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
                          const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles,
                          PsCounters* cnt)
