@@ -1326,7 +1326,7 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
 // shade: deferred FP of the last colour-writing pass per pixel
 // ---------------------------------------------------------------------------
 template <int PROG>
-__global__ void __launch_bounds__(256) k_shade(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
+__global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
                                                const float* __restrict__ ck, const PsTri* __restrict__ tris,
                                                const float* __restrict__ vary, int nvs, const float* __restrict__ grads,
                                                const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt)
