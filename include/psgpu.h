@@ -87,6 +87,11 @@ typedef struct {
 /* flags: PSGPU_CREATE_TINY_SCRATCH starts the intermediate buffers at a few
  * hundred entries, so the first flushes overflow and are replayed (tests). */
 #define PSGPU_CREATE_TINY_SCRATCH 0x1
+/* Display and viewport sizes are limited to PSGPU_MAX_VIEW_WIDTH x
+ * PSGPU_MAX_VIEW_HEIGHT (intermediate records keep a column in 16 bits and a
+ * row in 24); larger ones give PSGPU_ERR_RANGE. */
+#define PSGPU_MAX_VIEW_WIDTH  65535
+#define PSGPU_MAX_VIEW_HEIGHT 16777215
 int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** out);
 int psgpu_destroy(psgpu_ctx* ctx);
 const char* psgpu_last_error(psgpu_ctx* ctx);
@@ -167,11 +172,14 @@ int psgpu_swap_buffers(psgpu_ctx* ctx);           /* swapBuffers: flush; display
  * order: queued draws are flushed, then fn is called with the device
  * addresses of the current colour target and depth buffer and the context
  * stream, and enqueues its own work (kernels, copies) on that stream; later
- * draws and reads come after that work.  The call is logged like the draws:
- * when a flush before it is re-run after its scratch buffers grew, fn runs
- * again after it, so it must only enqueue stream work.  Inside a recorded
- * command list (psgpu_capture_begin/end) its work becomes part of the graph.
- * fn returns 0 or a negative status, which the call reports. */
+ * draws and reads come after that work.  fn sees finished frames only: it is
+ * called once the work before it has run, and when a flush before it
+ * overflowed its scratch buffers the call waits for the log replay that
+ * re-runs that flush, so fn runs exactly once per call and may change the
+ * frame in place.  fn must only enqueue stream work.  Inside a recorded
+ * command list (psgpu_capture_begin/end) its work becomes part of the graph
+ * and runs once per replay.  fn returns 0 or a negative status, which the
+ * call reports. */
 typedef struct {
     void* colour;          /* BGRA8 rows of colour_pitch bytes; display rows are top-down */
     int colour_pitch, colour_top_down;
@@ -251,7 +259,10 @@ int psgpu_profile_reset(psgpu_ctx* ctx);
  * replays it `count` times, stream-ordered; the snapshot of programme,
  * uniforms, behaviour and targets taken at record time is reused.  Resource
  * updates (textures, VBOs) are refused while recording.  Graphs are captured
- * again automatically when scratch buffers have been reallocated.  No extern
+ * again automatically when scratch buffers have been reallocated or a VBO
+ * they draw was updated (new geometry may need more scratch): the first
+ * launch after that runs the commands once eagerly, which counts as one of
+ * the `count` replays, and captures them again.  No extern
  * counterpart in the reference: it replaces the per-frame drawVAO calls of a
  * static scene by one launch. */
 int psgpu_capture_begin(psgpu_ctx* ctx);

@@ -29,6 +29,8 @@ const int kMaxSlots = 16;      // config.h:4
 const int kMaxUniforms = 1024; // config.h:5
 const int64_t kMaxFlushTris = (int64_t)8 << 20;  // triangles queued before a flush is forced
 const int64_t kSpanSearchTris = (int64_t)1 << 20; // larger flushes expand rows to triangles first
+// piece records hold a column in 16 bits and span records a row in 24 bits
+const int kMaxViewW = PSGPU_MAX_VIEW_WIDTH, kMaxViewH = PSGPU_MAX_VIEW_HEIGHT;
 
 struct Tex {
     bool alive = false;
@@ -111,12 +113,32 @@ struct Graph {
     std::vector<uint8_t*> blobs;
     hipGraphExec_t exec = nullptr;
     uint64_t gen = 0;
+    int64_t span_cap = 0, seg_cap = 0;   // the capacities its kernels were captured with
+    bool stale = false;                  // a vertex stream it draws was updated
     int64_t flushes = 0;
 };
 
 template <typename T> struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;
+};
+
+// The per-flush scratch of run_flush: everything it reads or writes besides
+// the command records, the render targets and the counters.  span_cap and
+// seg_cap are the span and piece capacities the kernels are told.
+struct FlushBufs {
+    DevBuf<PsTri> tris;
+    DevBuf<float> vary;
+    DevBuf<float> grad;   // per triangle: screen gradients for the mip level (8 floats)
+    DevBuf<uint32_t> nrows, rowoff, partial;
+    DevBuf<int> spantri;
+    DevBuf<PsSpan> spans;
+    DevBuf<uint32_t> segcount, segoff;
+    DevBuf<uint32_t> keys, vals, keys2, vals2, hist, histoff, kstart;
+    DevBuf<int> winner;
+    DevBuf<PsSeg> segs;
+    DevBuf<float> ck;
+    int64_t span_cap = 0, seg_cap = 0;
 };
 
 struct ProfRec { const char* name; hipEvent_t a, b; };
@@ -170,18 +192,8 @@ struct psgpu_ctx {
     psgpu_tiling tiling = {4, 8, 1};      // 8 x 32-pixel tiles: short per-tile lists for dense and sparse scenes alike
     // scratch
     DevBuf<uint8_t> d_blob;
-    DevBuf<PsTri> d_tris;
-    DevBuf<float> d_vary;
-    DevBuf<uint32_t> d_nrows, d_rowoff, d_partial;
-    DevBuf<int> d_spantri;
-    DevBuf<PsSpan> d_spans;
-    DevBuf<uint32_t> d_segcount, d_segoff;
-    DevBuf<uint32_t> d_keys, d_vals, d_keys2, d_vals2, d_hist, d_histoff, d_kstart;
-    DevBuf<int> d_winner;
-    DevBuf<PsSeg> d_segs;
-    DevBuf<float> d_ck;
-    DevBuf<float> d_grad;   // per triangle: screen gradients for the mip level (8 floats)
-    int64_t span_cap = 0, seg_cap = 0;
+    FlushBufs fb;
+    bool fixed_scratch = false;   // fb is carved from the static array: never reallocate it
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
     // per logged command: the overflow flag as it stood after the command ran
@@ -232,6 +244,9 @@ int fail(psgpu_ctx* c, int code, const char* fmt, ...)
 template <typename T> int ensure(psgpu_ctx* c, DevBuf<T>& b, size_t n)
 {
     if (n <= b.cap && b.p) return PSGPU_OK;
+    if (c->fixed_scratch)
+        return fail(c, PSGPU_ERR_NOMEM, "the flush needs %zu bytes in a buffer of %zu in the static scratch array",
+                    n * sizeof(T), b.cap * sizeof(T));
     size_t cap = std::max(n, (size_t)(b.cap * 3 / 2));
     if (cap < 64) cap = 64;
     if (c->in_stream_capture) return fail(c, PSGPU_ERR_STATE, "scratch buffer too small during graph capture");
@@ -452,9 +467,9 @@ int build_blob(psgpu_ctx* c, const Cmd& cmd, Blob& b, PsFrame& f, int64_t& T, in
     f.n_views = (int)views.size();
     f.n_tiles = tile_base;
     f.n_stiles = stile_base;
-    int rc = ensure(c, c->d_winner, (size_t)std::max<int64_t>(n_pix, 1));
+    int rc = ensure(c, c->fb.winner, (size_t)std::max<int64_t>(n_pix, 1));
     if (rc) return rc;
-    for (PsView& pv : views) pv.winner = c->d_winner.p + (intptr_t)pv.winner;
+    for (PsView& pv : views) pv.winner = c->fb.winner.p + (intptr_t)pv.winner;
     b.nd = (int)draws.size();
     b.nv = (int)views.size();
     b.nf = (int)cmd.fills.size();
@@ -496,6 +511,36 @@ int stage_blob(psgpu_ctx* c, const Blob& b, uint8_t** out)
     return PSGPU_OK;
 }
 
+// size the per-flush scratch for a flush of T triangles, nvs varying floats
+// per vertex and n_tiles raster tiles at the current span and piece capacities
+int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
+{
+    FlushBufs& b = c->fb;
+    int rc;
+    if ((rc = ensure(c, b.tris, T)) || (rc = ensure(c, b.vary, (size_t)T * 3 * nvs)) ||
+        (rc = ensure(c, b.grad, (size_t)T * 8)) || (rc = ensure(c, b.nrows, T)) || (rc = ensure(c, b.rowoff, T + 1)))
+        return rc;
+    if (b.span_cap == 0) {
+        b.span_cap = std::max<int64_t>(1 << 20, 6 * T);
+        b.seg_cap = b.span_cap * 3 / 2;
+    }
+    if (b.span_cap > ((int64_t)1 << 26) || b.seg_cap > ((int64_t)1 << 31))
+        return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans or 2^31 pieces in one flush");
+    int64_t scan_cap = std::max<int64_t>(T, b.span_cap);
+    if ((rc = ensure(c, b.partial, psk_scan_blocks(scan_cap) + 1)) ||
+        (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) || (rc = ensure(c, b.spans, b.span_cap)) ||
+        (rc = ensure(c, b.segcount, b.span_cap)) || (rc = ensure(c, b.segoff, b.span_cap + 1)) ||
+        (rc = ensure(c, b.keys, b.seg_cap)) || (rc = ensure(c, b.vals, b.seg_cap)) ||
+        (rc = ensure(c, b.keys2, b.seg_cap)) || (rc = ensure(c, b.vals2, b.seg_cap)) ||
+        (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)))
+        return rc;
+    int nbr = psk_rs_blocks(b.seg_cap);
+    if ((rc = ensure(c, b.hist, (size_t)nbr * 2048)) || (rc = ensure(c, b.histoff, (size_t)nbr * 2048 + 2048)) ||
+        (rc = ensure(c, b.kstart, std::max<int64_t>(n_tiles + 1, 2048))))
+        return rc;
+    return PSGPU_OK;
+}
+
 // run one flush: the whole pipeline over the draws of all its views
 int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
 {
@@ -511,30 +556,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     if (!dev && (rc = stage_blob(c, blob, &dev))) return rc;
     PsDraw* dd = (PsDraw*)dev;
     f.views = (const PsView*)(dev + blob.off_views);
-    if ((rc = ensure(c, c->d_tris, T)) || (rc = ensure(c, c->d_vary, (size_t)T * 3 * nvs)) ||
-        (rc = ensure(c, c->d_grad, (size_t)T * 8)) ||
-        (rc = ensure(c, c->d_nrows, T)) || (rc = ensure(c, c->d_rowoff, T + 1)))
-        return rc;
-    if (c->span_cap == 0) {
-        c->span_cap = std::max<int64_t>(1 << 20, 6 * T);
-        c->seg_cap = c->span_cap * 3 / 2;
-    }
-    if (c->span_cap > ((int64_t)1 << 26) || c->seg_cap > ((int64_t)1 << 31))
-        return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans or 2^31 pieces in one flush");
-    int64_t scan_cap = std::max<int64_t>(T, c->span_cap);
-    if ((rc = ensure(c, c->d_partial, psk_scan_blocks(scan_cap) + 1)) ||
-        (T > kSpanSearchTris && (rc = ensure(c, c->d_spantri, c->span_cap))) ||
-        (rc = ensure(c, c->d_spans, c->span_cap)) || (rc = ensure(c, c->d_segcount, c->span_cap)) ||
-        (rc = ensure(c, c->d_segoff, c->span_cap + 1)) || (rc = ensure(c, c->d_keys, c->seg_cap)) ||
-        (rc = ensure(c, c->d_vals, c->seg_cap)) || (rc = ensure(c, c->d_keys2, c->seg_cap)) ||
-        (rc = ensure(c, c->d_vals2, c->seg_cap)) || (rc = ensure(c, c->d_segs, c->seg_cap)) ||
-        (rc = ensure(c, c->d_ck, (size_t)c->seg_cap * nvs)))
-        return rc;
+    if ((rc = size_flush_scratch(c, T, nvs, f.n_tiles))) return rc;
     f.ck_stride = nvs;
-    int nbr = psk_rs_blocks(c->seg_cap);
-    if ((rc = ensure(c, c->d_hist, (size_t)nbr * 2048)) || (rc = ensure(c, c->d_histoff, (size_t)nbr * 2048 + 2048)) ||
-        (rc = ensure(c, c->d_kstart, std::max<int64_t>(f.n_tiles + 1, 2048))))
-        return rc;
     // debug planes record the flushes of the display pass: one view of the
     // debug viewport into the default depth buffer (a shadow-map pass into a
     // depth texture is not recorded)
@@ -558,14 +581,14 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         ProfScope p(c, "setup");
         for (int k = 1; k < PS_N_PROGS; k++)
             if (progs[k])
-                psk_setup(s, k, blob.shares[k], dd, nd, T, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, c->d_nrows.p,
+                psk_setup(s, k, blob.shares[k], dd, nd, T, c->fb.tris.p, c->fb.vary.p, nvs, c->fb.grad.p, c->fb.nrows.p,
                           c->d_cnt, dv);
     }
     {
         ProfScope p(c, "scan_rows");
         // row counts never depend on buffer capacities: always scanned, so the
         // largest row total of the log is known even after an overflow
-        psk_scan(s, c->d_nrows.p, nullptr, T, T, c->d_rowoff.p, c->d_partial.p, c->d_sizes + 0, nullptr);
+        psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
     }
     // up to kSpanSearchTris triangles k_span finds each row's triangle by a
     // search in the row offsets; beyond, where views leave long runs of
@@ -573,33 +596,33 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     const bool expand = T > kSpanSearchTris;
     if (expand) {
         ProfScope p(c, "expand");
-        psk_expand(s, c->d_nrows.p, c->d_rowoff.p, T, c->d_spantri.p, c->span_cap, c->d_cnt);
+        psk_expand(s, c->fb.nrows.p, c->fb.rowoff.p, T, c->fb.spantri.p, c->fb.span_cap, c->d_cnt);
     }
     {
         ProfScope p(c, "span");
-        psk_span(s, c->d_tris.p, dd, expand ? c->d_spantri.p : nullptr, c->d_rowoff.p, T, c->d_spans.p,
-                 c->d_segcount.p, dv, f.chunk_w, c->span_cap, c->d_cnt);
+        psk_span(s, c->fb.tris.p, dd, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p,
+                 c->fb.segcount.p, dv, f.chunk_w, c->fb.span_cap, c->d_cnt);
     }
     {
         ProfScope p(c, "scan_segs");
-        psk_scan(s, c->d_segcount.p, c->d_sizes + 0, 0, c->span_cap, c->d_segoff.p, c->d_partial.p, c->d_sizes + 1, c->d_cnt);
+        psk_scan(s, c->fb.segcount.p, c->d_sizes + 0, 0, c->fb.span_cap, c->fb.segoff.p, c->fb.partial.p, c->d_sizes + 1, c->d_cnt);
     }
     {
         ProfScope p(c, "segments");
-        psk_segments(s, c->d_spans.p, c->d_segoff.p, c->d_rowoff.p, T, f, c->d_tris.p, dd, c->d_vary.p, nvs,
-                     c->d_keys.p, c->d_vals.p, c->d_segs.p, c->d_ck.p, c->span_cap, c->seg_cap, c->d_cnt);
+        psk_segments(s, c->fb.spans.p, c->fb.segoff.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs,
+                     c->fb.keys.p, c->fb.vals.p, c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_cnt);
     }
     bool second = false;
     {
         ProfScope p(c, "radix_sort");
-        psk_radix_sort(s, c->d_keys.p, c->d_vals.p, c->d_keys2.p, c->d_vals2.p, c->d_sizes + 1, c->seg_cap,
-                       key_bits(f.n_tiles), c->d_hist.p, c->d_histoff.p, c->d_partial.p, c->d_cnt, second, c->d_kstart.p);
+        psk_radix_sort(s, c->fb.keys.p, c->fb.vals.p, c->fb.keys2.p, c->fb.vals2.p, c->d_sizes + 1, c->fb.seg_cap,
+                       key_bits(f.n_tiles), c->fb.hist.p, c->fb.histoff.p, c->fb.partial.p, c->d_cnt, second, c->fb.kstart.p);
     }
-    uint32_t* skeys = second ? c->d_keys2.p : c->d_keys.p;
-    uint32_t* svals = second ? c->d_vals2.p : c->d_vals.p;
+    uint32_t* skeys = second ? c->fb.keys2.p : c->fb.keys.p;
+    uint32_t* svals = second ? c->fb.vals2.p : c->fb.vals.p;
     if (psk_radix_passes(key_bits(f.n_tiles)) > 1) {   // one pass leaves the tile starts already
         ProfScope p(c, "bounds");
-        psk_bounds(s, skeys, c->d_sizes + 1, f.n_tiles, c->d_kstart.p, c->d_cnt);
+        psk_bounds(s, skeys, c->d_sizes + 1, f.n_tiles, c->fb.kstart.p, c->d_cnt);
     }
     int imm = 0;   // a flush of one programme that shades in the raster walk
     {
@@ -608,22 +631,22 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     if (imm) {
         ProfScope p(c, "raster");
-        psk_raster_imm(s, imm, dbg, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
-                       c->tiling.waves_per_group, c->d_cnt, c->d_vary.p, nvs, c->d_ck.p);
+        psk_raster_imm(s, imm, dbg, c->fb.spans.p, c->fb.segs.p, svals, c->fb.kstart.p, c->fb.tris.p, dd, f,
+                       c->tiling.waves_per_group, c->d_cnt, c->fb.vary.p, nvs, c->fb.ck.p);
         HIPCHK(c, hipGetLastError());
         c->stats.flushes++;
         return PSGPU_OK;
     }
     {
         ProfScope p(c, "raster");
-        psk_raster(s, dbg, c->d_spans.p, c->d_segs.p, svals, c->d_kstart.p, c->d_tris.p, dd, f,
+        psk_raster(s, dbg, c->fb.spans.p, c->fb.segs.p, svals, c->fb.kstart.p, c->fb.tris.p, dd, f,
                    c->tiling.waves_per_group, c->d_cnt);
     }
     {
         ProfScope p(c, "shade");
         for (int k = 1; k < PS_N_PROGS; k++)
             if (shade_progs[k])
-                psk_shade(s, k, c->d_spans.p, c->d_segs.p, c->d_ck.p, c->d_tris.p, c->d_vary.p, nvs, c->d_grad.p, dd, f,
+                psk_shade(s, k, c->fb.spans.p, c->fb.segs.p, c->fb.ck.p, c->fb.tris.p, c->fb.vary.p, nvs, c->fb.grad.p, dd, f,
                           c->d_cnt);
     }
     HIPCHK(c, hipGetLastError());
@@ -670,6 +693,19 @@ int run_cmd(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged = nullptr)
     case CMD_GRAPH: return graph_launch_n(c, cmd.graph, cmd.count);
     case CMD_COPY: return run_copy(c, cmd.copy);
     case CMD_POST: {
+        // The post-processor sees a finished frame only: after an overflow
+        // earlier in the log the frame lacks that flush's draws, and the log
+        // replay that follows runs this command again.  So it waits for the
+        // work before it and is skipped while the overflow flag is set; every
+        // logged post-process then runs its callback exactly once.  (Inside a
+        // graph capture its work is recorded instead; a prepared graph fits
+        // its scratch buffers.)
+        if (!c->in_stream_capture) {
+            int flag = 0;
+            HIPCHK(c, hipMemcpyAsync(&flag, &c->d_cnt->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (flag) return PSGPU_OK;
+        }
         int rc = cmd.post(cmd.post_user, &cmd.post_frame, (void*)c->stream);
         if (rc) return fail(c, rc < 0 ? rc : PSGPU_ERR_DEVICE, "post-processor returned %d", rc);
         HIPCHK(c, hipGetLastError());
@@ -757,6 +793,7 @@ int do_sync(psgpu_ctx* c)
     if (c->capturing) return fail(c, PSGPU_ERR_STATE, "not allowed while recording a command list");
     int rc = queue_to_log(c);
     if (rc) return rc;
+    const int64_t span_cap0 = c->fb.span_cap, seg_cap0 = c->fb.seg_cap;
     for (int attempt = 0; attempt < 16; attempt++) {
         HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(PsCounters), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->h_sizes, c->d_sizes, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -786,14 +823,14 @@ int do_sync(psgpu_ctx* c)
         }
         // grow and replay the log
         int64_t need_spans = (int64_t)c->h_sizes[2], need_segs = (int64_t)c->h_sizes[3];
-        if (need_spans > c->span_cap) {
-            c->span_cap = need_spans + need_spans / 4 + 1024;
-            c->seg_cap = std::max<int64_t>(c->seg_cap, c->span_cap * 3 / 2);
-        } else if (need_segs > c->seg_cap) {
-            c->seg_cap = need_segs + need_segs / 4 + 1024;
+        if (need_spans > c->fb.span_cap) {
+            c->fb.span_cap = need_spans + need_spans / 4 + 1024;
+            c->fb.seg_cap = std::max<int64_t>(c->fb.seg_cap, c->fb.span_cap * 3 / 2);
+        } else if (need_segs > c->fb.seg_cap) {
+            c->fb.seg_cap = need_segs + need_segs / 4 + 1024;
         } else {
-            c->span_cap *= 2;
-            c->seg_cap *= 2;
+            c->fb.span_cap *= 2;
+            c->fb.seg_cap *= 2;
         }
         c->stats.retries++;
         // commands that finished before the overflow are complete; replay the rest
@@ -807,6 +844,9 @@ int do_sync(psgpu_ctx* c)
             if ((rc = mark_done(c, (int)i))) return rc;
         }
     }
+    // give up with the capacities as they were, so later flushes stay usable
+    c->fb.span_cap = span_cap0;
+    c->fb.seg_cap = seg_cap0;
     return fail(c, PSGPU_ERR_NOMEM, "intermediate buffers kept overflowing");
 }
 
@@ -826,30 +866,41 @@ int graph_capture(psgpu_ctx* c, Graph& g)
     hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(c, PSGPU_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
     g.gen = c->buf_gen;
+    g.span_cap = c->fb.span_cap;
+    g.seg_cap = c->fb.seg_cap;
+    g.stale = false;
     return PSGPU_OK;
 }
 
-// upload the device records of every command of a graph for the current
-// scratch buffers, then capture it
+// size the scratch buffers for every flush of a graph at the current
+// capacities, then upload its commands' device records (their winner planes
+// point into the scratch) and capture it
 int graph_rebuild(psgpu_ctx* c, Graph& g)
 {
-    for (size_t i = 0; i < g.cmds.size(); i++) {
-        if (g.cmds[i].type != CMD_FLUSH || !g.blobs[i]) continue;
-        Blob b;
-        PsFrame f;
-        int64_t T, npx;
-        int nvs;
-        bool p1[PS_N_PROGS] = {false}, p2[PS_N_PROGS] = {false};
-        int rc = build_blob(c, g.cmds[i], b, f, T, nvs, p1, p2, npx);
-        if (rc) return rc;
-        HIPCHK(c, hipMemcpyAsync(g.blobs[i], b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int pass = 0; pass < 2; pass++) {   // sizing may move the winner plane: records come after it
+        for (size_t i = 0; i < g.cmds.size(); i++) {
+            if (g.cmds[i].type != CMD_FLUSH || !g.blobs[i]) continue;
+            Blob b;
+            PsFrame f;
+            int64_t T, npx;
+            int nvs;
+            bool p1[PS_N_PROGS] = {false}, p2[PS_N_PROGS] = {false};
+            int rc = build_blob(c, g.cmds[i], b, f, T, nvs, p1, p2, npx);
+            if (rc) return rc;
+            if (pass == 0) {
+                if (T > 0 && (rc = size_flush_scratch(c, T, nvs, f.n_tiles))) return rc;
+                continue;
+            }
+            HIPCHK(c, hipMemcpyAsync(g.blobs[i], b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
     }
     return graph_capture(c, g);
 }
 
-// a graph whose capture is stale is first run once eagerly: that sizes the
-// scratch buffers for the current tiling (overflow replays included)
+// A graph whose capture is stale is first run once eagerly: that sizes the
+// scratch buffers for the current tiling and geometry (overflow replays
+// included).  The eager run counts as the first of the launches asked for.
 int graph_prepare(psgpu_ctx* c, Graph& g)
 {
     int rc = do_sync(c);
@@ -864,11 +915,30 @@ int graph_prepare(psgpu_ctx* c, Graph& g)
     return graph_rebuild(c, g);
 }
 
+// whether a draw of the graph reads the vertex stream at ptr
+bool graph_reads(const Graph& g, const uint8_t* ptr)
+{
+    for (const Cmd& cmd : g.cmds)
+        for (const Segment& sg : cmd.views)
+            for (const PsDraw& d : sg.draws)
+                for (int s = 0; s < 5; s++)
+                    if (d.slot[s] == ptr) return true;
+    return false;
+}
+
+// a graph is current when it was captured against the scratch buffers and
+// capacities in use and no vertex stream it reads changed since
+bool graph_current(psgpu_ctx* c, const Graph& g)
+{
+    return g.exec && !g.stale && g.gen == c->buf_gen && g.span_cap == c->fb.span_cap && g.seg_cap == c->fb.seg_cap;
+}
+
 int graph_launch_n(psgpu_ctx* c, int id, int count)
 {
     Graph& g = c->graphs[id];
-    if (!g.exec || g.gen != c->buf_gen) {
-        // buffers only grew since this launch was logged: records and capture suffice
+    if (!graph_current(c, g)) {
+        // buffers or capacities only grew since this launch was logged (a
+        // replay of the log after an overflow): records and capture suffice
         int rc = graph_rebuild(c, g);
         if (rc) return rc;
     }
@@ -891,6 +961,7 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
     if (!out) return PSGPU_ERR_INVALID;
     *out = nullptr;
     if (width < 1 || height < 1) return PSGPU_ERR_INVALID;
+    if (width > kMaxViewW || height > kMaxViewH) return PSGPU_ERR_RANGE;
     // default depth scanline (pipeline.cpp:31); it is shorter than the row when
     // width % 4 == 1, which makes the reference alias the row ends
     int dscan = (int)(width / 4.0f + 0.5f) * 4;
@@ -904,8 +975,8 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
     c->dev_h = c->vp_h = height;
     c->depth_scan = dscan;
     if (flags & PSGPU_CREATE_TINY_SCRATCH) {   // start tiny: exercises growth and log replay
-        c->span_cap = 256;
-        c->seg_cap = 256;
+        c->fb.span_cap = 256;
+        c->fb.seg_cap = 256;
     }
     c->uniforms.resize(kMaxUniforms);
     memset(&c->stats, 0, sizeof(c->stats));
@@ -945,10 +1016,10 @@ int psgpu_destroy(psgpu_ctx* c)
     if (c->h_flags) hipHostFree(c->h_flags);
     for (auto& t : c->tex) fr(t.d);
     for (auto& v : c->vbo) fr(v.d);
-    fr(c->d_blob.p); fr(c->d_tris.p); fr(c->d_vary.p); fr(c->d_nrows.p); fr(c->d_rowoff.p); fr(c->d_partial.p);
-    fr(c->d_spantri.p); fr(c->d_spans.p); fr(c->d_segcount.p); fr(c->d_segoff.p); fr(c->d_keys.p); fr(c->d_vals.p);
-    fr(c->d_keys2.p); fr(c->d_vals2.p); fr(c->d_hist.p); fr(c->d_histoff.p); fr(c->d_kstart.p); fr(c->d_winner.p);
-    fr(c->d_segs.p); fr(c->d_ck.p); fr(c->d_grad.p);
+    fr(c->d_blob.p); fr(c->fb.tris.p); fr(c->fb.vary.p); fr(c->fb.nrows.p); fr(c->fb.rowoff.p); fr(c->fb.partial.p);
+    fr(c->fb.spantri.p); fr(c->fb.spans.p); fr(c->fb.segcount.p); fr(c->fb.segoff.p); fr(c->fb.keys.p); fr(c->fb.vals.p);
+    fr(c->fb.keys2.p); fr(c->fb.vals2.p); fr(c->fb.hist.p); fr(c->fb.histoff.p); fr(c->fb.kstart.p); fr(c->fb.winner.p);
+    fr(c->fb.segs.p); fr(c->fb.ck.p); fr(c->fb.grad.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
     for (int i = 0; i < psgpu_ctx::kStage; i++) {
         if (c->h_stage[i]) hipHostFree(c->h_stage[i]);
@@ -1126,6 +1197,10 @@ int psgpu_vbo_update(psgpu_ctx* c, int vbo, const void* src)
     Vbo& v = c->vbo[vbo];
     HIPCHK(c, hipMemcpyAsync(v.d, src, v.unit * v.count, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // new geometry can need more spans and pieces than a graph that draws it
+    // was captured with: such graphs are run eagerly and captured again
+    for (Graph& g : c->graphs)
+        if (g.alive && graph_reads(g, v.d)) g.stale = true;
     return PSGPU_OK;
 }
 
@@ -1260,6 +1335,8 @@ int psgpu_viewport(psgpu_ctx* c, int width, int height)
 {
     if (need_ctx(c)) return PSGPU_ERR_INVALID;
     if (width < 1 || height < 1) return fail(c, PSGPU_ERR_INVALID, "bad viewport");
+    if (width > kMaxViewW || height > kMaxViewH)
+        return fail(c, PSGPU_ERR_RANGE, "viewport %d x %d beyond %d x %d", width, height, kMaxViewW, kMaxViewH);
     c->vp_w = width;
     c->vp_h = height;
     return PSGPU_OK;
@@ -1898,11 +1975,14 @@ int psgpu_graph_launch(psgpu_ctx* c, int graph, int count)
     int rc = queue_to_log(c);
     if (rc) return rc;
     Graph& g = c->graphs[graph];
-    if (!g.exec || g.gen != c->buf_gen) {
+    if (!graph_current(c, g)) {
         if ((rc = graph_prepare(c, g))) return rc;
+        if (--count == 0) return PSGPU_OK;   // the eager run was the first launch
     }
-    // a prepared graph fits the scratch buffers, which only grow: a replay
-    // cannot overflow, so a launch of `count` replays is one log entry
+    // a prepared graph fits the scratch buffers, which only grow, and its
+    // vertex streams are those it was sized for (psgpu_vbo_update marks the
+    // graphs that read a stream stale): a replay cannot overflow, so a launch
+    // of `count` replays is one log entry
     Cmd cmd;
     cmd.type = CMD_GRAPH;
     cmd.graph = graph;

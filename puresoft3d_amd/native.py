@@ -461,8 +461,8 @@ class Context:
     def post_process(self, fn):
         """PuresoftPipeline::postProcess: fn(frame: PostFrame, stream: int) -> int
         runs in stream order after the draws queued so far and enqueues its own
-        device work on `stream` (it may run again when an earlier flush is
-        re-run).  The callback object is kept alive with the context."""
+        device work on `stream`; it sees the finished frame and runs once per
+        call.  The callback object is kept alive with the context."""
         cb = POST_FN(lambda user, frame, stream: int(fn(frame.contents, stream or 0)))
         self._post_cbs = getattr(self, "_post_cbs", []) + [cb]
         self._chk(self.L.psgpu_post_process(self.h, cb, None))
