@@ -90,14 +90,36 @@ def measured_valu_share(kernel, path=os.path.join(ROOT, "profiles", "valu_terrai
     return None
 
 
-def cpu_baseline(scene, seconds, threads):
+def host_cpus():
+    """logical CPUs of the host, those this process may run on, and the model"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return os.cpu_count() or 1, usable, model
+
+
+def cpu_baseline(scene, seconds, threads, warmup=3, max_frames=30):
+    """the oracle (1 producer + threads-1 row-interleaved spinning workers, the
+    reference's architecture) on `warmup` untimed and up to `max_frames` timed
+    frames, bounded by `seconds`"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     frames, t_frames = 0, []
     frag = 0
-    pyoracle.render_scene(scene, n_threads=threads)   # warm-up (page-in, thread pool)
+    for _ in range(warmup):   # page-in, thread pool
+        pyoracle.render_scene(scene, n_threads=threads)
     t_start = time.perf_counter()
-    while frames < 30 and time.perf_counter() - t_start < seconds:
+    while frames < max_frames and (frames < 3 or time.perf_counter() - t_start < seconds):
         t0 = time.perf_counter()
         _, st = pyoracle.render_scene(scene, n_threads=threads)
         t_frames.append(time.perf_counter() - t0)
@@ -106,9 +128,10 @@ def cpu_baseline(scene, seconds, threads):
     pyoracle.pool_shutdown()
     tot = sum(t_frames)
     return {"value": frames / tot, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "%d frames of the same 1080p workload (clear + 65 draws each), oracle with 1 producer + %d "
-                      "row-interleaved worker threads" % (frames, threads - 1),
-            "ms_per_frame_p50": float(np.median(t_frames) * 1e3),
+            "sample": "%d timed frames (after %d untimed) of the same 1080p workload (clear + 65 draws each), oracle "
+                      "with 1 producer + %d row-interleaved worker threads" % (frames, warmup, threads - 1),
+            "ms_per_frame_p50": round(float(np.median(t_frames) * 1e3), 2),
+            "ms_per_frame_p99": round(float(np.percentile(t_frames, 99) * 1e3), 2),
             "fragments_shaded_per_s": frag / tot}
 
 
@@ -362,6 +385,17 @@ def main():
     fps = args.steps / dt
     print("[bench] timed %d frames in %.3f s" % (args.steps, dt), flush=True)
 
+    # end to end: the same frames, each read back to host memory (8.3 MB of
+    # BGRA8 over PCIe) before the next starts
+    host_img = np.empty((scene.height, scene.width, 4), np.uint8)
+    n_e2e = max(10, min(args.steps // 4, 50))
+    t0 = time.perf_counter()
+    for _ in range(n_e2e):
+        frame()
+        ctx.read_colour(out=host_img)
+    e2e_fps = n_e2e / (time.perf_counter() - t0)
+    print("[bench] end to end (readback included): %.1f frames/s" % e2e_fps, flush=True)
+
     # per-kernel durations with HIP events on the context stream
     ctx.profile(True)
     ctx.profile_reset()
@@ -404,6 +438,8 @@ def main():
         "data": "synthetic (seeded procedural terrain, spheres and noise textures)",
         "config": {"workload": scene.name, "width": scene.width, "height": scene.height, "triangles": T,
                    "shading": "Blinn-Phong + normal map, trilinear 2048^2 mipmapped", "mode": args.mode},
+        "e2e_frames_per_s": round(e2e_fps, 1),
+        "e2e_note": "frames each followed by the 8.3 MB colour readback to host memory (PCIe-inclusive); not value",
         "fragments_shaded_per_s": round(st["frags_shaded"] / dt, 1),
         "fragments_rasterised_per_s": round(st["frags_rasterised"] / dt, 1),
         "frame_alg_bytes": B,
@@ -426,9 +462,18 @@ def main():
     if adj is not None:
         out["adjoint"] = adj
     if not args.no_cpu:
-        thr = args.cpu_threads or min(os.cpu_count() or 2, 16)
+        nproc, usable, model = host_cpus()
+        # every CPU this process may use, up to the 16 threads one GPU's share
+        # of the host holds; the reference's minimum (2: a producer and one
+        # worker) as a second point
+        thr = args.cpu_threads or max(2, min(usable, 16))
         print("[bench] cpu baseline: oracle with %d threads for <= %.0f s" % (thr, args.cpu_seconds), flush=True)
         cb = cpu_baseline(scene, args.cpu_seconds, thr)
+        cb.update(nproc=nproc, usable_cpus=usable, cpu_model=model)
+        if thr != 2:
+            print("[bench] cpu baseline: oracle with 2 threads for <= %.0f s" % (args.cpu_seconds / 2), flush=True)
+            c2 = cpu_baseline(scene, args.cpu_seconds / 2, 2, warmup=1, max_frames=10)
+            cb["n2"] = {k: c2[k] for k in ("value", "cores", "sample", "ms_per_frame_p50", "fragments_shaded_per_s")}
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = round(fps / cb["value"], 1)
     print(json.dumps(out), flush=True)

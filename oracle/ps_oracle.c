@@ -23,8 +23,14 @@
  *    form and the NaN/inf special cases.
  *  - samplers (samplr2d.cpp, samplrcube.cpp, fbo.cpp:552-594): pinned by the
  *    closed-form index formulas, exhaustive over small textures and the face table.
- *  - shading (tex1light1.cpp etc. and the two new processors): pinned by an
- *    FP64 variant of the same formulas (shade_fp64=1) within 1 LSB.
+ *  - shading (tex1light1.cpp etc. and the processors added here): pinned by
+ *    closed forms on flat quads (light and camera at set angles to the
+ *    normal, distinct B/G/R texels, turned TBN frames, turned models under a
+ *    real perspective, skybox faces), and by an FP64 variant of the same
+ *    formulas (shade_fp64=1) within 1 LSB.  shade_fp64=2 keeps FP32 but takes
+ *    rcpps / rsqrtss where the reference's DEF01-DEF04 call div_3_4 and
+ *    norm_3_4, to measure the colour against the reference's own arithmetic;
+ *    3 and 4 put those reciprocals at the two ends of their error bound.
  *  - bilinear / trilinear / mip chain / log2 approximation (not in the
  *    reference): pinned by textbook identities (bilinear of affine data is
  *    exact, mip of a constant is constant, log2 of powers of two is exact,
@@ -101,6 +107,42 @@ static inline void norm4(float* a)
     float r = 1.0f / sqrtf(s);
     a[0] *= r; a[1] *= r; a[2] *= r; a[3] *= r;
 }
+
+/* div_3_4 and norm_3_4 with the reference's own approximations, as the x86
+ * cores this runs on compute them: rcpps of the broadcast divisor
+ * (vector.cpp:158-169) and rsqrtss of the haddps-summed squares
+ * (vector.cpp:225-250).  Their bits are vendor-specific (|rel err| <=
+ * 1.5 * 2^-12 by the SDM), so they serve to measure how far the exact
+ * replacement above moves the colour, not as the parity reference. */
+static inline void div4_x86(float* a, float f)
+{
+    float r = _mm_cvtss_f32(_mm_rcp_ps(_mm_set1_ps(f)));
+    a[0] *= r; a[1] *= r; a[2] *= r; a[3] *= r;
+}
+static inline void norm4_x86(float* a)
+{
+    float r = _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(dot4(a, a))));
+    a[0] *= r; a[1] *= r; a[2] *= r; a[3] *= r;
+}
+
+/* the same with the SDM's error bound taken at one end: the exact
+ * reciprocal (square root) scaled by 1 + g_approx_rel, for modes 3 (+) and
+ * 4 (-) of shade_fp64 */
+static const double kApproxRel = 1.5 / 4096.0;
+static inline void div4_bound(float* a, float f, double rel)
+{
+    float r = (float)((1.0 / (double)f) * (1.0 + rel));
+    a[0] *= r; a[1] *= r; a[2] *= r; a[3] *= r;
+}
+static inline void norm4_bound(float* a, double rel)
+{
+    float r = (float)((1.0 / sqrt((double)dot4(a, a))) * (1.0 + rel));
+    a[0] *= r; a[1] *= r; a[2] *= r; a[3] *= r;
+}
+static inline void div4_hi(float* a, float f) { div4_bound(a, f, kApproxRel); }
+static inline void div4_lo(float* a, float f) { div4_bound(a, f, -kApproxRel); }
+static inline void norm4_hi(float* a) { norm4_bound(a, kApproxRel); }
+static inline void norm4_lo(float* a) { norm4_bound(a, -kApproxRel); }
 
 /* mcemaths_len_3_4, vector.cpp:196-223 */
 static inline float len4(const float* a) { return sqrtf(dot4(a, a)); }
@@ -827,7 +869,12 @@ static void run_vp(const or_draw* d, const or_prog* p, int64_t i, float* pos, fl
         float v[4];
         for (int k = 0; k < 4; k++) v[k] = src[k] - observer[k];
         v[3] = 0;
-        norm4(v);
+        switch (d->shade_fp64) {
+        case 2: norm4_x86(v); break;
+        case 3: norm4_hi(v); break;
+        case 4: norm4_lo(v); break;
+        default: norm4(v);
+        }
         m4v4(user, p->invView, v);
         memcpy(pos, src, 16);
         break;
@@ -943,9 +990,14 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
                 bump[0] = bb[2]; bump[1] = bb[1]; bump[2] = bb[0]; bump[3] = 0;
             }
         }
-        if (!fp64) {
+        if (fp64 != 1) {
+            /* modes 2-4: the reference's rcpps / rsqrtss in div_3_4 and
+               norm_3_4, as this CPU computes them or at either end of their
+               error bound */
+            void (*divf)(float*, float) = fp64 == 2 ? div4_x86 : fp64 == 3 ? div4_hi : fp64 == 4 ? div4_lo : div4;
+            void (*normf)(float*) = fp64 == 2 ? norm4_x86 : fp64 == 3 ? norm4_hi : fp64 == 4 ? norm4_lo : norm4;
             if (p->prog == OR_PROG_DEF03) {
-                div4(bump, 255.0f);
+                divf(bump, 255.0f);
                 for (int k = 0; k < 4; k++) bump[k] *= 2.0f;
                 for (int k = 0; k < 4; k++) bump[k] -= 1.0f;
                 float tbn[16];
@@ -953,17 +1005,17 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
                 tbn[12] = tbn[13] = tbn[14] = tbn[15] = 0.0f;
                 float t[4]; memcpy(t, bump, 16);
                 m4v4(bump, tbn, t);
-                norm4(bump);
+                normf(bump);
                 nrm = bump;
             }
             float L[4], E[4], H[4];
             for (int k = 0; k < 4; k++) L[k] = p->lightPos[k] - wp[k];
             float dist = len4(L);
-            div4(L, dist);
+            divf(L, dist);
             for (int k = 0; k < 4; k++) E[k] = p->cameraPos[k] - wp[k];
-            norm4(E);
+            normf(E);
             for (int k = 0; k < 4; k++) H[k] = E[k] + L[k];
-            norm4(H);
+            normf(H);
             float lambert = dot4(L, nrm);
             float spec = dot4(H, nrm);
             spec = spec < 0 ? 0 : spec;
@@ -1015,7 +1067,7 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
     case OR_PROG_DEF05:     /* shadow.cpp:76-78: writes nothing */
         return 0;
     case OR_PROG_LAMBERT: {
-        if (!fp64) {
+        if (fp64 != 1) {
             float t[4];
             or_sample_bilinear(p->tex0, 0, 0, in[3], in[4], t);
             float n[3] = {in[0], in[1], in[2]};
@@ -1044,7 +1096,7 @@ static int run_fp(const or_prog* p, const float* in, float r, const float* grad,
         float u = in[12], v = in[13];
         /* the shadow factor is a discrete lookup: both variants take it in FP32 */
         float sf = p->prog == OR_PROG_SHADOWED ? or_sample_projection(p->tex2, in + 14) : 1.0f;
-        if (!fp64) {
+        if (fp64 != 1) {
             float lod = lod_from_grad(grad, u, v, r, p->tex0->width, p->tex0->height);
             float alb[4], nm[4];
             sample_trilinear(p->tex0, u, v, lod, alb);

@@ -91,7 +91,9 @@ typedef struct {
     const or_texture* textures;                 /* texture pool          */
     int n_textures;
     int tri_base;                               /* id of this draw's first triangle */
-    int shade_fp64;                             /* 1: FP64 shading variant */
+    int shade_fp64;                             /* 1: FP64 shading variant; 2: FP32 with x86 rcpps/rsqrtss
+                                                   in DEF01-DEF04's div_3_4 and norm_3_4; 3, 4: those
+                                                   reciprocals at +/- 1.5 * 2^-12 relative error */
 } or_draw;
 
 typedef struct {

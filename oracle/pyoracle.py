@@ -293,9 +293,16 @@ def mesh_triangles(mesh):
     return min(a.shape[0] for a in mesh.values()) // 3
 
 
-def render_scene(scene, n_threads=1, debug=False, fp64=False, frame=None, clear=True):
+APPROX_MODES = {"x86": 2, "bound_hi": 3, "bound_lo": 4}
+
+
+def render_scene(scene, n_threads=1, debug=False, fp64=False, frame=None, clear=True, approx=None):
     """Run a puresoft3d_amd.scenes.Scene through the oracle: clearDepth +
-    clearColour + the draws in order.  Returns (Frame, Stats)."""
+    clearColour + the draws in order.  Returns (Frame, Stats).  fp64: shade
+    in FP64.  approx: FP32 with the reference's approximate reciprocals where
+    its DEF01-DEF04 call div_3_4 and norm_3_4: "x86" as this CPU's rcpps /
+    rsqrtss compute them, "bound_hi" / "bound_lo" at either end of their
+    1.5 * 2^-12 relative error bound."""
     f = frame if frame is not None else Frame(scene.width, scene.height, debug=debug)
     st = Stats()
     texs = [OracleTexture(t) for t in scene.textures]
@@ -320,7 +327,8 @@ def render_scene(scene, n_threads=1, debug=False, fp64=False, frame=None, clear=
         u = dict(scene.uniforms)
         u.update(d.uniforms)
         beh = DEFAULT_BEHAVIOR if d.behavior is None else d.behavior
-        dc = DrawCall(d.prog, scene.meshes[d.mesh], u, texs, behavior=beh, tri_base=base, shade_fp64=1 if fp64 else 0)
+        mode = APPROX_MODES[approx] if approx else (1 if fp64 else 0)
+        dc = DrawCall(d.prog, scene.meshes[d.mesh], u, texs, behavior=beh, tri_base=base, shade_fp64=mode)
         dc.run(f, n_threads, st)
         base += mesh_triangles(scene.meshes[d.mesh])
     return f, st

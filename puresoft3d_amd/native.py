@@ -351,8 +351,12 @@ class Context:
     def sync(self):
         self._chk(self.L.psgpu_sync(self.h))
 
-    def read_colour(self):
-        out = np.zeros((self.height, self.width, 4), np.uint8)
+    def read_colour(self, out=None):
+        """the display (top-down BGRA8 rows), into `out` when given"""
+        if out is None:
+            out = np.zeros((self.height, self.width, 4), np.uint8)
+        elif out.nbytes != self.width * self.height * 4 or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous %d x %d x 4 byte array" % (self.height, self.width))
         self._chk(self.L.psgpu_read_colour(self.h, _ptr(out), out.nbytes))
         return out
 

@@ -461,8 +461,9 @@ class GpuScene:
         ctx.depth_target(N.DEFAULT_DEPTH)
         ctx.viewport(self.scene.width, self.scene.height)
 
-    def frame(self, clear=True):
-        """clearDepth + clearColour + the scene's drawVAO calls (queued)"""
+    def frame(self, clear=True, flush=True):
+        """clearDepth + clearColour + the scene's drawVAO calls (queued, then
+        flushed unless flush=False)"""
         ctx, sc = self.ctx, self.scene
         if sc.shadow is not None:
             self.shadow_pass()
@@ -490,7 +491,8 @@ class GpuScene:
         if beh != N.DEFAULT_BEHAVIOR:
             ctx.enable(N.DEFAULT_BEHAVIOR)
             ctx.disable(~N.DEFAULT_BEHAVIOR & (N.UPDATE_DEPTH | N.TEST_DEPTH | N.FACE_CULLING | N.ALPHABLEND))
-        ctx.flush()
+        if flush:
+            ctx.flush()
 
 
 # ---------------------------------------------------------------- batches of views

@@ -285,3 +285,88 @@ def cloud_shadow_quad(red, w=64, h=48):
     sc.draws.append(S.DrawSpec("q", N.PROC_CLOUD_SHADOW, behavior=N.UPDATE_DEPTH | N.TEST_DEPTH))
     sc.clear_colour = 0x00112233
     return sc
+
+
+def reference_shader_scene(prog, width=320, height=240, seed=3):
+    """a lit sphere through one of the reference's built-in programmes:
+    DEF01 (textured), DEF02 (per-vertex colour), DEF03 (textured + bump map)
+    with a point light and a camera near it, or DEF04 (a skybox quad filling
+    the view, seen through a turned view matrix)"""
+    rng = np.random.default_rng(seed)
+    sc = S.Scene(width, height)
+    sc.name = "refshader_p%d_%dx%d" % (prog, width, height)
+    if prog == N.PROC_DEF04:
+        a = 1.0
+        sc.meshes["sky"] = {0: np.array([[-a, -a, 0.99, 1], [a, -a, 0.99, 1], [a, a, 0.99, 1],
+                                         [-a, -a, 0.99, 1], [a, a, 0.99, 1], [-a, a, 0.99, 1]], np.float32)}
+        sc.textures.append(S.TextureSpec([S.noise_texture(128, seed + L, cells=8) for L in range(6)], wrap=N.CLAMP))
+        V = S.look_at((0, 0, 0), (0.6, 0.3, -1.0))
+        sc.uniforms[0] = S.colmajor(S.perspective(np.radians(60.0), width / height, 0.1, 10.0))
+        sc.uniforms[1] = S.colmajor(V)
+        sc.uniforms[2] = np.array([0], np.int32)
+        sc.draws.append(S.DrawSpec("sky", prog))
+        return sc
+    m = S.uv_sphere(48, 48, 1.0, seed=seed, tangents=True)
+    if prog == N.PROC_DEF01:
+        sc.meshes["s"] = {0: m[0], 3: m[3], 4: m[4]}
+    elif prog == N.PROC_DEF02:
+        col = rng.uniform(0, 255, (m[0].shape[0], 4)).astype(np.float32)
+        sc.meshes["s"] = {0: m[0], 1: m[3], 2: col}
+    else:
+        sc.meshes["s"] = m
+    P = S.perspective(np.radians(60.0), width / height, 0.1, 10.0)
+    V = S.look_at((0.4, 0.3, 2.6), (0, 0, 0))
+    R = np.eye(4)
+    ang = 0.7
+    R[0, 0], R[0, 2], R[2, 0], R[2, 2] = np.cos(ang), np.sin(ang), -np.sin(ang), np.cos(ang)
+    sc.uniforms[3] = S.colmajor(P @ V)
+    sc.uniforms[4] = S.colmajor(S.translate(0.1, -0.05, 0.0) @ R)
+    sc.uniforms[5] = S.colmajor(R)
+    sc.uniforms[7] = S.vec4(1.5, 2.0, 2.5, 0.0)
+    sc.uniforms[8] = S.vec4(0.4, 0.3, 2.6, 0.0)
+    sc.textures.append(S.TextureSpec([S.noise_texture(256, seed)], wrap=N.WRAP))
+    sc.textures.append(S.TextureSpec([S.normal_map(256, seed + 1)], wrap=N.WRAP))
+    sc.uniforms[9] = np.array([0], np.int32)
+    sc.uniforms[10] = np.array([1], np.int32)
+    sc.draws.append(S.DrawSpec("s", prog))
+    sc.clear_colour = 0x00102030
+    return sc
+
+
+def hazard_triangles(rng, n):
+    """triangles the reference never clips (drawvao.cpp:39-63): one vertex
+    with w at or below zero (0, -0.5, +-1e-30; the divide gives infinities,
+    NaN or a mirrored vertex), or a coordinate that is NaN or infinite"""
+    P = random_clip_triangles(rng, n, kind=0).reshape(n, 3, 4)
+    for t in range(n):
+        v = int(rng.integers(0, 3))
+        if t % 2 == 0:
+            w = rng.choice([0.0, -0.0, -0.5, 1e-30, -1e-30])
+            P[t, v, 3] = w
+            if rng.integers(0, 3) == 0:
+                P[t, v, 2] = 0.0          # with w = 0: 0 * inf is NaN, which passes the z test
+        else:
+            P[t, v, int(rng.integers(0, 4))] = rng.choice([np.nan, np.inf, -np.inf])
+    return P.reshape(-1, 4)
+
+
+def hazard_scene(prog, seed, extra_vertices=0, width=96, height=64, n=90):
+    """a micro scene whose triangles include the hazards above, ordinary ones
+    around them, and a vertex stream of 3k + extra_vertices entries: the
+    trailing partial triangle is dropped (vertthrd.cpp:25-29)"""
+    sc = micro_scene(prog, seed, width, height, n=n)
+    rng = np.random.default_rng(seed + 1000)
+    m = sc.meshes["m"]
+    pos = np.concatenate([m[0], hazard_triangles(rng, n // 2)], 0)
+    rest = _slots_for(prog, pos, rng)
+    for k in rest:
+        if k != 0:
+            rest[k][: m[k].shape[0]] = m[k]
+    rest[0] = pos
+    if extra_vertices:
+        for k in rest:
+            tail = rest[k][:extra_vertices].copy()
+            rest[k] = np.concatenate([rest[k], tail], 0)
+    sc.meshes["m"] = rest
+    sc.name = "hazard_p%d_s%d_x%d" % (prog, seed, extra_vertices)
+    return sc

@@ -146,6 +146,18 @@ def test_micro_scenes(oracle, prog, seed):
     assert_same(sc, oracle)
 
 
+@pytest.mark.parametrize("prog", [N.PROC_DEF01, N.PROC_DEF02, N.PROC_DEF03, N.PROC_DEF04, N.PROC_DEF05,
+                                  N.PROC_LAMBERT, N.PROC_BLINN, N.PROC_SHADOWED, N.PROC_CLOUD,
+                                  N.PROC_CLOUD_SHADOW])
+@pytest.mark.parametrize("extra", [0, 1, 2])
+def test_hazard_scenes(oracle, prog, extra):
+    """vertices with w at or below zero and non-finite coordinates, which the
+    reference never clips (drawvao.cpp:39-63), among ordinary triangles, and
+    vertex streams of 3k + 1 and 3k + 2 entries (vertthrd.cpp:25-29)"""
+    sc = scenegen.hazard_scene(prog, 40 + prog, extra)
+    assert_same(sc, oracle)
+
+
 def test_sse_pack_selftest(oracle):
     """FP_Cloud's cvtps2dq/packusdw/packuswb tail on the device against the
     oracle's (which runs cvtps2dq itself): ties, the word and byte
@@ -229,7 +241,11 @@ def test_random_scene_sweep(oracle, seed):
     w, h = int(rng.integers(17, 230)), int(rng.integers(9, 170))
     if w % 4 == 1:   # the reference's default depth scanline would be shorter than the row
         w += 1
-    sc = scenegen.micro_scene(prog, 100 + seed, width=w, height=h, n=int(rng.integers(20, 400)))
+    n = int(rng.integers(20, 400))
+    if seed % 3 == 2:    # a third of the sweep carries the unclipped-vertex hazards and a partial triangle
+        sc = scenegen.hazard_scene(prog, 100 + seed, extra_vertices=seed % 5 % 3, width=w, height=h, n=n)
+    else:
+        sc = scenegen.micro_scene(prog, 100 + seed, width=w, height=h, n=n)
     cw, tr, wv = int(2 ** rng.integers(0, 7)), int(2 ** rng.integers(0, 7)), int(2 ** rng.integers(0, 4))
     while wv > 1 and wv * (cw * 128 + 128 * 7 + 128) * 4 > 65536:   # the raster kernel's LDS per group
         wv //= 2

@@ -283,10 +283,15 @@ def test_threads_do_not_change_results(oracle):
     assert sa.as_dict() == sb.as_dict()
 
 
-@pytest.mark.parametrize("which", ["sphere", "terrain", "shadowed"])
+@pytest.mark.parametrize("which", ["sphere", "terrain", "shadowed", "def01", "def02", "def03", "objx_plane"])
 def test_fp32_shading_within_one_lsb_of_fp64(oracle, which):
     if which == "sphere":
         sc = S.sphere_scene(width=320, height=240)
+    elif which in ("def01", "def02", "def03"):
+        sc = scenegen.reference_shader_scene({"def01": N.PROC_DEF01, "def02": N.PROC_DEF02,
+                                              "def03": N.PROC_DEF03}[which])
+    elif which == "objx_plane":   # the reference's own asset: DEF03 and DEF02 meshes
+        sc = scenegen.objx_scene("plane", 320, 240)
     elif which == "shadowed":   # the shadow factor is a discrete lookup, shared by both variants
         sc = S.shadow_terrain_scene(width=320, height=180, cells=40, n_spheres=4, tex_size=128, shadow_size=128)
     else:
@@ -541,3 +546,62 @@ def test_cloud_shadow_discard_and_shrink(oracle, red):
     ref = oracle.Frame(W, H)
     ref.clear_colour(sc.clear_colour)
     np.testing.assert_array_equal(f.colour, ref.colour)
+
+
+@pytest.mark.parametrize("extra", [1, 2])
+def test_trailing_partial_triangle_is_dropped(oracle, extra):
+    """processVertices fetches whole triangles only (vertthrd.cpp:25-29): a
+    stream of 3k + 1 or 3k + 2 vertices draws the same k triangles as one of
+    3k, with the same fragments"""
+    sc = scenegen.hazard_scene(N.PROC_DEF02, 5, extra_vertices=extra)
+    a, sa = oracle.render_scene(sc, debug=True)
+    n = sc.meshes["m"][0].shape[0]
+    assert n % 3 == extra
+    for k in sc.meshes["m"]:
+        sc.meshes["m"][k] = sc.meshes["m"][k][: n - extra]
+    b, sb = oracle.render_scene(sc, debug=True)
+    assert sa.tris_in == sb.tris_in == (n - extra) // 3
+    np.testing.assert_array_equal(a.colour, b.colour)
+    np.testing.assert_array_equal(a.pass_hash, b.pass_hash)
+
+
+@pytest.mark.parametrize("case", ["w_zero_z_nonzero", "w_zero_z_zero", "w_tiny", "nan_x", "inf_y", "w_negative"])
+def test_unclipped_vertices(oracle, case):
+    """The reference divides by w without clipping (vertthrd.cpp:42-47,
+    drawvao.cpp:39-63).  w = 0 makes z infinite, which the z test rejects,
+    unless z is 0 too: 0 * inf is NaN, NaN passes the z test, and the screen
+    row of a NaN vertex is undefined there, so the triangle is dropped here.
+    A tiny w sends the vertex far beyond the int range of rows: dropped.  A NaN
+    x is drawn from its finite rows (every NaN-edge span is skipped or
+    clamped); an infinite y is dropped.  A negative w mirrors the vertex
+    through the centre, as the divide says, with a negative 1/w weight in the
+    perspective correction."""
+    tri = np.array([[-0.5, -0.5, 0.2, 1.0], [0.5, -0.5, 0.2, 1.0], [0.0, 0.5, 0.2, 1.0]], np.float32)
+    t = tri.copy()
+    if case == "w_zero_z_nonzero":
+        t[2, 3] = 0.0
+    elif case == "w_zero_z_zero":
+        t[2, 2], t[2, 3] = 0.0, 0.0
+    elif case == "w_tiny":
+        t[2, 2], t[2, 3] = 0.0, 1e-30
+    elif case == "nan_x":
+        t[2, 0] = np.nan
+    elif case == "inf_y":
+        t[2, 1] = np.inf
+    else:
+        t[2] = [0.0, -0.5, -0.2, -1.0]     # mirrored: (0, 0.5, 0.2) after the divide
+    f, st = oracle.render_scene(_single_tri_scene(t), debug=True)
+    assert st.tris_in == 1
+    if case == "w_zero_z_nonzero":
+        assert st.tris_zrejected == 1 and st.frags_rasterised == 0
+    elif case in ("w_zero_z_zero", "w_tiny", "inf_y"):
+        assert st.tris_dropped == 1 and st.frags_rasterised == 0
+    elif case == "w_negative":
+        # the same pixels as the unmirrored triangle; its 1/w weight keeps the
+        # sign, so depth along the spans is not that triangle's
+        g, sg = oracle.render_scene(_single_tri_scene(tri), debug=True)
+        np.testing.assert_array_equal(f.raster_count, g.raster_count)
+        assert not np.array_equal(f.depth, g.depth)
+    else:
+        assert st.tris_culled == 0 and st.tris_zrejected == 0
+        assert (f.depth_image() < 1.0).sum() == st.frags_shaded
