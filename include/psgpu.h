@@ -270,6 +270,29 @@ int psgpu_capture_end(psgpu_ctx* ctx, int* out_graph);
 int psgpu_graph_launch(psgpu_ctx* ctx, int graph, int count);
 int psgpu_graph_destroy(psgpu_ctx* ctx, int graph);
 
+/* ---- synchronising without allocating ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they render (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): psgpu_sync_noalloc runs the
+ * queued clears and draws, as psgpu_sync would, with every per-flush buffer
+ * (triangle records, varyings, spans, pieces, sort keys, the winner plane,
+ * the draw records themselves) carved from a fixed array of
+ * PSGPU_STATIC_SCRATCH_BYTES in the device's memory, and waits for them.  It
+ * calls no hipMalloc, hipFree or other allocating routine.  Each GPU the
+ * library runs on holds one such array from the library's load until the
+ * process ends; all contexts on a GPU share it, and calls are serialised by a
+ * lock inside the library (the array keeps the last call's contents; nothing
+ * in it needs to start at zero).  The queued work must be one flush: if
+ * anything was submitted since the last sync (psgpu_flush, a flush psgpu_draw
+ * forces, a post-process, a graph launch) the call returns PSGPU_ERR_STATE.
+ * Work too large for the array returns PSGPU_ERR_NOMEM: before anything runs
+ * (triangle, view and pixel counts) with the queue left intact for
+ * psgpu_sync, or after the run when its spans or pieces overflowed (the
+ * clears were applied, the draws were not).  The 1080p benchmark frame
+ * (200,480 triangles) needs about 120 MB. */
+#define PSGPU_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int psgpu_sync_noalloc(psgpu_ctx* ctx);
+
 /* ---- arithmetic self-check: out[i] for op in {0:a/b, 1:sqrtf(a), 2:1/a, 3:(int)x86(a), 4:floorf(a),
  *      5:(int)(unsigned)x86(a), 6:log2 approximation of the trilinear LOD, 7:a (a plain copy kernel,
  *      the calibration point for HBM traffic counters), 8 and 9: the wrapped texel coordinates of v and

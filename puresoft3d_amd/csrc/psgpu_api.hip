@@ -17,11 +17,18 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/psgpu.h"
 #include "ps_kernels.h"
+
+// Per-flush scratch of psgpu_sync_noalloc: a fixed array in every GPU's
+// memory, held from the library's load until the process ends.  Host code
+// never uses this name as an address: hipGetSymbolAddress gives each
+// device's copy (static_scratch_init), and that address is what kernels get.
+static __device__ __align__(256) uint8_t scratch[PSGPU_STATIC_SCRATCH_BYTES];
 
 namespace {
 
@@ -274,11 +281,11 @@ struct ProfScope {
     psgpu_ctx* c; const char* name; hipEvent_t a = nullptr;
     ProfScope(psgpu_ctx* c_, const char* n) : c(c_), name(n)
     {
-        if (c->profiling && !c->in_stream_capture) { a = ev_get(c); hipEventRecord(a, c->stream); }
+        if (c->profiling && !c->in_stream_capture && !c->fixed_scratch) { a = ev_get(c); hipEventRecord(a, c->stream); }
     }
     ~ProfScope()
     {
-        if (c->profiling && !c->in_stream_capture) {
+        if (a) {
             hipEvent_t b = ev_get(c);
             hipEventRecord(b, c->stream);
             c->prof_pending.push_back({name, a, b});
@@ -949,6 +956,106 @@ int graph_launch_n(psgpu_ctx* c, int id, int count)
 
 int need_ctx(psgpu_ctx* c) { return c ? PSGPU_OK : PSGPU_ERR_INVALID; }
 
+// ---- the static scratch array ----------------------------------------------
+// Each device holds its own copy of `scratch`; its address is looked up once
+// per device, with that device current, when the first context on it is
+// created.  Calls of psgpu_sync_noalloc share the array of their device, so
+// they are serialised by g_static_lock.
+const int kMaxStaticDevices = 64;
+std::mutex g_static_lock;
+uint8_t* g_static_ptr[kMaxStaticDevices] = {nullptr};
+hipError_t g_static_err[kMaxStaticDevices] = {hipSuccess};
+
+void static_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxStaticDevices) return;
+    std::lock_guard<std::mutex> g(g_static_lock);
+    if (g_static_ptr[device]) return;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_static_ptr[device] = (uint8_t*)p;
+    else g_static_err[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+}
+
+// bump carving of the static array
+struct Arena {
+    uint8_t* base;
+    size_t cap, used = 0;
+    template <typename T> bool take(DevBuf<T>& b, size_t n)
+    {
+        size_t bytes = align256(std::max<size_t>(n, 1) * sizeof(T));
+        if (used + bytes > cap) return false;
+        b.p = (T*)(base + used);
+        b.cap = std::max<size_t>(n, 1);
+        used += bytes;
+        return true;
+    }
+    uint8_t* take_bytes(size_t n)
+    {
+        size_t bytes = align256(std::max<size_t>(n, 1));
+        if (used + bytes > cap) return nullptr;
+        uint8_t* p = base + used;
+        used += bytes;
+        return p;
+    }
+};
+
+// the quantities a flush's scratch depends on, as build_blob lays them out
+struct FlushShape { int64_t T = 0, n_pix = 0, n_tiles = 0; int nvs = 1; };
+
+FlushShape flush_shape(psgpu_ctx* c, const Cmd& cmd)
+{
+    FlushShape sh;
+    const int cw = c->tiling.chunk_width, tr = c->tiling.tile_rows, tc = 64 / tr;
+    for (const Segment& sg : cmd.views) {
+        int n_chunks = (sg.tg.vpw + cw - 1) / cw;
+        sh.n_tiles += (int64_t)((n_chunks + tc - 1) / tc) * ((sg.tg.vph + tr - 1) / tr);
+        sh.n_pix += (int64_t)sg.tg.vpw * sg.tg.vph;
+        for (const PsDraw& d : sg.draws) {
+            sh.T += d.tri_count;
+            sh.nvs = std::max(sh.nvs, d.nv);
+        }
+    }
+    return sh;
+}
+
+// carve every buffer of run_flush for a flush of this shape; the span and
+// piece capacities take what is left (pieces at 1.5 per span, as the
+// allocating path starts them)
+bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
+{
+    const int64_t T = sh.T;
+    if (!a.take(b.winner, sh.n_pix) || !a.take(b.tris, T) || !a.take(b.vary, (size_t)T * 3 * sh.nvs) ||
+        !a.take(b.grad, (size_t)T * 8) || !a.take(b.nrows, T) || !a.take(b.rowoff, T + 1) ||
+        !a.take(b.kstart, std::max<int64_t>(sh.n_tiles + 1, 2048)))
+        return false;
+    const bool expand = T > kSpanSearchTris;
+    // bytes per span: record, piece count and offset (+ row table), and 1.5
+    // pieces of key, value, both sort copies, record, checkpoint, histograms
+    const double per_span = 32 + 4 + 4 + (expand ? 4 : 0) + 1.5 * (4 * 4 + sizeof(PsSeg) + 4.0 * sh.nvs + 8);
+    size_t left = a.cap - a.used;
+    int64_t spans = (int64_t)((double)left / per_span);
+    spans = std::min<int64_t>(spans, (int64_t)1 << 26);
+    for (; spans >= 1024; spans -= spans / 16 + 1) {
+        Arena t = a;
+        FlushBufs x = b;
+        x.span_cap = spans;
+        x.seg_cap = spans * 3 / 2;
+        int nbr = psk_rs_blocks(x.seg_cap);
+        if (t.take(x.partial, psk_scan_blocks(std::max<int64_t>(T, x.span_cap)) + 1) &&
+            (!expand || t.take(x.spantri, x.span_cap)) && t.take(x.spans, x.span_cap) &&
+            t.take(x.segcount, x.span_cap) && t.take(x.segoff, x.span_cap + 1) && t.take(x.keys, x.seg_cap) &&
+            t.take(x.vals, x.seg_cap) && t.take(x.keys2, x.seg_cap) && t.take(x.vals2, x.seg_cap) &&
+            t.take(x.segs, x.seg_cap) && t.take(x.ck, (size_t)x.seg_cap * sh.nvs) &&
+            t.take(x.hist, (size_t)nbr * 2048) && t.take(x.histoff, (size_t)nbr * 2048 + 2048)) {
+            a = t;
+            b = x;
+            return true;
+        }
+    }
+    return false;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -969,6 +1076,7 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PSGPU_ERR_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return PSGPU_ERR_DEVICE;
+    static_scratch_init(device);
     psgpu_ctx* c = new psgpu_ctx();
     c->device = device;
     c->dev_w = c->vp_w = width;
@@ -2001,6 +2109,104 @@ int psgpu_graph_destroy(psgpu_ctx* c, int graph)
     if (g.exec) hipGraphExecDestroy(g.exec);
     for (auto p : g.blobs) if (p) hipFree(p);
     c->graphs[graph] = Graph();
+    return PSGPU_OK;
+}
+
+int psgpu_sync_noalloc(psgpu_ctx* c)
+{
+    if (need_ctx(c)) return PSGPU_ERR_INVALID;
+    if (c->capturing) return fail(c, PSGPU_ERR_STATE, "not allowed while recording a command list");
+    if (!c->log.empty())
+        return fail(c, PSGPU_ERR_STATE, "work submitted since the last sync (psgpu_flush, a flush forced by psgpu_draw, a "
+                                        "post-process or a graph launch) must be synchronised with psgpu_sync first");
+    if (c->pending_fills.empty() && c->queue.empty()) return PSGPU_OK;
+    uint8_t* base = (c->device >= 0 && c->device < kMaxStaticDevices) ? g_static_ptr[c->device] : nullptr;
+    if (!base)
+        return fail(c, PSGPU_ERR_DEVICE, "static scratch array of device %d unavailable: %s", c->device,
+                    hipGetErrorString(c->device < kMaxStaticDevices ? g_static_err[c->device] : hipErrorInvalidDevice));
+    Cmd clears, flush;
+    clears.type = CMD_CLEARS;
+    clears.fills = c->pending_fills;
+    flush.type = CMD_FLUSH;
+    flush.views = c->queue;
+    std::lock_guard<std::mutex> guard(g_static_lock);
+    Arena arena{base, PSGPU_STATIC_SCRATCH_BYTES};
+    uint8_t* fill_dev = nullptr;
+    if (!clears.fills.empty() && !(fill_dev = arena.take_bytes(clears.fills.size() * sizeof(PsFill))))
+        return fail(c, PSGPU_ERR_NOMEM, "%zu clears exceed the static scratch array", clears.fills.size());
+    FlushShape sh = flush_shape(c, flush);
+    FlushBufs sb;
+    Blob blob;
+    uint8_t* blob_dev = nullptr;
+    // the flush's records go first, then its buffers; the blob's size does
+    // not depend on where the buffers land
+    {
+        size_t draws = 0;
+        for (const Segment& sg : flush.views) draws += sg.draws.size();
+        size_t bytes = align256(draws * sizeof(PsDraw)) + align256(flush.views.size() * sizeof(PsView)) + 16;
+        if (sh.T > 0 && !(blob_dev = arena.take_bytes(bytes)))
+            return fail(c, PSGPU_ERR_NOMEM, "the draw records exceed the static scratch array");
+    }
+    if (sh.T > 0 && !carve_flush(arena, sh, sb))
+        return fail(c, PSGPU_ERR_NOMEM,
+                    "a flush of %lld triangles over %lld pixels needs more than the %zu-byte static scratch array",
+                    (long long)sh.T, (long long)sh.n_pix, (size_t)PSGPU_STATIC_SCRATCH_BYTES);
+    // from here on the queued work is consumed, as psgpu_flush consumes it
+    c->pending_fills.clear();
+    c->queue.clear();
+    c->queue_tris = 0;
+    c->queue_imm = 0;
+    std::swap(c->fb, sb);
+    c->fixed_scratch = true;
+    int rc = PSGPU_OK;
+    if (fill_dev) {
+        hipError_t e = hipMemcpyAsync(fill_dev, clears.fills.data(), clears.fills.size() * sizeof(PsFill),
+                                      hipMemcpyHostToDevice, c->stream);
+        rc = e != hipSuccess ? fail(c, PSGPU_ERR_DEVICE, "hipMemcpyAsync: %s", hipGetErrorString(e))
+                             : run_clears(c, clears, fill_dev);
+    }
+    if (!rc && blob_dev) {
+        PsFrame f;
+        int64_t T, n_pix;
+        int nvs;
+        bool p1[PS_N_PROGS] = {false}, p2[PS_N_PROGS] = {false};
+        rc = build_blob(c, flush, blob, f, T, nvs, p1, p2, n_pix);
+        if (!rc && (T != sh.T || n_pix != sh.n_pix || f.n_tiles != sh.n_tiles))
+            rc = fail(c, PSGPU_ERR_STATE, "internal: flush shape mismatch");
+        if (!rc) {
+            hipError_t e = hipMemcpyAsync(blob_dev, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice, c->stream);
+            rc = e != hipSuccess ? fail(c, PSGPU_ERR_DEVICE, "hipMemcpyAsync: %s", hipGetErrorString(e))
+                                 : run_flush(c, flush, blob_dev);
+        }
+    }
+    // wait, then read and reset the counters as psgpu_sync does
+    hipError_t e = hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(PsCounters), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    std::swap(c->fb, sb);
+    c->fixed_scratch = false;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(c, PSGPU_ERR_DEVICE, "%s", hipGetErrorString(e));
+    bool overflow = c->h_cnt->overflow != 0;
+    if (!overflow) {
+        int64_t sum[PS_ST_N] = {0};
+        for (int k = 0; k < PS_STRIPES; k++)
+            for (int f = 0; f < PS_ST_N; f++) sum[f] += (int64_t)c->h_cnt->stat[k][f];
+        c->stats.tris_culled += sum[PS_ST_CULLED];
+        c->stats.tris_zrejected += sum[PS_ST_ZREJ];
+        c->stats.tris_dropped += sum[PS_ST_DROPPED];
+        c->stats.spans += sum[PS_ST_SPANS];
+        c->stats.frags_rasterised += sum[PS_ST_FRAGS];
+        c->stats.frags_shaded += sum[PS_ST_SHADED];
+        c->stats.segments += sum[PS_ST_SEGS];
+    }
+    HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (overflow)
+        return fail(c, PSGPU_ERR_NOMEM,
+                    "the flush's spans or pieces exceed what the %zu-byte static scratch array holds (%lld spans, "
+                    "%lld pieces); its draws were not rendered (the clears were)",
+                    (size_t)PSGPU_STATIC_SCRATCH_BYTES, (long long)sb.span_cap, (long long)sb.seg_cap);
     return PSGPU_OK;
 }
 

@@ -104,6 +104,7 @@ _SIGS = {
     "psgpu_draw": ([C.c_void_p, C.c_int], C.c_int),
     "psgpu_flush": ([C.c_void_p], C.c_int),
     "psgpu_sync": ([C.c_void_p], C.c_int),
+    "psgpu_sync_noalloc": ([C.c_void_p], C.c_int),
     "psgpu_read_colour": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "psgpu_read_depth": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "psgpu_display_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
@@ -350,6 +351,11 @@ class Context:
 
     def sync(self):
         self._chk(self.L.psgpu_sync(self.h))
+
+    def sync_noalloc(self):
+        """psgpu_sync_noalloc: run the queued clears and draws with the fixed
+        device scratch array (no allocation) and wait"""
+        self._chk(self.L.psgpu_sync_noalloc(self.h))
 
     def read_colour(self, out=None):
         """the display (top-down BGRA8 rows), into `out` when given"""
