@@ -74,6 +74,7 @@ typedef struct {
     int64_t spans, frags_rasterised, frags_shaded;
     int64_t segments;           /* span pieces after splitting at bin-chunk borders */
     int64_t flushes, retries;   /* pipeline executions, capacity-growth re-runs      */
+    int64_t splits;             /* flushes split in two for the per-flush span limit */
 } psgpu_stats;
 
 /* tile configuration of the raster kernel; searched by the autotuner */
@@ -87,6 +88,10 @@ typedef struct {
 /* flags: PSGPU_CREATE_TINY_SCRATCH starts the intermediate buffers at a few
  * hundred entries, so the first flushes overflow and are replayed (tests). */
 #define PSGPU_CREATE_TINY_SCRATCH 0x1
+/* PSGPU_CREATE_SMALL_FLUSHES lowers the most spans one flush may hold from
+ * 2^26 to 4,096 (tests): a flush that needs more is split into flushes of
+ * about half its triangles, re-run in order, with the same result. */
+#define PSGPU_CREATE_SMALL_FLUSHES 0x2
 /* Display and viewport sizes are limited to PSGPU_MAX_VIEW_WIDTH x
  * PSGPU_MAX_VIEW_HEIGHT (intermediate records keep a column in 16 bits and a
  * row in 24); larger ones give PSGPU_ERR_RANGE. */
@@ -160,6 +165,11 @@ int psgpu_sync(psgpu_ctx* ctx);                   /* flush + wait; re-runs on ca
 /* display: top-down BGRA8 rows (the renderer's back buffer); depth: default depth buffer rows */
 int psgpu_read_colour(psgpu_ctx* ctx, void* host, size_t bytes);
 int psgpu_read_depth(psgpu_ctx* ctx, void* host, size_t bytes);
+/* the display and default depth buffer written from host memory (same layouts
+ * as the reads): the reference's are host FBOs that a post-processor may
+ * write in place (PuresoftFBO::getBuffer, post.cpp:3-18); synchronises first */
+int psgpu_write_colour(psgpu_ctx* ctx, const void* host, size_t bytes);
+int psgpu_write_depth(psgpu_ctx* ctx, const void* host, size_t bytes);
 int psgpu_display_device_ptr(psgpu_ctx* ctx, void** ptr);
 int psgpu_depth_device_ptr(psgpu_ctx* ctx, void** ptr, int* scanline_floats);
 int psgpu_get_stats(psgpu_ctx* ctx, psgpu_stats* out);

@@ -28,8 +28,6 @@ int psk_radix_passes(int key_bits);
 void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
                     int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
                     bool& result_in_second, uint32_t* kstart);
-void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
-                const PsCounters* c);
 size_t psk_raster_lds(int chunk_w, int waves);
 void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
                 const uint32_t* kstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
@@ -39,8 +37,9 @@ void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* seg
 void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
                     const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
                     PsCounters* cnt, const float* vary, int nvs, const float* ck);
+// clears: this launch also writes the views' fused colour clears (one launch of a flush does)
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
                const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f,
-               const PsCounters* cnt);
+               const PsCounters* cnt, bool clears);
 void psk_mip(hipStream_t s, const uint8_t* src, int ws, int hs, int ss, uint8_t* dst, int wd, int hd, int elem);
 void psk_selftest(hipStream_t s, int op, const float* a, const float* b, float* out, int n);

@@ -8,13 +8,14 @@
 //            depth / correction-factor start and step (drawvao.cpp:66-110,
 //            interp.cpp:26-80); counts the bin chunks the span touches
 //  segment : one thread per span writes (bin key, span) pairs
-//  radix   : stable LSD radix sort of the pairs by bin key (8-bit digits,
-//            wave ballots give stable in-block ranks)
-//  bounds  : first pair of every bin key
+//  radix   : stable LSD radix sort of the pairs by bin key (digits of up to
+//            11 bits, wave ballots give stable in-block ranks)
 //  raster  : one wave per tile of 64 bins, lane = bin (a row piece of
-//            chunk_w pixels); each lane walks its bin's spans in triangle
-//            order, keeping depth and the winning span in LDS
-//            (fragthrd.cpp:208-252); depth test -1 < z && z - d < -1e-4
+//            chunk_w pixels); the wave finds its tile's range of the sorted
+//            pairs (kept by a one-pass sort, else searched), and each lane
+//            walks its bin's spans in triangle order, keeping depth and the
+//            winning span in LDS (fragthrd.cpp:208-252); depth test
+//            -1 < z && z - d < -1e-4
 //  shade   : one thread per pixel whose last colour-writing depth-test pass
 //            is known: re-derives that span's varyings, walks them serially
 //            to the pixel, runs the FP twin and writes BGRA8.
@@ -35,11 +36,27 @@ __device__ __forceinline__ uint64_t ps_lt_mask()
     return __lanemask_lt();
 }
 
-__device__ __forceinline__ void ps_wave_add(PsCounters* c, int field, unsigned long long v)
+// a wave's statistic: into the committed counters (pending = false) or the
+// flush's pending ones; one atomic per wave, on a per-block stripe
+__device__ __forceinline__ void ps_wave_add(PsCounters* c, int field, unsigned long long v, bool pending = false)
 {
-    // one atomic per wave, on a per-block stripe
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PS_WAVE);
-    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v) atomicAdd(&c->stat[blockIdx.x & (PS_STRIPES - 1)][field], v);
+    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v)
+        atomicAdd(pending ? &c->pend[blockIdx.x & (PS_STRIPES - 1)][field] : &c->stat[blockIdx.x & (PS_STRIPES - 1)][field], v);
+}
+
+// the flush got past every overflow check: its pending statistics become
+// committed (blocks 0..63 of the raster launch take the stripes between them)
+__device__ __forceinline__ void ps_commit_pending(PsCounters* c)
+{
+    if (blockIdx.x >= PS_STRIPES || threadIdx.x >= PS_ST_N) return;
+    for (int s = blockIdx.x; s < PS_STRIPES; s += gridDim.x) {
+        unsigned long long v = c->pend[s][threadIdx.x];
+        if (v) {
+            atomicAdd(&c->stat[s][threadIdx.x], v);
+            c->pend[s][threadIdx.x] = 0;
+        }
+    }
 }
 
 __device__ __forceinline__ int ps_view_of_tile(const PsFrame& f, int64_t t)
@@ -184,49 +201,84 @@ __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draw
     return d;
 }
 
+// A block of 3 waves sets up 64 triangles: wave v runs the vertex twin for
+// vertex v of every triangle (with the perspective division), so the three
+// vertices of a triangle are processed in parallel, and leaves the clip
+// position and varyings in LDS; wave 0 then does the per-triangle work
+// (cull, z reject, pushVertex/pushTriangle, gradients) from LDS; finally the
+// three waves store the varyings of the triangles that need them, one
+// triangle per store instruction.
+#define SU_TRIS 64
 template <int PROG, bool SHARE>
-__global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
-                                               float* __restrict__ vary, int nvs, float* __restrict__ grads,
-                                               uint32_t* __restrict__ nrows, PsCounters* cnt,
-                                               const PsView* __restrict__ views)
+__global__ void __launch_bounds__(192) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
+                                             float* __restrict__ vary, int nvs, float* __restrict__ grads,
+                                             uint32_t* __restrict__ nrows, PsCounters* cnt,
+                                             const PsView* __restrict__ views)
 {
     typedef PsProc<PROG> P;
-    extern __shared__ float su[];   // per wave: the varyings of its 64 triangles, 3*nvs floats each
-    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool live = t < T;
-    int di = ps_find_draw_wave(draws, nd, live ? t : T - 1);
-    unsigned culled = 0, zrej = 0, dropped = 0;
-    const int lane = threadIdx.x & 63, n3 = 3 * nvs;
-    float* wl = su + (size_t)(threadIdx.x & ~63) * n3;
-    bool stage = false;
+    // LDS: the varyings of the block's triangles, 3*nvs floats each, then per
+    // vertex the divided position (all four lanes: w/w is 1 only for finite,
+    // nonzero w, and the back-face test reads it) and 1/w, and the mask of
+    // triangles whose varyings are stored
+    extern __shared__ float su[];
+    const int lane = threadIdx.x & 63, v = threadIdx.x >> 6, n3 = 3 * nvs;
+    float* spos = su + (size_t)SU_TRIS * n3;
+    float* srw = spos + 3 * SU_TRIS * 4;
+    __shared__ unsigned long long s_stage;
+    const int64_t t0 = blockIdx.x * (int64_t)SU_TRIS;
+    const int64_t t = t0 + lane;
+    const bool live = t < T;
+    const int di = ps_find_draw_wave(draws, nd, live ? t : T - 1);
+    bool mine = false;   // the triangle belongs to this programme
+    bool skip = false;   // positions only: another draw of the flush writes its varyings
     ps_with_draw(draws, di, [&](const PsDraw& d) {
-    if (live && d.prog == PROG) {
-        const int W = views[d.view].W, H = views[d.view].H;
-        int64_t lt = t - d.tri_start;
-        float pos[3][4];
-        float u[3][P::NV > 0 ? P::NV : 1];
-        float rw[3], z[3];
-        bool pieces = false;
-        const bool skip = SHARE && ps_shares_varyings(PROG) && d.vary_skip;
+        mine = live && d.prog == PROG;
+        if (!mine) return;
+        skip = SHARE && ps_shares_varyings(PROG) && d.vary_skip;
+        float pos[4];
+        float u[P::NV > 0 ? P::NV : 1];
+        const int64_t lt = t - d.tri_start;
+        if (skip) ps_vp_pos(d, lt * 3 + v, pos, u, ps_grad_iu<P>());
+        else P::vp(d, lt * 3 + v, pos, u);
+        // perspective division (vertthrd.cpp:37-47)
+        float r = 1.0f / pos[3];
+        pos[0] *= r; pos[1] *= r; pos[2] *= r; pos[3] *= r;
+        float* q = spos + (v * SU_TRIS + lane) * 4;
+        q[0] = pos[0]; q[1] = pos[1]; q[2] = pos[2]; q[3] = pos[3];
+        srw[v * SU_TRIS + lane] = r;
+        float* uo = su + lane * n3 + v * nvs;
+        if (skip) {
+            if constexpr (P::GRAD) { uo[P::IU] = u[P::IU]; uo[P::IU + 1] = u[P::IU + 1]; }
+        } else {
 #pragma unroll
-        for (int v = 0; v < 3; v++) {
-            if (skip) ps_vp_pos(d, lt * 3 + v, pos[v], u[v], ps_grad_iu<P>());
-            else P::vp(d, lt * 3 + v, pos[v], u[v]);
-            // perspective division (vertthrd.cpp:37-47)
-            float r = 1.0f / pos[v][3];
-            pos[v][0] *= r; pos[v][1] *= r; pos[v][2] *= r; pos[v][3] *= r;
-            z[v] = pos[v][2];
-            rw[v] = r;
+            for (int k = 0; k < P::NV; k++) uo[k] = u[k];
+        }
+    });
+    __syncthreads();
+    unsigned culled = 0, zrej = 0, dropped = 0;
+    if (v == 0) {
+        bool stage = false;
+        ps_with_draw(draws, di, [&](const PsDraw& d) {
+        if (!mine) return;
+        const int W = views[d.view].W, H = views[d.view].H;
+        float pos[3][4], rw[3], z[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float* q = spos + (k * SU_TRIS + lane) * 4;
+            pos[k][0] = q[0]; pos[k][1] = q[1]; pos[k][2] = q[2]; pos[k][3] = q[3];
+            rw[k] = srw[k * SU_TRIS + lane];
+            z[k] = q[2];
         }
         uint32_t n = 0;
         PsTri tr;
         bool keep = true;
+        bool pieces = false;
         if ((d.behavior & PSGPU_FACE_CULLING) && ps_back_face(pos[0], pos[1], pos[2])) { keep = false; culled = 1; }
         if (keep) {
             // drawvao.cpp:51-56
 #pragma unroll
-            for (int v = 0; v < 3; v++)
-                if (pos[v][2] < -1.0f || pos[v][2] > 1.0f) keep = false;
+            for (int k = 0; k < 3; k++)
+                if (pos[k][2] < -1.0f || pos[k][2] > 1.0f) keep = false;
             if (!keep) zrej = 1;
         }
         if (keep) {
@@ -235,18 +287,18 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
             float vx[3], vy[3];
             bool bad = false;
 #pragma unroll
-            for (int v = 0; v < 3; v++) {
-                vy[v] = ((float)hh * pos[v][1]) + (float)hh;
-                vx[v] = ((float)hw * pos[v][0]) + (float)hw;
+            for (int k = 0; k < 3; k++) {
+                vy[k] = ((float)hh * pos[k][1]) + (float)hh;
+                vx[k] = ((float)hw * pos[k][0]) + (float)hw;
                 // rows outside the int range make the reference read rows it
                 // never wrote (or crash); the triangle is dropped
-                if (!(vy[v] >= -2147483648.0f && vy[v] < 2147483648.0f)) bad = true;
+                if (!(vy[k] >= -2147483648.0f && vy[k] < 2147483648.0f)) bad = true;
             }
             int first = ps_cvt_i32(vy[0]), last = first;
 #pragma unroll
-            for (int v = 1; v < 3; v++) {
-                if (vy[v] > (float)last) last = ps_cvt_i32(vy[v]);
-                else if (vy[v] < (float)first) first = ps_cvt_i32(vy[v]);
+            for (int k = 1; k < 3; k++) {
+                if (vy[k] > (float)last) last = ps_cvt_i32(vy[k]);
+                else if (vy[k] < (float)first) first = ps_cvt_i32(vy[k]);
             }
             if (first >= H || last < 0) { first = 0; last = -1; }
             if (first < 0) first = 0;
@@ -274,7 +326,7 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
                     split = y3 > (float)(H - 1) ? H - 1 : ps_cvt_i32(y3);
                 }
 #pragma unroll
-                for (int v = 0; v < 3; v++) { tr.vx[v] = vx[v]; tr.vy[v] = vy[v]; tr.rw[v] = rw[v]; tr.z[v] = z[v]; }
+                for (int k = 0; k < 3; k++) { tr.vx[k] = vx[k]; tr.vy[k] = vy[k]; tr.rw[k] = rw[k]; tr.z[k] = z[k]; }
                 tr.first = first;
                 tr.nrows = last - first + 1;
                 tr.split = split;
@@ -285,15 +337,10 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
                 // extrapolate, so no x range test is safe here)
                 pieces = n > 0;
                 if (pieces) tris[t] = tr;
-                if (!SHARE && pieces) {   // staged: the wave stores whole triangles below
-#pragma unroll
-                    for (int v = 0; v < 3; v++)
-#pragma unroll
-                        for (int k = 0; k < P::NV; k++) wl[lane * n3 + v * nvs + k] = u[v][k];
-                }
                 if constexpr (P::GRAD) if (pieces) {   // screen gradients for the mip level, once per triangle
                     float g[6];
-                    ps_tri_grad(tr, u[0], u[1], u[2], P::IU, g);
+                    const float* ub = su + lane * n3;
+                    ps_tri_grad(tr, ub, ub + nvs, ub + 2 * nvs, P::IU, g);
                     float4* go = (float4*)(grads + t * 8);
                     go[0] = make_float4(g[0], g[1], g[2], g[3]);
                     go[1] = make_float4(g[4], g[5], 0.0f, 0.0f);
@@ -303,34 +350,31 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
         nrows[t] = n;
         // varyings of triangles with rows, or of all when other views read them
         stage = SHARE ? !skip && (pieces || d.vary_all) : pieces;
-        if (SHARE && stage) {
-#pragma unroll
-            for (int v = 0; v < 3; v++)
-#pragma unroll
-                for (int k = 0; k < P::NV; k++) wl[lane * n3 + v * nvs + k] = u[v][k];
-        }
+        });
+        uint64_t m = __ballot(stage);
+        if (lane == 0) s_stage = m;
     }
-    });
-    // varyings of the wave's staged triangles, one triangle per store
-    // instruction (3*nvs consecutive floats) instead of one 3*nvs-float stride
-    // per lane; floats past NV of each vertex are left as they are
-    uint64_t m = __ballot(stage);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __syncthreads();
+    // varyings of the staged triangles, one triangle per store instruction
+    // (3*nvs consecutive floats), the waves taking every third triangle;
+    // floats past NV of each vertex are left as they are
     {
+        uint64_t m = s_stage;
         int vv = lane / (nvs > 0 ? nvs : 1), kk = lane - vv * nvs;
-        bool mine = vv < 3 && kk < P::NV;
-        int64_t tw = t - lane;
+        bool part = vv < 3 && kk < P::NV;
+        int seen = 0;
         while (m) {
             int tl = __builtin_ctzll(m);
             m &= m - 1;
-            if (mine) vary[(tw + tl) * n3 + lane] = wl[tl * n3 + lane];
+            if (seen++ % 3 != v) continue;
+            if (part) vary[(t0 + tl) * n3 + lane] = su[tl * n3 + lane];
         }
     }
-    ps_wave_add(cnt, PS_ST_CULLED, culled);
-    ps_wave_add(cnt, PS_ST_ZREJ, zrej);
-    ps_wave_add(cnt, PS_ST_DROPPED, dropped);
+    if (v == 0) {
+        ps_wave_add(cnt, PS_ST_CULLED, culled, true);
+        ps_wave_add(cnt, PS_ST_ZREJ, zrej, true);
+        ps_wave_add(cnt, PS_ST_DROPPED, dropped, true);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -576,8 +620,8 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
             spans[i] = sp;
             seg_count[i] = segs;
         }
-        ps_wave_add(cnt, PS_ST_SPANS, nspan);
-        ps_wave_add(cnt, PS_ST_FRAGS, nfrag);
+        ps_wave_add(cnt, PS_ST_SPANS, nspan, true);
+        ps_wave_add(cnt, PS_ST_FRAGS, nfrag, true);
     }
 }
 
@@ -681,7 +725,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                 nseg = e - o;
             }
         }
-        ps_wave_add(cnt, PS_ST_SEGS, nseg);
+        ps_wave_add(cnt, PS_ST_SEGS, nseg, true);
     }
 }
 
@@ -851,37 +895,40 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
     }
 }
 
-// ---------------------------------------------------------------------------
-// bounds: first pair of each bin key (lower_bound)
-// ---------------------------------------------------------------------------
-// One wave per key k: the first position i with keys[i] >= k, by a 64-way
-// search (each step loads 64 pivots at once), so a lookup costs about four
-// dependent loads instead of twenty.
-__global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int64_t n_keys,
-                                                uint32_t* __restrict__ start, const PsCounters* c)
+// the first position i of keys[0, n) with keys[i] >= k, found by one wave
+// with a 64-way search (each step loads 64 pivots at once)
+__device__ __forceinline__ uint32_t ps_wave_lower_bound(const uint32_t* __restrict__ keys, int64_t n, uint32_t k, int lane)
 {
-    if (ps_poisoned(c)) return;
-    int lane = threadIdx.x & 63;
-    int64_t n = *n_ptr;
-    int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t k = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); k <= n_keys; k += waves) {
-        int64_t lo = 0, hi = n;
-        while (hi - lo > 64) {
-            int64_t step = (hi - lo + 63) / 64;
-            int64_t p = lo + lane * step;
-            bool below = p < hi && keys[p] < (uint32_t)k;
-            int cnt = __popcll(__ballot(below));
-            if (cnt == 0) { hi = lo; break; }
-            int64_t nlo = lo + (int64_t)(cnt - 1) * step + 1;
-            int64_t nhi = lo + (int64_t)cnt * step;
-            lo = nlo;
-            hi = nhi < hi ? nhi : hi;
-        }
-        int64_t q = lo + lane;
-        bool below = q < hi && keys[q] < (uint32_t)k;
+    int64_t lo = 0, hi = n;
+    while (hi - lo > 64) {
+        int64_t step = (hi - lo + 63) / 64;
+        int64_t p = lo + lane * step;
+        bool below = p < hi && keys[p] < k;
         int cnt = __popcll(__ballot(below));
-        if (lane == 0) start[k] = (uint32_t)(lo + cnt);
+        if (cnt == 0) return (uint32_t)lo;
+        int64_t nlo = lo + (int64_t)(cnt - 1) * step + 1;
+        int64_t nhi = lo + (int64_t)cnt * step;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
     }
+    int64_t q = lo + lane;
+    bool below = q < hi && keys[q] < k;
+    return (uint32_t)(lo + __popcll(__ballot(below)));
+}
+
+// the tile's range of the sorted list: from the starts a one-pass sort
+// leaves, or searched in the sorted keys
+__device__ __forceinline__ void ps_tile_range(const uint32_t* __restrict__ tstart, const PsFrame& f, int64_t tile, int lane,
+                                              uint32_t& s0, uint32_t& s1)
+{
+    if (tstart) {
+        s0 = tstart[tile];
+        s1 = tstart[tile + 1];
+        return;
+    }
+    const int64_t n = *f.n_sorted;
+    s0 = ps_wave_lower_bound(f.skeys, n, (uint32_t)tile, lane);
+    s1 = ps_wave_lower_bound(f.skeys, n, (uint32_t)tile + 1u, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -903,6 +950,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
 {
     extern __shared__ uint32_t lds[];
     if (ps_poisoned(cnt)) return;
+    ps_commit_pending(cnt);
     const int CW = f.chunk_w;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int waves = blockDim.x >> 6;
@@ -933,14 +981,18 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
     bool inside = y < V.H && X0 < VW;
     int ncol = inside ? min(CW, VW - X0) : 0;
     float* drow = V.depth + (int64_t)y * V.depth_scan;
+    // a clear fused into the flush: the tile starts from its value, and the
+    // whole tile is written back
+    const bool dclear = V.dclear != 0;
     for (int c = 0; c < ncol; c++) {
-        dep[c * 64 + lane] = drow[X0 + c];
+        dep[c * 64 + lane] = dclear ? V.dclear_value : drow[X0 + c];
         win[c * 64 + lane] = -1;
     }
     unsigned long long passes = 0;
-    bool dirty = false;
+    bool dirty = dclear;
     const uint64_t lt = ps_lt_mask();
-    uint32_t s0 = tstart[tile], s1 = tstart[tile + 1];
+    uint32_t s0, s1;
+    ps_tile_range(tstart, f, tile, lane, s0, s1);
     // software pipeline over chunks of the tile list: while chunk i is split and
     // walked, the records of chunk i+1 and the indices of chunk i+2 are in flight
     uint32_t vn[RA_PER], vnn[RA_PER];
@@ -988,50 +1040,56 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
             int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
             key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
         }
-        // stable split by lane key: count, offsets, place
+        // stable split by lane key: each entry's rank among the equal keys
+        // before it (the same-key lanes of its group of 64 by ballots, plus
+        // the count of earlier groups), then per-key offsets, then placement
         lcnt[lane] = 0;
         __builtin_amdgcn_wave_barrier();
-        for (int pass = 0; pass < 2; pass++) {
-            if (pass == 1) {
-                uint32_t cntv = lcnt[lane];
-                uint32_t x = cntv;
+        uint32_t rank[RA_PER];
 #pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    uint32_t yv = __shfl_up(x, o, 64);
-                    if (lane >= o) x += yv;
-                }
-                loff[lane] = x - cntv;
-                lcnt[lane] = 0;
-                __builtin_amdgcn_wave_barrier();
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + k * 64 + lane;
+            bool valid = e < ce;
+            uint32_t kk = key[k];
+            uint64_t same = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 6; bit++) {
+                uint64_t m = __ballot((kk >> bit) & 1u);
+                same &= ((kk >> bit) & 1u) ? m : ~m;
             }
+            uint32_t before = (uint32_t)__popcll(same & lt);
+            uint32_t cur = lcnt[kk];
+            rank[k] = cur + before;
+            __builtin_amdgcn_wave_barrier();
+            if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
+            __builtin_amdgcn_wave_barrier();
+        }
+        {
+            uint32_t cntv = lcnt[lane];
+            uint32_t x = cntv;
 #pragma unroll
-            for (int k = 0; k < RA_PER; k++) {
-                uint32_t e = cb + k * 64 + lane;
-                bool valid = e < ce;
-                uint32_t kk = key[k];
-                uint64_t same = __ballot(valid);
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t yv = __shfl_up(x, o, 64);
+                if (lane >= o) x += yv;
+            }
+            loff[lane] = x - cntv;
+            __builtin_amdgcn_wave_barrier();
+        }
 #pragma unroll
-                for (int bit = 0; bit < 6; bit++) {
-                    uint64_t m = __ballot((kk >> bit) & 1u);
-                    same &= ((kk >> bit) & 1u) ? m : ~m;
-                }
-                uint32_t before = (uint32_t)__popcll(same & lt);
-                uint32_t cur = lcnt[kk];
-                if (pass == 1 && valid) {
-                    uint32_t q = loff[kk] + cur + before;
-                    r_s[q] = v[k];
-                    r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
-                    r_x[q] = ra[k].y;
-                    r_cf2[q] = __uint_as_float(ra[k].z);
-                    r_pz[q] = __uint_as_float(ra[k].w);
-                    r_cst[q] = __uint_as_float(rb[k].x);
-                    r_pst[q] = __uint_as_float(rb[k].y);
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
-                __builtin_amdgcn_wave_barrier();
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + k * 64 + lane;
+            if (e < ce) {
+                uint32_t q = loff[key[k]] + rank[k];
+                r_s[q] = v[k];
+                r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
+                r_x[q] = ra[k].y;
+                r_cf2[q] = __uint_as_float(ra[k].z);
+                r_pz[q] = __uint_as_float(ra[k].w);
+                r_cst[q] = __uint_as_float(rb[k].x);
+                r_pst[q] = __uint_as_float(rb[k].y);
             }
         }
+        __builtin_amdgcn_wave_barrier();
         if (inside) {
             uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
             for (uint32_t q = l0; q < l1; q++) {
@@ -1101,6 +1159,7 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     typedef PsProc<PROG> P;
     extern __shared__ uint32_t lds[];
     if (ps_poisoned(cnt)) return;
+    ps_commit_pending(cnt);
     const int CW = f.chunk_w;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int waves = blockDim.x >> 6;
@@ -1131,14 +1190,18 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     bool inside = y < V.H && X0 < VW;
     int ncol = inside ? min(CW, VW - X0) : 0;
     float* drow = V.depth + (int64_t)y * V.depth_scan;
+    // a clear fused into the flush: the tile starts from its value, and the
+    // whole tile is written back
+    const bool dclear = V.dclear != 0;
     for (int c = 0; c < ncol; c++) {
-        dep[c * 64 + lane] = drow[X0 + c];
+        dep[c * 64 + lane] = dclear ? V.dclear_value : drow[X0 + c];
         win[c * 64 + lane] = -1;
     }
     unsigned long long passes = 0;
-    bool dirty = false;
+    bool dirty = dclear;
     const uint64_t lt = ps_lt_mask();
-    uint32_t s0 = tstart[tile], s1 = tstart[tile + 1];
+    uint32_t s0, s1;
+    ps_tile_range(tstart, f, tile, lane, s0, s1);
     // software pipeline over chunks of the tile list: while chunk i is split and
     // walked, the records of chunk i+1 and the indices of chunk i+2 are in flight
     uint32_t vn[RA_PER], vnn[RA_PER];
@@ -1186,50 +1249,56 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
             int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
             key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
         }
-        // stable split by lane key: count, offsets, place
+        // stable split by lane key: each entry's rank among the equal keys
+        // before it (the same-key lanes of its group of 64 by ballots, plus
+        // the count of earlier groups), then per-key offsets, then placement
         lcnt[lane] = 0;
         __builtin_amdgcn_wave_barrier();
-        for (int pass = 0; pass < 2; pass++) {
-            if (pass == 1) {
-                uint32_t cntv = lcnt[lane];
-                uint32_t x = cntv;
+        uint32_t rank[RA_PER];
 #pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    uint32_t yv = __shfl_up(x, o, 64);
-                    if (lane >= o) x += yv;
-                }
-                loff[lane] = x - cntv;
-                lcnt[lane] = 0;
-                __builtin_amdgcn_wave_barrier();
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + k * 64 + lane;
+            bool valid = e < ce;
+            uint32_t kk = key[k];
+            uint64_t same = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 6; bit++) {
+                uint64_t m = __ballot((kk >> bit) & 1u);
+                same &= ((kk >> bit) & 1u) ? m : ~m;
             }
+            uint32_t before = (uint32_t)__popcll(same & lt);
+            uint32_t cur = lcnt[kk];
+            rank[k] = cur + before;
+            __builtin_amdgcn_wave_barrier();
+            if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
+            __builtin_amdgcn_wave_barrier();
+        }
+        {
+            uint32_t cntv = lcnt[lane];
+            uint32_t x = cntv;
 #pragma unroll
-            for (int k = 0; k < RA_PER; k++) {
-                uint32_t e = cb + k * 64 + lane;
-                bool valid = e < ce;
-                uint32_t kk = key[k];
-                uint64_t same = __ballot(valid);
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t yv = __shfl_up(x, o, 64);
+                if (lane >= o) x += yv;
+            }
+            loff[lane] = x - cntv;
+            __builtin_amdgcn_wave_barrier();
+        }
 #pragma unroll
-                for (int bit = 0; bit < 6; bit++) {
-                    uint64_t m = __ballot((kk >> bit) & 1u);
-                    same &= ((kk >> bit) & 1u) ? m : ~m;
-                }
-                uint32_t before = (uint32_t)__popcll(same & lt);
-                uint32_t cur = lcnt[kk];
-                if (pass == 1 && valid) {
-                    uint32_t q = loff[kk] + cur + before;
-                    r_s[q] = v[k];
-                    r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
-                    r_x[q] = ra[k].y;
-                    r_cf2[q] = __uint_as_float(ra[k].z);
-                    r_pz[q] = __uint_as_float(ra[k].w);
-                    r_cst[q] = __uint_as_float(rb[k].x);
-                    r_pst[q] = __uint_as_float(rb[k].y);
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
-                __builtin_amdgcn_wave_barrier();
+        for (int k = 0; k < RA_PER; k++) {
+            uint32_t e = cb + k * 64 + lane;
+            if (e < ce) {
+                uint32_t q = loff[key[k]] + rank[k];
+                r_s[q] = v[k];
+                r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
+                r_x[q] = ra[k].y;
+                r_cf2[q] = __uint_as_float(ra[k].z);
+                r_pz[q] = __uint_as_float(ra[k].w);
+                r_cst[q] = __uint_as_float(rb[k].x);
+                r_pst[q] = __uint_as_float(rb[k].y);
             }
         }
+        __builtin_amdgcn_wave_barrier();
         if (inside) {
             uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
             for (uint32_t q = l0; q < l1; q++) {
@@ -1329,7 +1398,7 @@ template <int PROG>
 __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
                                                const float* __restrict__ ck, const PsTri* __restrict__ tris,
                                                const float* __restrict__ vary, int nvs, const float* __restrict__ grads,
-                                               const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt)
+                                               const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt, int clears)
 {
     typedef PsProc<PROG> P;
     if (ps_poisoned(cnt)) return;
@@ -1340,8 +1409,12 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
     int x = sx * 16 + (threadIdx.x & 15);
     int y = sy * 16 + (threadIdx.x >> 4);
     if (x >= V.W || y >= V.H) return;
+    const int64_t crow = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
     int w = V.winner[(int64_t)y * V.W + x];
-    if (w < 0) return;
+    if (w < 0) {   // no fragment colours the pixel: a fused clear of its row, if any, is all it gets
+        if (clears && crow < V.cclear_rows) V.colour[crow * V.colour_scan + x] = V.cclear_value;
+        return;
+    }
     const PsSeg sg = segs[w];
     const PsSpan sp = spans[sg.si & PS_SEG_SPAN_MASK];
     const PsTri tr = tris[sp.tri];
@@ -1392,10 +1465,8 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
         grad[0] = g0.x; grad[1] = g0.y; grad[2] = g0.z; grad[3] = g0.w; grad[4] = g1.x; grad[5] = g1.y;
     }
     uint32_t c;
-    if (P::fp(d, in, r, grad, c)) {
-        int64_t row = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
-        V.colour[row * V.colour_scan + x] = c;
-    }
+    if (P::fp(d, in, r, grad, c)) V.colour[crow * V.colour_scan + x] = c;
+    else if (crow < V.cclear_rows) V.colour[crow * V.colour_scan + x] = V.cclear_value;
     });
 }
 
@@ -1466,8 +1537,8 @@ void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, co
 void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
-    dim3 g(gsz(T, 128)), b(128);
-    size_t lds = (size_t)128 * 3 * nvs * sizeof(float);
+    dim3 g(gsz(T, SU_TRIS)), b(3 * 64);
+    size_t lds = ((size_t)SU_TRIS * 3 * nvs + (size_t)3 * SU_TRIS * 5) * sizeof(float);
 #define SU_CASE(K)                                                                                                  \
     case K:                                                                                                         \
         if (share && ps_shares_varyings(K))                                                                         \
@@ -1554,12 +1625,6 @@ void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* key
     result_in_second = (passes & 1) != 0;
 }
 
-void psk_bounds(hipStream_t s, const uint32_t* keys, const uint32_t* n_dev, int64_t n_keys, uint32_t* start,
-                const PsCounters* c)
-{
-    int g = gsz(n_keys + 1, 4); if (g > 16384) g = 16384;   // four keys (waves) per block
-    hipLaunchKernelGGL(k_bounds, dim3(g), dim3(256), 0, s, keys, n_dev, n_keys, start, c);
-}
 
 size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * RA_WORDS(chunk_w) * 4; }
 
@@ -1598,10 +1663,12 @@ void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, co
 }
 
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
-               const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt)
+               const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f, const PsCounters* cnt,
+               bool clears)
 {
     dim3 g((unsigned)f.n_stiles), b(256);
-#define SH_CASE(K) case K: hipLaunchKernelGGL(k_shade<K>, g, b, 0, s, spans, segs, ck, tris, vary, nvs, grads, draws, f, cnt); break;
+#define SH_CASE(K) case K: hipLaunchKernelGGL(k_shade<K>, g, b, 0, s, spans, segs, ck, tris, vary, nvs, grads, draws, f, cnt, \
+                                              clears ? 1 : 0); break;
     switch (prog) {
     SH_CASE(PSGPU_PROC_DEF01) SH_CASE(PSGPU_PROC_DEF02) SH_CASE(PSGPU_PROC_DEF03) SH_CASE(PSGPU_PROC_DEF04)
     SH_CASE(PSGPU_PROC_LAMBERT) SH_CASE(PSGPU_PROC_BLINN) SH_CASE(PSGPU_PROC_SHADOWED)

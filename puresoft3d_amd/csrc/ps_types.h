@@ -104,6 +104,12 @@ struct PsView {
     float* depth; int depth_scan;   // scan in floats
     int* winner;                    // W*H: segment index of the last colour-writing pass
     float* gbuf;                    // optional W*H*12 floats: N.xyz 1 | wp.xyz 0 | u v 0 0 of the winner
+    // clears that precede the flush's draws, applied by the raster and shade
+    // kernels instead of a fill: the whole depth buffer (W x H, no padding)
+    // to dclear_value when dclear, and memory rows [0, cclear_rows) of the
+    // colour target (W x H) to cclear_value
+    int dclear; float dclear_value;
+    int cclear_rows; uint32_t cclear_value;
 };
 
 // Flush-wide parameters passed by value to kernels.
@@ -113,6 +119,10 @@ struct PsFrame {
     int n_views;
     int64_t n_tiles, n_stiles;
     const PsView* views;
+    // the sorted tile keys and their count, where the raster kernels search
+    // their tile's range (sorts of more than one pass leave no tile starts)
+    const uint32_t* skeys;
+    const uint32_t* n_sorted;
     // optional debug planes (single-view flushes), index y*W+x
     int32_t* dbg_last; int32_t* dbg_rcount; uint32_t* dbg_rhash; int32_t* dbg_pcount; uint32_t* dbg_phash;
 };
@@ -126,11 +136,16 @@ struct PsFill {
 };
 
 // Device-side counters.  Statistics are striped over 64 slots (by block
-// index) so that waves of many blocks do not serialise on one address.
+// index) so that waves of many blocks do not serialise on one address.  The
+// kernels before a flush's raster pass count into `pend`; the raster pass,
+// which runs only once the flush is past every point where it can overflow
+// its scratch, moves `pend` into `stat`.  So `stat` holds the statistics of
+// completed flushes only, and a flush re-run after an overflow counts once.
 enum { PS_ST_CULLED = 0, PS_ST_ZREJ, PS_ST_DROPPED, PS_ST_SPANS, PS_ST_FRAGS, PS_ST_SHADED, PS_ST_SEGS, PS_ST_N = 8 };
 #define PS_STRIPES 64
 struct PsCounters {
     unsigned long long stat[PS_STRIPES][PS_ST_N];
     int overflow;
     int pad[15];
+    unsigned long long pend[PS_STRIPES][PS_ST_N];
 };

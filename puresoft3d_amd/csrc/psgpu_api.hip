@@ -75,10 +75,15 @@ struct Targets {
     }
 };
 
-// pending draws of one view (render targets + viewport) of the next flush
+// pending draws of one view (render targets + viewport) of the next flush,
+// with the clears of its buffers that the flush applies itself
 struct Segment {
     Targets tg;
     std::vector<PsDraw> draws;
+    bool dclear = false;
+    float dclear_value = 0.0f;
+    int cclear_rows = 0;
+    uint32_t cclear_value = 0;
 };
 
 enum CmdType { CMD_FLUSH = 0, CMD_CLEARS = 1, CMD_GRAPH = 3, CMD_COPY = 4, CMD_POST = 5 };
@@ -201,6 +206,9 @@ struct psgpu_ctx {
     DevBuf<uint8_t> d_blob;
     FlushBufs fb;
     bool fixed_scratch = false;   // fb is carved from the static array: never reallocate it
+    // most spans and pieces one flush may hold (span records keep their index
+    // in 26 bits); a flush that needs more is split in two and re-run
+    int64_t max_spans = (int64_t)1 << 26, max_segs = (int64_t)1 << 31;
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
     // per logged command: the overflow flag as it stood after the command ran
@@ -455,6 +463,10 @@ int build_blob(psgpu_ctx* c, const Cmd& cmd, Blob& b, PsFrame& f, int64_t& T, in
         pv.colour_h = sg.tg.colour_h;
         pv.depth = sg.tg.depth; pv.depth_scan = sg.tg.depth_scan;
         pv.gbuf = sg.tg.gbuf;
+        pv.dclear = sg.dclear ? 1 : 0;
+        pv.dclear_value = sg.dclear_value;
+        pv.cclear_rows = sg.cclear_rows;
+        pv.cclear_value = sg.cclear_value;
         pv.winner = (int*)(intptr_t)n_pix;   // offset; rebased once the winner buffer is known
         n_pix += (int64_t)pv.W * pv.H;
         views.push_back(pv);
@@ -528,11 +540,12 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
         (rc = ensure(c, b.grad, (size_t)T * 8)) || (rc = ensure(c, b.nrows, T)) || (rc = ensure(c, b.rowoff, T + 1)))
         return rc;
     if (b.span_cap == 0) {
-        b.span_cap = std::max<int64_t>(1 << 20, 6 * T);
-        b.seg_cap = b.span_cap * 3 / 2;
+        b.span_cap = std::min<int64_t>(std::max<int64_t>(1 << 20, 6 * T), c->max_spans);
+        b.seg_cap = std::min<int64_t>(b.span_cap * 3 / 2, c->max_segs);
     }
-    if (b.span_cap > ((int64_t)1 << 26) || b.seg_cap > ((int64_t)1 << 31))
-        return fail(c, PSGPU_ERR_NOMEM, "more than 2^26 spans or 2^31 pieces in one flush");
+    if (b.span_cap > c->max_spans || b.seg_cap > c->max_segs)
+        return fail(c, PSGPU_ERR_NOMEM, "more than %lld spans or %lld pieces in one flush", (long long)c->max_spans,
+                    (long long)c->max_segs);
     int64_t scan_cap = std::max<int64_t>(T, b.span_cap);
     if ((rc = ensure(c, b.partial, psk_scan_blocks(scan_cap) + 1)) ||
         (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) || (rc = ensure(c, b.spans, b.span_cap)) ||
@@ -627,10 +640,11 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     uint32_t* skeys = second ? c->fb.keys2.p : c->fb.keys.p;
     uint32_t* svals = second ? c->fb.vals2.p : c->fb.vals.p;
-    if (psk_radix_passes(key_bits(f.n_tiles)) > 1) {   // one pass leaves the tile starts already
-        ProfScope p(c, "bounds");
-        psk_bounds(s, skeys, c->d_sizes + 1, f.n_tiles, c->fb.kstart.p, c->d_cnt);
-    }
+    // one pass leaves the tile starts in kstart; after more, each raster wave
+    // searches its tile's range in the sorted keys
+    const uint32_t* tstart = psk_radix_passes(key_bits(f.n_tiles)) > 1 ? nullptr : c->fb.kstart.p;
+    f.skeys = skeys;
+    f.n_sorted = c->d_sizes + 1;
     int imm = 0;   // a flush of one programme that shades in the raster walk
     {
         const PsDraw* hd = (const PsDraw*)blob.bytes.data();
@@ -638,7 +652,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     if (imm) {
         ProfScope p(c, "raster");
-        psk_raster_imm(s, imm, dbg, c->fb.spans.p, c->fb.segs.p, svals, c->fb.kstart.p, c->fb.tris.p, dd, f,
+        psk_raster_imm(s, imm, dbg, c->fb.spans.p, c->fb.segs.p, svals, tstart, c->fb.tris.p, dd, f,
                        c->tiling.waves_per_group, c->d_cnt, c->fb.vary.p, nvs, c->fb.ck.p);
         HIPCHK(c, hipGetLastError());
         c->stats.flushes++;
@@ -646,15 +660,18 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     {
         ProfScope p(c, "raster");
-        psk_raster(s, dbg, c->fb.spans.p, c->fb.segs.p, svals, c->fb.kstart.p, c->fb.tris.p, dd, f,
+        psk_raster(s, dbg, c->fb.spans.p, c->fb.segs.p, svals, tstart, c->fb.tris.p, dd, f,
                    c->tiling.waves_per_group, c->d_cnt);
     }
     {
         ProfScope p(c, "shade");
+        bool clears = true;   // the first shading launch also writes the fused colour clears
         for (int k = 1; k < PS_N_PROGS; k++)
-            if (shade_progs[k])
+            if (shade_progs[k]) {
                 psk_shade(s, k, c->fb.spans.p, c->fb.segs.p, c->fb.ck.p, c->fb.tris.p, c->fb.vary.p, nvs, c->fb.grad.p, dd, f,
-                          c->d_cnt);
+                          c->d_cnt, clears);
+                clears = false;
+            }
     }
     HIPCHK(c, hipGetLastError());
     c->stats.flushes++;
@@ -724,6 +741,75 @@ int run_cmd(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged = nullptr)
 
 int do_sync(psgpu_ctx* c);
 
+// Split the flush at log[idx] into two flushes of about half its triangles
+// each, in submission order (the draw at the midpoint is cut into two draws
+// of the same programme and state, its vertex streams offset by the first
+// part's triangles).  Every pixel still sees its fragments in submission
+// order, so the result is unchanged.  False when there is nothing to split.
+bool split_flush(psgpu_ctx* c, size_t idx)
+{
+    if (idx >= c->log.size() || c->log[idx].type != CMD_FLUSH) return false;
+    const Cmd& cmd = c->log[idx];
+    int64_t T = 0;
+    for (const Segment& sg : cmd.views)
+        for (const PsDraw& d : sg.draws) T += d.tri_count;
+    if (T < 2) return false;
+    const int64_t half = T / 2;
+    int64_t acc = 0;
+    Cmd a, b;
+    a.type = b.type = CMD_FLUSH;
+    // clears the flush applied itself become a clear command before both parts
+    Cmd clears;
+    clears.type = CMD_CLEARS;
+    for (const Segment& sg : cmd.views) {
+        PsFill fl;
+        memset(&fl, 0, sizeof(fl));
+        if (sg.dclear) {
+            fl.ptr = (uint32_t*)sg.tg.depth;
+            fl.rows = 1;
+            fl.width = (int64_t)sg.tg.depth_scan * sg.tg.depth_h;
+            memcpy(&fl.value, &sg.dclear_value, 4);
+            clears.fills.push_back(fl);
+        }
+        if (sg.cclear_rows > 0) {
+            fl.ptr = sg.tg.colour;
+            fl.rows = sg.cclear_rows;
+            fl.width = sg.tg.colour_w;
+            fl.scan = sg.tg.colour_scan;
+            fl.value = sg.cclear_value;
+            clears.fills.push_back(fl);
+        }
+    }
+    for (const Segment& sg : cmd.views) {
+        Segment sa, sb;
+        sa.tg = sb.tg = sg.tg;
+        for (const PsDraw& d : sg.draws) {
+            if (acc + d.tri_count <= half) {
+                sa.draws.push_back(d);
+            } else if (acc >= half) {
+                sb.draws.push_back(d);
+            } else {
+                const int64_t k = half - acc;
+                PsDraw d1 = d, d2 = d;
+                d1.tri_count = k;
+                d2.tri_count = d.tri_count - k;
+                for (int s = 0; s < 5; s++)
+                    if (d2.slot[s]) d2.slot[s] += 3 * k * (int64_t)d2.unit[s];
+                d2.id_base += k;
+                sa.draws.push_back(d1);
+                sb.draws.push_back(d2);
+            }
+            acc += d.tri_count;
+        }
+        if (!sa.draws.empty()) a.views.push_back(std::move(sa));
+        if (!sb.draws.empty()) b.views.push_back(std::move(sb));
+    }
+    c->log[idx] = std::move(a);
+    c->log.insert(c->log.begin() + idx + 1, std::move(b));
+    if (!clears.fills.empty()) c->log.insert(c->log.begin() + idx, std::move(clears));
+    return true;
+}
+
 // remember whether the overflow flag was already set when command i finished:
 // the first command that finished with it set is where a replay starts
 int mark_done(psgpu_ctx* c, int i)
@@ -749,10 +835,65 @@ int submit(psgpu_ctx* c, Cmd&& cmd)
     return mark_done(c, (int)c->log.size() - 1);
 }
 
+// Clears that a flush can apply itself: a clear of a view's whole depth
+// buffer (when the buffer is exactly the viewport, without padding), taken
+// into the raster kernel's depth tiles, and a clear of its colour target's
+// rows (when the target is exactly the viewport), written by the shading
+// kernel to the pixels no fragment colours.  Only for flushes that shade
+// deferred (the raster walk that blends reads the colour beneath), and only
+// when no other pending clear touches the buffer.  The fills taken are
+// removed from `fills`.
+void fuse_clears(std::vector<PsFill>& fills, std::vector<Segment>& views, bool shades)
+{
+    if (fills.empty()) return;
+    std::vector<bool> taken(fills.size(), false);
+    auto only = [&](const void* p) {
+        int found = -1;
+        for (size_t i = 0; i < fills.size(); i++)
+            if ((const void*)fills[i].ptr == p) {
+                if (found >= 0) return -1;
+                found = (int)i;
+            }
+        return found;
+    };
+    for (Segment& sg : views) {
+        const Targets& tg = sg.tg;
+        int i = only(tg.depth);
+        if (i >= 0 && tg.depth_scan == tg.vpw && tg.depth_w == tg.vpw && tg.depth_h == tg.vph &&
+            fills[i].rows == 1 && fills[i].width == (int64_t)tg.depth_scan * tg.depth_h) {
+            sg.dclear = true;
+            memcpy(&sg.dclear_value, &fills[i].value, 4);
+            taken[i] = true;
+        }
+        i = only(tg.colour);
+        if (shades && i >= 0 && tg.colour_w == tg.vpw && tg.colour_h == tg.vph && fills[i].width == tg.colour_w &&
+            fills[i].scan == tg.colour_scan && fills[i].rows <= tg.colour_h && fills[i].rows > 0) {
+            sg.cclear_rows = (int)fills[i].rows;
+            sg.cclear_value = fills[i].value;
+            taken[i] = true;
+        }
+    }
+    size_t k = 0;
+    for (size_t i = 0; i < fills.size(); i++)
+        if (!taken[i]) fills[k++] = fills[i];
+    fills.resize(k);
+}
+
+// whether a flush of these views shades in the deferred kernel
+bool flush_shades(const std::vector<Segment>& views, int imm)
+{
+    if (imm) return false;
+    for (const Segment& sg : views)
+        for (const PsDraw& d : sg.draws)
+            if (d.writes_colour) return true;
+    return false;
+}
+
 // pending clears first, then the pending draws of every view as one flush
 int queue_to_log(psgpu_ctx* c)
 {
     int rc;
+    if (!c->queue.empty() && !c->queue_imm) fuse_clears(c->pending_fills, c->queue, flush_shades(c->queue, 0));
     if (!c->pending_fills.empty()) {
         Cmd cmd;
         cmd.type = CMD_CLEARS;
@@ -801,7 +942,7 @@ int do_sync(psgpu_ctx* c)
     int rc = queue_to_log(c);
     if (rc) return rc;
     const int64_t span_cap0 = c->fb.span_cap, seg_cap0 = c->fb.seg_cap;
-    for (int attempt = 0; attempt < 16; attempt++) {
+    for (int attempt = 0; attempt < 64; attempt++) {
         HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(PsCounters), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->h_sizes, c->d_sizes, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -828,23 +969,32 @@ int do_sync(psgpu_ctx* c)
             HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
             return PSGPU_OK;
         }
-        // grow and replay the log
+        // grow and replay the log; a flush that needs more than one flush may
+        // hold is split in two (and again, if a half still does)
         int64_t need_spans = (int64_t)c->h_sizes[2], need_segs = (int64_t)c->h_sizes[3];
-        if (need_spans > c->fb.span_cap) {
-            c->fb.span_cap = need_spans + need_spans / 4 + 1024;
-            c->fb.seg_cap = std::max<int64_t>(c->fb.seg_cap, c->fb.span_cap * 3 / 2);
-        } else if (need_segs > c->fb.seg_cap) {
-            c->fb.seg_cap = need_segs + need_segs / 4 + 1024;
-        } else {
-            c->fb.span_cap *= 2;
-            c->fb.seg_cap *= 2;
-        }
-        c->stats.retries++;
-        // commands that finished before the overflow are complete; replay the rest
         size_t first = 0;
         while (first < c->log.size() && !c->h_flags[first]) first++;
+        bool split = false;
+        if (need_spans > c->max_spans || need_segs > c->max_segs) {
+            if (c->log.size() + 2 >= (size_t)psgpu_ctx::kFlagCap || !split_flush(c, first)) break;
+            split = true;
+            c->stats.splits++;
+        }
+        if (need_spans > c->fb.span_cap) {
+            c->fb.span_cap = std::min<int64_t>(need_spans + need_spans / 4 + 1024, c->max_spans);
+            c->fb.seg_cap = std::min<int64_t>(std::max<int64_t>(c->fb.seg_cap, c->fb.span_cap * 3 / 2), c->max_segs);
+        } else if (need_segs > c->fb.seg_cap) {
+            c->fb.seg_cap = std::min<int64_t>(need_segs + need_segs / 4 + 1024, c->max_segs);
+        } else if (!split) {
+            c->fb.span_cap = std::min<int64_t>(c->fb.span_cap * 2, c->max_spans);
+            c->fb.seg_cap = std::min<int64_t>(c->fb.seg_cap * 2, c->max_segs);
+        }
+        c->stats.retries++;
+        // commands that finished before the overflow are complete (their
+        // statistics are committed and stay); replay the rest
         c->log.erase(c->log.begin(), c->log.begin() + first);
-        HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(PsCounters), c->stream));
+        HIPCHK(c, hipMemsetAsync(&c->d_cnt->overflow, 0, sizeof(int), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_cnt->pend, 0, sizeof(c->d_cnt->pend), c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
         for (size_t i = 0; i < c->log.size(); i++) {
             if ((rc = run_cmd(c, c->log[i]))) return rc;
@@ -905,6 +1055,14 @@ int graph_rebuild(psgpu_ctx* c, Graph& g)
     return graph_capture(c, g);
 }
 
+// a recorded flush that had to be split cannot be replayed as captured
+int graph_too_large(psgpu_ctx* c)
+{
+    return fail(c, PSGPU_ERR_NOMEM, "a flush of the recorded command list needs more than %lld spans or %lld pieces; "
+                "record it with fewer triangles per flush (psgpu_flush between draws)", (long long)c->max_spans,
+                (long long)c->max_segs);
+}
+
 // A graph whose capture is stale is first run once eagerly: that sizes the
 // scratch buffers for the current tiling and geometry (overflow replays
 // included).  The eager run counts as the first of the launches asked for.
@@ -912,6 +1070,7 @@ int graph_prepare(psgpu_ctx* c, Graph& g)
 {
     int rc = do_sync(c);
     if (rc) return rc;
+    const int64_t splits = c->stats.splits;
     for (const Cmd& cmd : g.cmds) {
         Cmd copy = cmd;
         c->log.push_back(std::move(copy));
@@ -919,6 +1078,7 @@ int graph_prepare(psgpu_ctx* c, Graph& g)
         if ((rc = mark_done(c, (int)c->log.size() - 1))) return rc;
     }
     if ((rc = do_sync(c))) return rc;
+    if (c->stats.splits != splits) return graph_too_large(c);
     return graph_rebuild(c, g);
 }
 
@@ -1085,6 +1245,10 @@ int psgpu_create(int device, int width, int height, unsigned flags, psgpu_ctx** 
     if (flags & PSGPU_CREATE_TINY_SCRATCH) {   // start tiny: exercises growth and log replay
         c->fb.span_cap = 256;
         c->fb.seg_cap = 256;
+    }
+    if (flags & PSGPU_CREATE_SMALL_FLUSHES) {  // a low ceiling per flush: exercises the splitting of flushes
+        c->max_spans = 4096;
+        c->max_segs = 6144;
     }
     c->uniforms.resize(kMaxUniforms);
     memset(&c->stats, 0, sizeof(c->stats));
@@ -1742,6 +1906,29 @@ int psgpu_read_depth(psgpu_ctx* c, void* host, size_t bytes)
     return PSGPU_OK;
 }
 
+// the reference's display and default depth buffer are host memory that a
+// post-processor may write (post.cpp:3-18, test2/testpost.cpp:9-44): the
+// host copies are written back through these
+int psgpu_write_colour(psgpu_ctx* c, const void* host, size_t bytes)
+{
+    if (need_ctx(c) || !host) return PSGPU_ERR_INVALID;
+    if (bytes != (size_t)c->dev_w * c->dev_h * 4) return fail(c, PSGPU_ERR_INVALID, "display is %d x %d BGRA8", c->dev_w, c->dev_h);
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(c->display, host, bytes, hipMemcpyHostToDevice));
+    return PSGPU_OK;
+}
+
+int psgpu_write_depth(psgpu_ctx* c, const void* host, size_t bytes)
+{
+    if (need_ctx(c) || !host) return PSGPU_ERR_INVALID;
+    if (bytes != (size_t)c->depth_scan * c->dev_h * 4) return fail(c, PSGPU_ERR_INVALID, "depth is %d x %d floats", c->depth_scan, c->dev_h);
+    int rc = do_sync(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(c->depth, host, bytes, hipMemcpyHostToDevice));
+    return PSGPU_OK;
+}
+
 // ---- asynchronous readback and fences --------------------------------------
 int psgpu_host_alloc(psgpu_ctx* c, size_t bytes, void** ptr)
 {
@@ -2036,6 +2223,7 @@ int psgpu_capture_end(psgpu_ctx* c, int* out_graph)
     g.alive = true;
     g.cmds.swap(c->capture_list);
     // first execution runs eagerly: it sizes every scratch buffer
+    const int64_t splits = c->stats.splits;
     for (const Cmd& cmd : g.cmds) {
         Cmd copy = cmd;
         c->log.push_back(std::move(copy));
@@ -2043,6 +2231,7 @@ int psgpu_capture_end(psgpu_ctx* c, int* out_graph)
         if ((rc = mark_done(c, (int)c->log.size() - 1))) return rc;
     }
     if ((rc = do_sync(c))) return rc;
+    if (c->stats.splits != splits) return graph_too_large(c);
     for (const Cmd& cmd : g.cmds) {
         uint8_t* p = nullptr;
         Blob b;
@@ -2129,6 +2318,7 @@ int psgpu_sync_noalloc(psgpu_ctx* c)
     clears.fills = c->pending_fills;
     flush.type = CMD_FLUSH;
     flush.views = c->queue;
+    if (!c->queue_imm) fuse_clears(clears.fills, flush.views, flush_shades(flush.views, 0));
     std::lock_guard<std::mutex> guard(g_static_lock);
     Arena arena{base, PSGPU_STATIC_SCRATCH_BYTES};
     uint8_t* fill_dev = nullptr;

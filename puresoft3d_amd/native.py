@@ -46,7 +46,7 @@ class Stats(C.Structure):
     _fields_ = [("tris_in", C.c_int64), ("tris_culled", C.c_int64), ("tris_zrejected", C.c_int64),
                 ("tris_dropped", C.c_int64), ("spans", C.c_int64), ("frags_rasterised", C.c_int64),
                 ("frags_shaded", C.c_int64), ("segments", C.c_int64), ("flushes", C.c_int64),
-                ("retries", C.c_int64)]
+                ("retries", C.c_int64), ("splits", C.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -107,6 +107,8 @@ _SIGS = {
     "psgpu_sync_noalloc": ([C.c_void_p], C.c_int),
     "psgpu_read_colour": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "psgpu_read_depth": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "psgpu_write_colour": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "psgpu_write_depth": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "psgpu_display_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "psgpu_depth_device_ptr": ([C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)], C.c_int),
     "psgpu_get_stats": ([C.c_void_p, C.POINTER(Stats)], C.c_int),

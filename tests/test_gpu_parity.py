@@ -653,3 +653,37 @@ def test_view_size_limits():
         ctx.viewport(64, 48)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("tiny", [False, True])
+def test_flushes_beyond_the_span_limit_are_split(oracle, tiny):
+    """a flush that needs more spans than one flush may hold is split into
+    flushes of about half its triangles (cutting a draw where needed) and
+    re-run in order, with the same colour, depth and fragment planes
+    (PSGPU_CREATE_SMALL_FLUSHES lowers the limit from 2^26 to 4,096 spans)"""
+    sc = S.terrain_scene(width=640, height=360, cells=60, n_spheres=6, tex_size=128)
+    announce("split flushes %s tiny=%s" % (sc.name, tiny))
+    f, ost = oracle.render_scene(sc, debug=True)
+    assert ost.spans > 3 * 4096
+    ctx = N.Context(sc.width, sc.height, flags=2 | (1 if tiny else 0))
+    try:
+        gs = S.GpuScene(ctx, sc)
+        ctx.debug_enable(True)
+        ctx.debug_reset()
+        gs.frame()
+        col, dep = ctx.read_colour(), ctx.read_depth()
+        dbg = ctx.debug_read(sc.width, sc.height)
+        st = ctx.stats()
+        assert st["splits"] >= 2
+        np.testing.assert_array_equal(col, f.colour)
+        np.testing.assert_array_equal(dep.view(np.uint32), f.depth.view(np.uint32))
+        for k in ("last_pass", "pass_count", "pass_hash", "raster_count", "raster_hash"):
+            np.testing.assert_array_equal(dbg[k], getattr(f, k))
+        assert st["frags_shaded"] == ost.frags_shaded and st["frags_rasterised"] == ost.frags_rasterised
+        # a recorded command list whose flush would need splitting is refused
+        ctx.capture_begin()
+        gs.frame()
+        with pytest.raises(MemoryError):
+            ctx.capture_end()
+    finally:
+        ctx.close()
