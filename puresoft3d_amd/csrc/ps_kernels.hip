@@ -201,84 +201,49 @@ __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draw
     return d;
 }
 
-// A block of 3 waves sets up 64 triangles: wave v runs the vertex twin for
-// vertex v of every triangle (with the perspective division), so the three
-// vertices of a triangle are processed in parallel, and leaves the clip
-// position and varyings in LDS; wave 0 then does the per-triangle work
-// (cull, z reject, pushVertex/pushTriangle, gradients) from LDS; finally the
-// three waves store the varyings of the triangles that need them, one
-// triangle per store instruction.
-#define SU_TRIS 64
 template <int PROG, bool SHARE>
-__global__ void __launch_bounds__(192) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
-                                             float* __restrict__ vary, int nvs, float* __restrict__ grads,
-                                             uint32_t* __restrict__ nrows, PsCounters* cnt,
-                                             const PsView* __restrict__ views)
+__global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
+                                               float* __restrict__ vary, int nvs, float* __restrict__ grads,
+                                               uint32_t* __restrict__ nrows, PsCounters* cnt,
+                                               const PsView* __restrict__ views)
 {
     typedef PsProc<PROG> P;
-    // LDS: the varyings of the block's triangles, 3*nvs floats each, then per
-    // vertex the divided position (all four lanes: w/w is 1 only for finite,
-    // nonzero w, and the back-face test reads it) and 1/w, and the mask of
-    // triangles whose varyings are stored
-    extern __shared__ float su[];
-    const int lane = threadIdx.x & 63, v = threadIdx.x >> 6, n3 = 3 * nvs;
-    float* spos = su + (size_t)SU_TRIS * n3;
-    float* srw = spos + 3 * SU_TRIS * 4;
-    __shared__ unsigned long long s_stage;
-    const int64_t t0 = blockIdx.x * (int64_t)SU_TRIS;
-    const int64_t t = t0 + lane;
-    const bool live = t < T;
-    const int di = ps_find_draw_wave(draws, nd, live ? t : T - 1);
-    bool mine = false;   // the triangle belongs to this programme
-    bool skip = false;   // positions only: another draw of the flush writes its varyings
-    ps_with_draw(draws, di, [&](const PsDraw& d) {
-        mine = live && d.prog == PROG;
-        if (!mine) return;
-        skip = SHARE && ps_shares_varyings(PROG) && d.vary_skip;
-        float pos[4];
-        float u[P::NV > 0 ? P::NV : 1];
-        const int64_t lt = t - d.tri_start;
-        if (skip) ps_vp_pos(d, lt * 3 + v, pos, u, ps_grad_iu<P>());
-        else P::vp(d, lt * 3 + v, pos, u);
-        // perspective division (vertthrd.cpp:37-47)
-        float r = 1.0f / pos[3];
-        pos[0] *= r; pos[1] *= r; pos[2] *= r; pos[3] *= r;
-        float* q = spos + (v * SU_TRIS + lane) * 4;
-        q[0] = pos[0]; q[1] = pos[1]; q[2] = pos[2]; q[3] = pos[3];
-        srw[v * SU_TRIS + lane] = r;
-        float* uo = su + lane * n3 + v * nvs;
-        if (skip) {
-            if constexpr (P::GRAD) { uo[P::IU] = u[P::IU]; uo[P::IU + 1] = u[P::IU + 1]; }
-        } else {
-#pragma unroll
-            for (int k = 0; k < P::NV; k++) uo[k] = u[k];
-        }
-    });
-    __syncthreads();
+    extern __shared__ float su[];   // per wave: the varyings of its 64 triangles, 3*nvs floats each
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool live = t < T;
+    int di = ps_find_draw_wave(draws, nd, live ? t : T - 1);
     unsigned culled = 0, zrej = 0, dropped = 0;
-    if (v == 0) {
-        bool stage = false;
-        ps_with_draw(draws, di, [&](const PsDraw& d) {
-        if (!mine) return;
+    const int lane = threadIdx.x & 63, n3 = 3 * nvs;
+    float* wl = su + (size_t)(threadIdx.x & ~63) * n3;
+    bool stage = false;
+    ps_with_draw(draws, di, [&](const PsDraw& d) {
+    if (live && d.prog == PROG) {
         const int W = views[d.view].W, H = views[d.view].H;
-        float pos[3][4], rw[3], z[3];
+        int64_t lt = t - d.tri_start;
+        float pos[3][4];
+        float u[3][P::NV > 0 ? P::NV : 1];
+        float rw[3], z[3];
+        bool pieces = false;
+        const bool skip = SHARE && ps_shares_varyings(PROG) && d.vary_skip;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const float* q = spos + (k * SU_TRIS + lane) * 4;
-            pos[k][0] = q[0]; pos[k][1] = q[1]; pos[k][2] = q[2]; pos[k][3] = q[3];
-            rw[k] = srw[k * SU_TRIS + lane];
-            z[k] = q[2];
+        for (int v = 0; v < 3; v++) {
+            if (skip) ps_vp_pos(d, lt * 3 + v, pos[v], u[v], ps_grad_iu<P>());
+            else P::vp(d, lt * 3 + v, pos[v], u[v]);
+            // perspective division (vertthrd.cpp:37-47)
+            float r = 1.0f / pos[v][3];
+            pos[v][0] *= r; pos[v][1] *= r; pos[v][2] *= r; pos[v][3] *= r;
+            z[v] = pos[v][2];
+            rw[v] = r;
         }
         uint32_t n = 0;
         PsTri tr;
         bool keep = true;
-        bool pieces = false;
         if ((d.behavior & PSGPU_FACE_CULLING) && ps_back_face(pos[0], pos[1], pos[2])) { keep = false; culled = 1; }
         if (keep) {
             // drawvao.cpp:51-56
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-                if (pos[k][2] < -1.0f || pos[k][2] > 1.0f) keep = false;
+            for (int v = 0; v < 3; v++)
+                if (pos[v][2] < -1.0f || pos[v][2] > 1.0f) keep = false;
             if (!keep) zrej = 1;
         }
         if (keep) {
@@ -287,18 +252,18 @@ __global__ void __launch_bounds__(192) k_setup(const PsDraw* __restrict__ draws,
             float vx[3], vy[3];
             bool bad = false;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                vy[k] = ((float)hh * pos[k][1]) + (float)hh;
-                vx[k] = ((float)hw * pos[k][0]) + (float)hw;
+            for (int v = 0; v < 3; v++) {
+                vy[v] = ((float)hh * pos[v][1]) + (float)hh;
+                vx[v] = ((float)hw * pos[v][0]) + (float)hw;
                 // rows outside the int range make the reference read rows it
                 // never wrote (or crash); the triangle is dropped
-                if (!(vy[k] >= -2147483648.0f && vy[k] < 2147483648.0f)) bad = true;
+                if (!(vy[v] >= -2147483648.0f && vy[v] < 2147483648.0f)) bad = true;
             }
             int first = ps_cvt_i32(vy[0]), last = first;
 #pragma unroll
-            for (int k = 1; k < 3; k++) {
-                if (vy[k] > (float)last) last = ps_cvt_i32(vy[k]);
-                else if (vy[k] < (float)first) first = ps_cvt_i32(vy[k]);
+            for (int v = 1; v < 3; v++) {
+                if (vy[v] > (float)last) last = ps_cvt_i32(vy[v]);
+                else if (vy[v] < (float)first) first = ps_cvt_i32(vy[v]);
             }
             if (first >= H || last < 0) { first = 0; last = -1; }
             if (first < 0) first = 0;
@@ -326,7 +291,7 @@ __global__ void __launch_bounds__(192) k_setup(const PsDraw* __restrict__ draws,
                     split = y3 > (float)(H - 1) ? H - 1 : ps_cvt_i32(y3);
                 }
 #pragma unroll
-                for (int k = 0; k < 3; k++) { tr.vx[k] = vx[k]; tr.vy[k] = vy[k]; tr.rw[k] = rw[k]; tr.z[k] = z[k]; }
+                for (int v = 0; v < 3; v++) { tr.vx[v] = vx[v]; tr.vy[v] = vy[v]; tr.rw[v] = rw[v]; tr.z[v] = z[v]; }
                 tr.first = first;
                 tr.nrows = last - first + 1;
                 tr.split = split;
@@ -337,10 +302,15 @@ __global__ void __launch_bounds__(192) k_setup(const PsDraw* __restrict__ draws,
                 // extrapolate, so no x range test is safe here)
                 pieces = n > 0;
                 if (pieces) tris[t] = tr;
+                if (!SHARE && pieces) {   // staged: the wave stores whole triangles below
+#pragma unroll
+                    for (int v = 0; v < 3; v++)
+#pragma unroll
+                        for (int k = 0; k < P::NV; k++) wl[lane * n3 + v * nvs + k] = u[v][k];
+                }
                 if constexpr (P::GRAD) if (pieces) {   // screen gradients for the mip level, once per triangle
                     float g[6];
-                    const float* ub = su + lane * n3;
-                    ps_tri_grad(tr, ub, ub + nvs, ub + 2 * nvs, P::IU, g);
+                    ps_tri_grad(tr, u[0], u[1], u[2], P::IU, g);
                     float4* go = (float4*)(grads + t * 8);
                     go[0] = make_float4(g[0], g[1], g[2], g[3]);
                     go[1] = make_float4(g[4], g[5], 0.0f, 0.0f);
@@ -350,31 +320,34 @@ __global__ void __launch_bounds__(192) k_setup(const PsDraw* __restrict__ draws,
         nrows[t] = n;
         // varyings of triangles with rows, or of all when other views read them
         stage = SHARE ? !skip && (pieces || d.vary_all) : pieces;
-        });
-        uint64_t m = __ballot(stage);
-        if (lane == 0) s_stage = m;
+        if (SHARE && stage) {
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+#pragma unroll
+                for (int k = 0; k < P::NV; k++) wl[lane * n3 + v * nvs + k] = u[v][k];
+        }
     }
-    __syncthreads();
-    // varyings of the staged triangles, one triangle per store instruction
-    // (3*nvs consecutive floats), the waves taking every third triangle;
-    // floats past NV of each vertex are left as they are
+    });
+    // varyings of the wave's staged triangles, one triangle per store
+    // instruction (3*nvs consecutive floats) instead of one 3*nvs-float stride
+    // per lane; floats past NV of each vertex are left as they are
+    uint64_t m = __ballot(stage);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     {
-        uint64_t m = s_stage;
         int vv = lane / (nvs > 0 ? nvs : 1), kk = lane - vv * nvs;
-        bool part = vv < 3 && kk < P::NV;
-        int seen = 0;
+        bool mine = vv < 3 && kk < P::NV;
+        int64_t tw = t - lane;
         while (m) {
             int tl = __builtin_ctzll(m);
             m &= m - 1;
-            if (seen++ % 3 != v) continue;
-            if (part) vary[(t0 + tl) * n3 + lane] = su[tl * n3 + lane];
+            if (mine) vary[(tw + tl) * n3 + lane] = wl[tl * n3 + lane];
         }
     }
-    if (v == 0) {
-        ps_wave_add(cnt, PS_ST_CULLED, culled, true);
-        ps_wave_add(cnt, PS_ST_ZREJ, zrej, true);
-        ps_wave_add(cnt, PS_ST_DROPPED, dropped, true);
-    }
+    ps_wave_add(cnt, PS_ST_CULLED, culled, true);
+    ps_wave_add(cnt, PS_ST_ZREJ, zrej, true);
+    ps_wave_add(cnt, PS_ST_DROPPED, dropped, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1537,8 +1510,8 @@ void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, co
 void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
 {
-    dim3 g(gsz(T, SU_TRIS)), b(3 * 64);
-    size_t lds = ((size_t)SU_TRIS * 3 * nvs + (size_t)3 * SU_TRIS * 5) * sizeof(float);
+    dim3 g(gsz(T, 128)), b(128);
+    size_t lds = (size_t)128 * 3 * nvs * sizeof(float);
 #define SU_CASE(K)                                                                                                  \
     case K:                                                                                                         \
         if (share && ps_shares_varyings(K))                                                                         \
