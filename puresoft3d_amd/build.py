@@ -1,6 +1,6 @@
 """Build recipe of the native libraries (hipcc for gfx950, in-tree outputs).
 
-  python -m puresoft3d_amd.build        # libpsgpu.so, libpuresoft3d.so, host/psrender
+  python -m puresoft3d_amd.build        # libpsgpu.so, libpuresoft3d.so, host/psrender, host/psdemo
 """
 import os
 import subprocess
@@ -53,16 +53,20 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 
 def build_facade():
     """C++ host library over libpsgpu (the PuresoftPipeline facade and the
-    serving loop), and the psrender program that drives the facade."""
+    serving loop), the psrender program that drives the facade, and psdemo,
+    the reference demos' resource idioms through it."""
     lib = os.path.join(HERE, "libpuresoft3d.so")
     run(["g++"] + CXX_FLAGS + ["-shared", "-o", lib + ".tmp", os.path.join(HOST, "puresoft3d.cpp"),
                                os.path.join(HOST, "psserve.cpp"), "-L" + HERE, "-lpsgpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(lib + ".tmp", lib)
-    exe = os.path.join(HOST, "psrender")
-    run(["g++"] + CXX_FLAGS + ["-o", exe + ".tmp", os.path.join(HOST, "psrender.cpp"),
-                               "-L" + HERE, "-lpuresoft3d", "-lpsgpu", "-Wl,-rpath,$ORIGIN/.."])
-    os.replace(exe + ".tmp", exe)
-    return lib, exe
+    exes = []
+    for prog in ("psrender", "psdemo"):
+        exe = os.path.join(HOST, prog)
+        run(["g++"] + CXX_FLAGS + ["-o", exe + ".tmp", os.path.join(HOST, prog + ".cpp"),
+                                   "-L" + HERE, "-lpuresoft3d", "-lpsgpu", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(exe + ".tmp", exe)
+        exes.append(exe)
+    return lib, exes
 
 
 def main():

@@ -8,6 +8,28 @@
 
 #include <new>
 #include <string>
+#include <thread>
+
+PuresoftFBO::PuresoftFBO(int width, int scanline, int height, int elemLen, bool topDown, void* externalBuffer,
+                         WRAPMODE wrapMode, int extraLayers)
+    : m_width(width), m_scanline(scanline), m_height(height), m_elemLen(elemLen), m_topDown(topDown),
+      m_wrap(wrapMode), m_extraLayers(extraLayers), m_buffer(nullptr)
+{
+    // fbo.cpp:11-74
+    if (width < 1 || height < 1 || scanline < width * elemLen || (elemLen != 1 && elemLen != 4))
+        throw std::invalid_argument("PuresoftFBO::PuresoftFBO");
+    setBuffer(externalBuffer);
+}
+
+void PuresoftFBO::setBuffer(void* externalBuffer)
+{
+    if (externalBuffer) {
+        m_buffer = externalBuffer;
+        return;
+    }
+    m_own.assign(getBytes(), 0);
+    m_buffer = m_own.data();
+}
 
 PuresoftVBO::PuresoftVBO(size_t unitBytes, size_t unitCount)
     : m_unitBytes(unitBytes), m_unitCount(unitCount), m_host(unitBytes * unitCount)
@@ -47,6 +69,11 @@ PuresoftPipeline::PuresoftPipeline(uintptr_t, int deviceWidth, int deviceHeight,
     }
 }
 
+struct PsPostCall {
+    PuresoftPostProcessor* proc;
+    bool host = false;   // the processor has the reference's form only
+};
+
 PuresoftPipeline::~PuresoftPipeline()
 {
     // the pipeline owns processors and attached VBOs (pipeline.cpp:81-87, 183-205)
@@ -66,23 +93,58 @@ int PuresoftPipeline::createTexture(const PURESOFTIMGBUFF32* image, int extraLay
     return idx;
 }
 
-void PuresoftPipeline::destroyTexture(int idx) { check(psgpu_texture_destroy(m_ctx, idx)); }
+void PuresoftPipeline::destroyTexture(int idx)
+{
+    check(psgpu_texture_destroy(m_ctx, idx));
+    for (int l = 0; l < PuresoftFBO::LAYER_MAX; l++) {
+        m_mirrors.erase({idx, l});
+        m_dirty.erase({idx, l});
+    }
+}
 
 void PuresoftPipeline::getTexture(int idx, PURESOFTIMGBUFF32* image, PuresoftFBO::LAYER layer)
 {
     int w, s, h, e;
     check(psgpu_texture_info(m_ctx, idx, 0, &w, &s, &h, &e));
-    if (image->pixels) check(psgpu_texture_download(m_ctx, idx, (int)layer, 0, image->pixels, (size_t)s * h));
+    if ((int)layer < 0 || (int)layer >= PuresoftFBO::LAYER_MAX) throw std::out_of_range("PuresoftPipeline::getTexture");
+    // the callers write texels through the pointer (test/scenobj.cpp:178-180,
+    // 203-215; test2/loadscene.cpp:363): hand out a mirror holding the layer's
+    // current texels, uploaded again before the GPU next uses textures
+    std::vector<unsigned char>& m = m_mirrors[{idx, (int)layer}];
+    m.resize((size_t)s * h);
+    if (!m_dirty.count({idx, (int)layer})) check(psgpu_texture_download(m_ctx, idx, (int)layer, 0, m.data(), m.size()));
+    m_dirty.insert({idx, (int)layer});
     image->width = w;
     image->scanline = s;
     image->height = h;
     image->elemLen = e;
+    image->pixels = m.data();
+}
+
+void PuresoftPipeline::flushMirrors(void)
+{
+    std::set<int> mipped;
+    for (const auto& k : m_dirty) {
+        std::vector<unsigned char>& m = m_mirrors[k];
+        check(psgpu_texture_upload(m_ctx, k.first, k.second, 0, m.data(), m.size()));
+        mipped.insert(k.first);
+    }
+    m_dirty.clear();
+    for (int idx : mipped) {   // a mip chain follows its level 0
+        int w, s, h, e;
+        if (psgpu_texture_info(m_ctx, idx, 1, &w, &s, &h, &e) == PSGPU_OK) check(psgpu_texture_generate_mips(m_ctx, idx));
+    }
 }
 
 void PuresoftPipeline::setTexturePixels(int idx, const void* pixels, PuresoftFBO::LAYER layer)
 {
     int w, s, h, e;
     check(psgpu_texture_info(m_ctx, idx, 0, &w, &s, &h, &e));
+    auto it = m_mirrors.find({idx, (int)layer});
+    if (it != m_mirrors.end()) {   // a handed-out mirror follows the texture
+        memcpy(it->second.data(), pixels, it->second.size());
+        m_dirty.erase({idx, (int)layer});
+    }
     check(psgpu_texture_upload(m_ctx, idx, (int)layer, 0, pixels, (size_t)s * h));
 }
 
@@ -191,6 +253,7 @@ void PuresoftPipeline::setRenderer(PuresoftHeadlessRenderer* rndr)
 // ---- debug api (dbg.cpp:3-61) ---------------------------------------------------
 void PuresoftPipeline::grab(int idx, std::vector<unsigned char>& bytes, int& w, int& scan, int& h, int& elem)
 {
+    flushMirrors();
     if (idx == -2) {   // the display, rows top-down as it lies in memory
         w = m_width; h = m_height; elem = 4; scan = w * 4;
         bytes.resize((size_t)scan * h);
@@ -279,7 +342,63 @@ void PuresoftPipeline::getTaskQCounters(unsigned int* taskPushCalled, unsigned i
     // there are no row-task queues here
     *taskPushCalled = *taskPushSpinned = *taskPopCalled = *taskPopSpinned = 0;
 }
-void PuresoftPipeline::setFBO(int idx, int tex) { check(psgpu_target_set(m_ctx, idx, tex)); }
+void PuresoftPipeline::setFBOTexture(int idx, int tex)
+{
+    check(psgpu_target_set(m_ctx, idx, tex));
+    if (idx == 0) m_fbo0 = nullptr;
+}
+
+// the GPU texture standing for a host FBO (created the first time it is bound)
+int PuresoftPipeline::fboTexture(PuresoftFBO* fbo)
+{
+    auto it = m_fboTex.find(fbo);
+    if (it != m_fboTex.end()) return it->second;
+    if (fbo->getElemLen() != 4) throw std::invalid_argument("PuresoftPipeline::setFBO: targets hold 4-byte texels");
+    int tex = -1;
+    check(psgpu_texture_create(m_ctx, fbo->getWidth(), fbo->getScanline(), fbo->getHeight(), 4, 0,
+                               (int)fbo->getWrapMode(), 1, nullptr, &tex));
+    m_fboTex[fbo] = tex;
+    return tex;
+}
+
+// texture rows are bottom-up: a top-down FBO's rows go across mirrored
+void PuresoftPipeline::fboToGpu(PuresoftFBO* fbo, int tex)
+{
+    const size_t scan = fbo->getScanline(), h = fbo->getHeight();
+    const unsigned char* src = (const unsigned char*)fbo->getBuffer();
+    if (!fbo->isTopDown()) {
+        check(psgpu_texture_upload(m_ctx, tex, 0, 0, src, scan * h));
+        return;
+    }
+    std::vector<unsigned char> rows(scan * h);
+    for (size_t y = 0; y < h; y++) memcpy(&rows[y * scan], src + (h - 1 - y) * scan, scan);
+    check(psgpu_texture_upload(m_ctx, tex, 0, 0, rows.data(), rows.size()));
+}
+
+void PuresoftPipeline::fboFromGpu(PuresoftFBO* fbo, int tex)
+{
+    const size_t scan = fbo->getScanline(), h = fbo->getHeight();
+    unsigned char* dst = (unsigned char*)fbo->getBuffer();
+    if (!fbo->isTopDown()) {
+        check(psgpu_texture_download(m_ctx, tex, 0, 0, dst, scan * h));
+        return;
+    }
+    std::vector<unsigned char> rows(scan * h);
+    check(psgpu_texture_download(m_ctx, tex, 0, 0, rows.data(), rows.size()));
+    for (size_t y = 0; y < h; y++) memcpy(dst + (h - 1 - y) * scan, &rows[y * scan], scan);
+}
+
+void PuresoftPipeline::setFBO(int idx, PuresoftFBO* fbo)
+{
+    if (idx < 0 || idx >= 16) throw std::out_of_range("PuresoftPipeline::setFBO");
+    if (!fbo) {   // the display for slot 0, none for the others (pipeline.cpp:266-274)
+        setFBOTexture(idx, PSGPU_DISPLAY);
+        return;
+    }
+    int tex = fboTexture(fbo);
+    check(psgpu_target_set(m_ctx, idx, tex));
+    if (idx == 0) m_fbo0 = fbo;
+}
 
 void PuresoftPipeline::setUniform(int idx, const void* data, size_t len)
 {
@@ -287,25 +406,93 @@ void PuresoftPipeline::setUniform(int idx, const void* data, size_t len)
     check(psgpu_uniform_set(m_ctx, idx, data, len));
 }
 
-void PuresoftPipeline::drawVAO(int vao, bool) { check(psgpu_draw(m_ctx, vao)); }
+void PuresoftPipeline::drawVAO(int vao, bool)
+{
+    flushMirrors();
+    if (!m_fbo0) {
+        check(psgpu_draw(m_ctx, vao));
+        return;
+    }
+    // a host FBO target: the draw sees and leaves its memory, as the
+    // reference's synchronous drawVAO does (drawvao.cpp:122-132)
+    int tex = m_fboTex.at(m_fbo0);
+    fboToGpu(m_fbo0, tex);
+    check(psgpu_draw(m_ctx, vao));
+    fboFromGpu(m_fbo0, tex);
+}
 void PuresoftPipeline::enable(int b) { check(psgpu_behavior_enable(m_ctx, b)); }
 void PuresoftPipeline::disable(int b) { check(psgpu_behavior_disable(m_ctx, b)); }
 void PuresoftPipeline::clearDepth(float f) { check(psgpu_clear_depth(m_ctx, f)); }
 void PuresoftPipeline::clearColour(PURESOFTBGRA c) { check(psgpu_clear_colour(m_ctx, c.i32)); }
-void PuresoftPipeline::finish(void) { check(psgpu_sync(m_ctx)); }
+void PuresoftPipeline::finish(void)
+{
+    flushMirrors();
+    check(psgpu_sync(m_ctx));
+}
 
-void PuresoftPipeline::readDisplay(void* out) { check(psgpu_read_colour(m_ctx, out, (size_t)m_width * m_height * 4)); }
+void PuresoftPipeline::readDisplay(void* out)
+{
+    flushMirrors();
+    check(psgpu_read_colour(m_ctx, out, (size_t)m_width * m_height * 4));
+}
 
 static int post_trampoline(void* user, const psgpu_post_frame* frame, void* stream)
 {
-    return static_cast<PuresoftPostProcessor*>(user)->process(*frame, stream);
+    PsPostCall* pc = static_cast<PsPostCall*>(user);
+    int rc = pc->proc->process(*frame, stream);
+    if (rc == PuresoftPostProcessor::kHostForm) {   // the reference's form: run by the pipeline afterwards
+        pc->host = true;
+        return PSGPU_OK;
+    }
+    return rc;
 }
 
 void PuresoftPipeline::postProcess(PuresoftPostProcessor* processor)
 {
     // post.cpp:3-18: after the draws so far, over the slot-0 target and the depth buffer
     if (!processor) throw std::invalid_argument("postProcess: no processor");
-    check(psgpu_post_process(m_ctx, post_trampoline, processor));
+    finish();   // the draws before it are done: the callback below runs within the call
+    std::unique_ptr<PsPostCall>& pc = m_postCalls[processor];
+    if (!pc) pc.reset(new PsPostCall{processor});
+    pc->host = false;
+    check(psgpu_post_process(m_ctx, post_trampoline, pc.get()));
+    if (pc->host) hostPostProcess(processor);
+}
+
+// the reference's form: the frame and the depth buffer as host FBOs, the
+// processor called from threadCount threads (the reference's workers plus
+// its main thread, post.cpp:3-18), both buffers written back
+void PuresoftPipeline::hostPostProcess(PuresoftPostProcessor* processor)
+{
+    check(psgpu_sync(m_ctx));
+    std::unique_ptr<PuresoftFBO> display, depth;
+    PuresoftFBO* frame = m_fbo0;   // a host FBO target already holds the frame (drawVAO)
+    if (!frame) {
+        display.reset(new PuresoftFBO(m_width, m_width * 4, m_height, 4, true));   // the display, top-down
+        frame = display.get();
+        check(psgpu_read_colour(m_ctx, frame->getBuffer(), frame->getBytes()));
+    }
+    int dscan = (int)(m_width / 4.0f + 0.5f) * 4 * 4;   // pipeline.cpp:31
+    if (m_depthTex < 0) {
+        depth.reset(new PuresoftFBO(m_width, dscan, m_height, 4));
+        check(psgpu_read_depth(m_ctx, depth->getBuffer(), depth->getBytes()));
+    } else {
+        int w, s, h, e;
+        check(psgpu_texture_info(m_ctx, m_depthTex, 0, &w, &s, &h, &e));
+        depth.reset(new PuresoftFBO(w, s, h, e));
+        check(psgpu_texture_download(m_ctx, m_depthTex, 0, 0, depth->getBuffer(), depth->getBytes()));
+    }
+    unsigned hw = std::thread::hardware_concurrency();
+    int n = (int)(hw < 2 ? 2 : (hw > 16 ? 16 : hw));
+    std::vector<std::thread> workers;
+    for (int i = 0; i < n - 1; i++)
+        workers.emplace_back([=, &depth] { processor->process(i, n, frame, depth.get()); });
+    processor->process(n - 1, n, frame, depth.get());
+    for (auto& t : workers) t.join();
+    if (m_fbo0) fboToGpu(frame, m_fboTex.at(m_fbo0));
+    else check(psgpu_write_colour(m_ctx, frame->getBuffer(), frame->getBytes()));
+    if (m_depthTex < 0) check(psgpu_write_depth(m_ctx, depth->getBuffer(), depth->getBytes()));
+    else check(psgpu_texture_upload(m_ctx, m_depthTex, 0, 0, depth->getBuffer(), depth->getBytes()));
 }
 
 void PuresoftPipeline::swapBuffers(void)

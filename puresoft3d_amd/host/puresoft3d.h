@@ -9,8 +9,11 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <map>
 #include <memory>
+#include <set>
 #include <stdexcept>
+#include <utility>
 #include <vector>
 
 #include "../../include/psgpu.h"
@@ -45,11 +48,37 @@ const int BEHAVIOR_TEST_DEPTH = PSGPU_TEST_DEPTH;
 const int BEHAVIOR_FACE_CULLING = PSGPU_FACE_CULLING;
 const int BEHAVIOR_ALPHABLEND = PSGPU_ALPHABLEND;
 
-// ---- fbo.h: the enums the texture API uses ---------------------------------------
-struct PuresoftFBO {
+// ---- fbo.h: a frame buffer in host memory ---------------------------------------
+// The reference's FBO (fbo.h:6-100) with its constructor and buffer access:
+// the frame and depth buffers a host post-processor is handed, and the
+// targets setFBO binds.  Rows are `scanline` bytes apart; a top-down FBO
+// holds display row y at memory row height-1-y (fbo.cpp:103-106).
+class PuresoftFBO {
+public:
     enum WRAPMODE { CLAMP = PSGPU_CLAMP, WRAP = PSGPU_WRAP };
     enum LAYER { LAYER_DEFAULT, LAYER_XPOS = LAYER_DEFAULT, LAYER_XNEG, LAYER_YPOS, LAYER_YNEG, LAYER_ZPOS, LAYER_ZNEG,
                  LAYER_MAX };
+
+    PuresoftFBO(int width, int scanline, int height, int elemLen, bool topDown = false, void* externalBuffer = nullptr,
+                WRAPMODE wrapMode = CLAMP, int extraLayers = 0);
+    void setBuffer(void* externalBuffer = nullptr);   // null: back to the FBO's own memory
+    void* getBuffer(void) { return m_buffer; }
+    int getWidth(void) const { return m_width; }
+    int getHeight(void) const { return m_height; }
+    int getScanline(void) const { return m_scanline; }
+    int getElemLen(void) const { return m_elemLen; }
+    size_t getBytes(void) const { return (size_t)m_scanline * m_height; }
+    bool isTopDown(void) const { return m_topDown; }
+    WRAPMODE getWrapMode(void) const { return m_wrap; }
+    int getExtraLayers(void) const { return m_extraLayers; }
+
+private:
+    int m_width, m_scanline, m_height, m_elemLen;
+    bool m_topDown;
+    WRAPMODE m_wrap;
+    int m_extraLayers;
+    std::vector<unsigned char> m_own;
+    void* m_buffer;
 };
 
 // ---- vbo.h: a vertex stream; contents live on the host until attached ----------
@@ -99,16 +128,23 @@ PS_DECLARE_PROCS(CLOUD, PSGPU_PROC_CLOUD)      // the demo's cloud, blended (tes
 PS_DECLARE_PROCS(CLOUD_SHADOW, PSGPU_PROC_CLOUD_SHADOW) // its shadow pass, discarding thin cloud (:589-686)
 #undef PS_DECLARE_PROCS
 
-// ---- proc.h:89-94: the post-processor, run on the device ---------------------------
-// The reference calls process(threadIndex, threadCount, frame, depth) from its
-// fragment threads over host FBOs (post.cpp:3-18).  Here the frame lives in
-// HBM: process() receives the device addresses of the colour target and depth
-// buffer and the context stream, and enqueues its own kernels or copies there.
-// It may be called again for the same frame when earlier draws are re-run.
+// ---- proc.h:89-94: the post-processor ----------------------------------------------
+// Two forms.  The reference's: process(threadIndex, threadCount, frame, depth)
+// is host code over the frame (the colour target of FBO slot 0) and the depth
+// buffer, called once per thread (post.cpp:3-18).  The pipeline finishes the
+// draws before it, copies both buffers to host FBOs, calls it from
+// threadCount threads and writes both buffers back.  The device form (no
+// counterpart in the reference): process(frame, stream) gets the device
+// addresses of the same buffers and the context stream and enqueues its own
+// kernels or copies there, with no copies through host memory.  A subclass
+// overrides one of the two; the device form's default returns kHostForm,
+// which selects the host form.
 class PuresoftPostProcessor {
 public:
+    enum { kHostForm = 1 };
     virtual ~PuresoftPostProcessor() {}
-    virtual int process(const psgpu_post_frame& frame, void* stream) = 0;   // 0 or a negative status
+    virtual void process(int threadIndex, int threadCount, PuresoftFBO* frame, PuresoftFBO* depth) {}
+    virtual int process(const psgpu_post_frame& frame, void* stream) { return kHostForm; }   // 0 or a negative status
 };
 
 // ---- rndr.h: a headless renderer owning the host copy of the display ------------
@@ -124,6 +160,7 @@ private:
 };
 
 // ---- pipeline.h ------------------------------------------------------------------
+struct PsPostCall;   // a post-processor as the C ABI calls it back
 class PuresoftPipeline {
 public:
     PuresoftPipeline(uintptr_t canvasWindow, int deviceWidth, int deviceHeight, PuresoftHeadlessRenderer* rndr = nullptr,
@@ -134,7 +171,11 @@ public:
     int createTexture(const PURESOFTIMGBUFF32* image, int extraLayers = 0,
                       PuresoftFBO::WRAPMODE mode = PuresoftFBO::CLAMP, int mipLevels = 1);
     void destroyTexture(int idx);
-    // copies the layer into image->pixels when it is non-null, else reports the layout
+    // tex.cpp:30-45: the layer's layout, and in image->pixels a writable
+    // pointer to its texels.  The pointer is to a host mirror of the layer
+    // that the pipeline keeps; what is written through it reaches the GPU
+    // before the next draw, read or post-process (and stays valid until the
+    // texture is destroyed).  Writes made after that need another getTexture.
     void getTexture(int idx, PURESOFTIMGBUFF32* image, PuresoftFBO::LAYER layer = PuresoftFBO::LAYER_DEFAULT);
     void setTexturePixels(int idx, const void* pixels, PuresoftFBO::LAYER layer = PuresoftFBO::LAYER_DEFAULT);
     void generateMipmaps(int idx);
@@ -157,7 +198,15 @@ public:
     void setRenderer(PuresoftHeadlessRenderer* rndr);   // not owned; null: a renderer of the device size
     void setViewport(int width, int height, uintptr_t canvasWindow = 0);
     void setDepth(int idx = -1);
-    void setFBO(int idx, int textureOrDisplay);      // texture index, or -1 for the display
+    // a texture of this pipeline as FBO slot idx (PSGPU_DISPLAY: the display for
+    // slot 0, none for the others); no counterpart in the reference, whose
+    // setFBO takes an FBO object (below)
+    void setFBOTexture(int idx, int textureOrDisplay);
+    // pipeline.cpp:260-275: a host FBO as target (null: the display for slot 0,
+    // none for slots 1..15).  Slot 0's FBO is rendered on the GPU; like the
+    // reference's synchronous drawVAO, each draw leaves the result in the FBO's
+    // memory (its contents go to the GPU before the draw and come back after).
+    void setFBO(int idx, PuresoftFBO* fbo);
     void setUniform(int idx, const void* data, size_t len);
     void drawVAO(int vao, bool callerThrdForFragProc = false);
     void swapBuffers(void);                            // display -> renderer buffer
@@ -182,6 +231,11 @@ public:
 
 private:
     void check(int rc) const;
+    void flushMirrors(void);                           // written getTexture mirrors -> GPU
+    void hostPostProcess(PuresoftPostProcessor* processor);
+    int fboTexture(PuresoftFBO* fbo);                  // the GPU texture standing for a host FBO
+    void fboToGpu(PuresoftFBO* fbo, int tex);
+    void fboFromGpu(PuresoftFBO* fbo, int tex);
     // bytes of texture idx, the depth buffer (-1) or the display (-2), level 0, layer 0, in memory row order
     void grab(int idx, std::vector<unsigned char>& bytes, int& w, int& scan, int& h, int& elem);
     psgpu_ctx* m_ctx = nullptr;
@@ -191,4 +245,9 @@ private:
     PuresoftHeadlessRenderer* m_renderer;
     std::vector<PuresoftProcessor*> m_processors;
     std::vector<std::vector<PuresoftVBO*>> m_vaoSlots;
+    std::map<std::pair<int, int>, std::vector<unsigned char>> m_mirrors;   // (texture, layer) -> texels
+    std::set<std::pair<int, int>> m_dirty;                               // handed out since the last upload
+    std::map<PuresoftFBO*, int> m_fboTex;
+    PuresoftFBO* m_fbo0 = nullptr;                                       // host FBO bound to slot 0, or none
+    std::map<PuresoftPostProcessor*, std::unique_ptr<PsPostCall>> m_postCalls;
 };

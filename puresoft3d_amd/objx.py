@@ -17,6 +17,8 @@ Colours come out as (R, G, B, A) clamped to [0, 1] (objxio.cpp:264-275).
 """
 import struct
 
+import os
+
 import numpy as np
 
 from . import native as N
@@ -213,8 +215,23 @@ def split_name(name):
     return (None, None) if i < 0 else (name[:i], name[i + 1:])
 
 
+def load_picture(path):
+    """PuresoftDefaultPictureLoader (picldr.cpp:33-133) with PIL in place of
+    GDI+: the picture as 32-bit ARGB texels (bytes B, G, R, A; alpha 255 for
+    pictures without one), rows flipped top to bottom (RotateNoneFlipY,
+    picldr.cpp:47), so memory row 0 holds the picture's bottom row; the
+    scanline is width * 4 (WIDTHBYTES of 32 bits).  Returns (h, w, 4) uint8."""
+    try:
+        from PIL import Image
+    except ImportError as e:   # the product decodes no pictures itself
+        raise ImportError("loading pictures needs PIL (Pillow)") from e
+    with Image.open(path) as im:
+        rgba = np.asarray(im.convert("RGBA"))
+    return np.ascontiguousarray(rgba[::-1, :, [2, 1, 0, 3]])
+
+
 def load_scene(objx, width=640, height=480, fov_deg=60.0, tex_size=256, seed=5, prog_textured=None,
-               name=None):
+               name=None, picture_dir=None, picture_wrap=N.CLAMP):
     """A Scene of an OBJX file, meshes as the demo loader prepares them:
       * meshes named 'object/mesh' only (loadscene.cpp:151-158), indices expanded;
       * model-space positions (w = 1) and the bounding-box translation, drawn
@@ -222,10 +239,16 @@ def load_scene(objx, width=640, height=480, fov_deg=60.0, tex_size=256, seed=5, 
       * binormal = norm(cross(normal, tangent)) (loadscene.cpp:176-184);
       * slots 0 position, 1 tangent, 2 binormal, 3 normal, 4 texcoord (or the
         material colour x 255 in slot 2 for colour-only meshes).
-    Textured meshes (a diffuse file) use tex1bump1light1 (DEF03) by default
-    with seeded albedo and normal-map textures (the image files are not part of
-    the format); the others use colr1light1 (DEF02) with their diffuse colour.
-    The camera looks from the file's camera position at the scene's centre."""
+    Textured meshes (a diffuse file) take their pictures from picture_dir
+    when it is given, as the demo loader does (loadscene.cpp:316-330): the
+    diffuse picture, and the bump picture when the mesh names one, each
+    loaded once (load_picture) into a texture of picture_wrap (CLAMP there,
+    loadscene.cpp:362; WRAP in the other demo, test/scenobj.cpp:177); a mesh with a bump
+    picture is drawn by tex1bump1light1 (DEF03), one without by tex1light1
+    (DEF01).  Without picture_dir, textured meshes use DEF03 (or
+    prog_textured) with seeded albedo and normal-map textures.  The others
+    use colr1light1 (DEF02) with their diffuse colour.  The camera looks from
+    the file's camera position at the scene's centre."""
     sc_in = objx if isinstance(objx, ObjxScene) else read_objx(objx)
     prog_textured = N.PROC_DEF03 if prog_textured is None else prog_textured
     sc = S.Scene(width, height)
@@ -268,19 +291,34 @@ def load_scene(objx, width=640, height=480, fov_deg=60.0, tex_size=256, seed=5, 
     sc.uniforms[9] = np.array([0], np.int32)
     sc.uniforms[10] = np.array([1], np.int32)
     sc.uniforms[11] = np.array([16], np.int32)
+    pictures = {}   # file name -> texture index (findOrCreateTexture, loadscene.cpp:349-366)
+
+    def picture(fname):
+        if fname not in pictures:
+            pictures[fname] = len(sc.textures)
+            sc.textures.append(S.TextureSpec([load_picture(os.path.join(picture_dir, fname))], wrap=picture_wrap))
+        return pictures[fname]
+
     for obj, part, m, mv, centre, norms, tans, tcs in items:
         key = "%s/%s" % (obj, part)
         n = mv.shape[0]
         normals = norms if norms is not None else np.tile(np.array([0, 0, 1, 0], np.float32), (n, 1))
         textured = bool(m.diffuse_file) and tcs is not None and tans is not None
+        uni = {4: S.colmajor(S.translate(*[float(x) for x in centre[:3]]))}
         if textured:
             binorm = _norm3(_cross3(normals, tans))
             sc.meshes[key] = {0: mv, 1: _f32(tans), 2: binorm, 3: _f32(normals), 4: _f32(tcs)}
             prog = prog_textured
+            if picture_dir is not None:
+                uni[9] = np.array([picture(m.diffuse_file)], np.int32)
+                if m.bump_file:
+                    uni[10] = np.array([picture(m.bump_file)], np.int32)
+                    prog = N.PROC_DEF03
+                else:
+                    prog = N.PROC_DEF01
         else:
             col = np.tile(m.diffuse * np.float32(255.0), (n, 1)).astype(np.float32)
             sc.meshes[key] = {0: mv, 1: _f32(normals), 2: col}
             prog = N.PROC_DEF02
-        M = S.translate(*[float(x) for x in centre[:3]])
-        sc.draws.append(S.DrawSpec(key, prog, uniforms={4: S.colmajor(M)}))
+        sc.draws.append(S.DrawSpec(key, prog, uniforms=uni))
     return sc

@@ -230,7 +230,7 @@ def sliver_scene(prog, seed, width=160, height=96, n=1000):
     return sc
 
 
-def objx_scene(which, width=640, height=480):
+def objx_scene(which, width=640, height=480, pictures=False):
     """the reference's own assets (src/test/sphere.objx, src/test2/plane.objx,
     kept gzipped in tests/golden) through the OBJX loader.  sphere.objx's one
     mesh has no 'object/' prefix, which the demo loader skips; it is drawn here
@@ -242,11 +242,16 @@ def objx_scene(which, width=640, height=480):
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "%s.objx.gz" % which)
     with gzip.open(path, "rb") as f:
         sc = X.read_objx(f.read())
+    pdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pictures") if pictures else None
     if which == "sphere":
+        # with pictures: the demo's Earth (test/testobjs.cpp:12-15), its diffuse and bump maps
         m = sc.meshes[0]
         sc.meshes = [X.ObjxMesh("ball/" + m.name, m.vertices, m.normals, m.tangents, m.texcoords, m.indices, m.ambient,
-                                m.diffuse, m.specular, m.specular_exponent, "sphere.jpg")]
-    return X.load_scene(sc, width, height, name="objx_%s_%dx%d" % (which, width, height))
+                                m.diffuse, m.specular, m.specular_exponent, "earth.jpg" if pictures else "sphere.jpg",
+                                bump_file="earth.dot3.png" if pictures else "")]
+    return X.load_scene(sc, width, height, name="objx_%s%s_%dx%d" % (which, "_pictures" if pictures else "", width,
+                                                                      height),
+                        picture_dir=pdir, picture_wrap=N.WRAP if which == "sphere" else N.CLAMP)
 
 
 def cloud_quad_scene(shadow_depth, normal_scale, red, bg=0x00406080, w=64, h=48):
@@ -369,4 +374,30 @@ def hazard_scene(prog, seed, extra_vertices=0, width=96, height=64, n=90):
             rest[k] = np.concatenate([rest[k], tail], 0)
     sc.meshes["m"] = rest
     sc.name = "hazard_p%d_s%d_x%d" % (prog, seed, extra_vertices)
+    return sc
+
+
+def skybox_scene(width=320, height=240, turn=0.6):
+    """the demo's skybox (test/testobjs.cpp:185-198): DEF04 over a full-view
+    quad, the six purplenebula pictures as layers +X, -X, +Y, -Y, +Z, -Z
+    (rt, lf, up, dn, ft, bk), loaded as the reference's loader leaves them,
+    seen through a view turned about y and x"""
+    import os
+    from puresoft3d_amd import objx as X
+    pdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pictures")
+    faces = [X.load_picture(os.path.join(pdir, "purplenebula_%s.png" % k)) for k in ("rt", "lf", "up", "dn", "ft", "bk")]
+    sc = S.Scene(width, height)
+    sc.name = "skybox_purplenebula_%dx%d" % (width, height)
+    z = 0.999
+    sc.meshes["sky"] = {0: np.array([[-1, -1, z, 1], [1, -1, z, 1], [1, 1, z, 1],
+                                     [-1, -1, z, 1], [1, 1, z, 1], [-1, 1, z, 1]], np.float32)}
+    sc.textures.append(S.TextureSpec(faces, wrap=N.CLAMP))
+    c, s_ = np.cos(turn), np.sin(turn)
+    Ry = np.array([[c, 0, s_, 0], [0, 1, 0, 0], [-s_, 0, c, 0], [0, 0, 0, 1]])
+    c2, s2 = np.cos(0.35), np.sin(0.35)
+    Rx = np.array([[1, 0, 0, 0], [0, c2, -s2, 0], [0, s2, c2, 0], [0, 0, 0, 1]])
+    sc.uniforms[0] = S.colmajor(np.eye(4))
+    sc.uniforms[1] = S.colmajor(Rx @ Ry)
+    sc.uniforms[2] = np.array([0], np.int32)
+    sc.draws.append(S.DrawSpec("sky", N.PROC_DEF04))
     return sc

@@ -115,3 +115,78 @@ def test_debug_api_dumps(oracle, tmp_path):
     grey = gb[54 + 1024:].reshape(H, line)[:, :W]
     want = (f.depth[:, :W] * np.float32(255.0)).astype(np.int32).astype(np.uint8)
     np.testing.assert_array_equal(grey, want)
+
+
+PSDEMO = os.path.join(PKG, "host", "psdemo")
+
+
+def _stub_picture(name, w, h):
+    """the pixels psdemo's stub picture loader writes for a file name"""
+    s = 0
+    for ch in name:
+        s = (s * 31 + ord(ch)) & 255
+    y, x = np.mgrid[0:h, 0:w].astype(np.int64)
+    img = np.empty((h, w, 4), np.uint8)
+    img[..., 0] = (7 * x + 3 * y + s) & 255
+    img[..., 1] = (5 * x + 11 * y + 2 * s) & 255
+    img[..., 2] = (13 * x + y + 3 * s) & 255
+    img[..., 3] = 255
+    return img
+
+
+def _psdemo_scene(W, H):
+    """psdemo's frame as a scene: the skybox (DEF04, six stub pictures) and
+    the textured DEF01 quad, with the same floats"""
+    f32 = np.float32
+    sc = S.Scene(W, H)
+    sc.name = "psdemo_%dx%d" % (W, H)
+    z = f32(0.999)
+    sky = np.array([[-1, -1, z, 1], [1, -1, z, 1], [1, 1, z, 1], [-1, -1, z, 1], [1, 1, z, 1], [-1, 1, z, 1]], np.float32)
+    a = f32(0.6)
+    pos = np.array([[-a, -a, 0.2, 1], [a, -a, 0.2, 1], [a, a, 0.2, 1], [-a, -a, 0.2, 1], [a, a, 0.2, 1], [-a, a, 0.2, 1]],
+                   np.float32)
+    uv = ((pos[:, :2] + a) / (f32(2) * a)).astype(np.float32)
+    nrm = np.tile(np.array([0, 0, 1, 0], np.float32), (6, 1))
+    sc.meshes["sky"] = {0: sky}
+    sc.meshes["quad"] = {0: pos, 3: nrm, 4: uv}
+    faces = ["sky_rt.png", "sky_lf.png", "sky_up.png", "sky_dn.png", "sky_ft.png", "sky_bk.png"]
+    sc.textures.append(S.TextureSpec([_stub_picture(n, 32, 32) for n in faces], wrap=N.CLAMP))
+    sc.textures.append(S.TextureSpec([_stub_picture("quad.png", 64, 32)], wrap=N.WRAP))
+    I = S.colmajor(np.eye(4))
+    c, s = f32(0.87758256), f32(0.47942554)
+    V = np.array([c, 0, -s, 0, 0, 1, 0, 0, s, 0, c, 0, 0, 0, 0, 1], np.float32)
+    sc.draws.append(S.DrawSpec("sky", N.PROC_DEF04, uniforms={0: I, 1: V, 2: np.array([0], np.int32)}))
+    sc.draws.append(S.DrawSpec("quad", N.PROC_DEF01, uniforms={3: I, 4: I, 5: I, 7: S.vec4(-3, 4, 6, 0),
+                                                               8: S.vec4(0, 0, 5, 0), 9: np.array([1], np.int32)}))
+    sc.clear_colour = 0x00203040
+    return sc
+
+
+@pytest.mark.gpu
+def test_reference_resource_idioms(oracle, tmp_path):
+    """psdemo: textures filled through getTexture's pointer, once for a 2D
+    picture created without pixels and once per layer of a skybox created
+    from a NULL image (test/scenobj.cpp:165-219), then a post-processor of
+    the reference's own form (test2/testpost.h: paddb +50 over every row):
+    the presented frame equals the oracle's frame plus 50 per byte"""
+    if not os.path.exists(PSDEMO):
+        pytest.skip("psdemo not built (python -m puresoft3d_amd.build)")
+    W, H = 160, 120
+    out = str(tmp_path / "demo.bgra")
+    announce("psdemo %dx%d" % (W, H))
+    fbo = str(tmp_path / "fbo.bgra")
+    r = subprocess.run([PSDEMO, out, str(W), str(H), fbo], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, np.uint8).reshape(H, W, 4)
+    sc = _psdemo_scene(W, H)
+    f, st = oracle.render_scene(sc)
+    assert st.frags_shaded > W * H // 2
+    np.testing.assert_array_equal(got, (f.colour.astype(np.int64) + 50).astype(np.uint8))
+    # the quad drawn again into a host FBO bound with setFBO(0, fbo): its
+    # memory holds the draw over the 0x11 bytes it was filled with
+    sc.draws = sc.draws[1:]
+    g = oracle.Frame(W, H, depth_scanline=W, top_down=False)
+    g.colour[:] = 0x11
+    g.clear_depth(1.0)
+    oracle.render_scene(sc, frame=g, clear=False)
+    np.testing.assert_array_equal(np.fromfile(fbo, np.uint8).reshape(H, W, 4), g.colour)

@@ -297,6 +297,22 @@ def test_objx_assets_bitexact(oracle, which):
     assert st["frags_shaded"] > 10000
 
 
+@pytest.mark.parametrize("which", ["plane", "sphere"])
+def test_objx_assets_with_pictures_bitexact(oracle, which):
+    """the reference's own meshes with its own pictures (decoded by PIL as
+    its loader would leave them): plane.objx with marmite.jpg, top.jpg and
+    penhold.jpg (tex1light1 and colr1light1), sphere.objx as the demo's
+    Earth (earth.jpg and earth.dot3.png, tex1bump1light1)"""
+    pytest.importorskip("PIL")
+    assert_same(scenegen.objx_scene(which, 640, 480, pictures=True), oracle)
+
+
+def test_skybox_pictures_bitexact(oracle):
+    """the demo's purplenebula skybox through DEF04"""
+    pytest.importorskip("PIL")
+    assert_same(scenegen.skybox_scene(), oracle)
+
+
 def test_two_frames_accumulate(oracle):
     sc = S.sphere_scene(width=200, height=150)
     assert_same(sc, oracle, frames=2)
