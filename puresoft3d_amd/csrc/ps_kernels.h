@@ -16,27 +16,27 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
 // row item -> triangle table for k_span (large flushes; k_span searches row_off without it)
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
                 PsCounters* cnt);
+// exclusive scan of the per-tile counts (pieces | runs << 32) into tstart
+// (n + 1 entries); the piece total into *total_out, its running maximum at
+// total_out[2]
+void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, unsigned long long* tstart,
+                    unsigned long long* partial, uint32_t* total_out, const PsCounters* c);
+// spans of every row item; tcnt[t] += the pieces in tile t | the waves with pieces there << 32
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
-              int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt);
-void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
-                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, uint32_t* keys,
-                  uint32_t* vals, PsSeg* segs, float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
-int psk_rs_blocks(int64_t cap);
-int psk_radix_passes(int key_bits);
-// stable sort by key; with a single pass, kstart[k] receives the first position
-// of key k for every k < 2^key_bits (the lower bounds), else it is not written
-void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
-                    int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
-                    bool& result_in_second, uint32_t* kstart);
+              int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
+              int64_t cap, PsCounters* cnt);
+// the pieces, into runs of the tile lists f.tstart / f.runs; counts tcnt back down to zero
+void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
+                  const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,
+                  float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
 size_t psk_raster_lds(int chunk_w, int waves);
-void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
-                const uint32_t* kstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
-                PsCounters* cnt);
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris, const PsDraw* draws,
+                const PsFrame& f, int waves_per_group, PsCounters* cnt);
 // the raster walk with the FP of `prog` on every pass (blending / discarding
 // programmes), writing colour at once; no winner buffer, no shading pass
-void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
-                    const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
-                    PsCounters* cnt, const float* vary, int nvs, const float* ck);
+void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris,
+                    const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt, const float* vary, int nvs,
+                    const float* ck);
 // clears: this launch also writes the views' fused colour clears (one launch of a flush does)
 void psk_shade(hipStream_t s, int prog, const PsSpan* spans, const PsSeg* segs, const float* ck, const PsTri* tris,
                const float* vary, int nvs, const float* grads, const PsDraw* draws, const PsFrame& f,

@@ -6,16 +6,18 @@
 //  expand  : one wave per 64 triangles writes the triangle id of every row
 //  span    : one thread per (triangle,row): span ends, clamping and the
 //            depth / correction-factor start and step (drawvao.cpp:66-110,
-//            interp.cpp:26-80); counts the bin chunks the span touches
-//  segment : one thread per span writes (bin key, span) pairs
-//  radix   : stable LSD radix sort of the pairs by bin key (digits of up to
-//            11 bits, wave ballots give stable in-block ranks)
+//            interp.cpp:26-80); counts, per tile, the pieces (bin chunks) of
+//            the spans and the waves that have pieces there
+//  tiles   : exclusive scan of the per-tile counts: each tile's piece and
+//            run ranges
+//  segment : one thread per span walks it and writes its pieces; a wave's
+//            pieces of one tile form one run of consecutive pieces, placed by
+//            one atomic on the tile's counter
 //  raster  : one wave per tile of 64 bins, lane = bin (a row piece of
-//            chunk_w pixels); the wave finds its tile's range of the sorted
-//            pairs (kept by a one-pass sort, else searched), and each lane
-//            walks its bin's spans in triangle order, keeping depth and the
-//            winning span in LDS (fragthrd.cpp:208-252); depth test
-//            -1 < z && z - d < -1e-4
+//            chunk_w pixels); the wave puts its tile's runs in wave order,
+//            which is submission order, and each lane walks its bin's spans
+//            in triangle order, keeping depth and the winning span in LDS
+//            (fragthrd.cpp:208-252); depth test -1 < z && z - d < -1e-4
 //  shade   : one thread per pixel whose last colour-writing depth-test pass
 //            is known: re-derives that span's varyings, walks them serially
 //            to the pixel, runs the FP twin and writes BGRA8.
@@ -113,6 +115,103 @@ __device__ __forceinline__ void ps_with_draw(const PsDraw* __restrict__ draws, i
     int d0 = __builtin_amdgcn_readfirstlane(di);
     if (__all(di == d0)) f(draws[d0]);
     else f(draws[di]);
+}
+
+// Wave-wide min / max / inclusive sum over 64 lanes with DPP row operations
+// (no LDS crossbar round trips).  Control codes of GFX9 DPP: quad_perm
+// 0x00-0xff, row_shr:n 0x110+n, row_mirror 0x140, row_half_mirror 0x141,
+// row_bcast:15 0x142, row_bcast:31 0x143.
+#define PS_DPP(v, ctrl, rmask, bound) \
+    __builtin_amdgcn_update_dpp(0u, (v), (ctrl), (rmask), 0xf, (bound))
+
+// the minimum, the same in every lane
+__device__ __forceinline__ uint32_t ps_wave_min(uint32_t v)
+{
+    // within rows of 16: quad swaps, then the half-row and row mirrors
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false));  // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false));  // row_mirror
+    // the four row minima
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return min(min(r0, r1), min(r2, r3));
+}
+
+__device__ __forceinline__ uint32_t ps_wave_max(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false));
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return max(max(r0, r1), max(r2, r3));
+}
+
+// inclusive prefix sum over the lanes (every lane of the wave active)
+__device__ __forceinline__ uint32_t ps_wave_incl_scan(uint32_t v)
+{
+    v += PS_DPP(v, 0x111, 0xf, true);   // row_shr:1, zero from outside the row
+    v += PS_DPP(v, 0x112, 0xf, true);   // row_shr:2
+    v += PS_DPP(v, 0x114, 0xf, true);   // row_shr:4
+    v += PS_DPP(v, 0x118, 0xf, true);   // row_shr:8
+    v += PS_DPP(v, 0x142, 0xa, false);  // row_bcast:15 into rows 1 and 3
+    v += PS_DPP(v, 0x143, 0xc, false);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// The tiles a span's pieces fall in: tiles g0 .. g1 (consecutive tile
+// indices of one tile row), chunk range cx0 .. cx1 of the row.
+struct PsSpanTiles {
+    uint32_t g0, g1;
+    int cx0, cx1, tx0;
+};
+
+__device__ __forceinline__ PsSpanTiles ps_span_tiles(const PsView& v, int tile_rows, int tile_cols, int chunk_w, int y,
+                                                     int x1, int x2)
+{
+    PsSpanTiles s;
+    s.cx0 = x1 / chunk_w;
+    s.cx1 = x2 / chunk_w;
+    s.tx0 = s.cx0 / tile_cols;
+    s.g0 = (uint32_t)(v.tile_base + (int64_t)(y / tile_rows) * v.n_tiles_x + s.tx0);
+    s.g1 = s.g0 + (uint32_t)(s.cx1 / tile_cols - s.tx0);
+    return s;
+}
+
+// the chunks [lo, hi] of the span inside tile k (g0 <= k <= g1)
+__device__ __forceinline__ void ps_span_tile_chunks(const PsSpanTiles& s, int tile_cols, uint32_t k, int& lo, int& hi)
+{
+    int tx = s.tx0 + (int)(k - s.g0);
+    lo = max(s.cx0, tx * tile_cols);
+    hi = min(s.cx1, tx * tile_cols + tile_cols - 1);
+}
+
+// The distinct tiles of the wave's spans, in increasing order, up to 64 at a
+// time: lane j receives the j-th tile (bk) and the wave's pieces in it (bn);
+// returns how many.  cur is the lane's next tile (0xffffffff: none left) and
+// moves on past the batch.  Every lane of the wave calls it.  A span's tiles
+// are consecutive, so taking the smallest pending tile each round visits each
+// (wave, tile) pair exactly once.
+__device__ __forceinline__ int ps_wave_tile_batch(const PsSpanTiles& s, int tile_cols, uint32_t& cur, uint32_t& bk,
+                                                  uint32_t& bn)
+{
+    const int lane = threadIdx.x & (PS_WAVE - 1);
+    bk = 0xffffffffu;
+    bn = 0;
+    int j = 0;
+    for (; j < PS_WAVE; j++) {
+        const uint32_t k = ps_wave_min(cur);
+        if (k == 0xffffffffu) break;
+        const bool m = cur == k;
+        int lo = 0, hi = -1;
+        if (m) ps_span_tile_chunks(s, tile_cols, k, lo, hi);
+        const uint32_t n = __builtin_amdgcn_readlane(ps_wave_incl_scan((uint32_t)(hi - lo + 1)), 63);
+        if (lane == j) { bk = k; bn = n; }
+        if (m) cur = k < s.g1 ? k + 1 : 0xffffffffu;
+    }
+    return j;
 }
 
 // ---------------------------------------------------------------------------
@@ -351,113 +450,72 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
 }
 
 // ---------------------------------------------------------------------------
-// exclusive scan of uint32 with a device-side length; out[n] = total
+// exclusive scan (uint32, or uint64 holding two 32-bit counts side by side)
+// with a device-side length; out[n] = total
 // ---------------------------------------------------------------------------
 #define SCAN_ITEMS 16
 #define SCAN_BLOCK (256 * SCAN_ITEMS)
+#define SCAN1_THREADS 1024
+#define SCAN1_MAX (SCAN1_THREADS * SCAN_ITEMS)
 
-// exclusive scan of one value per thread over a 256-thread block
-__device__ __forceinline__ uint32_t ps_block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total)
+// exclusive scan of one value per thread over a block of NW waves
+template <typename T, int NW>
+__device__ __forceinline__ T ps_block_excl_scan(T v, T* lds, T& total)
 {
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
+    T x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
+        T y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
     if (lane == 63) lds[wave] = x;
     __syncthreads();
-    uint32_t wbase = 0;
+    T wbase = 0;
+    total = 0;
 #pragma unroll
-    for (int w = 0; w < 4; w++) wbase += (w < wave) ? lds[w] : 0u;
-    total = lds[0] + lds[1] + lds[2] + lds[3];
+    for (int w = 0; w < NW; w++) {
+        T l = lds[w];
+        wbase += (w < wave) ? l : (T)0;
+        total += l;
+    }
     __syncthreads();
     return wbase + x - v;
 }
 
-// thread t of block b owns items [b*SCAN_BLOCK + t*SCAN_ITEMS, +SCAN_ITEMS)
-__device__ __forceinline__ void ps_load_items(const uint32_t* in, int64_t base, int64_t n, uint32_t* v)
+// thread t of a block owns items [base, base + SCAN_ITEMS)
+template <typename T>
+__device__ __forceinline__ void ps_load_items(const T* in, int64_t base, int64_t n, T* v)
 {
+    constexpr int PER = 16 / sizeof(T);
     if (base + SCAN_ITEMS <= n) {
         const uint4* p = (const uint4*)(in + base);
 #pragma unroll
-        for (int q = 0; q < SCAN_ITEMS / 4; q++) {
+        for (int q = 0; q < SCAN_ITEMS / PER; q++) {
             uint4 u = p[q];
-            v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+            const T* t = (const T*)&u;
+#pragma unroll
+            for (int j = 0; j < PER; j++) v[PER * q + j] = t[j];
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0u;
+        for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : (T)0;
     }
 }
 
-__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
-                                                     uint32_t* __restrict__ partial, const PsCounters* c)
+// writes the exclusive prefix of v (starting at run) to out[base ...]
+template <typename T>
+__device__ __forceinline__ void ps_store_scan(T* out, int64_t base, int64_t n, const T* v, T run)
 {
-    __shared__ uint32_t lds[4];
-    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t s = 0;
-    if (!(c && ps_poisoned(c)) && base < n) {
-        uint32_t v[SCAN_ITEMS];
-        ps_load_items(in, base, n, v);
-#pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
-    }
-    uint32_t tot;
-    ps_block_excl_scan(s, lds, tot);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
-}
-
-// sum of partial[0, end) over a 256-thread block
-__device__ __forceinline__ uint32_t ps_block_sum_prefix(const uint32_t* partial, int end, uint32_t* lds)
-{
-    uint32_t s = 0;
-    for (int i = threadIdx.x; i < end; i += 256) s += partial[i];
-    uint32_t tot;
-    ps_block_excl_scan(s, lds, tot);
-    return tot;
-}
-
-// block b: prefix of the block sums before it, then the block's own scan.
-// Block 0 also writes out[n] = total (and the running maximum of totals).
-__global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
-                                                    const uint32_t* __restrict__ partial, int nb, uint32_t* __restrict__ out,
-                                                    uint32_t* total_out, int64_t cap, const PsCounters* c)
-{
-    __shared__ uint32_t lds[4];
-    if (c && ps_poisoned(c)) return;
-    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    if (blockIdx.x == 0) {
-        uint32_t tot = ps_block_sum_prefix(partial, nb, lds);
-        if (threadIdx.x == 0 && n <= cap) {
-            out[n] = tot;
-            if (total_out) {
-                *total_out = tot;
-                atomicMax(total_out + 2, tot);   // largest total since the counters were reset
-            }
-        }
-    }
-    if (blockIdx.x * (int64_t)SCAN_BLOCK >= n) return;
-    uint32_t before = blockIdx.x ? ps_block_sum_prefix(partial, blockIdx.x, lds) : 0u;
-    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    ps_load_items(in, base, n, v);
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
-    uint32_t tot;
-    uint32_t run = before + ps_block_excl_scan(s, lds, tot);
+    constexpr int PER = 16 / sizeof(T);
     if (base + SCAN_ITEMS <= n) {
         uint4* p = (uint4*)(out + base);
 #pragma unroll
-        for (int q = 0; q < SCAN_ITEMS / 4; q++) {
+        for (int q = 0; q < SCAN_ITEMS / PER; q++) {
             uint4 u;
-            u.x = run; run += v[4 * q];
-            u.y = run; run += v[4 * q + 1];
-            u.z = run; run += v[4 * q + 2];
-            u.w = run; run += v[4 * q + 3];
+            T* t = (T*)&u;
+#pragma unroll
+            for (int j = 0; j < PER; j++) { t[j] = run; run += v[PER * q + j]; }
             p[q] = u;
         }
     } else {
@@ -466,6 +524,114 @@ __global__ void __launch_bounds__(256) k_scan_final(const uint32_t* __restrict__
             run += v[i];
         }
     }
+}
+
+// the total: out[n], and its low 32 bits into *total_out with their running
+// maximum at total_out[2] (largest total since the counters were reset)
+// The tile scan (T = uint64) adds 32-bit piece counts: when the flush's
+// pieces, which k_span also counts in 64 bits into the pending statistics,
+// do not fit, it reports the largest total and flags an overflow, so that
+// the flush is split.
+// Called by the 64 lanes of one wave.
+template <typename T>
+__device__ __forceinline__ void ps_scan_total(T tot, int64_t n, int64_t cap, T* out, uint32_t* total_out,
+                                              const PsCounters* c)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane == 0 && n <= cap) {
+        out[n] = tot;
+        if (total_out) {
+            *total_out = (uint32_t)tot;
+            atomicMax(total_out + 2, (uint32_t)tot);
+        }
+    }
+    if constexpr (sizeof(T) == 8) {
+        static_assert(PS_STRIPES == 64, "one stripe per lane");
+        unsigned long long pieces = c->pend[lane][PS_ST_SEGS];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) pieces += __shfl_xor(pieces, o, 64);
+        if (lane == 0 && pieces != (uint32_t)tot) {
+            if (total_out) atomicMax(total_out + 2, 0xffffffffu);
+            atomicOr((int*)&c->overflow, 1);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scan_reduce(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
+                                                     T* __restrict__ partial, const PsCounters* c)
+{
+    __shared__ T lds[4];
+    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    T s = 0;
+    if (!(c && ps_poisoned(c)) && base < n) {
+        T v[SCAN_ITEMS];
+        ps_load_items(in, base, n, v);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
+    }
+    T tot;
+    ps_block_excl_scan<T, 4>(s, lds, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// sum of partial[0, end) over a 256-thread block
+template <typename T>
+__device__ __forceinline__ T ps_block_sum_prefix(const T* partial, int end, T* lds)
+{
+    T s = 0;
+    for (int i = threadIdx.x; i < end; i += 256) s += partial[i];
+    T tot;
+    ps_block_excl_scan<T, 4>(s, lds, tot);
+    return tot;
+}
+
+// block b: prefix of the block sums before it, then the block's own scan.
+// Block 0 also writes the total.
+template <typename T>
+__global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
+                                                    const T* __restrict__ partial, int nb, T* __restrict__ out,
+                                                    uint32_t* total_out, int64_t cap, const PsCounters* c)
+{
+    __shared__ T lds[4];
+    if (c && ps_poisoned(c)) return;
+    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    if (blockIdx.x == 0) {
+        T tot = ps_block_sum_prefix(partial, nb, lds);
+        if (threadIdx.x < 64) ps_scan_total(tot, n, cap, out, total_out, c);
+    }
+    if (blockIdx.x * (int64_t)SCAN_BLOCK >= n) return;
+    T before = blockIdx.x ? ps_block_sum_prefix(partial, blockIdx.x, lds) : (T)0;
+    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    ps_load_items(in, base, n, v);
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
+    T tot;
+    ps_store_scan(out, base, n, v, before + ps_block_excl_scan<T, 4>(s, lds, tot));
+}
+
+// the whole scan in one block, for up to SCAN1_MAX items
+template <typename T>
+__global__ void __launch_bounds__(SCAN1_THREADS) k_scan_one(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
+                                                            T* __restrict__ out, uint32_t* total_out, int64_t cap,
+                                                            const PsCounters* c)
+{
+    __shared__ T lds[SCAN1_THREADS / 64];
+    if (c && ps_poisoned(c)) return;
+    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    int64_t base = (int64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    ps_load_items(in, base, n, v);
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
+    T tot;
+    T run = ps_block_excl_scan<T, SCAN1_THREADS / 64>(s, lds, tot);
+    if (base < n) ps_store_scan(out, base, n, v, run);
+    if (threadIdx.x < 64) ps_scan_total(tot, n, cap, out, total_out, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -491,8 +657,8 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nro
     uint32_t myn = mt < T ? nrows[mt] : 0;
     uint32_t myo = mt < T ? row_off[mt] : 0;
     for (int j = 0; j < 64; j++) {
-        uint32_t n = __shfl(myn, j, 64);
-        uint32_t o = __shfl(myo, j, 64);
+        uint32_t n = __builtin_amdgcn_readlane(myn, j);
+        uint32_t o = __builtin_amdgcn_readlane(myo, j);
         for (uint32_t r = lane; r < n; r += 64) span_tri[o + r] = (int)(t0 + j);
     }
 }
@@ -542,12 +708,15 @@ __device__ __forceinline__ int64_t ps_row_tri(const uint32_t* __restrict__ row_o
 // ---------------------------------------------------------------------------
 // span setup
 // ---------------------------------------------------------------------------
+// Also counts, per tile, the span pieces in it and the waves (of 64 span
+// items) that have pieces there: tcnt[t] += pieces | waves << 32.
 template <bool SEARCH>
 __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
                                               const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
                                               int64_t T, int64_t cap,
-                                              PsSpan* __restrict__ spans, uint32_t* __restrict__ seg_count,
-                                              const PsView* __restrict__ views, int chunk_w, PsCounters* cnt)
+                                              PsSpan* __restrict__ spans, unsigned long long* __restrict__ tcnt,
+                                              const PsView* __restrict__ views, int chunk_w, int tile_rows, int tile_cols,
+                                              PsCounters* cnt)
 {
     if (ps_poisoned(cnt)) return;
     int64_t R = row_off[T];
@@ -561,17 +730,19 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         // the item's triangle: from k_expand's table, or searched in row_off
         int64_t tw = -1;
         if constexpr (SEARCH) tw = ps_row_tri(row_off, T, R, base + (threadIdx.x & ~63));
+        PsSpanTiles st = {};
+        bool has = false;
         if (i < R) {
             int t = SEARCH ? (int)tw : span_tri[i];
             const PsTri tr = tris[t];
             const PsDraw& d = draws[tr.code >> 8];
-            const int W = views[d.view].W;
+            const PsView& v = views[d.view];
+            const int W = v.W;
             int row = tr.first + (int)(i - (int64_t)row_off[t]);
             PsEdge L, Rr;
             ps_row_edges(tr, row, L, Rr);
             int left = ps_cvt_i32(ps_edge_eval(L, (float)row) + 0.5f);
             int right = ps_cvt_i32(ps_edge_eval(Rr, (float)row) + 0.5f);
-            uint32_t segs = 0;
             PsSpan sp;
             sp.tri = -1;
             if (left != right) {   // drawvao.cpp:72-75
@@ -586,14 +757,25 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
                     sp.y = row | ((d.behavior & 3) << 24) | (d.writes_colour ? (1 << 28) : 0);
                     sp.x1 = x1;
                     sp.x2 = x2;
-                    segs = (uint32_t)(x2 / chunk_w - x1 / chunk_w + 1);
                     nfrag = (unsigned long long)(x2 - x1 + 1);
+                    st = ps_span_tiles(v, tile_rows, tile_cols, chunk_w, row, x1, x2);
+                    has = true;
                 }
             }
             spans[i] = sp;
-            seg_count[i] = segs;
+        }
+        // per tile of the wave's spans: its pieces there and one run (lane j
+        // adds the j-th tile's counts, so a batch of tiles is one atomic)
+        const unsigned long long npieces = has ? (unsigned long long)(st.cx1 - st.cx0 + 1) : 0ull;
+        uint32_t cur = has ? st.g0 : 0xffffffffu;
+        for (;;) {
+            uint32_t bk, bn;
+            const int nb = ps_wave_tile_batch(st, tile_cols, cur, bk, bn);
+            if ((int)(threadIdx.x & (PS_WAVE - 1)) < nb) atomicAdd(&tcnt[bk], (unsigned long long)bn | (1ull << 32));
+            if (nb < PS_WAVE) break;
         }
         ps_wave_add(cnt, PS_ST_SPANS, nspan, true);
+        ps_wave_add(cnt, PS_ST_SEGS, npieces, true);
         ps_wave_add(cnt, PS_ST_FRAGS, nfrag, true);
     }
 }
@@ -622,286 +804,278 @@ __device__ __forceinline__ uint32_t ps_bin_lane(const PsFrame& f, int y, int cx)
 // pixel by pixel, so each piece starts with exactly the accumulators the
 // reference holds at its first pixel; spans of colour-writing draws that
 // cross chunks also keep the varying accumulators at each later piece.
-__global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ seg_off,
-                                                  const uint32_t* __restrict__ row_off, int64_t T, PsFrame f,
-                                                  const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
-                                                  const float* __restrict__ vary, int nvs, uint32_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ vals, PsSeg* __restrict__ segs,
+// Placement: the wave visits its tiles in increasing order, SG_BATCH at a
+// time.  Per tile, each lane's pieces go after those of the lanes before it
+// (offsets kept in LDS), and lane j reserves the j-th tile's run of
+// consecutive pieces with one atomic on the tile's counter (counting down
+// from k_span's count), all lanes at once.  Then every lane walks its pieces
+// of the batch's tiles into their slots; a span's tiles are consecutive in
+// the visit order, so each walk goes on from tile to tile.
+#define SG_BATCH 16
+__global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ row_off,
+                                                  int64_t T, PsFrame f, const PsTri* __restrict__ tris,
+                                                  const PsDraw* __restrict__ draws, const float* __restrict__ vary, int nvs,
+                                                  unsigned long long* __restrict__ tcnt, PsSeg* __restrict__ segs,
                                                   float* __restrict__ ck, int64_t cap, PsCounters* cnt)
 {
+    __shared__ uint32_t rel_all[4][SG_BATCH][PS_WAVE];   // per wave: [iteration][lane] offset in the run
     if (ps_poisoned(cnt)) return;
-    int64_t R = row_off[T];
-    int64_t S = seg_off[R];
+    const int64_t R = row_off[T];
+    const int64_t S = (uint32_t)f.tstart[f.n_tiles];
     if (S > cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->overflow, 1);
         return;
     }
-    const int CW = f.chunk_w;
+    const int CW = f.chunk_w, TC = f.tile_cols;
+    const int lane = threadIdx.x & (PS_WAVE - 1);
+    uint32_t (*rel)[PS_WAVE] = rel_all[threadIdx.x >> 6];
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = base + threadIdx.x;
-        unsigned long long nseg = 0;
-        if (i < R) {
-            uint32_t o = seg_off[i], e = seg_off[i + 1];
-            if (o != e) {
-                const PsSpan sp = spans[i];
-                const PsTri tr = tris[sp.tri];
-                const PsDraw& d = draws[tr.code >> 8];
-                const PsView& v = f.views[d.view];
-                int y = sp.y & 0xffffff;
-                uint32_t flags = ((uint32_t)(sp.y >> 24) & 3u) << 26 | ((uint32_t)(sp.y >> 28) & 1u) << 28;
-                bool need_ck = (e - o > 1) && ((sp.y >> 28) & 1) && ps_span_has_ck(sp);
-                int nv = d.nv;
-                float vs[PS_MAX_NV], vt[PS_MAX_NV];
-                if (need_ck) {
-                    PsEdge L, Rr;
-                    ps_row_edges(tr, y, L, Rr);
-                    int left = ps_cvt_i32(ps_edge_eval(L, (float)y) + 0.5f);
-                    int right = ps_cvt_i32(ps_edge_eval(Rr, (float)y) + 0.5f);
-                    PsSpanGeom g;
-                    ps_span_geom(tr, y, left, right, L, Rr, g);
-                    const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
-                    ps_span_varyings_n(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt, nv);
+        const int64_t i = base + threadIdx.x;
+        const uint32_t wave_id = (uint32_t)((base + (threadIdx.x & ~(PS_WAVE - 1))) >> 6);
+        PsSpan sp;
+        sp.tri = -1;
+        if (i < R) sp = spans[i];
+        const bool has = sp.tri >= 0;
+        PsSpanTiles st = {};
+        int nv = 0, y = 0;
+        uint32_t flags = 0;
+        bool need_ck = false;
+        float vs[PS_MAX_NV], vt[PS_MAX_NV];
+        if (has) {
+            const PsTri tr = tris[sp.tri];
+            const PsDraw& d = draws[tr.code >> 8];
+            const PsView& v = f.views[d.view];
+            y = sp.y & 0xffffff;
+            flags = ((uint32_t)(sp.y >> 24) & 3u) << 26 | ((uint32_t)(sp.y >> 28) & 1u) << 28;
+            st = ps_span_tiles(v, f.tile_rows, TC, CW, y, sp.x1, sp.x2);
+            need_ck = (st.cx1 > st.cx0) && ((sp.y >> 28) & 1) && ps_span_has_ck(sp);
+            nv = d.nv;
+            if (need_ck) {
+                PsEdge L, Rr;
+                ps_row_edges(tr, y, L, Rr);
+                int left = ps_cvt_i32(ps_edge_eval(L, (float)y) + 0.5f);
+                int right = ps_cvt_i32(ps_edge_eval(Rr, (float)y) + 0.5f);
+                PsSpanGeom g;
+                ps_span_geom(tr, y, left, right, L, Rr, g);
+                const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
+                ps_span_varyings_n(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt, nv);
+            }
+        }
+        float cf2 = sp.cf2, pz = sp.pz;
+        int x = sp.x1;
+        int cx = st.cx0;                     // the lane's next chunk
+        uint32_t cur = has ? st.g0 : 0xffffffffu;
+        for (;;) {
+            // the batch's tiles: lane j takes the j-th (bk) and its pieces (bn);
+            // each lane notes its offset in the runs of its tiles, and which
+            // iterations those are (consecutive: jt0 .. jt0 + ntb - 1)
+            uint32_t bk = 0xffffffffu, bn = 0;
+            int nb = 0, jt0 = -1, ntb = 0;
+            for (; nb < SG_BATCH; nb++) {
+                const uint32_t k = ps_wave_min(cur);
+                if (k == 0xffffffffu) break;
+                const bool m = cur == k;
+                int lo = 0, hi = -1;
+                if (m) ps_span_tile_chunks(st, TC, k, lo, hi);
+                const uint32_t n = (uint32_t)(hi - lo + 1);
+                const uint32_t incl = ps_wave_incl_scan(n);
+                const uint32_t tot = __builtin_amdgcn_readlane(incl, 63);
+                if (lane == nb) { bk = k; bn = tot; }
+                if (m) {
+                    rel[nb][lane] = incl - n;
+                    if (jt0 < 0) jt0 = nb;
+                    ntb++;
+                    cur = k < st.g1 ? k + 1 : 0xffffffffu;
                 }
-                float cf2 = sp.cf2, pz = sp.pz;
-                int x = sp.x1;
-                int cx = sp.x1 / CW;
-                for (uint32_t s = o; s < e; s++, cx++) {
-                    int xa = max(sp.x1, cx * CW), xb = min(sp.x2, cx * CW + CW - 1);
-                    for (; x < xa; x++) {
-                        cf2 += sp.cf2_step;
-                        pz += sp.pz_step;
-                        if (need_ck) {
+            }
+            if (nb == 0) break;
+            uint32_t first = 0xffffffffu;
+            if (lane < nb) {
+                const unsigned long long take = (unsigned long long)bn | (1ull << 32);
+                const unsigned long long left = atomicAdd(&tcnt[bk], (unsigned long long)0 - take);
+                const unsigned long long end = f.tstart[bk + 1];
+                const uint32_t fp = (uint32_t)end - (uint32_t)left;
+                const uint32_t slot = (uint32_t)(end >> 32) - (uint32_t)(left >> 32);
+                // the counts k_span left must cover this run (they always do:
+                // both kernels visit the same tiles); never write outside
+                if ((uint32_t)left < bn || (uint32_t)(left >> 32) == 0u || (uint64_t)fp + bn > (uint64_t)cap) {
+                    atomicOr(&cnt->overflow, 2);
+                } else {
+                    first = fp;
+                    f.runs[2 * (size_t)slot] = fp;
+                    f.runs[2 * (size_t)slot + 1] = (wave_id << 12) | (bn - 1u);
+                }
+            }
+            // every lane walks its pieces of the batch's tiles
+            const int maxtb = __builtin_amdgcn_readfirstlane(ps_wave_max((uint32_t)ntb));
+            for (int q = 0; q < maxtb; q++) {
+                const int jj = jt0 + q;
+                const uint32_t run0 = __shfl(first, jj < 0 ? 0 : jj, PS_WAVE);
+                if (q < ntb && run0 != 0xffffffffu) {
+                    uint32_t s = run0 + rel[jj][lane];
+                    const int hi = min(st.cx1, (cx / TC) * TC + TC - 1);   // the tile's last chunk of the span
+                    for (; cx <= hi; cx++, s++) {
+                        const int xa = max(sp.x1, cx * CW), xb = min(sp.x2, cx * CW + CW - 1);
+                        for (; x < xa; x++) {
+                            cf2 += sp.cf2_step;
+                            pz += sp.pz_step;
+                            if (need_ck) {
+#pragma unroll
+                                for (int k = 0; k < PS_MAX_NV; k++)
+                                    if (k < nv) vs[k] = vs[k] + vt[k];
+                            }
+                        }
+                        PsSeg sg;
+                        sg.si = (uint32_t)i | flags;
+                        sg.xab = (uint32_t)xa | ((uint32_t)xb << 16);
+                        sg.cf2 = cf2;
+                        sg.pz = pz;
+                        sg.cst = sp.cf2_step;
+                        sg.pst = sp.pz_step;
+                        sg.y = y;
+                        sg.pad = 0;
+                        segs[s] = sg;
+                        if (need_ck && xa > sp.x1) {
+                            float* c = ck + (int64_t)s * f.ck_stride;
 #pragma unroll
                             for (int k = 0; k < PS_MAX_NV; k++)
-                                if (k < nv) vs[k] = vs[k] + vt[k];
+                                if (k < nv) c[k] = vs[k];
                         }
                     }
-                    PsSeg sg;
-                    sg.si = (uint32_t)i | flags;
-                    sg.xab = (uint32_t)xa | ((uint32_t)xb << 16);
-                    sg.cf2 = cf2;
-                    sg.pz = pz;
-                    sg.cst = sp.cf2_step;
-                    sg.pst = sp.pz_step;
-                    sg.y = y;
-                    sg.pad = 0;
-                    segs[s] = sg;
-                    if (need_ck && s > o) {
-                        float* c = ck + (int64_t)s * f.ck_stride;
-#pragma unroll
-                        for (int k = 0; k < PS_MAX_NV; k++)
-                            if (k < nv) c[k] = vs[k];
-                    }
-                    keys[s] = ps_bin_tile(f, v, y, cx);
-                    vals[s] = s;
                 }
-                nseg = e - o;
             }
+            if (nb < SG_BATCH) break;
         }
-        ps_wave_add(cnt, PS_ST_SEGS, nseg, true);
     }
 }
 
 // ---------------------------------------------------------------------------
-// stable LSD radix sort, digits of up to 11 bits, 4096 pairs per block
+// tile lists: the runs of a tile in submission order
 // ---------------------------------------------------------------------------
-#define RS_ITEMS 8
-#define RS_BLOCK (256 * RS_ITEMS)
-#define RS_MAXB 2048
-
-// per-block digit counts; blocks past the used part of the key array write nothing
-__global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr, int shift, int bits,
-                                                 uint32_t* __restrict__ hist, int nb_cap, const PsCounters* c)
+// bitonic sort of one (key, value) pair per lane over lanes [0, np), np a
+// power of two up to 64, ascending by key
+__device__ __forceinline__ void ps_sort_lanes(uint32_t& key, uint32_t& val, int np)
 {
-    __shared__ uint32_t h[RS_MAXB];
-    if (ps_poisoned(c)) return;
-    int64_t n = *n_ptr;
-    int64_t base = blockIdx.x * (int64_t)RS_BLOCK;
-    if (base >= n) return;
-    const int nbk = 1 << bits;
-    const uint32_t mask = (uint32_t)nbk - 1u;
-    for (int d = threadIdx.x; d < nbk; d += 256) h[d] = 0;
-    __syncthreads();
-    for (int i = 0; i < RS_ITEMS; i++) {
-        int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+    const int lane = threadIdx.x & 63;
+    for (int k = 2; k <= np; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t ok = __shfl_xor(key, j, 64), ov = __shfl_xor(val, j, 64);
+            const bool up = (lane & k) == 0, low = (lane & j) == 0;
+            if (low == up ? ok < key : ok > key) { key = ok; val = ov; }
+        }
     }
-    __syncthreads();
-    for (int d = threadIdx.x; d < nbk; d += 256) hist[(int64_t)d * nb_cap + blockIdx.x] = h[d];
 }
 
-// one wave per digit: exclusive prefix of the digit's counts over the used
-// blocks (into offs) and the digit's total (into totals[d])
-__global__ void __launch_bounds__(256) k_rs_digits(const uint32_t* __restrict__ hist, const uint32_t* n_ptr, int bits,
-                                                   int nb_cap, uint32_t* __restrict__ offs, uint32_t* __restrict__ totals,
-                                                   const PsCounters* c)
+// The runs of a tile are written in the order their atomics happened.  Ordered
+// by wave they give the tile's pieces in submission order: wave order is span
+// order, and inside a run the pieces are in span order.
+//
+// Window of up to 64 runs from run w0 on, one per lane: rbase = the run's
+// first piece, rv = the list position of that piece (0xffffffff past the last
+// run), vhi = the list position where the window ends, np = the smallest
+// power of two at least the window's runs.  A tile of at most 64 runs is
+// sorted here; longer lists come ordered by ps_order_runs, with each run's
+// key replaced by its list position.
+__device__ __forceinline__ void ps_run_window(const uint32_t* rl, uint32_t nr, uint32_t w0, uint32_t n_list,
+                                              uint32_t& rbase, uint32_t& rv, uint32_t& vhi, int& np)
 {
-    if (ps_poisoned(c)) return;
-    int lane = threadIdx.x & 63;
-    int d = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (d >= (1 << bits)) return;
-    int64_t n = *n_ptr;
-    int nb = (int)((n + RS_BLOCK - 1) / RS_BLOCK);
-    const uint32_t* hrow = hist + (int64_t)d * nb_cap;
-    uint32_t* orow = offs + (int64_t)d * nb_cap;
-    uint32_t run = 0;
-    // the counts of up to 8 x 64 blocks are loaded before any is scanned, so
-    // the loads overlap instead of costing one round trip per 64 blocks
-    constexpr int PF = 8;
-    for (int b0 = 0; b0 < nb; b0 += 64 * PF) {
-        uint32_t v[PF];
-#pragma unroll
-        for (int k = 0; k < PF; k++) {
-            int b = b0 + k * 64 + lane;
-            v[k] = b < nb ? hrow[b] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < PF; k++) {
-            int b = b0 + k * 64 + lane;
-            uint32_t x = v[k];
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
-            }
-            if (b < nb) orow[b] = run + x - v[k];
-            run += __shfl(x, 63, 64);
-        }
+    const uint32_t j = w0 + (threadIdx.x & 63);
+    const uint32_t nw = min(nr - w0, 64u);
+    np = 1;
+    while ((uint32_t)np < nw) np <<= 1;
+    if (nr <= 64) {
+        uint32_t key = j < nr ? rl[2 * j + 1] : 0xffffffffu;
+        rbase = j < nr ? rl[2 * j] : 0u;
+        ps_sort_lanes(key, rbase, np);
+        const uint32_t c = j < nr ? (key & 0xfffu) + 1u : 0u;
+        const uint32_t incl = ps_wave_incl_scan(c);
+        rv = j < nr ? incl - c : 0xffffffffu;
+        vhi = n_list;
+    } else {
+        rbase = j < nr ? rl[2 * j] : 0u;
+        rv = j < nr ? rl[2 * j + 1] : 0xffffffffu;
+        vhi = w0 + 64 < nr ? rl[2 * (w0 + 64) + 1] : n_list;
     }
-    if (lane == 0) totals[d] = run;
 }
 
-template <int BITS>
-__global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                    const uint32_t* n_ptr, int shift, const uint32_t* __restrict__ offs,
-                                                    const uint32_t* __restrict__ totals, int nb_cap,
-                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                    uint32_t* __restrict__ kstart, const PsCounters* c)
+// the piece at list position v of the window (rv[0] <= v): the last run that
+// starts at or before v, found by a binary search over lanes [0, np)
+__device__ __forceinline__ uint32_t ps_run_piece(uint32_t rv, uint32_t rbase, uint32_t v, int np)
 {
-    constexpr int NBK = 1 << BITS;
-    constexpr int PER = (NBK + 255) / 256;   // digits per thread in the digit prefix
-    __shared__ uint32_t cntl[4][NBK];
-    __shared__ uint32_t base[NBK];
-    __shared__ uint32_t wsum[4];
-    if (ps_poisoned(c)) return;
-    int64_t n = *n_ptr;
-    int64_t b0 = blockIdx.x * (int64_t)RS_BLOCK;
-    // block 0 always runs: on a single-pass sort it also writes the key starts
-    if (b0 >= n && !(kstart && blockIdx.x == 0)) return;
-    int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    // exclusive prefix of the digit totals: thread t owns digits [t*PER, t*PER+PER)
-    {
-        uint32_t tv[PER];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            int d = tid * PER + k;
-            tv[k] = d < NBK ? totals[d] : 0u;
-            sum += tv[k];
-        }
-        uint32_t x = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        uint32_t wbase = 0;
-        for (int w = 0; w < wave; w++) wbase += wsum[w];
-        uint32_t run = wbase + x - sum;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            int d = tid * PER + k;
-            if (d < NBK) {
-                // digit = whole key: the prefix is where each key's run starts
-                if (kstart && blockIdx.x == 0) kstart[d] = run;
-                if (b0 < n) base[d] = run + offs[(int64_t)d * nb_cap + blockIdx.x];
-            }
-            run += tv[k];
-        }
+    if (np == 1) return (uint32_t)__builtin_amdgcn_readlane(rbase, 0) + (v - (uint32_t)__builtin_amdgcn_readlane(rv, 0));
+    int pos = 0;
+    for (int st = np >> 1; st >= 1; st >>= 1)
+        if (__shfl(rv, pos + st, 64) <= v) pos += st;
+    return __shfl(rbase, pos, 64) + (v - __shfl(rv, pos, 64));
+}
+
+// A tile with more than 64 runs: one wave sorts them stably by wave (LSD
+// radix, 7-bit digits of the wave index above the smallest, ping-ponging
+// between the tile's parts a and b of the two run arrays), then replaces each
+// run's key by the list position of its first piece.  cnt128: 128 words of
+// the wave's LDS.  Returns the array that holds the result.
+__device__ uint32_t* ps_order_runs(uint32_t* a, uint32_t* b, uint32_t nr, uint32_t* cnt128)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = ps_lt_mask();
+    uint32_t mn = 0xffffffffu, mx = 0;
+    for (uint32_t j = lane; j < nr; j += 64) {
+        const uint32_t w = a[2 * j + 1] >> 12;
+        mn = min(mn, w);
+        mx = max(mx, w);
     }
-    if (b0 >= n) return;
-    for (int d = tid; d < NBK; d += 256) cntl[0][d] = cntl[1][d] = cntl[2][d] = cntl[3][d] = 0;
-    __syncthreads();
-    uint32_t k[RS_ITEMS], v[RS_ITEMS], rank[RS_ITEMS];
-    uint64_t lt = ps_lt_mask();
-#pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
-        int64_t idx = b0 + wave * (64 * RS_ITEMS) + r * 64 + lane;
-        bool valid = idx < n;
-        k[r] = valid ? kin[idx] : 0u;
-        v[r] = valid ? vin[idx] : 0u;
-        uint32_t dig = (k[r] >> shift) & (NBK - 1);
-        uint64_t same = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < BITS; bit++) {
-            uint64_t m = __ballot((dig >> bit) & 1u);
-            same &= ((dig >> bit) & 1u) ? m : ~m;
-        }
-        uint32_t before = (uint32_t)__popcll(same & lt);
-        uint32_t cur = cntl[wave][dig];
-        rank[r] = cur + before;
+    mn = ps_wave_min(mn);
+    mx = ps_wave_max(mx);
+    const int bits = mx > mn ? 32 - __clz(mx - mn) : 0;
+    for (int shift = 0; shift < bits; shift += 7) {
+        cnt128[lane] = 0;
+        cnt128[lane + 64] = 0;
         __builtin_amdgcn_wave_barrier();
-        if (valid && before == 0) cntl[wave][dig] = cur + (uint32_t)__popcll(same);
+        for (uint32_t j = lane; j < nr; j += 64)
+            atomicAdd(&cnt128[(((a[2 * j + 1] >> 12) - mn) >> shift) & 127u], 1u);
         __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
-    for (int d = tid; d < NBK; d += 256) {
-        uint32_t run = 0;
-        for (int w = 0; w < 4; w++) {
-            uint32_t t = cntl[w][d];
-            cntl[w][d] = run;
-            run += t;
+        {
+            const uint32_t c0 = cnt128[2 * lane], c1 = cnt128[2 * lane + 1];
+            const uint32_t ex = ps_wave_incl_scan(c0 + c1) - c0 - c1;
+            __builtin_amdgcn_wave_barrier();
+            cnt128[2 * lane] = ex;
+            cnt128[2 * lane + 1] = ex + c0;
+            __builtin_amdgcn_wave_barrier();
         }
-    }
-    __syncthreads();
+        for (uint32_t j0 = 0; j0 < nr; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            const bool valid = j < nr;
+            const uint32_t e0 = valid ? a[2 * j] : 0u, e1 = valid ? a[2 * j + 1] : 0u;
+            const uint32_t dig = valid ? (((e1 >> 12) - mn) >> shift) & 127u : 0u;
+            uint64_t same = __ballot(valid);
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
-        int64_t idx = b0 + wave * (64 * RS_ITEMS) + r * 64 + lane;
-        if (idx < n) {
-            uint32_t dig = (k[r] >> shift) & (NBK - 1);
-            uint32_t pos = base[dig] + cntl[wave][dig] + rank[r];
-            kout[pos] = k[r];
-            vout[pos] = v[r];
+            for (int bit = 0; bit < 7; bit++) {
+                const uint64_t m = __ballot((dig >> bit) & 1u);
+                same &= ((dig >> bit) & 1u) ? m : ~m;
+            }
+            const uint32_t before = (uint32_t)__popcll(same & lt);
+            const uint32_t cur = cnt128[dig];
+            __builtin_amdgcn_wave_barrier();
+            if (valid) {
+                b[2 * (cur + before)] = e0;
+                b[2 * (cur + before) + 1] = e1;
+                if (before == 0) cnt128[dig] = cur + (uint32_t)__popcll(same);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
+        __threadfence();   // the next pass reads what this one wrote
+        uint32_t* t = a; a = b; b = t;
     }
-}
-
-// the first position i of keys[0, n) with keys[i] >= k, found by one wave
-// with a 64-way search (each step loads 64 pivots at once)
-__device__ __forceinline__ uint32_t ps_wave_lower_bound(const uint32_t* __restrict__ keys, int64_t n, uint32_t k, int lane)
-{
-    int64_t lo = 0, hi = n;
-    while (hi - lo > 64) {
-        int64_t step = (hi - lo + 63) / 64;
-        int64_t p = lo + lane * step;
-        bool below = p < hi && keys[p] < k;
-        int cnt = __popcll(__ballot(below));
-        if (cnt == 0) return (uint32_t)lo;
-        int64_t nlo = lo + (int64_t)(cnt - 1) * step + 1;
-        int64_t nhi = lo + (int64_t)cnt * step;
-        lo = nlo;
-        hi = nhi < hi ? nhi : hi;
+    uint32_t carry = 0;
+    for (uint32_t j0 = 0; j0 < nr; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const uint32_t c = j < nr ? (a[2 * j + 1] & 0xfffu) + 1u : 0u;
+        const uint32_t incl = ps_wave_incl_scan(c);
+        if (j < nr) a[2 * j + 1] = carry + incl - c;
+        carry += __builtin_amdgcn_readlane(incl, 63);
     }
-    int64_t q = lo + lane;
-    bool below = q < hi && keys[q] < k;
-    return (uint32_t)(lo + __popcll(__ballot(below)));
-}
-
-// the tile's range of the sorted list: from the starts a one-pass sort
-// leaves, or searched in the sorted keys
-__device__ __forceinline__ void ps_tile_range(const uint32_t* __restrict__ tstart, const PsFrame& f, int64_t tile, int lane,
-                                              uint32_t& s0, uint32_t& s1)
-{
-    if (tstart) {
-        s0 = tstart[tile];
-        s1 = tstart[tile + 1];
-        return;
-    }
-    const int64_t n = *f.n_sorted;
-    s0 = ps_wave_lower_bound(f.skeys, n, (uint32_t)tile, lane);
-    s1 = ps_wave_lower_bound(f.skeys, n, (uint32_t)tile + 1u, lane);
+    __threadfence();
+    return a;
 }
 
 // ---------------------------------------------------------------------------
@@ -917,7 +1091,6 @@ __device__ __forceinline__ void ps_tile_range(const uint32_t* __restrict__ tstar
 
 template <bool DEBUG>
 __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
-                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
                          const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles,
                          PsCounters* cnt)
 {
@@ -964,149 +1137,152 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
     unsigned long long passes = 0;
     bool dirty = dclear;
     const uint64_t lt = ps_lt_mask();
-    uint32_t s0, s1;
-    ps_tile_range(tstart, f, tile, lane, s0, s1);
-    // software pipeline over chunks of the tile list: while chunk i is split and
-    // walked, the records of chunk i+1 and the indices of chunk i+2 are in flight
-    uint32_t vn[RA_PER], vnn[RA_PER];
-    uint4 ran[RA_PER], rbn[RA_PER];
-#pragma unroll
-    for (int k = 0; k < RA_PER; k++) {
-        uint32_t e = s0 + k * 64 + lane;
-        vn[k] = e < s1 ? vals[e] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < RA_PER; k++) {
-        uint32_t e = s0 + k * 64 + lane;
-        const uint4* p = (const uint4*)(segs + vn[k]);
-        ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
-        rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int k = 0; k < RA_PER; k++) {
-        uint32_t e = s0 + RA_CHUNK + k * 64 + lane;
-        vnn[k] = e < s1 ? vals[e] : 0u;
-    }
-    for (uint32_t cb = s0; cb < s1; cb += RA_CHUNK) {
-        uint32_t ce = min(s1, cb + RA_CHUNK);
-        uint32_t v[RA_PER], key[RA_PER];
-        uint4 ra[RA_PER], rb[RA_PER];
+    // the tile's runs in submission order, up to 64 at a time (one per lane)
+    const unsigned long long ta = f.tstart[tile], tb = f.tstart[tile + 1];
+    const uint32_t n_list = (uint32_t)tb - (uint32_t)ta;
+    const uint32_t r0 = (uint32_t)(ta >> 32), nr = (uint32_t)(tb >> 32) - r0;
+    const uint32_t* rl = f.runs + 2 * (size_t)r0;
+    if (nr > 64) rl = ps_order_runs(f.runs + 2 * (size_t)r0, f.runs2 + 2 * (size_t)r0, nr, lcnt);
+    for (uint32_t w0 = 0; w0 < nr; w0 += 64) {
+        uint32_t rbase, rv, vhi;
+        int np;
+        ps_run_window(rl, nr, w0, n_list, rbase, rv, vhi, np);
+        const uint32_t vlo = __builtin_amdgcn_readlane(rv, 0);
+        // software pipeline over chunks of the window: while chunk i is split
+        // and walked, the records of chunk i+1 are in flight
+        uint32_t vn[RA_PER];
+        uint4 ran[RA_PER], rbn[RA_PER];
 #pragma unroll
         for (int k = 0; k < RA_PER; k++) {
-            v[k] = vn[k];
-            ra[k] = ran[k];
-            rb[k] = rbn[k];
-            vn[k] = vnn[k];
-        }
-#pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + RA_CHUNK + k * 64 + lane;
+            uint32_t e = vlo + k * 64 + lane;
+            uint32_t pp = ps_run_piece(rv, rbase, e, np);
+            vn[k] = e < vhi ? pp : 0u;
             const uint4* p = (const uint4*)(segs + vn[k]);
-            ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
-            rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
-            uint32_t e2 = cb + 2 * RA_CHUNK + k * 64 + lane;
-            vnn[k] = e2 < s1 ? vals[e2] : 0u;
+            ran[k] = e < vhi ? p[0] : make_uint4(0, 0, 0, 0);
+            rbn[k] = e < vhi ? p[1] : make_uint4(0, 0, 0, 0);
         }
+        for (uint32_t cb = vlo; cb < vhi; cb += RA_CHUNK) {
+            uint32_t ce = min(vhi, cb + RA_CHUNK);
+            uint32_t v[RA_PER], key[RA_PER];
+            uint4 ra[RA_PER], rb[RA_PER];
 #pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            // the piece's lane: its row inside the tile and its chunk inside the row
-            int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
-            key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
-        }
-        // stable split by lane key: each entry's rank among the equal keys
-        // before it (the same-key lanes of its group of 64 by ballots, plus
-        // the count of earlier groups), then per-key offsets, then placement
-        lcnt[lane] = 0;
-        __builtin_amdgcn_wave_barrier();
-        uint32_t rank[RA_PER];
-#pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + k * 64 + lane;
-            bool valid = e < ce;
-            uint32_t kk = key[k];
-            uint64_t same = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < 6; bit++) {
-                uint64_t m = __ballot((kk >> bit) & 1u);
-                same &= ((kk >> bit) & 1u) ? m : ~m;
+            for (int k = 0; k < RA_PER; k++) {
+                v[k] = vn[k];
+                ra[k] = ran[k];
+                rb[k] = rbn[k];
             }
-            uint32_t before = (uint32_t)__popcll(same & lt);
-            uint32_t cur = lcnt[kk];
-            rank[k] = cur + before;
-            __builtin_amdgcn_wave_barrier();
-            if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
-            __builtin_amdgcn_wave_barrier();
-        }
-        {
-            uint32_t cntv = lcnt[lane];
-            uint32_t x = cntv;
+            if (cb + RA_CHUNK < vhi) {
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t yv = __shfl_up(x, o, 64);
-                if (lane >= o) x += yv;
-            }
-            loff[lane] = x - cntv;
-            __builtin_amdgcn_wave_barrier();
-        }
-#pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + k * 64 + lane;
-            if (e < ce) {
-                uint32_t q = loff[key[k]] + rank[k];
-                r_s[q] = v[k];
-                r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
-                r_x[q] = ra[k].y;
-                r_cf2[q] = __uint_as_float(ra[k].z);
-                r_pz[q] = __uint_as_float(ra[k].w);
-                r_cst[q] = __uint_as_float(rb[k].x);
-                r_pst[q] = __uint_as_float(rb[k].y);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (inside) {
-            uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
-            for (uint32_t q = l0; q < l1; q++) {
-                int sidx = (int)r_s[q];
-                uint32_t fl = r_fl[q];
-                int beh = (fl >> 26) & 3;
-                bool wc = (fl >> 28) & 1;
-                uint32_t xab = r_x[q];
-                int a = (int)(xab & 0xffffu), b = (int)(xab >> 16);
-                float cf2 = r_cf2[q], pz = r_pz[q];
-                const float cst = r_cst[q], pst = r_pst[q];
-                uint32_t th = 0;
-                int tid = 0;
-                if (DEBUG) {
-                    int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
-                    int tri = spans[si].tri;
-                    const PsTri tr = tris[tri];
-                    const PsDraw& d = draws[tr.code >> 8];
-                    tid = (int)(d.id_base + (tri - d.tri_start));
-                    uint32_t h = (uint32_t)tid;
-                    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
-                    th = h;
+                for (int k = 0; k < RA_PER; k++) {
+                    uint32_t e = cb + RA_CHUNK + k * 64 + lane;
+                    uint32_t pp = ps_run_piece(rv, rbase, e, np);
+                    vn[k] = e < vhi ? pp : 0u;
+                    const uint4* p = (const uint4*)(segs + vn[k]);
+                    ran[k] = e < vhi ? p[0] : make_uint4(0, 0, 0, 0);
+                    rbn[k] = e < vhi ? p[1] : make_uint4(0, 0, 0, 0);
                 }
-                for (int x = a; x <= b; x++) {
-                    // interpolateNextStep (interp.cpp:82-92)
-                    float r = 1.0f / cf2;
-                    cf2 += cst;
-                    float z = pz * r;
-                    pz += pst;
-                    int c = x - X0;
-                    float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
-                    int64_t pix = (int64_t)y * VW + x;
-                    if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
-                    // fragthrd.cpp:227
-                    if (-1.0f < z && (z - cur < -0.0001f)) {
-                        passes++;
-                        if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
-                        if (wc) win[c * 64 + lane] = sidx;
-                        if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+            }
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                // the piece's lane: its row inside the tile and its chunk inside the row
+                int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
+                key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
+            }
+            // stable split by lane key: each entry's rank among the equal keys
+            // before it (the same-key lanes of its group of 64 by ballots, plus
+            // the count of earlier groups), then per-key offsets, then placement
+            lcnt[lane] = 0;
+            __builtin_amdgcn_wave_barrier();
+            uint32_t rank[RA_PER];
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                uint32_t e = cb + k * 64 + lane;
+                bool valid = e < ce;
+                uint32_t kk = key[k];
+                uint64_t same = __ballot(valid);
+#pragma unroll
+                for (int bit = 0; bit < 6; bit++) {
+                    uint64_t m = __ballot((kk >> bit) & 1u);
+                    same &= ((kk >> bit) & 1u) ? m : ~m;
+                }
+                uint32_t before = (uint32_t)__popcll(same & lt);
+                uint32_t cur = lcnt[kk];
+                rank[k] = cur + before;
+                __builtin_amdgcn_wave_barrier();
+                if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
+                __builtin_amdgcn_wave_barrier();
+            }
+            {
+                uint32_t cntv = lcnt[lane];
+                uint32_t x = cntv;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint32_t yv = __shfl_up(x, o, 64);
+                    if (lane >= o) x += yv;
+                }
+                loff[lane] = x - cntv;
+                __builtin_amdgcn_wave_barrier();
+            }
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                uint32_t e = cb + k * 64 + lane;
+                if (e < ce) {
+                    uint32_t q = loff[key[k]] + rank[k];
+                    r_s[q] = v[k];
+                    r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
+                    r_x[q] = ra[k].y;
+                    r_cf2[q] = __uint_as_float(ra[k].z);
+                    r_pz[q] = __uint_as_float(ra[k].w);
+                    r_cst[q] = __uint_as_float(rb[k].x);
+                    r_pst[q] = __uint_as_float(rb[k].y);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (inside) {
+                uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
+                for (uint32_t q = l0; q < l1; q++) {
+                    int sidx = (int)r_s[q];
+                    uint32_t fl = r_fl[q];
+                    int beh = (fl >> 26) & 3;
+                    bool wc = (fl >> 28) & 1;
+                    uint32_t xab = r_x[q];
+                    int a = (int)(xab & 0xffffu), b = (int)(xab >> 16);
+                    float cf2 = r_cf2[q], pz = r_pz[q];
+                    const float cst = r_cst[q], pst = r_pst[q];
+                    uint32_t th = 0;
+                    int tid = 0;
+                    if (DEBUG) {
+                        int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
+                        int tri = spans[si].tri;
+                        const PsTri tr = tris[tri];
+                        const PsDraw& d = draws[tr.code >> 8];
+                        tid = (int)(d.id_base + (tri - d.tri_start));
+                        uint32_t h = (uint32_t)tid;
+                        h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+                        th = h;
+                    }
+                    for (int x = a; x <= b; x++) {
+                        // interpolateNextStep (interp.cpp:82-92)
+                        float r = 1.0f / cf2;
+                        cf2 += cst;
+                        float z = pz * r;
+                        pz += pst;
+                        int c = x - X0;
+                        float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
+                        int64_t pix = (int64_t)y * VW + x;
+                        if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                        // fragthrd.cpp:227
+                        if (-1.0f < z && (z - cur < -0.0001f)) {
+                            passes++;
+                            if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
+                            if (wc) win[c * 64 + lane] = sidx;
+                            if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                        }
                     }
                 }
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
     int* wrow = V.winner + (int64_t)y * VW;
     for (int c = 0; c < ncol; c++) {
@@ -1125,7 +1301,6 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
 // pass, in submission order, and writes (or blends) the colour at once.
 template <int PROG, bool DEBUG>
 __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
-                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
                          const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles,
                          PsCounters* cnt, const float* __restrict__ vary, int nvs, const float* __restrict__ ck)
 {
@@ -1173,191 +1348,194 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     unsigned long long passes = 0;
     bool dirty = dclear;
     const uint64_t lt = ps_lt_mask();
-    uint32_t s0, s1;
-    ps_tile_range(tstart, f, tile, lane, s0, s1);
-    // software pipeline over chunks of the tile list: while chunk i is split and
-    // walked, the records of chunk i+1 and the indices of chunk i+2 are in flight
-    uint32_t vn[RA_PER], vnn[RA_PER];
-    uint4 ran[RA_PER], rbn[RA_PER];
-#pragma unroll
-    for (int k = 0; k < RA_PER; k++) {
-        uint32_t e = s0 + k * 64 + lane;
-        vn[k] = e < s1 ? vals[e] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < RA_PER; k++) {
-        uint32_t e = s0 + k * 64 + lane;
-        const uint4* p = (const uint4*)(segs + vn[k]);
-        ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
-        rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int k = 0; k < RA_PER; k++) {
-        uint32_t e = s0 + RA_CHUNK + k * 64 + lane;
-        vnn[k] = e < s1 ? vals[e] : 0u;
-    }
-    for (uint32_t cb = s0; cb < s1; cb += RA_CHUNK) {
-        uint32_t ce = min(s1, cb + RA_CHUNK);
-        uint32_t v[RA_PER], key[RA_PER];
-        uint4 ra[RA_PER], rb[RA_PER];
+    // the tile's runs in submission order, up to 64 at a time (one per lane)
+    const unsigned long long ta = f.tstart[tile], tb = f.tstart[tile + 1];
+    const uint32_t n_list = (uint32_t)tb - (uint32_t)ta;
+    const uint32_t r0 = (uint32_t)(ta >> 32), nr = (uint32_t)(tb >> 32) - r0;
+    const uint32_t* rl = f.runs + 2 * (size_t)r0;
+    if (nr > 64) rl = ps_order_runs(f.runs + 2 * (size_t)r0, f.runs2 + 2 * (size_t)r0, nr, lcnt);
+    for (uint32_t w0 = 0; w0 < nr; w0 += 64) {
+        uint32_t rbase, rv, vhi;
+        int np;
+        ps_run_window(rl, nr, w0, n_list, rbase, rv, vhi, np);
+        const uint32_t vlo = __builtin_amdgcn_readlane(rv, 0);
+        // software pipeline over chunks of the window: while chunk i is split
+        // and walked, the records of chunk i+1 are in flight
+        uint32_t vn[RA_PER];
+        uint4 ran[RA_PER], rbn[RA_PER];
 #pragma unroll
         for (int k = 0; k < RA_PER; k++) {
-            v[k] = vn[k];
-            ra[k] = ran[k];
-            rb[k] = rbn[k];
-            vn[k] = vnn[k];
-        }
-#pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + RA_CHUNK + k * 64 + lane;
+            uint32_t e = vlo + k * 64 + lane;
+            uint32_t pp = ps_run_piece(rv, rbase, e, np);
+            vn[k] = e < vhi ? pp : 0u;
             const uint4* p = (const uint4*)(segs + vn[k]);
-            ran[k] = e < s1 ? p[0] : make_uint4(0, 0, 0, 0);
-            rbn[k] = e < s1 ? p[1] : make_uint4(0, 0, 0, 0);
-            uint32_t e2 = cb + 2 * RA_CHUNK + k * 64 + lane;
-            vnn[k] = e2 < s1 ? vals[e2] : 0u;
+            ran[k] = e < vhi ? p[0] : make_uint4(0, 0, 0, 0);
+            rbn[k] = e < vhi ? p[1] : make_uint4(0, 0, 0, 0);
         }
+        for (uint32_t cb = vlo; cb < vhi; cb += RA_CHUNK) {
+            uint32_t ce = min(vhi, cb + RA_CHUNK);
+            uint32_t v[RA_PER], key[RA_PER];
+            uint4 ra[RA_PER], rb[RA_PER];
 #pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            // the piece's lane: its row inside the tile and its chunk inside the row
-            int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
-            key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
-        }
-        // stable split by lane key: each entry's rank among the equal keys
-        // before it (the same-key lanes of its group of 64 by ballots, plus
-        // the count of earlier groups), then per-key offsets, then placement
-        lcnt[lane] = 0;
-        __builtin_amdgcn_wave_barrier();
-        uint32_t rank[RA_PER];
-#pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + k * 64 + lane;
-            bool valid = e < ce;
-            uint32_t kk = key[k];
-            uint64_t same = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < 6; bit++) {
-                uint64_t m = __ballot((kk >> bit) & 1u);
-                same &= ((kk >> bit) & 1u) ? m : ~m;
+            for (int k = 0; k < RA_PER; k++) {
+                v[k] = vn[k];
+                ra[k] = ran[k];
+                rb[k] = rbn[k];
             }
-            uint32_t before = (uint32_t)__popcll(same & lt);
-            uint32_t cur = lcnt[kk];
-            rank[k] = cur + before;
-            __builtin_amdgcn_wave_barrier();
-            if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
-            __builtin_amdgcn_wave_barrier();
-        }
-        {
-            uint32_t cntv = lcnt[lane];
-            uint32_t x = cntv;
+            if (cb + RA_CHUNK < vhi) {
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t yv = __shfl_up(x, o, 64);
-                if (lane >= o) x += yv;
-            }
-            loff[lane] = x - cntv;
-            __builtin_amdgcn_wave_barrier();
-        }
-#pragma unroll
-        for (int k = 0; k < RA_PER; k++) {
-            uint32_t e = cb + k * 64 + lane;
-            if (e < ce) {
-                uint32_t q = loff[key[k]] + rank[k];
-                r_s[q] = v[k];
-                r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
-                r_x[q] = ra[k].y;
-                r_cf2[q] = __uint_as_float(ra[k].z);
-                r_pz[q] = __uint_as_float(ra[k].w);
-                r_cst[q] = __uint_as_float(rb[k].x);
-                r_pst[q] = __uint_as_float(rb[k].y);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (inside) {
-            uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
-            for (uint32_t q = l0; q < l1; q++) {
-                int sidx = (int)r_s[q];
-                uint32_t fl = r_fl[q];
-                int beh = (fl >> 26) & 3;
-                bool wc = (fl >> 28) & 1;
-                uint32_t xab = r_x[q];
-                int a = (int)(xab & 0xffffu), b = (int)(xab >> 16);
-                float cf2 = r_cf2[q], pz = r_pz[q];
-                const float cst = r_cst[q], pst = r_pst[q];
-                uint32_t th = 0;
-                int tid = 0;
-                if (DEBUG) {
-                    int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
-                    int tri = spans[si].tri;
-                    const PsTri tr = tris[tri];
-                    const PsDraw& d = draws[tr.code >> 8];
-                    tid = (int)(d.id_base + (tri - d.tri_start));
-                    uint32_t h = (uint32_t)tid;
-                    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
-                    th = h;
+                for (int k = 0; k < RA_PER; k++) {
+                    uint32_t e = cb + RA_CHUNK + k * 64 + lane;
+                    uint32_t pp = ps_run_piece(rv, rbase, e, np);
+                    vn[k] = e < vhi ? pp : 0u;
+                    const uint4* p = (const uint4*)(segs + vn[k]);
+                    ran[k] = e < vhi ? p[0] : make_uint4(0, 0, 0, 0);
+                    rbn[k] = e < vhi ? p[1] : make_uint4(0, 0, 0, 0);
                 }
-                // the span's varyings at pixel a: from the piece's checkpoint or
-                // by the serial walk from x1 (interp.cpp:82-92), as the deferred
-                // shading does
-                const int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
-                const PsSpan sp = spans[si];
-                const PsTri tr = tris[sp.tri];
-                const PsDraw& d = draws[tr.code >> 8];
-                float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
-                {
-                    PsEdge Le, Re;
-                    ps_row_edges(tr, y, Le, Re);
-                    int left = ps_cvt_i32(ps_edge_eval(Le, (float)y) + 0.5f);
-                    int right = ps_cvt_i32(ps_edge_eval(Re, (float)y) + 0.5f);
-                    PsSpanGeom g;
-                    ps_span_geom(tr, y, left, right, Le, Re, g);
-                    const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
-                    ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
-                    int xs = sp.x1;
-                    if (a > sp.x1 && ps_span_has_ck(sp) && wc) {
-                        const float* cp = ck + (int64_t)sidx * f.ck_stride;
+            }
 #pragma unroll
-                        for (int i = 0; i < P::NV; i++) vs[i] = cp[i];
-                        xs = a;
+            for (int k = 0; k < RA_PER; k++) {
+                // the piece's lane: its row inside the tile and its chunk inside the row
+                int py = (int)rb[k].z, pcx = (int)(ra[k].y & 0xffffu) >> cw_shift;
+                key[k] = (uint32_t)(((py & (f.tile_rows - 1)) << tc_shift) | (pcx & (f.tile_cols - 1)));
+            }
+            // stable split by lane key: each entry's rank among the equal keys
+            // before it (the same-key lanes of its group of 64 by ballots, plus
+            // the count of earlier groups), then per-key offsets, then placement
+            lcnt[lane] = 0;
+            __builtin_amdgcn_wave_barrier();
+            uint32_t rank[RA_PER];
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                uint32_t e = cb + k * 64 + lane;
+                bool valid = e < ce;
+                uint32_t kk = key[k];
+                uint64_t same = __ballot(valid);
+#pragma unroll
+                for (int bit = 0; bit < 6; bit++) {
+                    uint64_t m = __ballot((kk >> bit) & 1u);
+                    same &= ((kk >> bit) & 1u) ? m : ~m;
+                }
+                uint32_t before = (uint32_t)__popcll(same & lt);
+                uint32_t cur = lcnt[kk];
+                rank[k] = cur + before;
+                __builtin_amdgcn_wave_barrier();
+                if (valid && before == 0) lcnt[kk] = cur + (uint32_t)__popcll(same);
+                __builtin_amdgcn_wave_barrier();
+            }
+            {
+                uint32_t cntv = lcnt[lane];
+                uint32_t x = cntv;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint32_t yv = __shfl_up(x, o, 64);
+                    if (lane >= o) x += yv;
+                }
+                loff[lane] = x - cntv;
+                __builtin_amdgcn_wave_barrier();
+            }
+#pragma unroll
+            for (int k = 0; k < RA_PER; k++) {
+                uint32_t e = cb + k * 64 + lane;
+                if (e < ce) {
+                    uint32_t q = loff[key[k]] + rank[k];
+                    r_s[q] = v[k];
+                    r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
+                    r_x[q] = ra[k].y;
+                    r_cf2[q] = __uint_as_float(ra[k].z);
+                    r_pz[q] = __uint_as_float(ra[k].w);
+                    r_cst[q] = __uint_as_float(rb[k].x);
+                    r_pst[q] = __uint_as_float(rb[k].y);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (inside) {
+                uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
+                for (uint32_t q = l0; q < l1; q++) {
+                    int sidx = (int)r_s[q];
+                    uint32_t fl = r_fl[q];
+                    int beh = (fl >> 26) & 3;
+                    bool wc = (fl >> 28) & 1;
+                    uint32_t xab = r_x[q];
+                    int a = (int)(xab & 0xffffu), b = (int)(xab >> 16);
+                    float cf2 = r_cf2[q], pz = r_pz[q];
+                    const float cst = r_cst[q], pst = r_pst[q];
+                    uint32_t th = 0;
+                    int tid = 0;
+                    if (DEBUG) {
+                        int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
+                        int tri = spans[si].tri;
+                        const PsTri tr = tris[tri];
+                        const PsDraw& d = draws[tr.code >> 8];
+                        tid = (int)(d.id_base + (tri - d.tri_start));
+                        uint32_t h = (uint32_t)tid;
+                        h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+                        th = h;
                     }
-                    for (int xx = xs; xx < a; xx++) {
+                    // the span's varyings at pixel a: from the piece's checkpoint or
+                    // by the serial walk from x1 (interp.cpp:82-92), as the deferred
+                    // shading does
+                    const int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
+                    const PsSpan sp = spans[si];
+                    const PsTri tr = tris[sp.tri];
+                    const PsDraw& d = draws[tr.code >> 8];
+                    float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
+                    {
+                        PsEdge Le, Re;
+                        ps_row_edges(tr, y, Le, Re);
+                        int left = ps_cvt_i32(ps_edge_eval(Le, (float)y) + 0.5f);
+                        int right = ps_cvt_i32(ps_edge_eval(Re, (float)y) + 0.5f);
+                        PsSpanGeom g;
+                        ps_span_geom(tr, y, left, right, Le, Re, g);
+                        const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
+                        ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
+                        int xs = sp.x1;
+                        if (a > sp.x1 && ps_span_has_ck(sp) && wc) {
+                            const float* cp = ck + (int64_t)sidx * f.ck_stride;
+#pragma unroll
+                            for (int i = 0; i < P::NV; i++) vs[i] = cp[i];
+                            xs = a;
+                        }
+                        for (int xx = xs; xx < a; xx++) {
+#pragma unroll
+                            for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
+                        }
+                    }
+                    const bool blend = (d.behavior & PSGPU_ALPHABLEND) != 0;
+                    const int64_t crow = (V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y) * V.colour_scan;
+                    for (int x = a; x <= b; x++) {
+                        // interpolateNextStep (interp.cpp:82-92)
+                        float r = 1.0f / cf2;
+                        cf2 += cst;
+                        float z = pz * r;
+                        pz += pst;
+                        int c = x - X0;
+                        float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
+                        int64_t pix = (int64_t)y * VW + x;
+                        if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                        // fragthrd.cpp:227-236: the FP runs on a pass; a discarded
+                        // fragment leaves depth as it was
+                        if (-1.0f < z && (z - cur < -0.0001f)) {
+                            passes++;
+                            float in[P::NV > 0 ? P::NV : 1];
+#pragma unroll
+                            for (int i = 0; i < P::NV; i++) in[i] = vs[i] * r;
+                            uint32_t col = 0;
+                            int fr = P::fpi(d, in, col);
+                            if (fr != 2 && (beh & PSGPU_UPDATE_DEPTH)) { dep[c * 64 + lane] = z; dirty = true; }
+                            if (fr == 1 && wc) {   // FBOBridge::write4 (fragthrd.cpp:70-82)
+                                uint32_t* cp = V.colour + crow + x;
+                                *cp = blend ? ps_blend4(*cp, col) : col;
+                            }
+                            if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                        }
 #pragma unroll
                         for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
                     }
                 }
-                const bool blend = (d.behavior & PSGPU_ALPHABLEND) != 0;
-                const int64_t crow = (V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y) * V.colour_scan;
-                for (int x = a; x <= b; x++) {
-                    // interpolateNextStep (interp.cpp:82-92)
-                    float r = 1.0f / cf2;
-                    cf2 += cst;
-                    float z = pz * r;
-                    pz += pst;
-                    int c = x - X0;
-                    float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
-                    int64_t pix = (int64_t)y * VW + x;
-                    if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
-                    // fragthrd.cpp:227-236: the FP runs on a pass; a discarded
-                    // fragment leaves depth as it was
-                    if (-1.0f < z && (z - cur < -0.0001f)) {
-                        passes++;
-                        float in[P::NV > 0 ? P::NV : 1];
-#pragma unroll
-                        for (int i = 0; i < P::NV; i++) in[i] = vs[i] * r;
-                        uint32_t col = 0;
-                        int fr = P::fpi(d, in, col);
-                        if (fr != 2 && (beh & PSGPU_UPDATE_DEPTH)) { dep[c * 64 + lane] = z; dirty = true; }
-                        if (fr == 1 && wc) {   // FBOBridge::write4 (fragthrd.cpp:70-82)
-                            uint32_t* cp = V.colour + crow + x;
-                            *cp = blend ? ps_blend4(*cp, col) : col;
-                        }
-                        if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
-                    }
-#pragma unroll
-                    for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
-                }
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
     for (int c = 0; c < ncol; c++)
         if (dirty) drow[X0 + c] = dep[c * 64 + lane];
@@ -1533,8 +1711,23 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
               uint32_t* partial, uint32_t* total_out, const PsCounters* c)
 {
     int nb = psk_scan_blocks(cap);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, out, total_out, cap, c);
+    hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
+    hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, out, total_out,
+                       cap, c);
+}
+
+void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, unsigned long long* tstart,
+                    unsigned long long* partial, uint32_t* total_out, const PsCounters* c)
+{
+    if (n <= SCAN1_MAX) {
+        hipLaunchKernelGGL(k_scan_one<unsigned long long>, dim3(1), dim3(SCAN1_THREADS), 0, s, tcnt, nullptr, n, tstart,
+                           total_out, n, c);
+        return;
+    }
+    int nb = psk_scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_reduce<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, c);
+    hipLaunchKernelGGL(k_scan_final<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, nb, tstart,
+                       total_out, n, c);
 }
 
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
@@ -1544,77 +1737,43 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
 }
 
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
-              int64_t T, PsSpan* spans, uint32_t* seg_count, const PsView* views, int chunk_w, int64_t cap, PsCounters* cnt)
+              int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
+              int64_t cap, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 2048) g = 2048;
+    const int tile_cols = 64 / tile_rows;
     if (span_tri)
-        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans,
-                           seg_count, views, chunk_w, cnt);
+        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans, tcnt,
+                           views, chunk_w, tile_rows, tile_cols, cnt);
     else
-        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans,
-                           seg_count, views, chunk_w, cnt);
+        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans, tcnt,
+                           views, chunk_w, tile_rows, tile_cols, cnt);
 }
 
-void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* seg_off, const uint32_t* row_off, int64_t T,
-                  const PsFrame& f, const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, uint32_t* keys,
-                  uint32_t* vals, PsSeg* segs, float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
+void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
+                  const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,
+                  float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
 {
     int g = gsz(span_cap, 256); if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, seg_off, row_off, T, f, tris, draws, vary, nvs, keys,
-                       vals, segs, ck, seg_cap, cnt);
+    hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, row_off, T, f, tris, draws, vary, nvs, tcnt, segs, ck,
+                       seg_cap, cnt);
 }
-
-int psk_rs_blocks(int64_t cap) { return gsz(cap, RS_BLOCK); }
-
-int psk_radix_passes(int key_bits)
-{
-    int passes = (key_bits + 10) / 11;
-    return passes < 1 ? 1 : passes;
-}
-
-void psk_radix_sort(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys2, uint32_t* vals2, const uint32_t* n_dev,
-                    int64_t cap, int key_bits, uint32_t* hist, uint32_t* hist_off, uint32_t* partial, const PsCounters* c,
-                    bool& result_in_second, uint32_t* kstart)
-{
-    int nb = psk_rs_blocks(cap);
-    int passes = psk_radix_passes(key_bits);
-    int bits = (key_bits + passes - 1) / passes;
-    if (passes > 1) kstart = nullptr;
-    uint32_t *ki = keys, *vi = vals, *ko = keys2, *vo = vals2;
-    uint32_t* totals = hist_off + ((int64_t)nb << 11);
-    for (int p = 0, shift = 0; p < passes; p++, shift += bits) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(256), 0, s, ki, n_dev, shift, bits, hist, nb, c);
-        hipLaunchKernelGGL(k_rs_digits, dim3(((1 << bits) + 3) / 4), dim3(256), 0, s, hist, n_dev, bits, nb, hist_off,
-                           totals, c);
-        switch (bits) {
-#define RS_CASE(B) case B: hipLaunchKernelGGL(k_rs_scatter<B>, dim3(nb), dim3(256), 0, s, ki, vi, n_dev, shift, hist_off, totals, nb, ko, vo, kstart, c); break;
-        RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4) RS_CASE(5) RS_CASE(6) RS_CASE(7) RS_CASE(8) RS_CASE(9) RS_CASE(10) RS_CASE(11)
-#undef RS_CASE
-        }
-        uint32_t* t;
-        t = ki; ki = ko; ko = t;
-        t = vi; vi = vo; vo = t;
-    }
-    result_in_second = (passes & 1) != 0;
-}
-
 
 size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * RA_WORDS(chunk_w) * 4; }
 
-void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
-                const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
-                PsCounters* cnt)
+void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris, const PsDraw* draws,
+                const PsFrame& f, int waves_per_group, PsCounters* cnt)
 {
     int64_t n_tiles = f.n_tiles;
     size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
     dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
-    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, segs, vals, tstart, tris, draws, f, n_tiles, cnt);
-    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, segs, vals, tstart, tris, draws, f, n_tiles, cnt);
+    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt);
+    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt);
 }
 
-void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const uint32_t* vals,
-                    const uint32_t* tstart, const PsTri* tris, const PsDraw* draws, const PsFrame& f, int waves_per_group,
-                    PsCounters* cnt, const float* vary, int nvs, const float* ck)
+void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris,
+                    const PsDraw* draws, const PsFrame& f, int waves_per_group, PsCounters* cnt, const float* vary, int nvs,
+                    const float* ck)
 {
     int64_t n_tiles = f.n_tiles;
     size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
@@ -1622,11 +1781,11 @@ void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, co
 #define RI_CASE(K)                                                                                                   \
     case K:                                                                                                          \
         if (debug)                                                                                                   \
-            hipLaunchKernelGGL((k_raster_imm<K, true>), g, b, lds, s, spans, segs, vals, tstart, tris, draws, f,     \
-                               n_tiles, cnt, vary, nvs, ck);                                                         \
+            hipLaunchKernelGGL((k_raster_imm<K, true>), g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt,     \
+                               vary, nvs, ck);                                                                       \
         else                                                                                                         \
-            hipLaunchKernelGGL((k_raster_imm<K, false>), g, b, lds, s, spans, segs, vals, tstart, tris, draws, f,    \
-                               n_tiles, cnt, vary, nvs, ck);                                                         \
+            hipLaunchKernelGGL((k_raster_imm<K, false>), g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt,    \
+                               vary, nvs, ck);                                                                       \
         break;
     switch (prog) {
     RI_CASE(PSGPU_PROC_CLOUD) RI_CASE(PSGPU_PROC_CLOUD_SHADOW)

@@ -119,10 +119,14 @@ struct PsFrame {
     int n_views;
     int64_t n_tiles, n_stiles;
     const PsView* views;
-    // the sorted tile keys and their count, where the raster kernels search
-    // their tile's range (sorts of more than one pass leave no tile starts)
-    const uint32_t* skeys;
-    const uint32_t* n_sorted;
+    // tile lists.  Each wave of 64 span items writes its pieces of a tile as
+    // one run of consecutive pieces.  tstart[t] holds the first piece (low 32
+    // bits) and the first run (high 32 bits) of tile t (n_tiles + 1 entries);
+    // run r is runs[2r] = its first piece, runs[2r+1] = wave << 12 | pieces - 1.
+    // runs2 is scratch of the same size for ordering long run lists.
+    const unsigned long long* tstart;
+    uint32_t* runs;
+    uint32_t* runs2;
     // optional debug planes (single-view flushes), index y*W+x
     int32_t* dbg_last; int32_t* dbg_rcount; uint32_t* dbg_rhash; int32_t* dbg_pcount; uint32_t* dbg_phash;
 };

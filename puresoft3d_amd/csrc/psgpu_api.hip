@@ -145,8 +145,11 @@ struct FlushBufs {
     DevBuf<uint32_t> nrows, rowoff, partial;
     DevBuf<int> spantri;
     DevBuf<PsSpan> spans;
-    DevBuf<uint32_t> segcount, segoff;
-    DevBuf<uint32_t> keys, vals, keys2, vals2, hist, histoff, kstart;
+    // tile lists: per-tile piece and run counts (zero between flushes: the
+    // piece kernel counts them back down), their scan, the scan's block sums,
+    // and the run records (two words each) with scratch for reordering them
+    DevBuf<unsigned long long> tcnt, tstart, tpartial;
+    DevBuf<uint32_t> runs, runs2;
     DevBuf<int> winner;
     DevBuf<PsSeg> segs;
     DevBuf<float> ck;
@@ -206,9 +209,10 @@ struct psgpu_ctx {
     DevBuf<uint8_t> d_blob;
     FlushBufs fb;
     bool fixed_scratch = false;   // fb is carved from the static array: never reallocate it
-    // most spans and pieces one flush may hold (span records keep their index
-    // in 26 bits); a flush that needs more is split in two and re-run
-    int64_t max_spans = (int64_t)1 << 26, max_segs = (int64_t)1 << 31;
+    // most spans and pieces one flush may hold (pieces keep their span's index
+    // in 26 bits, run records the index of its wave of 64 below 2^20 - 1); a
+    // flush that needs more is split in two and re-run
+    int64_t max_spans = ((int64_t)1 << 26) - 64, max_segs = (int64_t)1 << 31;
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
     // per logged command: the overflow flag as it stood after the command ran
@@ -271,7 +275,14 @@ template <typename T> int ensure(psgpu_ctx* c, DevBuf<T>& b, size_t n)
         b.p = nullptr;
         b.cap = 0;
     }
-    HIPCHK(c, hipMalloc((void**)&b.p, cap * sizeof(T)));
+    hipError_t e = hipMalloc((void**)&b.p, cap * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // the failure is reported here, not by the next launch check
+        b.p = nullptr;
+        return fail(c, e == hipErrorOutOfMemory ? PSGPU_ERR_NOMEM : PSGPU_ERR_DEVICE,
+                    "hipMalloc of %zu bytes (span capacity %lld, piece capacity %lld): %s", cap * sizeof(T),
+                    (long long)c->fb.span_cap, (long long)c->fb.seg_cap, hipGetErrorString(e));
+    }
     b.cap = cap;
     c->buf_gen++;
     return PSGPU_OK;
@@ -378,13 +389,6 @@ PsTex tex_desc(const Tex& x)
     d.layers = x.layers; d.wrap = x.wrap; d.levels = x.levels; d.layer_bytes = x.layer_bytes;
     for (int l = 0; l < PS_MAX_LEVELS; l++) d.level_off[l] = x.level_off[l];
     return d;
-}
-
-int key_bits(int64_t n_keys)
-{
-    int b = 0;
-    while (((int64_t)1 << b) < n_keys + 1) b++;
-    return b < 1 ? 1 : b;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -546,18 +550,17 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
     if (b.span_cap > c->max_spans || b.seg_cap > c->max_segs)
         return fail(c, PSGPU_ERR_NOMEM, "more than %lld spans or %lld pieces in one flush", (long long)c->max_spans,
                     (long long)c->max_segs);
-    int64_t scan_cap = std::max<int64_t>(T, b.span_cap);
-    if ((rc = ensure(c, b.partial, psk_scan_blocks(scan_cap) + 1)) ||
+    if ((rc = ensure(c, b.partial, psk_scan_blocks(T) + 1)) ||
         (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) || (rc = ensure(c, b.spans, b.span_cap)) ||
-        (rc = ensure(c, b.segcount, b.span_cap)) || (rc = ensure(c, b.segoff, b.span_cap + 1)) ||
-        (rc = ensure(c, b.keys, b.seg_cap)) || (rc = ensure(c, b.vals, b.seg_cap)) ||
-        (rc = ensure(c, b.keys2, b.seg_cap)) || (rc = ensure(c, b.vals2, b.seg_cap)) ||
-        (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)))
+        (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)) ||
+        (rc = ensure(c, b.runs, (size_t)b.seg_cap * 2)) || (rc = ensure(c, b.runs2, (size_t)b.seg_cap * 2)) ||
+        (rc = ensure(c, b.tstart, n_tiles + 1)) || (rc = ensure(c, b.tpartial, psk_scan_blocks(n_tiles) + 1)))
         return rc;
-    int nbr = psk_rs_blocks(b.seg_cap);
-    if ((rc = ensure(c, b.hist, (size_t)nbr * 2048)) || (rc = ensure(c, b.histoff, (size_t)nbr * 2048 + 2048)) ||
-        (rc = ensure(c, b.kstart, std::max<int64_t>(n_tiles + 1, 2048))))
-        return rc;
+    // a new tile-count array starts at zero (a reallocation may return the
+    // old address, so the test is whether one happened)
+    const uint64_t gen = c->buf_gen;
+    if ((rc = ensure(c, b.tcnt, n_tiles + 1))) return rc;
+    if (c->buf_gen != gen) HIPCHK(c, hipMemsetAsync(b.tcnt.p, 0, b.tcnt.cap * sizeof(unsigned long long), c->stream));
     return PSGPU_OK;
 }
 
@@ -620,31 +623,21 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     {
         ProfScope p(c, "span");
-        psk_span(s, c->fb.tris.p, dd, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p,
-                 c->fb.segcount.p, dv, f.chunk_w, c->fb.span_cap, c->d_cnt);
+        psk_span(s, c->fb.tris.p, dd, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p, c->fb.tcnt.p,
+                 dv, f.chunk_w, f.tile_rows, c->fb.span_cap, c->d_cnt);
     }
     {
-        ProfScope p(c, "scan_segs");
-        psk_scan(s, c->fb.segcount.p, c->d_sizes + 0, 0, c->fb.span_cap, c->fb.segoff.p, c->fb.partial.p, c->d_sizes + 1, c->d_cnt);
+        ProfScope p(c, "scan_tiles");
+        psk_scan_tiles(s, c->fb.tcnt.p, f.n_tiles, c->fb.tstart.p, c->fb.tpartial.p, c->d_sizes + 1, c->d_cnt);
     }
+    f.tstart = c->fb.tstart.p;
+    f.runs = c->fb.runs.p;
+    f.runs2 = c->fb.runs2.p;
     {
         ProfScope p(c, "segments");
-        psk_segments(s, c->fb.spans.p, c->fb.segoff.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs,
-                     c->fb.keys.p, c->fb.vals.p, c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_cnt);
+        psk_segments(s, c->fb.spans.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs, c->fb.tcnt.p,
+                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_cnt);
     }
-    bool second = false;
-    {
-        ProfScope p(c, "radix_sort");
-        psk_radix_sort(s, c->fb.keys.p, c->fb.vals.p, c->fb.keys2.p, c->fb.vals2.p, c->d_sizes + 1, c->fb.seg_cap,
-                       key_bits(f.n_tiles), c->fb.hist.p, c->fb.histoff.p, c->fb.partial.p, c->d_cnt, second, c->fb.kstart.p);
-    }
-    uint32_t* skeys = second ? c->fb.keys2.p : c->fb.keys.p;
-    uint32_t* svals = second ? c->fb.vals2.p : c->fb.vals.p;
-    // one pass leaves the tile starts in kstart; after more, each raster wave
-    // searches its tile's range in the sorted keys
-    const uint32_t* tstart = psk_radix_passes(key_bits(f.n_tiles)) > 1 ? nullptr : c->fb.kstart.p;
-    f.skeys = skeys;
-    f.n_sorted = c->d_sizes + 1;
     int imm = 0;   // a flush of one programme that shades in the raster walk
     {
         const PsDraw* hd = (const PsDraw*)blob.bytes.data();
@@ -652,16 +645,15 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     }
     if (imm) {
         ProfScope p(c, "raster");
-        psk_raster_imm(s, imm, dbg, c->fb.spans.p, c->fb.segs.p, svals, tstart, c->fb.tris.p, dd, f,
-                       c->tiling.waves_per_group, c->d_cnt, c->fb.vary.p, nvs, c->fb.ck.p);
+        psk_raster_imm(s, imm, dbg, c->fb.spans.p, c->fb.segs.p, c->fb.tris.p, dd, f, c->tiling.waves_per_group,
+                       c->d_cnt, c->fb.vary.p, nvs, c->fb.ck.p);
         HIPCHK(c, hipGetLastError());
         c->stats.flushes++;
         return PSGPU_OK;
     }
     {
         ProfScope p(c, "raster");
-        psk_raster(s, dbg, c->fb.spans.p, c->fb.segs.p, svals, tstart, c->fb.tris.p, dd, f,
-                   c->tiling.waves_per_group, c->d_cnt);
+        psk_raster(s, dbg, c->fb.spans.p, c->fb.segs.p, c->fb.tris.p, dd, f, c->tiling.waves_per_group, c->d_cnt);
     }
     {
         ProfScope p(c, "shade");
@@ -969,6 +961,8 @@ int do_sync(psgpu_ctx* c)
             HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
             return PSGPU_OK;
         }
+        if (c->h_cnt->overflow & 2)
+            return fail(c, PSGPU_ERR_STATE, "internal: a tile list did not match its counts");
         // grow and replay the log; a flush that needs more than one flush may
         // hold is split in two (and again, if a half still does)
         int64_t need_spans = (int64_t)c->h_sizes[2], need_segs = (int64_t)c->h_sizes[3];
@@ -996,6 +990,9 @@ int do_sync(psgpu_ctx* c)
         HIPCHK(c, hipMemsetAsync(&c->d_cnt->overflow, 0, sizeof(int), c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_cnt->pend, 0, sizeof(c->d_cnt->pend), c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_sizes, 0, 4 * sizeof(uint32_t), c->stream));
+        // a flush cut short leaves tile counts behind
+        if (c->fb.tcnt.p)
+            HIPCHK(c, hipMemsetAsync(c->fb.tcnt.p, 0, c->fb.tcnt.cap * sizeof(unsigned long long), c->stream));
         for (size_t i = 0; i < c->log.size(); i++) {
             if ((rc = run_cmd(c, c->log[i]))) return rc;
             if ((rc = mark_done(c, (int)i))) return rc;
@@ -1187,12 +1184,13 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
     const int64_t T = sh.T;
     if (!a.take(b.winner, sh.n_pix) || !a.take(b.tris, T) || !a.take(b.vary, (size_t)T * 3 * sh.nvs) ||
         !a.take(b.grad, (size_t)T * 8) || !a.take(b.nrows, T) || !a.take(b.rowoff, T + 1) ||
-        !a.take(b.kstart, std::max<int64_t>(sh.n_tiles + 1, 2048)))
+        !a.take(b.partial, psk_scan_blocks(T) + 1) || !a.take(b.tcnt, sh.n_tiles + 1) ||
+        !a.take(b.tstart, sh.n_tiles + 1) || !a.take(b.tpartial, psk_scan_blocks(sh.n_tiles) + 1))
         return false;
     const bool expand = T > kSpanSearchTris;
-    // bytes per span: record, piece count and offset (+ row table), and 1.5
-    // pieces of key, value, both sort copies, record, checkpoint, histograms
-    const double per_span = 32 + 4 + 4 + (expand ? 4 : 0) + 1.5 * (4 * 4 + sizeof(PsSeg) + 4.0 * sh.nvs + 8);
+    // bytes per span: record (+ row table), and 1.5 pieces of record,
+    // checkpoint and two run records
+    const double per_span = 32 + (expand ? 4 : 0) + 1.5 * (sizeof(PsSeg) + 4.0 * sh.nvs + 16);
     size_t left = a.cap - a.used;
     int64_t spans = (int64_t)((double)left / per_span);
     spans = std::min<int64_t>(spans, (int64_t)1 << 26);
@@ -1201,13 +1199,9 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
         FlushBufs x = b;
         x.span_cap = spans;
         x.seg_cap = spans * 3 / 2;
-        int nbr = psk_rs_blocks(x.seg_cap);
-        if (t.take(x.partial, psk_scan_blocks(std::max<int64_t>(T, x.span_cap)) + 1) &&
-            (!expand || t.take(x.spantri, x.span_cap)) && t.take(x.spans, x.span_cap) &&
-            t.take(x.segcount, x.span_cap) && t.take(x.segoff, x.span_cap + 1) && t.take(x.keys, x.seg_cap) &&
-            t.take(x.vals, x.seg_cap) && t.take(x.keys2, x.seg_cap) && t.take(x.vals2, x.seg_cap) &&
-            t.take(x.segs, x.seg_cap) && t.take(x.ck, (size_t)x.seg_cap * sh.nvs) &&
-            t.take(x.hist, (size_t)nbr * 2048) && t.take(x.histoff, (size_t)nbr * 2048 + 2048)) {
+        if ((!expand || t.take(x.spantri, x.span_cap)) && t.take(x.spans, x.span_cap) && t.take(x.segs, x.seg_cap) &&
+            t.take(x.ck, (size_t)x.seg_cap * sh.nvs) && t.take(x.runs, (size_t)x.seg_cap * 2) &&
+            t.take(x.runs2, (size_t)x.seg_cap * 2)) {
             a = t;
             b = x;
             return true;
@@ -1289,8 +1283,8 @@ int psgpu_destroy(psgpu_ctx* c)
     for (auto& t : c->tex) fr(t.d);
     for (auto& v : c->vbo) fr(v.d);
     fr(c->d_blob.p); fr(c->fb.tris.p); fr(c->fb.vary.p); fr(c->fb.nrows.p); fr(c->fb.rowoff.p); fr(c->fb.partial.p);
-    fr(c->fb.spantri.p); fr(c->fb.spans.p); fr(c->fb.segcount.p); fr(c->fb.segoff.p); fr(c->fb.keys.p); fr(c->fb.vals.p);
-    fr(c->fb.keys2.p); fr(c->fb.vals2.p); fr(c->fb.hist.p); fr(c->fb.histoff.p); fr(c->fb.kstart.p); fr(c->fb.winner.p);
+    fr(c->fb.spantri.p); fr(c->fb.spans.p); fr(c->fb.tcnt.p); fr(c->fb.tstart.p); fr(c->fb.tpartial.p);
+    fr(c->fb.runs.p); fr(c->fb.runs2.p); fr(c->fb.winner.p);
     fr(c->fb.segs.p); fr(c->fb.ck.p); fr(c->fb.grad.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
     for (int i = 0; i < psgpu_ctx::kStage; i++) {
@@ -2354,6 +2348,11 @@ int psgpu_sync_noalloc(psgpu_ctx* c)
                                       hipMemcpyHostToDevice, c->stream);
         rc = e != hipSuccess ? fail(c, PSGPU_ERR_DEVICE, "hipMemcpyAsync: %s", hipGetErrorString(e))
                              : run_clears(c, clears, fill_dev);
+    }
+    // the tile counts start at zero (the array may have held anything)
+    if (!rc && blob_dev) {
+        hipError_t e = hipMemsetAsync(c->fb.tcnt.p, 0, c->fb.tcnt.cap * sizeof(unsigned long long), c->stream);
+        if (e != hipSuccess) rc = fail(c, PSGPU_ERR_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
     }
     if (!rc && blob_dev) {
         PsFrame f;
