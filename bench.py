@@ -305,6 +305,9 @@ def main():
     ap.add_argument("--small", action="store_true", help="reduced scene for quick checks")
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
     ap.add_argument("--no-tune", action="store_true", help="keep the default raster tiling (frame and batch)")
+    ap.add_argument("--tiling", type=str, default="", metavar="CW,TR,W",
+                    help="fixed raster tiling of the 1080p frame (chunk width, tile rows, waves per group); implies "
+                         "no tuning of it")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-view batch workload")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
@@ -364,7 +367,9 @@ def main():
             ctx.graph_launch(graph, 1)
 
     tuned = None
-    if not args.no_tune:
+    if args.tiling:
+        ctx.tiling(*[int(v) for v in args.tiling.split(",")])
+    elif not args.no_tune:
         from puresoft3d_amd import autotune as AT
         best, table = AT.autotune(ctx, frame, reps=5)
         top = sorted(table, key=lambda r: r[3])[:5]

@@ -7,12 +7,18 @@ void psk_fill_u32(hipStream_t s, uint32_t* p, int64_t n, uint32_t v, const PsCou
 // work pieces of a fill (1024 values of a row); psk_fills takes the largest
 int64_t psk_fill_pieces(int64_t rows, int64_t width);
 void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, const PsCounters* c);
+// row_part (optional, flushes of one programme): each block of psk_setup_block()
+// triangles writes its row total there
+int psk_setup_block();
 void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
-               float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views);
+               float* grads, uint32_t* nrows, uint32_t* row_part, PsCounters* cnt, const PsView* views);
 int psk_scan_blocks(int64_t cap);
 // exclusive scan; out[n] and *total_out (if given) receive the total
 void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
               uint32_t* partial, uint32_t* total_out, const PsCounters* c);
+// the same scan from given partial sums of pgran items each (pgran divides 4096)
+void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int64_t n_parts,
+                    int pgran, uint32_t* total_out, const PsCounters* c);
 // row item -> triangle table for k_span (large flushes; k_span searches row_off without it)
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
                 PsCounters* cnt);
@@ -25,10 +31,12 @@ void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, un
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
               int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
               int64_t cap, PsCounters* cnt);
-// the pieces, into runs of the tile lists f.tstart / f.runs; counts tcnt back down to zero
+// the pieces, into runs of the tile lists f.tstart / f.runs; counts tcnt back down to zero.
+// sizes: the flush's size record (sizes[3] receives 0xffffffff when the pieces
+// do not fit the 32-bit tile totals)
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
                   const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,
-                  float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
+                  float* ck, int64_t span_cap, int64_t seg_cap, uint32_t* sizes, PsCounters* cnt);
 size_t psk_raster_lds(int chunk_w, int waves);
 void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris, const PsDraw* draws,
                 const PsFrame& f, int waves_per_group, PsCounters* cnt);

@@ -300,18 +300,22 @@ __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draw
     return d;
 }
 
+// row_part (flushes of one programme): the block's row total, for the scan
+// of the row counts
 template <int PROG, bool SHARE>
 __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
                                                float* __restrict__ vary, int nvs, float* __restrict__ grads,
-                                               uint32_t* __restrict__ nrows, PsCounters* cnt,
-                                               const PsView* __restrict__ views)
+                                               uint32_t* __restrict__ nrows, uint32_t* __restrict__ row_part,
+                                               PsCounters* cnt, const PsView* __restrict__ views)
 {
     typedef PsProc<PROG> P;
     extern __shared__ float su[];   // per wave: the varyings of its 64 triangles, 3*nvs floats each
+    __shared__ uint32_t wrows[2];
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool live = t < T;
     int di = ps_find_draw_wave(draws, nd, live ? t : T - 1);
     unsigned culled = 0, zrej = 0, dropped = 0;
+    uint32_t myrows = 0;
     const int lane = threadIdx.x & 63, n3 = 3 * nvs;
     float* wl = su + (size_t)(threadIdx.x & ~63) * n3;
     bool stage = false;
@@ -417,6 +421,7 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
             }
         }
         nrows[t] = n;
+        myrows = n;
         // varyings of triangles with rows, or of all when other views read them
         stage = SHARE ? !skip && (pieces || d.vary_all) : pieces;
         if (SHARE && stage) {
@@ -447,6 +452,12 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
     ps_wave_add(cnt, PS_ST_CULLED, culled, true);
     ps_wave_add(cnt, PS_ST_ZREJ, zrej, true);
     ps_wave_add(cnt, PS_ST_DROPPED, dropped, true);
+    if (row_part) {
+        const uint32_t w = __builtin_amdgcn_readlane(ps_wave_incl_scan(myrows), 63);
+        if (lane == 0) wrows[threadIdx.x >> 6] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) row_part[blockIdx.x] = wrows[0] + wrows[1];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -455,20 +466,35 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
 // ---------------------------------------------------------------------------
 #define SCAN_ITEMS 16
 #define SCAN_BLOCK (256 * SCAN_ITEMS)
-#define SCAN1_THREADS 1024
-#define SCAN1_MAX (SCAN1_THREADS * SCAN_ITEMS)
+
+// inclusive wave scan of uint32, or of the two 32-bit halves of a uint64
+// separately (the packed counts never carry from the low half)
+__device__ __forceinline__ uint32_t ps_wave_incl_scan_t(uint32_t v) { return ps_wave_incl_scan(v); }
+__device__ __forceinline__ unsigned long long ps_wave_incl_scan_t(unsigned long long v)
+{
+    return (unsigned long long)ps_wave_incl_scan((uint32_t)v) |
+           ((unsigned long long)ps_wave_incl_scan((uint32_t)(v >> 32)) << 32);
+}
+
+__device__ __forceinline__ uint32_t ps_wave_bcast63(uint32_t v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ uint32_t ps_wave_bcast0(uint32_t v) { return __builtin_amdgcn_readlane(v, 0); }
+__device__ __forceinline__ unsigned long long ps_wave_bcast0(unsigned long long v)
+{
+    return (unsigned long long)(uint32_t)__builtin_amdgcn_readlane((uint32_t)v, 0) |
+           ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 0) << 32);
+}
+__device__ __forceinline__ unsigned long long ps_wave_bcast63(unsigned long long v)
+{
+    return (unsigned long long)(uint32_t)__builtin_amdgcn_readlane((uint32_t)v, 63) |
+           ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 63) << 32);
+}
 
 // exclusive scan of one value per thread over a block of NW waves
 template <typename T, int NW>
 __device__ __forceinline__ T ps_block_excl_scan(T v, T* lds, T& total)
 {
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    T x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        T y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    T x = ps_wave_incl_scan_t(v);
     if (lane == 63) lds[wave] = x;
     __syncthreads();
     T wbase = 0;
@@ -483,56 +509,64 @@ __device__ __forceinline__ T ps_block_excl_scan(T v, T* lds, T& total)
     return wbase + x - v;
 }
 
-// thread t of a block owns items [base, base + SCAN_ITEMS)
+// Wave-striped ownership: the wave owns items [wbase, wbase + 64 * SCAN_ITEMS),
+// in groups of 64 * PER (PER = the items in 16 bytes); in group g the lane
+// owns the PER consecutive items wbase + g * 64 * PER + lane * PER + j, so
+// each group is one coalesced 16-byte load or store per lane.
 template <typename T>
-__device__ __forceinline__ void ps_load_items(const T* in, int64_t base, int64_t n, T* v)
+__device__ __forceinline__ void ps_load_items(const T* in, int64_t wbase, int64_t n, T* v)
 {
     constexpr int PER = 16 / sizeof(T);
-    if (base + SCAN_ITEMS <= n) {
-        const uint4* p = (const uint4*)(in + base);
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-        for (int q = 0; q < SCAN_ITEMS / PER; q++) {
-            uint4 u = p[q];
+    for (int g = 0; g < SCAN_ITEMS / PER; g++) {
+        const int64_t i = wbase + (int64_t)g * 64 * PER + lane * PER;
+        if (i + PER <= n) {
+            const uint4 u = *(const uint4*)(in + i);
             const T* t = (const T*)&u;
 #pragma unroll
-            for (int j = 0; j < PER; j++) v[PER * q + j] = t[j];
-        }
-    } else {
+            for (int j = 0; j < PER; j++) v[g * PER + j] = t[j];
+        } else {
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : (T)0;
+            for (int j = 0; j < PER; j++) v[g * PER + j] = i + j < n ? in[i + j] : (T)0;
+        }
     }
 }
 
-// writes the exclusive prefix of v (starting at run) to out[base ...]
+// the wave's exclusive prefix of its items (from `run` on) into out
 template <typename T>
-__device__ __forceinline__ void ps_store_scan(T* out, int64_t base, int64_t n, const T* v, T run)
+__device__ __forceinline__ void ps_store_scan(T* out, int64_t wbase, int64_t n, const T* v, T run)
 {
     constexpr int PER = 16 / sizeof(T);
-    if (base + SCAN_ITEMS <= n) {
-        uint4* p = (uint4*)(out + base);
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-        for (int q = 0; q < SCAN_ITEMS / PER; q++) {
+    for (int g = 0; g < SCAN_ITEMS / PER; g++) {
+        T sum = 0;
+#pragma unroll
+        for (int j = 0; j < PER; j++) sum += v[g * PER + j];
+        const T x = ps_wave_incl_scan_t(sum);
+        T r = run + x - sum;
+        const int64_t i = wbase + (int64_t)g * 64 * PER + lane * PER;
+        if (i + PER <= n) {
             uint4 u;
             T* t = (T*)&u;
 #pragma unroll
-            for (int j = 0; j < PER; j++) { t[j] = run; run += v[PER * q + j]; }
-            p[q] = u;
+            for (int j = 0; j < PER; j++) { t[j] = r; r += v[g * PER + j]; }
+            *(uint4*)(out + i) = u;
+        } else {
+            for (int j = 0; j < PER; j++) {
+                if (i + j < n) out[i + j] = r;
+                r += v[g * PER + j];
+            }
         }
-    } else {
-        for (int i = 0; i < SCAN_ITEMS; i++) {
-            if (base + i < n) out[base + i] = run;
-            run += v[i];
-        }
+        run += ps_wave_bcast63(x);
     }
 }
 
-// the total: out[n], and its low 32 bits into *total_out with their running
-// maximum at total_out[2] (largest total since the counters were reset)
 // The tile scan (T = uint64) adds 32-bit piece counts: when the flush's
 // pieces, which k_span also counts in 64 bits into the pending statistics,
 // do not fit, it reports the largest total and flags an overflow, so that
-// the flush is split.
-// Called by the 64 lanes of one wave.
+// the flush is split.  Called by the 64 lanes of one wave.
 template <typename T>
 __device__ __forceinline__ void ps_scan_total(T tot, int64_t n, int64_t cap, T* out, uint32_t* total_out,
                                               const PsCounters* c)
@@ -563,11 +597,11 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const T* __restrict__ in, c
 {
     __shared__ T lds[4];
     int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t wbase = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)(threadIdx.x >> 6) * (64 * SCAN_ITEMS);
     T s = 0;
-    if (!(c && ps_poisoned(c)) && base < n) {
+    if (!(c && ps_poisoned(c)) && wbase < n) {
         T v[SCAN_ITEMS];
-        ps_load_items(in, base, n, v);
+        ps_load_items(in, wbase, n, v);
 #pragma unroll
         for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     }
@@ -587,11 +621,12 @@ __device__ __forceinline__ T ps_block_sum_prefix(const T* partial, int end, T* l
     return tot;
 }
 
-// block b: prefix of the block sums before it, then the block's own scan.
-// Block 0 also writes the total.
+// block b: prefix of the partial sums before it (partial i sums items
+// [i * pgran, (i + 1) * pgran), pgran dividing SCAN_BLOCK), then the block's
+// own scan.  Block 0 also writes the total.
 template <typename T>
 __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
-                                                    const T* __restrict__ partial, int nb, T* __restrict__ out,
+                                                    const T* __restrict__ partial, int nb, int pgran, T* __restrict__ out,
                                                     uint32_t* total_out, int64_t cap, const PsCounters* c)
 {
     __shared__ T lds[4];
@@ -602,36 +637,51 @@ __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, co
         if (threadIdx.x < 64) ps_scan_total(tot, n, cap, out, total_out, c);
     }
     if (blockIdx.x * (int64_t)SCAN_BLOCK >= n) return;
-    T before = blockIdx.x ? ps_block_sum_prefix(partial, blockIdx.x, lds) : (T)0;
-    int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    T before = blockIdx.x ? ps_block_sum_prefix(partial, blockIdx.x * (SCAN_BLOCK / pgran), lds) : (T)0;
+    int64_t wbase = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)(threadIdx.x >> 6) * (64 * SCAN_ITEMS);
     T v[SCAN_ITEMS];
-    ps_load_items(in, base, n, v);
+    ps_load_items(in, wbase, n, v);
     T s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     T tot;
-    ps_store_scan(out, base, n, v, before + ps_block_excl_scan<T, 4>(s, lds, tot));
+    // the waves before this one: the block scan of the lane sums, taken at lane 0
+    const T wpre = ps_block_excl_scan<T, 4>(s, lds, tot);
+    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre));
 }
 
-// the whole scan in one block, for up to SCAN1_MAX items
+// One pass for short inputs (up to SCAN_DIRECT_MAX items): block b sums the
+// items before its own SCAN_BLOCK directly, then scans its part; the last
+// block writes the total.
+#define SCAN_DIRECT_MAX (16 * SCAN_BLOCK)
 template <typename T>
-__global__ void __launch_bounds__(SCAN1_THREADS) k_scan_one(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
-                                                            T* __restrict__ out, uint32_t* total_out, int64_t cap,
-                                                            const PsCounters* c)
+__global__ void __launch_bounds__(256) k_scan_direct(const T* __restrict__ in, int64_t n, T* __restrict__ out,
+                                                     uint32_t* total_out, const PsCounters* c)
 {
-    __shared__ T lds[SCAN1_THREADS / 64];
+    __shared__ T lds[4];
     if (c && ps_poisoned(c)) return;
-    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    int64_t base = (int64_t)threadIdx.x * SCAN_ITEMS;
-    T v[SCAN_ITEMS];
-    ps_load_items(in, base, n, v);
-    T s = 0;
+    constexpr int PER = 16 / sizeof(T);
+    const int64_t b0 = blockIdx.x * (int64_t)SCAN_BLOCK;
+    T pre = 0;
+#pragma unroll 4
+    for (int64_t j = (int64_t)threadIdx.x * PER; j < b0; j += 256 * PER) {   // b0 is a multiple of PER
+        const uint4 u = *(const uint4*)(in + j);
+        const T* t = (const T*)&u;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
+        for (int q = 0; q < PER; q++) pre += t[q];
+    }
+    T before;
+    ps_block_excl_scan<T, 4>(pre, lds, before);
+    const int64_t wbase = b0 + (int64_t)(threadIdx.x >> 6) * (64 * SCAN_ITEMS);
+    T v[SCAN_ITEMS];
+    ps_load_items(in, wbase, n, v);
+    T sum = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) sum += v[i];
     T tot;
-    T run = ps_block_excl_scan<T, SCAN1_THREADS / 64>(s, lds, tot);
-    if (base < n) ps_store_scan(out, base, n, v, run);
-    if (threadIdx.x < 64) ps_scan_total(tot, n, cap, out, total_out, c);
+    const T wpre = ps_block_excl_scan<T, 4>(sum, lds, tot);
+    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre));
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) ps_scan_total(before + tot, n, n, out, total_out, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,7 +866,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                                                   int64_t T, PsFrame f, const PsTri* __restrict__ tris,
                                                   const PsDraw* __restrict__ draws, const float* __restrict__ vary, int nvs,
                                                   unsigned long long* __restrict__ tcnt, PsSeg* __restrict__ segs,
-                                                  float* __restrict__ ck, int64_t cap, PsCounters* cnt)
+                                                  float* __restrict__ ck, int64_t cap, uint32_t* sizes, PsCounters* cnt)
 {
     __shared__ uint32_t rel_all[4][SG_BATCH][PS_WAVE];   // per wave: [iteration][lane] offset in the run
     if (ps_poisoned(cnt)) return;
@@ -1685,17 +1735,19 @@ void psk_fills(hipStream_t s, const PsFill* fills, int n, int64_t max_pieces, co
 }
 
 
+int psk_setup_block() { return 128; }
+
 void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
-               float* grads, uint32_t* nrows, PsCounters* cnt, const PsView* views)
+               float* grads, uint32_t* nrows, uint32_t* row_part, PsCounters* cnt, const PsView* views)
 {
     dim3 g(gsz(T, 128)), b(128);
     size_t lds = (size_t)128 * 3 * nvs * sizeof(float);
 #define SU_CASE(K)                                                                                                  \
     case K:                                                                                                         \
         if (share && ps_shares_varyings(K))                                                                         \
-            hipLaunchKernelGGL((k_setup<K, true>), g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); \
+            hipLaunchKernelGGL((k_setup<K, true>), g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, row_part, cnt, views); \
         else                                                                                                        \
-            hipLaunchKernelGGL((k_setup<K, false>), g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, cnt, views); \
+            hipLaunchKernelGGL((k_setup<K, false>), g, b, lds, s, draws, nd, T, tris, vary, nvs, grads, nrows, row_part, cnt, views); \
         break;
     switch (prog) {
     SU_CASE(PSGPU_PROC_DEF01) SU_CASE(PSGPU_PROC_DEF02) SU_CASE(PSGPU_PROC_DEF03) SU_CASE(PSGPU_PROC_DEF04)
@@ -1712,22 +1764,29 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
 {
     int nb = psk_scan_blocks(cap);
     hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
-    hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, out, total_out,
-                       cap, c);
+    hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, SCAN_BLOCK, out,
+                       total_out, cap, c);
+}
+
+void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int64_t n_parts,
+                    int pgran, uint32_t* total_out, const PsCounters* c)
+{
+    hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, in, nullptr, n, parts,
+                       (int)n_parts, pgran, out, total_out, n, c);
 }
 
 void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, unsigned long long* tstart,
                     unsigned long long* partial, uint32_t* total_out, const PsCounters* c)
 {
-    if (n <= SCAN1_MAX) {
-        hipLaunchKernelGGL(k_scan_one<unsigned long long>, dim3(1), dim3(SCAN1_THREADS), 0, s, tcnt, nullptr, n, tstart,
-                           total_out, n, c);
+    if (n <= SCAN_DIRECT_MAX) {
+        hipLaunchKernelGGL(k_scan_direct<unsigned long long>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, tcnt, n, tstart,
+                           total_out, c);
         return;
     }
     int nb = psk_scan_blocks(n);
     hipLaunchKernelGGL(k_scan_reduce<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, c);
-    hipLaunchKernelGGL(k_scan_final<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, nb, tstart,
-                       total_out, n, c);
+    hipLaunchKernelGGL(k_scan_final<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, nb,
+                       SCAN_BLOCK, tstart, total_out, n, c);
 }
 
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
@@ -1752,11 +1811,11 @@ void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* 
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
                   const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,
-                  float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
+                  float* ck, int64_t span_cap, int64_t seg_cap, uint32_t* sizes, PsCounters* cnt)
 {
     int g = gsz(span_cap, 256); if (g > 2048) g = 2048;
     hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, row_off, T, f, tris, draws, vary, nvs, tcnt, segs, ck,
-                       seg_cap, cnt);
+                       seg_cap, sizes, cnt);
 }
 
 size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * RA_WORDS(chunk_w) * 4; }

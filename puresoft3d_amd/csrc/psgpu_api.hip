@@ -534,6 +534,13 @@ int stage_blob(psgpu_ctx* c, const Blob& b, uint8_t** out)
     return PSGPU_OK;
 }
 
+// partial sums of the row scan: one per scan block, or one per setup block
+int64_t row_partials(int64_t T)
+{
+    const int sb = psk_setup_block();
+    return std::max<int64_t>(psk_scan_blocks(T), (T + sb - 1) / sb) + 1;
+}
+
 // size the per-flush scratch for a flush of T triangles, nvs varying floats
 // per vertex and n_tiles raster tiles at the current span and piece capacities
 int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
@@ -550,7 +557,7 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
     if (b.span_cap > c->max_spans || b.seg_cap > c->max_segs)
         return fail(c, PSGPU_ERR_NOMEM, "more than %lld spans or %lld pieces in one flush", (long long)c->max_spans,
                     (long long)c->max_segs);
-    if ((rc = ensure(c, b.partial, psk_scan_blocks(T) + 1)) ||
+    if ((rc = ensure(c, b.partial, row_partials(T))) ||
         (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) || (rc = ensure(c, b.spans, b.span_cap)) ||
         (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)) ||
         (rc = ensure(c, b.runs, (size_t)b.seg_cap * 2)) || (rc = ensure(c, b.runs2, (size_t)b.seg_cap * 2)) ||
@@ -600,18 +607,27 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     hipStream_t s = c->stream;
     int nd = blob.nd;
     const PsView* dv = f.views;
+    // a flush of one programme has one setup launch, which also leaves the
+    // row totals of its blocks for the row scan
+    int n_progs = 0;
+    for (int k = 1; k < PS_N_PROGS; k++) n_progs += progs[k] ? 1 : 0;
+    uint32_t* row_part = n_progs == 1 ? c->fb.partial.p : nullptr;
     {
         ProfScope p(c, "setup");
         for (int k = 1; k < PS_N_PROGS; k++)
             if (progs[k])
                 psk_setup(s, k, blob.shares[k], dd, nd, T, c->fb.tris.p, c->fb.vary.p, nvs, c->fb.grad.p, c->fb.nrows.p,
-                          c->d_cnt, dv);
+                          row_part, c->d_cnt, dv);
     }
     {
         ProfScope p(c, "scan_rows");
         // row counts never depend on buffer capacities: always scanned, so the
         // largest row total of the log is known even after an overflow
-        psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
+        const int sb = psk_setup_block();
+        if (row_part)
+            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, (T + sb - 1) / sb, sb, c->d_sizes + 0, nullptr);
+        else
+            psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
     }
     // up to kSpanSearchTris triangles k_span finds each row's triangle by a
     // search in the row offsets; beyond, where views leave long runs of
@@ -636,7 +652,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     {
         ProfScope p(c, "segments");
         psk_segments(s, c->fb.spans.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs, c->fb.tcnt.p,
-                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_cnt);
+                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_sizes, c->d_cnt);
     }
     int imm = 0;   // a flush of one programme that shades in the raster walk
     {
@@ -1184,7 +1200,7 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
     const int64_t T = sh.T;
     if (!a.take(b.winner, sh.n_pix) || !a.take(b.tris, T) || !a.take(b.vary, (size_t)T * 3 * sh.nvs) ||
         !a.take(b.grad, (size_t)T * 8) || !a.take(b.nrows, T) || !a.take(b.rowoff, T + 1) ||
-        !a.take(b.partial, psk_scan_blocks(T) + 1) || !a.take(b.tcnt, sh.n_tiles + 1) ||
+        !a.take(b.partial, row_partials(T)) || !a.take(b.tcnt, sh.n_tiles + 1) ||
         !a.take(b.tstart, sh.n_tiles + 1) || !a.take(b.tpartial, psk_scan_blocks(sh.n_tiles) + 1))
         return false;
     const bool expand = T > kSpanSearchTris;
