@@ -25,12 +25,14 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
 // exclusive scan of the per-tile counts (pieces | runs << 32) into tstart
 // (n + 1 entries); the piece total into *total_out, its running maximum at
 // total_out[2]
+// heavy (optional, 1 + psk_heavy_cap() words): the heavy-tile list of PsFrame.heavy
+int psk_heavy_cap();
 void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, unsigned long long* tstart,
-                    unsigned long long* partial, uint32_t* total_out, const PsCounters* c);
+                    unsigned long long* partial, uint32_t* total_out, const PsCounters* c, uint32_t* heavy);
 // spans of every row item; tcnt[t] += the pieces in tile t | the waves with pieces there << 32
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
               int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
-              int64_t cap, PsCounters* cnt);
+              int64_t cap, uint32_t* heavy, PsCounters* cnt);
 // the pieces, into runs of the tile lists f.tstart / f.runs; counts tcnt back down to zero.
 // sizes: the flush's size record (sizes[3] receives 0xffffffff when the pieces
 // do not fit the 32-bit tile totals)

@@ -125,6 +125,9 @@ __device__ __forceinline__ void ps_with_draw(const PsDraw* __restrict__ draws, i
     __builtin_amdgcn_update_dpp(0u, (v), (ctrl), (rmask), 0xf, (bound))
 
 // the minimum, the same in every lane
+// a tile start without the heavy-tile mark (bit 63)
+#define PS_TS(x) ((x) & 0x7fffffffffffffffull)
+
 __device__ __forceinline__ uint32_t ps_wave_min(uint32_t v)
 {
     // within rows of 16: quad swaps, then the half-row and row mirrors
@@ -466,6 +469,23 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
 // ---------------------------------------------------------------------------
 #define SCAN_ITEMS 16
 #define SCAN_BLOCK (256 * SCAN_ITEMS)
+#define PS_HEAVY_CAP 1024
+
+// Pieces that make a tile heavy: three times the flush's mean per tile (at
+// least 96), the flush's pieces from k_span's count (in units of 256, which
+// is plenty for a threshold).  ps_heavy_load reads one count stripe per lane
+// early; ps_heavy_min (every lane of the wave) sums them.
+__device__ __forceinline__ uint32_t ps_heavy_load(const PsCounters* c, const uint32_t* heavy)
+{
+    return heavy ? (uint32_t)min(c->pend[threadIdx.x & 63][PS_ST_SEGS] >> 8, 0xffffffffull) : 0u;
+}
+
+__device__ __forceinline__ uint32_t ps_heavy_min(uint32_t pieces256, int64_t n_tiles)
+{
+    const unsigned long long p = (unsigned long long)__builtin_amdgcn_readlane(ps_wave_incl_scan(pieces256), 63) << 8;
+    const unsigned long long h = 3 * p / (unsigned long long)(n_tiles > 0 ? n_tiles : 1);
+    return (uint32_t)min(max(h, 96ull), 0xffffffffull);
+}
 
 // inclusive wave scan of uint32, or of the two 32-bit halves of a uint64
 // separately (the packed counts never carry from the low half)
@@ -533,12 +553,36 @@ __device__ __forceinline__ void ps_load_items(const T* in, int64_t wbase, int64_
     }
 }
 
-// the wave's exclusive prefix of its items (from `run` on) into out
+// The wave's exclusive prefix of its items (from `run` on) into out.  Tile
+// scans (T = uint64, heavy given) also list the tiles of at least heavy_min
+// pieces in heavy[1 ..] (heavy[0] counts them; up to PS_HEAVY_CAP are listed,
+// one atomic per wave) and mark the listed ones with bit 63 of their start.
 template <typename T>
-__device__ __forceinline__ void ps_store_scan(T* out, int64_t wbase, int64_t n, const T* v, T run)
+__device__ __forceinline__ void ps_store_scan(T* out, int64_t wbase, int64_t n, const T* v, T run,
+                                              uint32_t heavy_min = 0xffffffffu, uint32_t* heavy = nullptr)
 {
     constexpr int PER = 16 / sizeof(T);
     const int lane = threadIdx.x & 63;
+    uint32_t hslot = 0xffffffffu;   // the lane's first slot in the heavy list
+    if constexpr (sizeof(T) == 8) {
+        if (heavy) {
+            uint32_t nh = 0;
+#pragma unroll
+            for (int g = 0; g < SCAN_ITEMS / PER; g++)
+#pragma unroll
+                for (int j = 0; j < PER; j++) {
+                    const int64_t i = wbase + (int64_t)g * 64 * PER + lane * PER + j;
+                    nh += (i < n && (uint32_t)v[g * PER + j] >= heavy_min) ? 1u : 0u;
+                }
+            const uint32_t incl = ps_wave_incl_scan(nh);
+            const uint32_t tot = __builtin_amdgcn_readlane(incl, 63);
+            if (tot) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(heavy, tot);
+                hslot = __builtin_amdgcn_readlane(base, 0) + incl - nh;
+            }
+        }
+    }
 #pragma unroll
     for (int g = 0; g < SCAN_ITEMS / PER; g++) {
         T sum = 0;
@@ -547,17 +591,30 @@ __device__ __forceinline__ void ps_store_scan(T* out, int64_t wbase, int64_t n, 
         const T x = ps_wave_incl_scan_t(sum);
         T r = run + x - sum;
         const int64_t i = wbase + (int64_t)g * 64 * PER + lane * PER;
+        T o[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            o[j] = r;
+            if constexpr (sizeof(T) == 8) {
+                if (hslot != 0xffffffffu && i + j < n && (uint32_t)v[g * PER + j] >= heavy_min) {
+                    if (hslot < PS_HEAVY_CAP) {
+                        heavy[1 + hslot] = (uint32_t)(i + j);
+                        o[j] |= 1ull << 63;
+                    }
+                    hslot++;
+                }
+            }
+            r += v[g * PER + j];
+        }
         if (i + PER <= n) {
             uint4 u;
             T* t = (T*)&u;
 #pragma unroll
-            for (int j = 0; j < PER; j++) { t[j] = r; r += v[g * PER + j]; }
+            for (int j = 0; j < PER; j++) t[j] = o[j];
             *(uint4*)(out + i) = u;
         } else {
-            for (int j = 0; j < PER; j++) {
-                if (i + j < n) out[i + j] = r;
-                r += v[g * PER + j];
-            }
+            for (int j = 0; j < PER; j++)
+                if (i + j < n) out[i + j] = o[j];
         }
         run += ps_wave_bcast63(x);
     }
@@ -627,10 +684,12 @@ __device__ __forceinline__ T ps_block_sum_prefix(const T* partial, int end, T* l
 template <typename T>
 __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
                                                     const T* __restrict__ partial, int nb, int pgran, T* __restrict__ out,
-                                                    uint32_t* total_out, int64_t cap, const PsCounters* c)
+                                                    uint32_t* total_out, int64_t cap, const PsCounters* c,
+                                                    uint32_t* heavy)
 {
     __shared__ T lds[4];
     if (c && ps_poisoned(c)) return;
+    const uint32_t hp = ps_heavy_load(c, heavy);
     int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
     if (blockIdx.x == 0) {
         T tot = ps_block_sum_prefix(partial, nb, lds);
@@ -647,7 +706,8 @@ __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, co
     T tot;
     // the waves before this one: the block scan of the lane sums, taken at lane 0
     const T wpre = ps_block_excl_scan<T, 4>(s, lds, tot);
-    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre));
+    const uint32_t hm = heavy ? ps_heavy_min(hp, n) : 0xffffffffu;
+    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre), hm, heavy);
 }
 
 // One pass for short inputs (up to SCAN_DIRECT_MAX items): block b sums the
@@ -656,10 +716,11 @@ __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, co
 #define SCAN_DIRECT_MAX (16 * SCAN_BLOCK)
 template <typename T>
 __global__ void __launch_bounds__(256) k_scan_direct(const T* __restrict__ in, int64_t n, T* __restrict__ out,
-                                                     uint32_t* total_out, const PsCounters* c)
+                                                     uint32_t* total_out, const PsCounters* c, uint32_t* heavy)
 {
     __shared__ T lds[4];
     if (c && ps_poisoned(c)) return;
+    const uint32_t hp = ps_heavy_load(c, heavy);
     constexpr int PER = 16 / sizeof(T);
     const int64_t b0 = blockIdx.x * (int64_t)SCAN_BLOCK;
     T pre = 0;
@@ -680,7 +741,8 @@ __global__ void __launch_bounds__(256) k_scan_direct(const T* __restrict__ in, i
     for (int i = 0; i < SCAN_ITEMS; i++) sum += v[i];
     T tot;
     const T wpre = ps_block_excl_scan<T, 4>(sum, lds, tot);
-    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre));
+    const uint32_t hm = heavy ? ps_heavy_min(hp, n) : 0xffffffffu;
+    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre), hm, heavy);
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) ps_scan_total(before + tot, n, n, out, total_out, c);
 }
 
@@ -766,9 +828,10 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
                                               int64_t T, int64_t cap,
                                               PsSpan* __restrict__ spans, unsigned long long* __restrict__ tcnt,
                                               const PsView* __restrict__ views, int chunk_w, int tile_rows, int tile_cols,
-                                              PsCounters* cnt)
+                                              uint32_t* heavy, PsCounters* cnt)
 {
     if (ps_poisoned(cnt)) return;
+    if (heavy && blockIdx.x == 0 && threadIdx.x == 0) heavy[0] = 0;   // the tile scan lists this flush's heavy tiles
     int64_t R = row_off[T];
     if (R > cap) {   // more rows than span records: flag it, write nothing
         if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&cnt->overflow, 1);
@@ -943,7 +1006,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
             if (lane < nb) {
                 const unsigned long long take = (unsigned long long)bn | (1ull << 32);
                 const unsigned long long left = atomicAdd(&tcnt[bk], (unsigned long long)0 - take);
-                const unsigned long long end = f.tstart[bk + 1];
+                const unsigned long long end = PS_TS(f.tstart[bk + 1]);
                 const uint32_t fp = (uint32_t)end - (uint32_t)left;
                 const uint32_t slot = (uint32_t)(end >> 32) - (uint32_t)(left >> 32);
                 // the counts k_span left must cover this run (they always do:
@@ -1139,7 +1202,10 @@ __device__ uint32_t* ps_order_runs(uint32_t* a, uint32_t* b, uint32_t nr, uint32
 // counts and offsets
 #define RA_WORDS(cw) ((cw) * 64 * 2 + RA_CHUNK * 7 + 128)
 
-template <bool DEBUG>
+// KCW: the chunk width when it is 1, 2, 4 or 8, else 0.  With a known width
+// each lane keeps its pixels' depth and winner in registers, so the walk has
+// no LDS round trip per pixel; the LDS columns serve wider chunks.
+template <bool DEBUG, int KCW>
 __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
                          const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws, PsFrame f, int64_t n_tiles,
                          PsCounters* cnt)
@@ -1147,15 +1213,27 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
     extern __shared__ uint32_t lds[];
     if (ps_poisoned(cnt)) return;
     ps_commit_pending(cnt);
-    const int CW = f.chunk_w;
+    const int CW = KCW ? KCW : f.chunk_w;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int waves = blockDim.x >> 6;
-    int64_t tile = ps_xcd_remap<4>(blockIdx.x, gridDim.x) * (int64_t)waves + wave;   // neighbouring tiles share an L2
+    // neighbouring tiles share an L2; with a heavy-tile list, the first
+    // PS_HEAVY_CAP waves take the listed tiles (they start first, so the
+    // longest walks do not end the kernel), the others every unlisted tile
+    int64_t tile = ps_xcd_remap<4>(blockIdx.x, gridDim.x) * (int64_t)waves + wave;
+    if (f.heavy) {
+        if (tile < PS_HEAVY_CAP) {
+            if (tile >= min(f.heavy[0], (uint32_t)PS_HEAVY_CAP)) return;
+            tile = f.heavy[1 + tile];
+        } else {
+            tile -= PS_HEAVY_CAP;
+            if (tile >= n_tiles || (f.tstart[tile] >> 63)) return;
+        }
+    }
     if (tile >= n_tiles) return;
-    uint32_t* wl = lds + (size_t)wave * RA_WORDS(CW);
-    float* dep = (float*)wl;
-    int* win = (int*)(dep + CW * 64);
-    uint32_t* r_s = (uint32_t*)(win + CW * 64);   // segment index
+    uint32_t* wl = lds + (size_t)wave * RA_WORDS(KCW ? 0 : CW);
+    float* dep = (float*)wl;                       // LDS columns: chunks wider than 8 only
+    int* win = (int*)(dep + (KCW ? 0 : CW) * 64);
+    uint32_t* r_s = (uint32_t*)(win + (KCW ? 0 : CW) * 64);   // segment index
     uint32_t* r_fl = r_s + RA_CHUNK;               // behaviour | writes colour (PsSeg.si high bits)
     uint32_t* r_x = r_fl + RA_CHUNK;               // xa | xb << 16
     float* r_cf2 = (float*)(r_x + RA_CHUNK);
@@ -1180,15 +1258,25 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
     // a clear fused into the flush: the tile starts from its value, and the
     // whole tile is written back
     const bool dclear = V.dclear != 0;
-    for (int c = 0; c < ncol; c++) {
-        dep[c * 64 + lane] = dclear ? V.dclear_value : drow[X0 + c];
-        win[c * 64 + lane] = -1;
+    float dv[KCW ? KCW : 1];
+    int wv[KCW ? KCW : 1];
+    if constexpr (KCW > 0) {
+#pragma unroll
+        for (int k = 0; k < KCW; k++) {
+            dv[k] = k < ncol ? (dclear ? V.dclear_value : drow[X0 + k]) : 1.0f;
+            wv[k] = -1;
+        }
+    } else {
+        for (int c = 0; c < ncol; c++) {
+            dep[c * 64 + lane] = dclear ? V.dclear_value : drow[X0 + c];
+            win[c * 64 + lane] = -1;
+        }
     }
     unsigned long long passes = 0;
     bool dirty = dclear;
     const uint64_t lt = ps_lt_mask();
     // the tile's runs in submission order, up to 64 at a time (one per lane)
-    const unsigned long long ta = f.tstart[tile], tb = f.tstart[tile + 1];
+    const unsigned long long ta = PS_TS(f.tstart[tile]), tb = PS_TS(f.tstart[tile + 1]);
     const uint32_t n_list = (uint32_t)tb - (uint32_t)ta;
     const uint32_t r0 = (uint32_t)(ta >> 32), nr = (uint32_t)(tb >> 32) - r0;
     const uint32_t* rl = f.runs + 2 * (size_t)r0;
@@ -1311,22 +1399,46 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
                         h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
                         th = h;
                     }
-                    for (int x = a; x <= b; x++) {
-                        // interpolateNextStep (interp.cpp:82-92)
-                        float r = 1.0f / cf2;
-                        cf2 += cst;
-                        float z = pz * r;
-                        pz += pst;
-                        int c = x - X0;
-                        float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
-                        int64_t pix = (int64_t)y * VW + x;
-                        if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
-                        // fragthrd.cpp:227
-                        if (-1.0f < z && (z - cur < -0.0001f)) {
-                            passes++;
-                            if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
-                            if (wc) win[c * 64 + lane] = sidx;
-                            if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                    if constexpr (KCW > 0) {
+                        // the chunk's pixels in order, those of [a, b] stepping the accumulators
+#pragma unroll
+                        for (int k = 0; k < KCW; k++) {
+                            const int x = X0 + k;
+                            if (x < a || x > b) continue;
+                            // interpolateNextStep (interp.cpp:82-92)
+                            float r = 1.0f / cf2;
+                            cf2 += cst;
+                            float z = pz * r;
+                            pz += pst;
+                            float cur = (beh & PSGPU_TEST_DEPTH) ? dv[k] : 1.0f;
+                            int64_t pix = (int64_t)y * VW + x;
+                            if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                            // fragthrd.cpp:227
+                            if (-1.0f < z && (z - cur < -0.0001f)) {
+                                passes++;
+                                if (beh & PSGPU_UPDATE_DEPTH) { dv[k] = z; dirty = true; }
+                                if (wc) wv[k] = sidx;
+                                if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                            }
+                        }
+                    } else {
+                        for (int x = a; x <= b; x++) {
+                            // interpolateNextStep (interp.cpp:82-92)
+                            float r = 1.0f / cf2;
+                            cf2 += cst;
+                            float z = pz * r;
+                            pz += pst;
+                            int c = x - X0;
+                            float cur = (beh & PSGPU_TEST_DEPTH) ? dep[c * 64 + lane] : 1.0f;
+                            int64_t pix = (int64_t)y * VW + x;
+                            if (DEBUG) { f.dbg_rcount[pix] += 1; f.dbg_rhash[pix] ^= th; }
+                            // fragthrd.cpp:227
+                            if (-1.0f < z && (z - cur < -0.0001f)) {
+                                passes++;
+                                if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
+                                if (wc) win[c * 64 + lane] = sidx;
+                                if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
+                            }
                         }
                     }
                 }
@@ -1335,9 +1447,18 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
         }
     }
     int* wrow = V.winner + (int64_t)y * VW;
-    for (int c = 0; c < ncol; c++) {
-        if (dirty) drow[X0 + c] = dep[c * 64 + lane];
-        wrow[X0 + c] = win[c * 64 + lane];
+    if constexpr (KCW > 0) {
+#pragma unroll
+        for (int k = 0; k < KCW; k++)
+            if (k < ncol) {
+                if (dirty) drow[X0 + k] = dv[k];
+                wrow[X0 + k] = wv[k];
+            }
+    } else {
+        for (int c = 0; c < ncol; c++) {
+            if (dirty) drow[X0 + c] = dep[c * 64 + lane];
+            wrow[X0 + c] = win[c * 64 + lane];
+        }
     }
     ps_wave_add(cnt, PS_ST_SHADED, passes);
 }
@@ -1399,7 +1520,7 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     bool dirty = dclear;
     const uint64_t lt = ps_lt_mask();
     // the tile's runs in submission order, up to 64 at a time (one per lane)
-    const unsigned long long ta = f.tstart[tile], tb = f.tstart[tile + 1];
+    const unsigned long long ta = PS_TS(f.tstart[tile]), tb = PS_TS(f.tstart[tile + 1]);
     const uint32_t n_list = (uint32_t)tb - (uint32_t)ta;
     const uint32_t r0 = (uint32_t)(ta >> 32), nr = (uint32_t)(tb >> 32) - r0;
     const uint32_t* rl = f.runs + 2 * (size_t)r0;
@@ -1765,28 +1886,30 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
     int nb = psk_scan_blocks(cap);
     hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
     hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, SCAN_BLOCK, out,
-                       total_out, cap, c);
+                       total_out, cap, c, (uint32_t*)nullptr);
 }
 
 void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int64_t n_parts,
                     int pgran, uint32_t* total_out, const PsCounters* c)
 {
     hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, in, nullptr, n, parts,
-                       (int)n_parts, pgran, out, total_out, n, c);
+                       (int)n_parts, pgran, out, total_out, n, c, (uint32_t*)nullptr);
 }
 
+int psk_heavy_cap() { return PS_HEAVY_CAP; }
+
 void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, unsigned long long* tstart,
-                    unsigned long long* partial, uint32_t* total_out, const PsCounters* c)
+                    unsigned long long* partial, uint32_t* total_out, const PsCounters* c, uint32_t* heavy)
 {
     if (n <= SCAN_DIRECT_MAX) {
         hipLaunchKernelGGL(k_scan_direct<unsigned long long>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, tcnt, n, tstart,
-                           total_out, c);
+                           total_out, c, heavy);
         return;
     }
     int nb = psk_scan_blocks(n);
     hipLaunchKernelGGL(k_scan_reduce<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, c);
     hipLaunchKernelGGL(k_scan_final<unsigned long long>, dim3(nb), dim3(256), 0, s, tcnt, nullptr, n, partial, nb,
-                       SCAN_BLOCK, tstart, total_out, n, c);
+                       SCAN_BLOCK, tstart, total_out, n, c, heavy);
 }
 
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
@@ -1797,16 +1920,16 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
 
 void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
               int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
-              int64_t cap, PsCounters* cnt)
+              int64_t cap, uint32_t* heavy, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 2048) g = 2048;
     const int tile_cols = 64 / tile_rows;
     if (span_tri)
         hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans, tcnt,
-                           views, chunk_w, tile_rows, tile_cols, cnt);
+                           views, chunk_w, tile_rows, tile_cols, heavy, cnt);
     else
         hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans, tcnt,
-                           views, chunk_w, tile_rows, tile_cols, cnt);
+                           views, chunk_w, tile_rows, tile_cols, heavy, cnt);
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
@@ -1824,10 +1947,19 @@ void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* seg
                 const PsFrame& f, int waves_per_group, PsCounters* cnt)
 {
     int64_t n_tiles = f.n_tiles;
-    size_t lds = psk_raster_lds(f.chunk_w, waves_per_group);
-    dim3 g(gsz(n_tiles, waves_per_group)), b(64 * waves_per_group);
-    if (debug) hipLaunchKernelGGL(k_raster<true>, g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt);
-    else hipLaunchKernelGGL(k_raster<false>, g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt);
+    const int cw = f.chunk_w;
+    const bool known = cw == 1 || cw == 2 || cw == 4 || cw == 8;
+    size_t lds = psk_raster_lds(known ? 0 : cw, waves_per_group);
+    dim3 g(gsz(n_tiles + (f.heavy ? PS_HEAVY_CAP : 0), waves_per_group)), b(64 * waves_per_group);
+#define RA_LAUNCH(D, K) hipLaunchKernelGGL((k_raster<D, K>), g, b, lds, s, spans, segs, tris, draws, f, n_tiles, cnt)
+    switch (known ? cw : 0) {
+    case 1: if (debug) RA_LAUNCH(true, 1); else RA_LAUNCH(false, 1); break;
+    case 2: if (debug) RA_LAUNCH(true, 2); else RA_LAUNCH(false, 2); break;
+    case 4: if (debug) RA_LAUNCH(true, 4); else RA_LAUNCH(false, 4); break;
+    case 8: if (debug) RA_LAUNCH(true, 8); else RA_LAUNCH(false, 8); break;
+    default: if (debug) RA_LAUNCH(true, 0); else RA_LAUNCH(false, 0); break;
+    }
+#undef RA_LAUNCH
 }
 
 void psk_raster_imm(hipStream_t s, int prog, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris,

@@ -121,12 +121,15 @@ struct PsFrame {
     const PsView* views;
     // tile lists.  Each wave of 64 span items writes its pieces of a tile as
     // one run of consecutive pieces.  tstart[t] holds the first piece (low 32
-    // bits) and the first run (high 32 bits) of tile t (n_tiles + 1 entries);
+    // bits) and the first run (bits 32-62) of tile t (n_tiles + 1 entries);
     // run r is runs[2r] = its first piece, runs[2r+1] = wave << 12 | pieces - 1.
     // runs2 is scratch of the same size for ordering long run lists.
     const unsigned long long* tstart;
     uint32_t* runs;
     uint32_t* runs2;
+    // heavy tiles, rastered first (null: every tile in order): [0] = how
+    // many, then the listed tiles, whose tstart entries carry bit 63
+    const uint32_t* heavy;
     // optional debug planes (single-view flushes), index y*W+x
     int32_t* dbg_last; int32_t* dbg_rcount; uint32_t* dbg_rhash; int32_t* dbg_pcount; uint32_t* dbg_phash;
 };

@@ -150,6 +150,7 @@ struct FlushBufs {
     // and the run records (two words each) with scratch for reordering them
     DevBuf<unsigned long long> tcnt, tstart, tpartial;
     DevBuf<uint32_t> runs, runs2;
+    DevBuf<uint32_t> heavy;   // count and list of the heavy tiles, rastered first
     DevBuf<int> winner;
     DevBuf<PsSeg> segs;
     DevBuf<float> ck;
@@ -561,7 +562,8 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
         (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) || (rc = ensure(c, b.spans, b.span_cap)) ||
         (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)) ||
         (rc = ensure(c, b.runs, (size_t)b.seg_cap * 2)) || (rc = ensure(c, b.runs2, (size_t)b.seg_cap * 2)) ||
-        (rc = ensure(c, b.tstart, n_tiles + 1)) || (rc = ensure(c, b.tpartial, psk_scan_blocks(n_tiles) + 1)))
+        (rc = ensure(c, b.tstart, n_tiles + 1)) || (rc = ensure(c, b.tpartial, psk_scan_blocks(n_tiles) + 1)) ||
+        (rc = ensure(c, b.heavy, psk_heavy_cap() + 1)))
         return rc;
     // a new tile-count array starts at zero (a reallocation may return the
     // old address, so the test is whether one happened)
@@ -637,27 +639,30 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         ProfScope p(c, "expand");
         psk_expand(s, c->fb.nrows.p, c->fb.rowoff.p, T, c->fb.spantri.p, c->fb.span_cap, c->d_cnt);
     }
-    {
-        ProfScope p(c, "span");
-        psk_span(s, c->fb.tris.p, dd, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p, c->fb.tcnt.p,
-                 dv, f.chunk_w, f.tile_rows, c->fb.span_cap, c->d_cnt);
-    }
-    {
-        ProfScope p(c, "scan_tiles");
-        psk_scan_tiles(s, c->fb.tcnt.p, f.n_tiles, c->fb.tstart.p, c->fb.tpartial.p, c->d_sizes + 1, c->d_cnt);
-    }
-    f.tstart = c->fb.tstart.p;
-    f.runs = c->fb.runs.p;
-    f.runs2 = c->fb.runs2.p;
-    {
-        ProfScope p(c, "segments");
-        psk_segments(s, c->fb.spans.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs, c->fb.tcnt.p,
-                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_sizes, c->d_cnt);
-    }
     int imm = 0;   // a flush of one programme that shades in the raster walk
     {
         const PsDraw* hd = (const PsDraw*)blob.bytes.data();
         if (nd > 0 && (hd[0].prog == PSGPU_PROC_CLOUD || hd[0].prog == PSGPU_PROC_CLOUD_SHADOW)) imm = hd[0].prog;
+    }
+    // the deferred raster takes the tiles with the longest lists first
+    uint32_t* heavy = imm ? nullptr : c->fb.heavy.p;
+    {
+        ProfScope p(c, "span");
+        psk_span(s, c->fb.tris.p, dd, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p, c->fb.tcnt.p,
+                 dv, f.chunk_w, f.tile_rows, c->fb.span_cap, heavy, c->d_cnt);
+    }
+    {
+        ProfScope p(c, "scan_tiles");
+        psk_scan_tiles(s, c->fb.tcnt.p, f.n_tiles, c->fb.tstart.p, c->fb.tpartial.p, c->d_sizes + 1, c->d_cnt, heavy);
+    }
+    f.tstart = c->fb.tstart.p;
+    f.runs = c->fb.runs.p;
+    f.runs2 = c->fb.runs2.p;
+    f.heavy = heavy;
+    {
+        ProfScope p(c, "segments");
+        psk_segments(s, c->fb.spans.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs, c->fb.tcnt.p,
+                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_sizes, c->d_cnt);
     }
     if (imm) {
         ProfScope p(c, "raster");
@@ -1201,7 +1206,8 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
     if (!a.take(b.winner, sh.n_pix) || !a.take(b.tris, T) || !a.take(b.vary, (size_t)T * 3 * sh.nvs) ||
         !a.take(b.grad, (size_t)T * 8) || !a.take(b.nrows, T) || !a.take(b.rowoff, T + 1) ||
         !a.take(b.partial, row_partials(T)) || !a.take(b.tcnt, sh.n_tiles + 1) ||
-        !a.take(b.tstart, sh.n_tiles + 1) || !a.take(b.tpartial, psk_scan_blocks(sh.n_tiles) + 1))
+        !a.take(b.tstart, sh.n_tiles + 1) || !a.take(b.tpartial, psk_scan_blocks(sh.n_tiles) + 1) ||
+        !a.take(b.heavy, psk_heavy_cap() + 1))
         return false;
     const bool expand = T > kSpanSearchTris;
     // bytes per span: record (+ row table), and 1.5 pieces of record,
@@ -1300,7 +1306,7 @@ int psgpu_destroy(psgpu_ctx* c)
     for (auto& v : c->vbo) fr(v.d);
     fr(c->d_blob.p); fr(c->fb.tris.p); fr(c->fb.vary.p); fr(c->fb.nrows.p); fr(c->fb.rowoff.p); fr(c->fb.partial.p);
     fr(c->fb.spantri.p); fr(c->fb.spans.p); fr(c->fb.tcnt.p); fr(c->fb.tstart.p); fr(c->fb.tpartial.p);
-    fr(c->fb.runs.p); fr(c->fb.runs2.p); fr(c->fb.winner.p);
+    fr(c->fb.runs.p); fr(c->fb.runs2.p); fr(c->fb.heavy.p); fr(c->fb.winner.p);
     fr(c->fb.segs.p); fr(c->fb.ck.p); fr(c->fb.grad.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
     for (int i = 0; i < psgpu_ctx::kStage; i++) {
