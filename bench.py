@@ -52,8 +52,8 @@ def kernel_alg_bytes(name, scene):
     return {"setup": vb, "raster": px * 4, "shade": px * 4 + tb}.get(name)
 
 
-PROFILE_NAMES = {"setup": "k_setup<7>", "raster": "k_raster<false>", "shade": "k_shade<7>", "span": "k_span",
-                 "segments": "k_segments", "expand": "k_expand"}
+PROFILE_NAMES = {"setup": "k_setup<7, false>", "raster": "k_raster<false, 2>", "shade": "k_shade<7>",
+                 "span": "k_span<true>", "segments": "k_segments", "expand": "k_expand"}
 
 
 def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "traffic_terrain1080p.md")):
