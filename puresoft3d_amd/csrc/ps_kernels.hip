@@ -32,6 +32,10 @@ __device__ __forceinline__ bool ps_poisoned(const PsCounters* c)
 {
     return *((volatile const int*)&c->overflow) != 0;
 }
+// Kernels read the flag together with their first loads that stay in bounds
+// whatever an overflow left behind, and test it before the first store or
+// the first load whose address depends on the flush's device-side sizes:
+// the flag's round trip then overlaps those loads instead of preceding them.
 
 __device__ __forceinline__ uint64_t ps_lt_mask()
 {
@@ -472,18 +476,33 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
 #define PS_HEAVY_CAP 1024
 
 // Pieces that make a tile heavy: three times the flush's mean per tile (at
-// least 96), the flush's pieces from k_span's count (in units of 256, which
-// is plenty for a threshold).  ps_heavy_load reads one count stripe per lane
-// early; ps_heavy_min (every lane of the wave) sums them.
-__device__ __forceinline__ uint32_t ps_heavy_load(const PsCounters* c, const uint32_t* heavy)
+// least 96).  The flush's pieces are k_span's 64-bit count in the pending
+// statistics: ps_pend_pieces loads one stripe per lane (0 without counters),
+// ps_wave_sum64 adds the wave's stripes.
+__device__ __forceinline__ unsigned long long ps_pend_pieces(const PsCounters* c)
 {
-    return heavy ? (uint32_t)min(c->pend[threadIdx.x & 63][PS_ST_SEGS] >> 8, 0xffffffffull) : 0u;
+    static_assert(PS_STRIPES == 64, "one stripe per lane");
+    return c ? c->pend[threadIdx.x & 63][PS_ST_SEGS] : 0ull;
 }
 
-__device__ __forceinline__ uint32_t ps_heavy_min(uint32_t pieces256, int64_t n_tiles)
+__device__ __forceinline__ unsigned long long ps_wave_sum64(unsigned long long x)
 {
-    const unsigned long long p = (unsigned long long)__builtin_amdgcn_readlane(ps_wave_incl_scan(pieces256), 63) << 8;
-    const unsigned long long h = 3 * p / (unsigned long long)(n_tiles > 0 ? n_tiles : 1);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// the wave's pieces rounded down to multiples of 256 (plenty for a
+// threshold): a 32-bit DPP scan instead of 64-bit shuffles
+__device__ __forceinline__ unsigned long long ps_pend_approx(unsigned long long stripe)
+{
+    const uint32_t p256 = (uint32_t)min(stripe >> 8, (unsigned long long)0x3ffffff);
+    return (unsigned long long)__builtin_amdgcn_readlane(ps_wave_incl_scan(p256), 63) << 8;
+}
+
+__device__ __forceinline__ uint32_t ps_heavy_min(unsigned long long pieces, int64_t n_tiles)
+{
+    const unsigned long long h = 3 * pieces / (unsigned long long)(n_tiles > 0 ? n_tiles : 1);
     return (uint32_t)min(max(h, 96ull), 0xffffffffull);
 }
 
@@ -553,36 +572,47 @@ __device__ __forceinline__ void ps_load_items(const T* in, int64_t wbase, int64_
     }
 }
 
+// Heavy tiles among the lane's items (wave-striped as above): those of at
+// least heavy_min pieces.
+__device__ __forceinline__ uint32_t ps_heavy_count(int64_t wbase, int64_t n, const unsigned long long* v, uint32_t heavy_min)
+{
+    constexpr int PER = 2;
+    const int lane = threadIdx.x & 63;
+    uint32_t nh = 0;
+#pragma unroll
+    for (int g = 0; g < SCAN_ITEMS / PER; g++)
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int64_t i = wbase + (int64_t)g * 64 * PER + lane * PER + j;
+            nh += (i < n && (uint32_t)v[g * PER + j] >= heavy_min) ? 1u : 0u;
+        }
+    return nh;
+}
+
+// The lane's first slot in the heavy list for its nh heavy tiles, reserved
+// with one atomic per wave on heavy[0] (0xffffffff: the wave has none).
+__device__ __forceinline__ uint32_t ps_heavy_reserve(uint32_t nh, uint32_t* heavy)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t incl = ps_wave_incl_scan(nh);
+    const uint32_t tot = __builtin_amdgcn_readlane(incl, 63);
+    if (!tot) return 0xffffffffu;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(heavy, tot);
+    return __builtin_amdgcn_readlane(base, 0) + incl - nh;
+}
+
 // The wave's exclusive prefix of its items (from `run` on) into out.  Tile
 // scans (T = uint64, heavy given) also list the tiles of at least heavy_min
-// pieces in heavy[1 ..] (heavy[0] counts them; up to PS_HEAVY_CAP are listed,
-// one atomic per wave) and mark the listed ones with bit 63 of their start.
+// pieces in heavy[1 ..], the lane's from slot hslot on (up to PS_HEAVY_CAP
+// are listed), and mark the listed ones with bit 63 of their start.
 template <typename T>
 __device__ __forceinline__ void ps_store_scan(T* out, int64_t wbase, int64_t n, const T* v, T run,
-                                              uint32_t heavy_min = 0xffffffffu, uint32_t* heavy = nullptr)
+                                              uint32_t heavy_min = 0xffffffffu, uint32_t* heavy = nullptr,
+                                              uint32_t hslot = 0xffffffffu)
 {
     constexpr int PER = 16 / sizeof(T);
     const int lane = threadIdx.x & 63;
-    uint32_t hslot = 0xffffffffu;   // the lane's first slot in the heavy list
-    if constexpr (sizeof(T) == 8) {
-        if (heavy) {
-            uint32_t nh = 0;
-#pragma unroll
-            for (int g = 0; g < SCAN_ITEMS / PER; g++)
-#pragma unroll
-                for (int j = 0; j < PER; j++) {
-                    const int64_t i = wbase + (int64_t)g * 64 * PER + lane * PER + j;
-                    nh += (i < n && (uint32_t)v[g * PER + j] >= heavy_min) ? 1u : 0u;
-                }
-            const uint32_t incl = ps_wave_incl_scan(nh);
-            const uint32_t tot = __builtin_amdgcn_readlane(incl, 63);
-            if (tot) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(heavy, tot);
-                hslot = __builtin_amdgcn_readlane(base, 0) + incl - nh;
-            }
-        }
-    }
 #pragma unroll
     for (int g = 0; g < SCAN_ITEMS / PER; g++) {
         T sum = 0;
@@ -626,7 +656,7 @@ __device__ __forceinline__ void ps_store_scan(T* out, int64_t wbase, int64_t n, 
 // the flush is split.  Called by the 64 lanes of one wave.
 template <typename T>
 __device__ __forceinline__ void ps_scan_total(T tot, int64_t n, int64_t cap, T* out, uint32_t* total_out,
-                                              const PsCounters* c)
+                                              const PsCounters* c, unsigned long long pieces = 0)
 {
     const int lane = threadIdx.x & 63;
     if (lane == 0 && n <= cap) {
@@ -637,10 +667,6 @@ __device__ __forceinline__ void ps_scan_total(T tot, int64_t n, int64_t cap, T* 
         }
     }
     if constexpr (sizeof(T) == 8) {
-        static_assert(PS_STRIPES == 64, "one stripe per lane");
-        unsigned long long pieces = c->pend[lane][PS_ST_SEGS];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) pieces += __shfl_xor(pieces, o, 64);
         if (lane == 0 && pieces != (uint32_t)tot) {
             if (total_out) atomicMax(total_out + 2, 0xffffffffu);
             atomicOr((int*)&c->overflow, 1);
@@ -672,6 +698,7 @@ template <typename T>
 __device__ __forceinline__ T ps_block_sum_prefix(const T* partial, int end, T* lds)
 {
     T s = 0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < end; i += 256) s += partial[i];
     T tot;
     ps_block_excl_scan<T, 4>(s, lds, tot);
@@ -680,7 +707,9 @@ __device__ __forceinline__ T ps_block_sum_prefix(const T* partial, int end, T* l
 
 // block b: prefix of the partial sums before it (partial i sums items
 // [i * pgran, (i + 1) * pgran), pgran dividing SCAN_BLOCK), then the block's
-// own scan.  Block 0 also writes the total.
+// own scan.  Block 0 also writes the total.  With a host-side length the
+// overflow flag is tested only before the first store: the loads stay in
+// bounds either way, so its round trip overlaps theirs.
 template <typename T>
 __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, const uint32_t* n_ptr, int64_t n_host,
                                                     const T* __restrict__ partial, int nb, int pgran, T* __restrict__ out,
@@ -688,62 +717,128 @@ __global__ void __launch_bounds__(256) k_scan_final(const T* __restrict__ in, co
                                                     uint32_t* heavy)
 {
     __shared__ T lds[4];
-    if (c && ps_poisoned(c)) return;
-    const uint32_t hp = ps_heavy_load(c, heavy);
-    int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
-    if (blockIdx.x == 0) {
-        T tot = ps_block_sum_prefix(partial, nb, lds);
-        if (threadIdx.x < 64) ps_scan_total(tot, n, cap, out, total_out, c);
-    }
-    if (blockIdx.x * (int64_t)SCAN_BLOCK >= n) return;
-    T before = blockIdx.x ? ps_block_sum_prefix(partial, blockIdx.x * (SCAN_BLOCK / pgran), lds) : (T)0;
-    int64_t wbase = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)(threadIdx.x >> 6) * (64 * SCAN_ITEMS);
+    const bool poisoned = c && ps_poisoned(c);
+    if (n_ptr && poisoned) return;
+    const unsigned long long pend = sizeof(T) == 8 ? ps_pend_pieces(c) : 0ull;
+    const int64_t n = n_ptr ? (int64_t)*n_ptr : n_host;
+    const int64_t wbase = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)(threadIdx.x >> 6) * (64 * SCAN_ITEMS);
+    const bool mine = blockIdx.x * (int64_t)SCAN_BLOCK < n;
     T v[SCAN_ITEMS];
-    ps_load_items(in, wbase, n, v);
+    if (mine) ps_load_items(in, wbase, n, v);
+    T total = 0, before = 0;
+    if (blockIdx.x == 0) total = ps_block_sum_prefix(partial, nb, lds);
+    else if (mine) before = ps_block_sum_prefix(partial, blockIdx.x * (SCAN_BLOCK / pgran), lds);
+    if (poisoned) return;
+    const unsigned long long pieces = ps_wave_sum64(pend);
+    if (blockIdx.x == 0 && threadIdx.x < 64) ps_scan_total(total, n, cap, out, total_out, c, pieces);
+    if (!mine) return;
     T s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     T tot;
     // the waves before this one: the block scan of the lane sums, taken at lane 0
     const T wpre = ps_block_excl_scan<T, 4>(s, lds, tot);
-    const uint32_t hm = heavy ? ps_heavy_min(hp, n) : 0xffffffffu;
-    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre), hm, heavy);
+    uint32_t hm = 0xffffffffu, hslot = 0xffffffffu;
+    if constexpr (sizeof(T) == 8) {
+        if (heavy) {
+            hm = ps_heavy_min(ps_pend_approx(pend), n);
+            hslot = ps_heavy_reserve(ps_heavy_count(wbase, n, v, hm), heavy);
+        }
+    }
+    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre), hm, heavy, hslot);
 }
 
-// One pass for short inputs (up to SCAN_DIRECT_MAX items): block b sums the
-// items before its own SCAN_BLOCK directly, then scans its part; the last
-// block writes the total.
-#define SCAN_DIRECT_MAX (16 * SCAN_BLOCK)
-template <typename T>
-__global__ void __launch_bounds__(256) k_scan_direct(const T* __restrict__ in, int64_t n, T* __restrict__ out,
-                                                     uint32_t* total_out, const PsCounters* c, uint32_t* heavy)
+// One pass for tile counts up to SCAN_DIRECT_MAX.  Blocks of 16 waves; each
+// lane owns two adjacent tiles (one 16-byte load), so a block covers
+// DIRECT_BLOCK tiles and a wave needs one wave scan.  Block b adds the counts
+// before its own tiles directly (up to 8 16-byte loads per thread in flight
+// at a time, one round trip below 16k tiles) and counts the heavy tiles among
+// them on the way, so each lane knows its slots in the heavy list without
+// atomics (the list is in tile order).  The waves combine their totals in
+// one LDS exchange; the last block writes the totals.
+#define DIRECT_BLOCK 2048
+#define SCAN_DIRECT_MAX (32 * DIRECT_BLOCK)
+struct PsDirectWave {
+    unsigned long long pre, own;   // counts before the block (the wave's share), the wave's own tiles
+    uint32_t hpre, hown;           // heavy tiles among them
+};
+__global__ void __launch_bounds__(1024) k_scan_direct(const unsigned long long* __restrict__ in, int64_t n,
+                                                      unsigned long long* __restrict__ out, uint32_t* total_out,
+                                                      const PsCounters* c, uint32_t* heavy)
 {
-    __shared__ T lds[4];
-    if (c && ps_poisoned(c)) return;
-    const uint32_t hp = ps_heavy_load(c, heavy);
-    constexpr int PER = 16 / sizeof(T);
-    const int64_t b0 = blockIdx.x * (int64_t)SCAN_BLOCK;
+    typedef unsigned long long T;
+    __shared__ PsDirectWave xw[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // tested before the first store: every load stays in bounds either way,
+    // so the flag's round trip overlaps theirs
+    const bool poisoned = ps_poisoned(c);
+    const int64_t b0 = blockIdx.x * (int64_t)DIRECT_BLOCK;
+    const int64_t i = b0 + 2 * (int64_t)threadIdx.x;   // the lane's tiles i, i + 1
+    const T a = in[min(i, n - 1)], b = in[min(i + 1, n - 1)];
+    // tiles before the block: 8 loads per chunk, addresses past b0 clamped
+    // (b0 - 2, a multiple of DIRECT_BLOCK less 2) and their values dropped
+    const int nk = (int)(b0 / DIRECT_BLOCK);   // 16-byte loads per thread in all
+    ulonglong2 u[8];
+    auto chunk_load = [&](int k0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            u[k] = *(const ulonglong2*)(in + min(((int64_t)(k0 + k) * 1024 + threadIdx.x) * 2, b0 - 2));
+    };
+    if (nk > 0) chunk_load(0);
+    const unsigned long long pend = ps_pend_pieces(c);   // after the loads: one wait covers them all
+    const uint32_t hm = heavy ? ps_heavy_min(ps_pend_approx(pend), n) : 0xffffffffu;
     T pre = 0;
-#pragma unroll 4
-    for (int64_t j = (int64_t)threadIdx.x * PER; j < b0; j += 256 * PER) {   // b0 is a multiple of PER
-        const uint4 u = *(const uint4*)(in + j);
-        const T* t = (const T*)&u;
+    uint32_t hpre = 0;
+    for (int k0 = 0; k0 < nk; k0 += 8) {
+        if (k0 > 0) chunk_load(k0);
 #pragma unroll
-        for (int q = 0; q < PER; q++) pre += t[q];
+        for (int k = 0; k < 8; k++) {
+            const bool ok = k0 + k < nk;
+            const T x = ok ? u[k].x : 0ull, y = ok ? u[k].y : 0ull;
+            pre += x + y;
+            hpre += ((uint32_t)x >= hm ? 1u : 0u) + ((uint32_t)y >= hm ? 1u : 0u);   // hm >= 96: dropped values never count
+        }
     }
-    T before;
-    ps_block_excl_scan<T, 4>(pre, lds, before);
-    const int64_t wbase = b0 + (int64_t)(threadIdx.x >> 6) * (64 * SCAN_ITEMS);
-    T v[SCAN_ITEMS];
-    ps_load_items(in, wbase, n, v);
-    T sum = 0;
+    if (poisoned) return;
+    const T va = i < n ? a : 0ull, vb = i + 1 < n ? b : 0ull;
+    const uint32_t ha = (uint32_t)va >= hm ? 1u : 0u, hb = (uint32_t)vb >= hm ? 1u : 0u;
+    // the wave's inclusive scans of its own tiles and heavy marks, and its
+    // totals before the block
+    const T own = ps_wave_incl_scan_t(va + vb);
+    const uint32_t hown = ps_wave_incl_scan(ha + hb);
+    const T pre_w = ps_wave_incl_scan_t(pre);
+    const uint32_t hpre_w = ps_wave_incl_scan(hpre);
+    if (lane == 63) xw[wave] = PsDirectWave{pre_w, own, hpre_w, hown};
+    __syncthreads();
+    T before = 0, woff = 0, btot = 0;
+    uint32_t hbefore = 0, hwoff = 0, hbtot = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) sum += v[i];
-    T tot;
-    const T wpre = ps_block_excl_scan<T, 4>(sum, lds, tot);
-    const uint32_t hm = heavy ? ps_heavy_min(hp, n) : 0xffffffffu;
-    ps_store_scan(out, wbase, n, v, before + ps_wave_bcast0(wpre), hm, heavy);
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) ps_scan_total(before + tot, n, n, out, total_out, c);
+    for (int w = 0; w < 16; w++) {
+        const PsDirectWave e = xw[w];
+        before += e.pre;
+        hbefore += e.hpre;
+        woff += w < wave ? e.own : 0ull;
+        hwoff += w < wave ? e.hown : 0u;
+        btot += e.own;
+        hbtot += e.hown;
+    }
+    T r = before + woff + own - (va + vb);                 // tile i's start
+    uint32_t hs = hbefore + hwoff + hown - (ha + hb);      // its first heavy slot
+    T oa = r, ob = r + va;
+    if (heavy) {
+        if (ha && hs < PS_HEAVY_CAP) { heavy[1 + hs] = (uint32_t)i; oa |= 1ull << 63; }
+        hs += ha;
+        if (hb && hs < PS_HEAVY_CAP) { heavy[1 + hs] = (uint32_t)(i + 1); ob |= 1ull << 63; }
+    }
+    if (i + 2 <= n) {
+        *(ulonglong2*)(out + i) = make_ulonglong2(oa, ob);
+    } else if (i < n) {
+        out[i] = oa;
+    }
+    if (blockIdx.x == gridDim.x - 1 && wave == 0) {
+        ps_scan_total(before + btot, n, n, out, total_out, c, ps_wave_sum64(pend));
+        if (heavy && lane == 0) heavy[0] = hbefore + hbtot;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -830,9 +925,10 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
                                               const PsView* __restrict__ views, int chunk_w, int tile_rows, int tile_cols,
                                               uint32_t* heavy, PsCounters* cnt)
 {
-    if (ps_poisoned(cnt)) return;
-    if (heavy && blockIdx.x == 0 && threadIdx.x == 0) heavy[0] = 0;   // the tile scan lists this flush's heavy tiles
-    int64_t R = row_off[T];
+    const bool poisoned = ps_poisoned(cnt);
+    const int64_t R = row_off[T];
+    if (poisoned) return;
+    if (heavy && blockIdx.x == 0 && threadIdx.x == 0) heavy[0] = 0;   // the two-pass tile scan lists heavy tiles atomically
     if (R > cap) {   // more rows than span records: flag it, write nothing
         if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&cnt->overflow, 1);
         return;
@@ -932,9 +1028,10 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                                                   float* __restrict__ ck, int64_t cap, uint32_t* sizes, PsCounters* cnt)
 {
     __shared__ uint32_t rel_all[4][SG_BATCH][PS_WAVE];   // per wave: [iteration][lane] offset in the run
-    if (ps_poisoned(cnt)) return;
+    const bool poisoned = ps_poisoned(cnt);
     const int64_t R = row_off[T];
     const int64_t S = (uint32_t)f.tstart[f.n_tiles];
+    if (poisoned) return;
     if (S > cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->overflow, 1);
         return;
@@ -1211,25 +1308,30 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
                          PsCounters* cnt)
 {
     extern __shared__ uint32_t lds[];
-    if (ps_poisoned(cnt)) return;
-    ps_commit_pending(cnt);
+    const bool poisoned = ps_poisoned(cnt);
     const int CW = KCW ? KCW : f.chunk_w;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int waves = blockDim.x >> 6;
     // neighbouring tiles share an L2; with a heavy-tile list, the first
     // PS_HEAVY_CAP waves take the listed tiles (they start first, so the
     // longest walks do not end the kernel), the others every unlisted tile
+    // (the list and the marks are in bounds, stale or not)
     int64_t tile = ps_xcd_remap<4>(blockIdx.x, gridDim.x) * (int64_t)waves + wave;
+    bool skip = false;
     if (f.heavy) {
         if (tile < PS_HEAVY_CAP) {
-            if (tile >= min(f.heavy[0], (uint32_t)PS_HEAVY_CAP)) return;
-            tile = f.heavy[1 + tile];
+            const uint32_t listed = min(f.heavy[0], (uint32_t)PS_HEAVY_CAP);
+            const uint32_t t = f.heavy[1 + tile];
+            skip = tile >= listed;
+            tile = t;
         } else {
             tile -= PS_HEAVY_CAP;
-            if (tile >= n_tiles || (f.tstart[tile] >> 63)) return;
+            skip = tile >= n_tiles || (f.tstart[min(tile, n_tiles)] >> 63);
         }
     }
-    if (tile >= n_tiles) return;
+    if (poisoned) return;
+    ps_commit_pending(cnt);
+    if (skip || tile >= n_tiles) return;
     uint32_t* wl = lds + (size_t)wave * RA_WORDS(KCW ? 0 : CW);
     float* dep = (float*)wl;                       // LDS columns: chunks wider than 8 only
     int* win = (int*)(dep + (KCW ? 0 : CW) * 64);
@@ -1723,7 +1825,7 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
                                                const PsDraw* __restrict__ draws, PsFrame f, const PsCounters* cnt, int clears)
 {
     typedef PsProc<PROG> P;
-    if (ps_poisoned(cnt)) return;
+    const bool poisoned = ps_poisoned(cnt);
     const int64_t lb = ps_xcd_remap<4>(blockIdx.x, gridDim.x);
     const PsView& V = f.views[ps_view_of_stile(f, lb)];
     int64_t st = lb - V.stile_base;
@@ -1733,6 +1835,7 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
     if (x >= V.W || y >= V.H) return;
     const int64_t crow = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
     int w = V.winner[(int64_t)y * V.W + x];
+    if (poisoned) return;
     if (w < 0) {   // no fragment colours the pixel: a fused clear of its row, if any, is all it gets
         if (clears && crow < V.cclear_rows) V.colour[crow * V.colour_scan + x] = V.cclear_value;
         return;
@@ -1902,7 +2005,7 @@ void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, un
                     unsigned long long* partial, uint32_t* total_out, const PsCounters* c, uint32_t* heavy)
 {
     if (n <= SCAN_DIRECT_MAX) {
-        hipLaunchKernelGGL(k_scan_direct<unsigned long long>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, tcnt, n, tstart,
+        hipLaunchKernelGGL(k_scan_direct, dim3(gsz(n, DIRECT_BLOCK)), dim3(1024), 0, s, tcnt, n, tstart,
                            total_out, c, heavy);
         return;
     }

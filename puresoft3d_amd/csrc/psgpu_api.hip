@@ -824,9 +824,17 @@ bool split_flush(psgpu_ctx* c, size_t idx)
 }
 
 // remember whether the overflow flag was already set when command i finished:
-// the first command that finished with it set is where a replay starts
+// the first command that finished with it set is where a replay starts.
+// Only an eager flush can set it (clears, copies and post-processes have no
+// intermediate buffers, and a graph replay fits the buffers it was sized
+// for), so the first command found set is always a flush and the others are
+// marked clear without a copy.
 int mark_done(psgpu_ctx* c, int i)
 {
+    if (c->log[i].type != CMD_FLUSH) {
+        c->h_flags[i] = 0;
+        return PSGPU_OK;
+    }
     HIPCHK(c, hipMemcpyAsync(c->h_flags + i, &c->d_cnt->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     return PSGPU_OK;
 }
