@@ -17,8 +17,8 @@ int psk_scan_blocks(int64_t cap);
 void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
               uint32_t* partial, uint32_t* total_out, const PsCounters* c);
 // the same scan from given partial sums of pgran items each (pgran divides 4096)
-void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int64_t n_parts,
-                    int pgran, uint32_t* total_out, const PsCounters* c);
+void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int pgran,
+                    uint32_t* total_out);
 // row item -> triangle table for k_span (large flushes; k_span searches row_off without it)
 void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
                 PsCounters* cnt);
@@ -34,11 +34,9 @@ void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* 
               int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
               int64_t cap, uint32_t* heavy, PsCounters* cnt);
 // the pieces, into runs of the tile lists f.tstart / f.runs; counts tcnt back down to zero.
-// sizes: the flush's size record (sizes[3] receives 0xffffffff when the pieces
-// do not fit the 32-bit tile totals)
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
                   const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,
-                  float* ck, int64_t span_cap, int64_t seg_cap, uint32_t* sizes, PsCounters* cnt);
+                  float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt);
 size_t psk_raster_lds(int chunk_w, int waves);
 void psk_raster(hipStream_t s, bool debug, const PsSpan* spans, const PsSeg* segs, const PsTri* tris, const PsDraw* draws,
                 const PsFrame& f, int waves_per_group, PsCounters* cnt);

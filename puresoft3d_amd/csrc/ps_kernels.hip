@@ -841,6 +841,63 @@ __global__ void __launch_bounds__(1024) k_scan_direct(const unsigned long long* 
     }
 }
 
+// Row scan from per-block partial sums (partial i sums items [i * pgran,
+// (i + 1) * pgran), pgran dividing SCAN_BLOCK): blocks of 16 waves, each lane
+// owning four adjacent items (one 16-byte load), so a block covers
+// SCAN_BLOCK items and a wave needs one wave scan.  Block b adds the partials
+// before it (two loads per thread below 2048 of them), the waves combine in
+// one LDS exchange, and the last block writes the total.
+struct PsRowWave {
+    uint32_t pre, own;
+};
+__global__ void __launch_bounds__(1024) k_scan_parts(const uint32_t* __restrict__ in, int64_t n,
+                                                     const uint32_t* __restrict__ partial, int pgran,
+                                                     uint32_t* __restrict__ out, uint32_t* total_out)
+{
+    __shared__ PsRowWave xw[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = blockIdx.x * (int64_t)SCAN_BLOCK + 4 * (int64_t)threadIdx.x;   // items i .. i + 3
+    uint32_t v[4];
+    if (i + 4 <= n) {
+        const uint4 q = *(const uint4*)(in + i);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = in[min(i + j, n - 1)];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = i + j < n ? v[j] : 0u;
+    }
+    const int np = blockIdx.x * (SCAN_BLOCK / pgran);   // partials before the block
+    uint32_t pre = 0;
+#pragma unroll 4
+    for (int k = threadIdx.x; k < np; k += 1024) pre += partial[k];
+    const uint32_t sum = v[0] + v[1] + v[2] + v[3];
+    const uint32_t own = ps_wave_incl_scan(sum);
+    const uint32_t pre_w = ps_wave_incl_scan(pre);
+    if (lane == 63) xw[wave] = PsRowWave{pre_w, own};
+    __syncthreads();
+    uint32_t before = 0, woff = 0, btot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+        const PsRowWave e = xw[w];
+        before += e.pre;
+        woff += w < wave ? e.own : 0u;
+        btot += e.own;
+    }
+    uint32_t r = before + woff + own - sum;
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) { o[j] = r; r += v[j]; }
+    if (i + 4 <= n) {
+        *(uint4*)(out + i) = make_uint4(o[0], o[1], o[2], o[3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (i + j < n) out[i + j] = o[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && wave == 0) ps_scan_total(before + btot, n, n, out, total_out, (const PsCounters*)nullptr);
+}
+
 // ---------------------------------------------------------------------------
 // expand: triangle id of every drawn row
 // ---------------------------------------------------------------------------
@@ -1025,7 +1082,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                                                   int64_t T, PsFrame f, const PsTri* __restrict__ tris,
                                                   const PsDraw* __restrict__ draws, const float* __restrict__ vary, int nvs,
                                                   unsigned long long* __restrict__ tcnt, PsSeg* __restrict__ segs,
-                                                  float* __restrict__ ck, int64_t cap, uint32_t* sizes, PsCounters* cnt)
+                                                  float* __restrict__ ck, int64_t cap, PsCounters* cnt)
 {
     __shared__ uint32_t rel_all[4][SG_BATCH][PS_WAVE];   // per wave: [iteration][lane] offset in the run
     const bool poisoned = ps_poisoned(cnt);
@@ -1992,11 +2049,10 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
                        total_out, cap, c, (uint32_t*)nullptr);
 }
 
-void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int64_t n_parts,
-                    int pgran, uint32_t* total_out, const PsCounters* c)
+void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int pgran,
+                    uint32_t* total_out)
 {
-    hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, in, nullptr, n, parts,
-                       (int)n_parts, pgran, out, total_out, n, c, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_scan_parts, dim3(psk_scan_blocks(n)), dim3(1024), 0, s, in, n, parts, pgran, out, total_out);
 }
 
 int psk_heavy_cap() { return PS_HEAVY_CAP; }
@@ -2037,11 +2093,11 @@ void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* 
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
                   const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,
-                  float* ck, int64_t span_cap, int64_t seg_cap, uint32_t* sizes, PsCounters* cnt)
+                  float* ck, int64_t span_cap, int64_t seg_cap, PsCounters* cnt)
 {
     int g = gsz(span_cap, 256); if (g > 2048) g = 2048;
     hipLaunchKernelGGL(k_segments, dim3(g), dim3(256), 0, s, spans, row_off, T, f, tris, draws, vary, nvs, tcnt, segs, ck,
-                       seg_cap, sizes, cnt);
+                       seg_cap, cnt);
 }
 
 size_t psk_raster_lds(int chunk_w, int waves) { return (size_t)waves * RA_WORDS(chunk_w) * 4; }

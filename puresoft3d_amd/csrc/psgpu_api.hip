@@ -627,7 +627,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         // largest row total of the log is known even after an overflow
         const int sb = psk_setup_block();
         if (row_part)
-            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, (T + sb - 1) / sb, sb, c->d_sizes + 0, nullptr);
+            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, sb, c->d_sizes + 0);
         else
             psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
     }
@@ -662,7 +662,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     {
         ProfScope p(c, "segments");
         psk_segments(s, c->fb.spans.p, c->fb.rowoff.p, T, f, c->fb.tris.p, dd, c->fb.vary.p, nvs, c->fb.tcnt.p,
-                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_sizes, c->d_cnt);
+                     c->fb.segs.p, c->fb.ck.p, c->fb.span_cap, c->fb.seg_cap, c->d_cnt);
     }
     if (imm) {
         ProfScope p(c, "raster");
