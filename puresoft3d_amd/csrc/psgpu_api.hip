@@ -997,6 +997,9 @@ int do_sync(psgpu_ctx* c)
         int64_t need_spans = (int64_t)c->h_sizes[2], need_segs = (int64_t)c->h_sizes[3];
         size_t first = 0;
         while (first < c->log.size() && !c->h_flags[first]) first++;
+        // only eager flushes carry a copied flag; the others cannot overflow
+        if (first == c->log.size())
+            return fail(c, PSGPU_ERR_STATE, "internal: the scratch buffers overflowed outside an eager flush");
         bool split = false;
         if (need_spans > c->max_spans || need_segs > c->max_segs) {
             if (c->log.size() + 2 >= (size_t)psgpu_ctx::kFlagCap || !split_flush(c, first)) break;
