@@ -439,21 +439,30 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
         }
     }
     });
-    // varyings of the wave's staged triangles, one triangle per store
-    // instruction (3*nvs consecutive floats) instead of one 3*nvs-float stride
-    // per lane; floats past NV of each vertex are left as they are
-    uint64_t m = __ballot(stage);
+    // varyings of the wave's staged triangles: the wave's slots are one
+    // contiguous block of 64 * 3*nvs floats, stored 16 bytes per lane and
+    // instruction.  A float is written only when its triangle staged in this
+    // launch (the slots of other programmes' triangles belong to their own
+    // launches); floats past NV of a staged vertex receive what the LDS
+    // holds, and nothing reads them.
+    const uint64_t m = __ballot(stage);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    {
-        int vv = lane / (nvs > 0 ? nvs : 1), kk = lane - vv * nvs;
-        bool mine = vv < 3 && kk < P::NV;
-        int64_t tw = t - lane;
-        while (m) {
-            int tl = __builtin_ctzll(m);
-            m &= m - 1;
-            if (mine) vary[(tw + tl) * n3 + lane] = wl[tl * n3 + lane];
+    if (m) {
+        const int64_t tw = t - lane;
+        const int nf = (int)min((int64_t)PS_WAVE, T - tw) * n3;   // the wave's floats
+        const float rn3 = 1.0f / (float)n3;   // (f + 0.5) * rn3 floors to f / n3 exactly for f < 64 * 60
+        float* dst = vary + tw * n3;
+        auto staged = [&](int f) { return (m >> (int)(((float)f + 0.5f) * rn3)) & 1ull; };
+        for (int f = lane * 4; f < nf; f += 4 * PS_WAVE) {
+            if (f + 4 <= nf && staged(f) && staged(f + 3)) {
+                *(float4*)(dst + f) = *(const float4*)(wl + f);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (f + j < nf && staged(f + j)) dst[f + j] = wl[f + j];
+            }
         }
     }
     ps_wave_add(cnt, PS_ST_CULLED, culled, true);
