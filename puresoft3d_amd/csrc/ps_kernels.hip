@@ -278,32 +278,48 @@ __global__ void __launch_bounds__(256) k_fills(const PsFill* __restrict__ fills,
 // setup
 // ---------------------------------------------------------------------------
 // Draw of triangle t (the last draw with tri_start <= t; draws are sorted by
-// tri_start and every t is covered).  The wave first finds the draw of its
-// first triangle with a 64-way search (64 pivots per step, so a couple of
-// dependent loads for thousands of draws); each lane then steps over the draw
-// boundaries between that triangle and its own, usually none.
+// tri_start and every t is covered).  The lanes' triangles lie in
+// [t0, t0 + 63], t0 that of lane 0.  While more than 63 draws remain, a
+// 64-way search (lane j loads pivot lo + j * step) narrows them to the
+// interval holding t0's draw.  The last step loads the 63 draws from lo on
+// into lanes 0..62 and the next one into lane 63: each lane's draw is then
+// the last of those starting at or before its t, found from ballots and
+// read lanes without another load, unless a lane's t reaches the draw after
+// the 63 (then it steps on).  So one dependent load for up to 63 draws, two
+// for up to 4,032.
 __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draws, int nd, int64_t t)
 {
-    int lane = threadIdx.x & 63;
-    int64_t t0 = (int64_t)__builtin_amdgcn_readfirstlane((int)(t >> 32)) << 32 |
-                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t);
-    // last index with tri_start <= t0 = (first index with tri_start > t0) - 1
+    const int lane = threadIdx.x & 63;
+    const int64_t t0 = (int64_t)__builtin_amdgcn_readfirstlane((int)(t >> 32)) << 32 |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t);
     int lo = 0, hi = nd;
-    while (hi - lo > 1) {
-        int step = (hi - lo + 63) / 64;
-        int p = lo + lane * step;
-        bool le = p < hi && draws[p].tri_start <= t0;
-        int cnt = __popcll(__ballot(le));
-        // pivots lo, lo+step, ... below cnt are <= t0: the answer is in
-        // [lo + (cnt-1)*step, lo + cnt*step)
-        int nlo = lo + (cnt - 1) * step;
-        int nhi = lo + cnt * step;
-        lo = nlo;
+    while (hi - lo > 63) {
+        const int step = (hi - lo + 63) / 64;
+        const int p = lo + lane * step;
+        const bool le = p < hi && draws[p].tri_start <= t0;
+        const int cnt = __popcll(__ballot(le));   // pivot 0 (lo) is always <= t0
+        // pivots below cnt are <= t0: t0's draw is in [lo + (cnt-1)*step, lo + cnt*step)
+        const int nhi = lo + cnt * step;
+        lo = lo + (cnt - 1) * step;
         hi = nhi < hi ? nhi : hi;
-        if (step == 1) break;
     }
-    int d = lo;
-    while (d + 1 < nd && draws[d + 1].tri_start <= t) d++;
+    const int p = lo + lane;
+    const int64_t sp = p < nd ? draws[p].tri_start : INT64_MAX;
+    // draws lo .. lo+62 starting in (t0, t] for some lane
+    const uint64_t later = __ballot(lane < 63 && sp > t0 && sp <= t0 + 63) ;
+    int d = lo + __popcll(__ballot(lane < 63 && sp <= t0)) - 1;
+    for (uint64_t m = later; m;) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        const int64_t sb = (int64_t)__builtin_amdgcn_readlane((int)(sp >> 32), b) << 32 |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sp, b);
+        if (t >= sb) d = lo + b;
+    }
+    // the draw after the 63 starts at or before some lane's t: step on
+    const int64_t s63 = (int64_t)__builtin_amdgcn_readlane((int)(sp >> 32), 63) << 32 |
+                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sp, 63);
+    if (s63 <= t0 + 63)
+        while (d + 1 < nd && draws[d + 1].tri_start <= t) d++;
     return d;
 }
 
