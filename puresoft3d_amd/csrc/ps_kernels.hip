@@ -175,14 +175,14 @@ struct PsSpanTiles {
     int cx0, cx1, tx0;
 };
 
-__device__ __forceinline__ PsSpanTiles ps_span_tiles(const PsView& v, int tile_rows, int tile_cols, int chunk_w, int y,
-                                                     int x1, int x2)
+__device__ __forceinline__ PsSpanTiles ps_span_tiles(int64_t tile_base, int n_tiles_x, int tile_rows, int tile_cols,
+                                                     int chunk_w, int y, int x1, int x2)
 {
     PsSpanTiles s;
     s.cx0 = x1 / chunk_w;
     s.cx1 = x2 / chunk_w;
     s.tx0 = s.cx0 / tile_cols;
-    s.g0 = (uint32_t)(v.tile_base + (int64_t)(y / tile_rows) * v.n_tiles_x + s.tx0);
+    s.g0 = (uint32_t)(tile_base + (int64_t)(y / tile_rows) * n_tiles_x + s.tx0);
     s.g1 = s.g0 + (uint32_t)(s.cx1 / tile_cols - s.tx0);
     return s;
 }
@@ -422,6 +422,9 @@ __global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ dra
                 tr.nrows = last - first + 1;
                 tr.split = split;
                 tr.code = code | ((uint32_t)di << 8);
+                tr.view = d.view;
+                tr.vflags = (uint32_t)(d.behavior & 3) | (d.writes_colour ? 4u : 0u);
+                tr.pad[0] = tr.pad[1] = 0;
                 n = (uint32_t)(last - first + 1);
                 // a triangle without rows has no spans: nothing reads its records
                 // (rows can lie outside the vertices' y range, where edges
@@ -1000,15 +1003,18 @@ __device__ __forceinline__ int64_t ps_row_tri(const uint32_t* __restrict__ row_o
 // Also counts, per tile, the span pieces in it and the waves (of 64 span
 // items) that have pieces there: tcnt[t] += pieces | waves << 32.
 template <bool SEARCH>
-__global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const PsDraw* __restrict__ draws,
-                                              const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
+__global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
                                               int64_t T, int64_t cap,
                                               PsSpan* __restrict__ spans, unsigned long long* __restrict__ tcnt,
-                                              const PsView* __restrict__ views, int chunk_w, int tile_rows, int tile_cols,
-                                              uint32_t* heavy, PsCounters* cnt)
+                                              const PsView* __restrict__ views, int n_views, int chunk_w, int tile_rows,
+                                              int tile_cols, uint32_t* heavy, PsCounters* cnt)
 {
     const bool poisoned = ps_poisoned(cnt);
     const int64_t R = row_off[T];
+    // the only view of a one-view flush, read once (otherwise each triangle's)
+    int64_t tb1 = 0;
+    int ntx1 = 0, W1 = 0;
+    if (n_views == 1) { tb1 = views[0].tile_base; ntx1 = views[0].n_tiles_x; W1 = views[0].W; }
     if (poisoned) return;
     if (heavy && blockIdx.x == 0 && threadIdx.x == 0) heavy[0] = 0;   // the two-pass tile scan lists heavy tiles atomically
     if (R > cap) {   // more rows than span records: flag it, write nothing
@@ -1026,9 +1032,9 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         if (i < R) {
             int t = SEARCH ? (int)tw : span_tri[i];
             const PsTri tr = tris[t];
-            const PsDraw& d = draws[tr.code >> 8];
-            const PsView& v = views[d.view];
-            const int W = v.W;
+            int64_t vtb = tb1;
+            int vntx = ntx1, W = W1;
+            if (n_views != 1) { const PsView& v = views[tr.view]; vtb = v.tile_base; vntx = v.n_tiles_x; W = v.W; }
             int row = tr.first + (int)(i - (int64_t)row_off[t]);
             PsEdge L, Rr;
             ps_row_edges(tr, row, L, Rr);
@@ -1045,11 +1051,11 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
                     ps_span_geom(tr, row, left, right, L, Rr, g);
                     ps_span_vis(tr, g, sp.cf2, sp.cf2_step, sp.pz, sp.pz_step);
                     sp.tri = t;
-                    sp.y = row | ((d.behavior & 3) << 24) | (d.writes_colour ? (1 << 28) : 0);
+                    sp.y = row | (int)((tr.vflags & 3u) << 24) | (int)(((tr.vflags >> 2) & 1u) << 28);
                     sp.x1 = x1;
                     sp.x2 = x2;
                     nfrag = (unsigned long long)(x2 - x1 + 1);
-                    st = ps_span_tiles(v, tile_rows, tile_cols, chunk_w, row, x1, x2);
+                    st = ps_span_tiles(vtb, vntx, tile_rows, tile_cols, chunk_w, row, x1, x2);
                     has = true;
                 }
             }
@@ -1119,6 +1125,9 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
         return;
     }
     const int CW = f.chunk_w, TC = f.tile_cols;
+    int64_t tb1 = 0;   // the only view of a one-view flush, read once
+    int ntx1 = 0;
+    if (f.n_views == 1) { tb1 = f.views[0].tile_base; ntx1 = f.views[0].n_tiles_x; }
     const int lane = threadIdx.x & (PS_WAVE - 1);
     uint32_t (*rel)[PS_WAVE] = rel_all[threadIdx.x >> 6];
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
@@ -1134,15 +1143,25 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
         bool need_ck = false;
         float vs[PS_MAX_NV], vt[PS_MAX_NV];
         if (has) {
-            const PsTri tr = tris[sp.tri];
-            const PsDraw& d = draws[tr.code >> 8];
-            const PsView& v = f.views[d.view];
+            // the triangle only for a flush of several views (its view) or
+            // for varying checkpoints
+            PsTri tr;
+            int64_t vtb = tb1;
+            int vntx = ntx1;
+            if (f.n_views != 1) {
+                tr = tris[sp.tri];
+                const PsView& v = f.views[tr.view];
+                vtb = v.tile_base;
+                vntx = v.n_tiles_x;
+            }
             y = sp.y & 0xffffff;
             flags = ((uint32_t)(sp.y >> 24) & 3u) << 26 | ((uint32_t)(sp.y >> 28) & 1u) << 28;
-            st = ps_span_tiles(v, f.tile_rows, TC, CW, y, sp.x1, sp.x2);
+            st = ps_span_tiles(vtb, vntx, f.tile_rows, TC, CW, y, sp.x1, sp.x2);
             need_ck = (st.cx1 > st.cx0) && ((sp.y >> 28) & 1) && ps_span_has_ck(sp);
-            nv = d.nv;
             if (need_ck) {
+                if (f.n_views == 1) tr = tris[sp.tri];
+                const PsDraw& d = draws[tr.code >> 8];
+                nv = d.nv;
                 PsEdge L, Rr;
                 ps_row_edges(tr, y, L, Rr);
                 int left = ps_cvt_i32(ps_edge_eval(L, (float)y) + 0.5f);
@@ -2102,18 +2121,18 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
     hipLaunchKernelGGL(k_expand, dim3(gsz(T, 256)), dim3(256), 0, s, nrows, row_off, T, span_tri, cap, cnt);
 }
 
-void psk_span(hipStream_t s, const PsTri* tris, const PsDraw* draws, const int* span_tri, const uint32_t* row_off,
-              int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int chunk_w, int tile_rows,
-              int64_t cap, uint32_t* heavy, PsCounters* cnt)
+void psk_span(hipStream_t s, const PsTri* tris, const int* span_tri, const uint32_t* row_off, int64_t T, PsSpan* spans,
+              unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w, int tile_rows, int64_t cap,
+              uint32_t* heavy, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 2048) g = 2048;
     const int tile_cols = 64 / tile_rows;
     if (span_tri)
-        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans, tcnt,
-                           views, chunk_w, tile_rows, tile_cols, heavy, cnt);
+        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, T, cap, spans, tcnt, views,
+                           n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
     else
-        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, draws, span_tri, row_off, T, cap, spans, tcnt,
-                           views, chunk_w, tile_rows, tile_cols, heavy, cnt);
+        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, T, cap, spans, tcnt, views,
+                           n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,

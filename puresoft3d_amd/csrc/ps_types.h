@@ -68,6 +68,11 @@ struct PsTri {
     int nrows;     // drawn rows (0 = no rows)
     int split;     // last row of the lower half (standing triangles)
     uint32_t code; // edge code | draw index << 8
+    // what the span and piece kernels need of the draw, without reading it:
+    // its view, and its depth behaviour bits | writes colour << 2
+    int view;
+    uint32_t vflags;
+    int pad[2];
 };
 
 // One scanline span with at least one pixel (drawvao.cpp:66-110, interp.cpp:26-80).

@@ -648,8 +648,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     uint32_t* heavy = imm ? nullptr : c->fb.heavy.p;
     {
         ProfScope p(c, "span");
-        psk_span(s, c->fb.tris.p, dd, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p, c->fb.tcnt.p,
-                 dv, f.chunk_w, f.tile_rows, c->fb.span_cap, heavy, c->d_cnt);
+        psk_span(s, c->fb.tris.p, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p, c->fb.tcnt.p, dv,
+                 f.n_views, f.chunk_w, f.tile_rows, c->fb.span_cap, heavy, c->d_cnt);
     }
     {
         ProfScope p(c, "scan_tiles");
