@@ -42,15 +42,6 @@ __device__ __forceinline__ uint64_t ps_lt_mask()
     return __lanemask_lt();
 }
 
-// a wave's statistic: into the committed counters (pending = false) or the
-// flush's pending ones; one atomic per wave, on a per-block stripe
-__device__ __forceinline__ void ps_wave_add(PsCounters* c, int field, unsigned long long v, bool pending = false)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PS_WAVE);
-    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v)
-        atomicAdd(pending ? &c->pend[blockIdx.x & (PS_STRIPES - 1)][field] : &c->stat[blockIdx.x & (PS_STRIPES - 1)][field], v);
-}
-
 // the flush got past every overflow check: its pending statistics become
 // committed (blocks 0..63 of the raster launch take the stripes between them)
 __device__ __forceinline__ void ps_commit_pending(PsCounters* c)
@@ -166,6 +157,24 @@ __device__ __forceinline__ uint32_t ps_wave_incl_scan(uint32_t v)
     v += PS_DPP(v, 0x142, 0xa, false);  // row_bcast:15 into rows 1 and 3
     v += PS_DPP(v, 0x143, 0xc, false);  // row_bcast:31 into rows 2 and 3
     return v;
+}
+
+// a wave's statistic (lane values whose wave total fits 32 bits, every lane
+// active): into the committed counters (pending = false) or the flush's
+// pending ones; one atomic per wave, on a per-block stripe
+__device__ __forceinline__ void ps_wave_add(PsCounters* c, int field, uint32_t v, bool pending = false)
+{
+    const uint32_t tot = __builtin_amdgcn_readlane(ps_wave_incl_scan(v), 63);
+    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && tot)
+        atomicAdd(pending ? &c->pend[blockIdx.x & (PS_STRIPES - 1)][field] : &c->stat[blockIdx.x & (PS_STRIPES - 1)][field],
+                  (unsigned long long)tot);
+}
+
+// the same for 64-bit lane values, into the committed counters
+__device__ __forceinline__ void ps_wave_add64(PsCounters* c, int field, unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PS_WAVE);
+    if ((threadIdx.x & (PS_WAVE - 1)) == 0 && v) atomicAdd(&c->stat[blockIdx.x & (PS_STRIPES - 1)][field], v);
 }
 
 // The tiles a span's pieces fall in: tiles g0 .. g1 (consecutive tile
@@ -1021,9 +1030,10 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&cnt->overflow, 1);
         return;
     }
+    uint32_t all_spans = 0, all_pieces = 0, all_frags = 0;   // the lane's statistics, added up once per wave
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = base + threadIdx.x;
-        unsigned long long nspan = 0, nfrag = 0;
+        uint32_t nspan = 0, nfrag = 0;
         // the item's triangle: from k_expand's table, or searched in row_off
         int64_t tw = -1;
         if constexpr (SEARCH) tw = ps_row_tri(row_off, T, R, base + (threadIdx.x & ~63));
@@ -1054,7 +1064,7 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
                     sp.y = row | (int)((tr.vflags & 3u) << 24) | (int)(((tr.vflags >> 2) & 1u) << 28);
                     sp.x1 = x1;
                     sp.x2 = x2;
-                    nfrag = (unsigned long long)(x2 - x1 + 1);
+                    nfrag = (uint32_t)(x2 - x1 + 1);
                     st = ps_span_tiles(vtb, vntx, tile_rows, tile_cols, chunk_w, row, x1, x2);
                     has = true;
                 }
@@ -1063,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         }
         // per tile of the wave's spans: its pieces there and one run (lane j
         // adds the j-th tile's counts, so a batch of tiles is one atomic)
-        const unsigned long long npieces = has ? (unsigned long long)(st.cx1 - st.cx0 + 1) : 0ull;
+        const uint32_t npieces = has ? (uint32_t)(st.cx1 - st.cx0 + 1) : 0u;
         uint32_t cur = has ? st.g0 : 0xffffffffu;
         for (;;) {
             uint32_t bk, bn;
@@ -1071,10 +1081,13 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
             if ((int)(threadIdx.x & (PS_WAVE - 1)) < nb) atomicAdd(&tcnt[bk], (unsigned long long)bn | (1ull << 32));
             if (nb < PS_WAVE) break;
         }
-        ps_wave_add(cnt, PS_ST_SPANS, nspan, true);
-        ps_wave_add(cnt, PS_ST_SEGS, npieces, true);
-        ps_wave_add(cnt, PS_ST_FRAGS, nfrag, true);
+        all_spans += nspan;
+        all_pieces += npieces;
+        all_frags += nfrag;
     }
+    ps_wave_add(cnt, PS_ST_SPANS, all_spans, true);
+    ps_wave_add(cnt, PS_ST_SEGS, all_pieces, true);
+    ps_wave_add(cnt, PS_ST_FRAGS, all_frags, true);
 }
 
 // Spans at least this long keep varying checkpoints at each later piece, so
@@ -1663,7 +1676,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
             wrow[X0 + c] = win[c * 64 + lane];
         }
     }
-    ps_wave_add(cnt, PS_ST_SHADED, passes);
+    ps_wave_add64(cnt, PS_ST_SHADED, passes);
 }
 
 // ---------------------------------------------------------------------------
@@ -1913,7 +1926,7 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     }
     for (int c = 0; c < ncol; c++)
         if (dirty) drow[X0 + c] = dep[c * 64 + lane];
-    ps_wave_add(cnt, PS_ST_SHADED, passes);
+    ps_wave_add64(cnt, PS_ST_SHADED, passes);
 }
 
 // ---------------------------------------------------------------------------
