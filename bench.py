@@ -204,8 +204,9 @@ def batch_section(args, N, S, log):
 def serve_section(args, N, S, log):
     """2,000 Poisson view requests (area log-uniform 128^2..2048^2) over 32
     resident scenes (1k-1M triangles, 512^2-4096^2 mipmapped textures),
-    continuous batching with HBM-budget admission, against the plain
-    one-view-per-launch path on the same GPU."""
+    continuous batching with HBM-budget admission on one render stream,
+    against the plain one-view-per-launch path on the same GPU; then the same
+    batching over two render streams (contexts) at the lower loads."""
     from puresoft3d_amd import serving as SV
     t0 = time.perf_counter()
     assets = S.serving_assets(n_scenes=32, seed=21)
@@ -213,27 +214,34 @@ def serve_section(args, N, S, log):
     views, _ = S.poisson_requests(assets, n=n, rate=1.0, seed=23)
     log("[bench] serve: %d scenes (%.2f M tris), %d requests, generated in %.1f s" % (
         len(assets), sum(a.triangles() for a in assets) / 1e6, n, time.perf_counter() - t0))
-    ctx = N.Context(64, 64)
-    srv = SV.RenderServer(ctx, assets)
+    ctxs = [N.Context(64, 64) for _ in range(max(1, args.serve_streams))]
+    ctx = ctxs[0]
+    srv = SV.RenderServer(ctx, assets, extra_ctxs=ctxs[1:])
     cfg = dict(max_batch_views=args.serve_batch_views, max_batch_pixels=args.serve_batch_pixels << 20,
-               max_inflight=2, hbm_budget=args.serve_budget_gib << 30)
+               max_inflight=2, hbm_budget=args.serve_budget_gib << 30, streams=1)
+    plain_cfg = dict(max_batch_views=1, max_inflight=1, streams=1)
+    # two batches in flight in all: queued on one stream, or one on each of two
+    cfg2 = dict(cfg, max_inflight=1, streams=len(ctxs))
     zeros = np.zeros(n)
     srv.serve(views, zeros, **cfg)                       # warm-up: target pools and scratch sizes
-    srv.serve(views, zeros, max_batch_views=1, max_inflight=1)
+    if len(ctxs) > 1:
+        srv.serve(views, zeros, **cfg2)
+    srv.serve(views, zeros, **plain_cfg)
     out = {"requests": n, "scenes": len(assets), "config": {k: int(v) for k, v in cfg.items()}}
     if not args.no_tune:   # raster tiling for this mix of views, on a sample of 64 requests
         from puresoft3d_amd import autotune as AT
         sample = views[:64]
         best, _ = AT.autotune(ctx, lambda: srv.serve(sample, zeros[:64], to_host=False, **cfg), reps=2,
                               cands=AT.candidates(waves=(1, 2)))
+        for c in ctxs[1:]:
+            c.tiling(*best)
         out["tiling"] = {"chunk_width": best[0], "tile_rows": best[1], "waves_per_group": best[2]}
     keep = list(range(0, n, max(1, n // 8)))
     for mode in ("host", "device"):
         to_host = mode == "host"
         # plain path: one view per launch, back to back
         n_plain = min(n, 600)
-        res_p, st_p, _ = srv.serve(views[:n_plain], zeros[:n_plain], max_batch_views=1, max_inflight=1,
-                                   to_host=to_host)
+        res_p, st_p, _ = srv.serve(views[:n_plain], zeros[:n_plain], to_host=to_host, **plain_cfg)
         single = res_p["done_s"] - res_p["launched_s"]
         plain_vps = n_plain / st_p["wall_s"]
         sec = {"plain": {"views_per_s": round(plain_vps, 1),
@@ -255,7 +263,8 @@ def serve_section(args, N, S, log):
                                                    float(np.percentile(single, 99)), 2),
                    "batches": st["batches"], "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1),
                    "host_submit_ms_per_batch": round(st["host_submit_s"] * 1e3 / max(1, st["batches"]), 4),
-                   "peak_target_bytes": st["peak_bytes"]}
+                   "peak_target_bytes": st["peak_bytes"], "target_allocs": st["target_allocs"],
+                   "target_evictions": st["target_evictions"]}
             if to_host:
                 same = sum(int(np.array_equal(imgs[i], srv.render_alone(views[i]))) for i in keep)
                 run["images_identical_to_alone"] = "%d/%d" % (same, len(keep))
@@ -275,8 +284,28 @@ def serve_section(args, N, S, log):
                       "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1)})
         log("[bench] serve (device, 2x) cap %d MPix: %s" % (cap, json.dumps(sweep[-1])))
     out["batch_cap_sweep_device_2x"] = sweep
+    # the same batching over two render streams, one batch in flight on each,
+    # at the loads below saturation (above it the streams' separate target
+    # pools churn inside the budget)
+    if len(ctxs) > 1:
+        for mode in ("host", "device"):
+            to_host = mode == "host"
+            plain_vps = out["deliver_" + mode]["plain"]["views_per_s"]
+            two = []
+            for load in [x for x in args.serve_loads if x <= 2.0]:
+                _, arr = S.poisson_requests(assets, n=n, rate=load * plain_vps, seed=23)
+                res, st, _ = srv.serve(views, arr, to_host=to_host, **cfg2)
+                summ = SV.latency_summary(res)
+                two.append({"offered_x_plain": load, "views_per_s": round(summ["views_per_s"], 1),
+                            "speedup_vs_plain": round(summ["views_per_s"] / plain_vps, 2),
+                            "latency_ms_p50": round(summ["p50_ms"], 3), "latency_ms_p99": round(summ["p99_ms"], 3),
+                            "mean_batch_views": round(st["views"] / max(1, st["batches"]), 1),
+                            "target_allocs": st["target_allocs"], "target_evictions": st["target_evictions"]})
+                log("[bench] serve (%s, two streams) load %.1fx: %s" % (mode, load, json.dumps(two[-1])))
+            out["two_streams_" + mode] = two
     srv.close()
-    ctx.close()
+    for c in ctxs:
+        c.close()
     return out
 
 
@@ -320,6 +349,7 @@ def main():
                     help="batch pixel caps swept at twice the plain rate (device delivery)")
     ap.add_argument("--serve-batch-pixels", type=int, default=16, metavar="MPIX")
     ap.add_argument("--serve-budget-gib", type=int, default=8)
+    ap.add_argument("--serve-streams", type=int, default=2, help="render streams (contexts) of the batched server")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--frame-only", action="store_true", help="only the 1080p frame (and the CPU baseline)")
     ap.add_argument("--adjoint-steps", type=int, default=200)

@@ -7,7 +7,8 @@
  * many such views in flight: scenes and textures stay resident on one
  * psgpu context; each request names a scene, a resolution and a camera.
  *
- * Policy (one GPU, one context, one host thread):
+ * Policy (one GPU, one host thread driving one or more contexts, each a
+ * render stream with its own copy of the scenes):
  *  - arrivals join a FIFO; the head is admitted when its render targets
  *    (colour + depth, rounded up to a size class) plus a per-pixel scratch
  *    estimate fit in the HBM budget, and fewer than max_batch_views requests
@@ -20,8 +21,12 @@
  *    Requests arriving while batches run are admitted and launched as soon as
  *    targets and a batch slot free up, so the GPU always has the next batch
  *    queued behind the current one and readback copies overlap it.
- * max_batch_views = 1 with max_inflight = 1 is the plain one-view-per-launch
- * path.  A view renders to the same pixels alone or in any batch.
+ * With several render streams, admitted requests go to the stream with the
+ * fewest batches in flight, so a small batch need not wait behind a large
+ * one and the streams' kernels overlap on the GPU.
+ * max_batch_views = 1 with max_inflight = 1 and one render stream is the
+ * plain one-view-per-launch path.  A view renders to the same pixels alone
+ * or in any batch, on any stream.
  */
 #ifndef PSSERVE_H
 #define PSSERVE_H
@@ -62,6 +67,7 @@ typedef struct {
     uint32_t clear_bgra;
     int deliver_to_host;        /* 1: each image is copied to pinned host memory before the
                                  * request completes (PCIe-inclusive); 0: images stay in HBM */
+    int render_streams;         /* contexts used, 1 .. the server's (0: all of them) */
 } psserve_config;
 
 typedef struct {
@@ -85,6 +91,11 @@ typedef struct psserve_server psserve_server;
 
 int psserve_create(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_config* cfg,
                    psserve_server** out);
+/* The same over n_ctx contexts of one device (render streams): scenes holds
+ * n_ctx * n_scenes entries, scene k of context c at [c * n_scenes + k]; the
+ * configuration's programme id must name the same programme on each. */
+int psserve_create_multi(psgpu_ctx* const* ctxs, int n_ctx, const psserve_scene* scenes, int n_scenes,
+                         const psserve_config* cfg, psserve_server** out);
 int psserve_set_config(psserve_server* server, const psserve_config* cfg);
 /* frees the pooled targets (syncs the context) */
 int psserve_destroy(psserve_server* server);

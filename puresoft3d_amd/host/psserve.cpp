@@ -1,10 +1,12 @@
 // psserve.cpp -- the render-serving loop declared in include/psserve.h.
 //
-// One host thread drives one psgpu context.  Requests move through
-//   waiting (FIFO) -> admitted (targets held) -> in flight (in a batch) -> done.
-// Render targets are pooled per size class and reused; a target class is
-// allocated lazily while the HBM budget allows, and idle targets of other
-// classes are evicted when it does not.
+// One host thread drives one or more psgpu contexts (render streams).
+// Requests move through
+//   waiting (FIFO) -> admitted (targets held on a stream) -> in flight (in a
+//   batch) -> done.
+// Render targets are pooled per stream and size class and reused; a target
+// class is allocated lazily while the HBM budget (shared by the streams)
+// allows, and idle targets of other classes are evicted when it does not.
 #include "../../include/psserve.h"
 
 #include <stdarg.h>
@@ -15,6 +17,7 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <string>
 #include <thread>
 #include <vector>
@@ -44,6 +47,7 @@ int class_side(int v)
 }
 
 struct Slot {
+    int ctx = 0;               // the render stream (context) holding it
     int colour = -1, depth = -1;
     uint8_t* host = nullptr;   // pinned, cw*ch*4 bytes
     int cw = 0, ch = 0;
@@ -58,9 +62,8 @@ struct Batch {
 }  // namespace
 
 struct psserve_server {
-    psgpu_ctx* ctx;
-    std::vector<psserve_scene> scene_list;
-    const psserve_scene* scenes;
+    std::vector<psgpu_ctx*> ctxs;
+    std::vector<psserve_scene> scene_list;   // scene k of context c at [c * n_scenes + k]
     int n_scenes;
     const psserve_request* reqs;
     int n;
@@ -69,33 +72,41 @@ struct psserve_server {
     uint8_t* const* images;
     psserve_stats st;
 
-    std::map<std::pair<int, int>, std::vector<Slot>> idle;   // by class
+    // idle targets by (stream, class width, class height)
+    std::map<std::tuple<int, int, int>, std::vector<Slot>> idle;
     std::vector<Slot> held;                                  // per request
     size_t target_bytes = 0;       // all allocated targets (held + idle)
     size_t inflight_scratch = 0;   // scratch estimate of admitted requests
 
-    int psgpu(int rc)
+    int psgpu(int rc, int c)
     {
-        if (rc) g_err = std::string("psgpu: ") + psgpu_last_error(ctx);
+        if (rc) g_err = std::string("psgpu: ") + psgpu_last_error(ctxs[c]);
         return rc;
+    }
+
+    int streams() const
+    {
+        const int n = (int)ctxs.size();
+        return cfg.render_streams >= 1 && cfg.render_streams < n ? cfg.render_streams : n;
     }
 
     int free_slot(Slot& s)
     {
         int rc;
-        if ((rc = psgpu(psgpu_texture_destroy(ctx, s.colour))) || (rc = psgpu(psgpu_texture_destroy(ctx, s.depth))) ||
-            (rc = psgpu(psgpu_host_free(ctx, s.host))))
+        psgpu_ctx* ctx = ctxs[s.ctx];
+        if ((rc = psgpu(psgpu_texture_destroy(ctx, s.colour), s.ctx)) ||
+            (rc = psgpu(psgpu_texture_destroy(ctx, s.depth), s.ctx)) || (rc = psgpu(psgpu_host_free(ctx, s.host), s.ctx)))
             return rc;
         target_bytes -= s.bytes();
         st.target_evictions++;
         return PSGPU_OK;
     }
 
-    // a target for request i, within the budget; *ok = 0 when it must wait
-    int acquire(int i, int* ok)
+    // a target for request i on stream c, within the budget; *ok = 0 when it must wait
+    int acquire(int i, int c, int* ok)
     {
         const psserve_request& r = reqs[i];
-        std::pair<int, int> key(class_side(r.width), class_side(r.height));
+        std::tuple<int, int, int> key(c, class_side(r.width), class_side(r.height));
         size_t scratch = (size_t)r.width * r.height * (size_t)cfg.scratch_bytes_per_px;
         *ok = 0;
         auto& pool = idle[key];
@@ -108,8 +119,9 @@ struct psserve_server {
             return PSGPU_OK;
         }
         Slot s;
-        s.cw = key.first;
-        s.ch = key.second;
+        s.ctx = c;
+        s.cw = std::get<1>(key);
+        s.ch = std::get<2>(key);
         size_t need = s.bytes() + scratch;
         // evict idle targets of other classes while over budget
         while (target_bytes + inflight_scratch + need > cfg.hbm_budget) {
@@ -128,9 +140,10 @@ struct psserve_server {
         bool empty = target_bytes == 0 && inflight_scratch == 0;
         if (target_bytes + inflight_scratch + need > cfg.hbm_budget && !empty) return PSGPU_OK;
         int rc;
-        if ((rc = psgpu(psgpu_texture_create(ctx, s.cw, s.cw * 4, s.ch, 4, 0, PSGPU_CLAMP, 1, nullptr, &s.colour))) ||
-            (rc = psgpu(psgpu_texture_create(ctx, s.cw, s.cw * 4, s.ch, 4, 0, PSGPU_CLAMP, 1, nullptr, &s.depth))) ||
-            (rc = psgpu(psgpu_host_alloc(ctx, (size_t)s.cw * s.ch * 4, (void**)&s.host))))
+        psgpu_ctx* ctx = ctxs[c];
+        if ((rc = psgpu(psgpu_texture_create(ctx, s.cw, s.cw * 4, s.ch, 4, 0, PSGPU_CLAMP, 1, nullptr, &s.colour), c)) ||
+            (rc = psgpu(psgpu_texture_create(ctx, s.cw, s.cw * 4, s.ch, 4, 0, PSGPU_CLAMP, 1, nullptr, &s.depth), c)) ||
+            (rc = psgpu(psgpu_host_alloc(ctx, (size_t)s.cw * s.ch * 4, (void**)&s.host), c)))
             return rc;
         target_bytes += s.bytes();
         st.target_allocs++;
@@ -146,7 +159,7 @@ struct psserve_server {
         const psserve_request& r = reqs[i];
         Slot s = held[i];
         held[i] = Slot();
-        idle[std::make_pair(s.cw, s.ch)].push_back(s);
+        idle[std::make_tuple(s.ctx, s.cw, s.ch)].push_back(s);
         inflight_scratch -= (size_t)r.width * r.height * (size_t)cfg.scratch_bytes_per_px;
     }
 
@@ -155,14 +168,17 @@ struct psserve_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     }
 
-    int launch(const std::vector<int>& ids, uint64_t* fence)
+    // the batch ids on stream c
+    int launch(int c, const std::vector<int>& ids, uint64_t* fence)
     {
         int rc;
         double t0 = secs();
+        psgpu_ctx* ctx = ctxs[c];
+        auto psgpu = [&](int e) { return this->psgpu(e, c); };
         if ((rc = psgpu(psgpu_program_use(ctx, cfg.program)))) return rc;
         for (int i : ids) {
             const psserve_request& r = reqs[i];
-            const psserve_scene& sc = scenes[r.scene];
+            const psserve_scene& sc = scene_list[(size_t)c * n_scenes + r.scene];
             const Slot& s = held[i];
             static const float I4[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
             if ((rc = psgpu(psgpu_target_set(ctx, 0, s.colour))) || (rc = psgpu(psgpu_depth_set(ctx, s.depth))) ||
@@ -200,59 +216,72 @@ struct psserve_server {
     {
         using clk = std::chrono::steady_clock;
         held.assign(n, Slot());
+        const int ns = streams();
         std::deque<int> waiting;
-        std::vector<int> admitted;   // holding targets, not launched yet
-        std::deque<Batch> inflight;
+        std::vector<std::vector<int>> admitted(ns);   // per stream: holding targets, not launched yet
+        std::vector<std::deque<Batch>> inflight(ns);
         int next = 0, done = 0;
         auto t0 = clk::now();
         auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
         int rc;
         while (done < n) {
             bool progress = false;
-            // completions, in launch order
-            while (!inflight.empty()) {
-                int fin = 0;
-                if ((rc = psgpu(psgpu_fence_query(ctx, inflight.front().fence, &fin)))) return rc;
-                if (!fin) break;
-                double t = now();
-                for (int i : inflight.front().reqs) {
-                    const psserve_request& r = reqs[i];
-                    res[i].done_s = t;
-                    if (images && images[i] && cfg.deliver_to_host)
-                        memcpy(images[i], held[i].host, (size_t)r.width * r.height * 4);
-                    release(i);
-                    done++;
+            // completions, in launch order on each stream
+            for (int c = 0; c < ns; c++)
+                while (!inflight[c].empty()) {
+                    int fin = 0;
+                    if ((rc = psgpu(psgpu_fence_query(ctxs[c], inflight[c].front().fence, &fin), c))) return rc;
+                    if (!fin) break;
+                    double t = now();
+                    for (int i : inflight[c].front().reqs) {
+                        const psserve_request& r = reqs[i];
+                        res[i].done_s = t;
+                        if (images && images[i] && cfg.deliver_to_host)
+                            memcpy(images[i], held[i].host, (size_t)r.width * r.height * 4);
+                        release(i);
+                        done++;
+                    }
+                    inflight[c].pop_front();
+                    progress = true;
                 }
-                inflight.pop_front();
-                progress = true;
-            }
             double t = now();
             while (next < n && reqs[next].arrival_s <= t) {
                 res[next].arrival_s = reqs[next].arrival_s;
                 waiting.push_back(next++);
             }
-            // targets are held only by requests the next batch can take
-            while (!waiting.empty() && (int)admitted.size() < cfg.max_batch_views) {
+            // targets are held only by requests the next batch of a stream can
+            // take; the head of the queue goes to the stream with the fewest
+            // batches in flight (then the fewest admitted)
+            while (!waiting.empty()) {
+                int c = -1;
+                for (int k = 0; k < ns; k++) {
+                    if ((int)admitted[k].size() >= cfg.max_batch_views) continue;
+                    if (c < 0 || inflight[k].size() < inflight[c].size() ||
+                        (inflight[k].size() == inflight[c].size() && admitted[k].size() < admitted[c].size()))
+                        c = k;
+                }
+                if (c < 0) break;
                 int ok = 0;
-                if ((rc = acquire(waiting.front(), &ok))) return rc;
+                if ((rc = acquire(waiting.front(), c, &ok))) return rc;
                 if (!ok) break;
                 res[waiting.front()].admitted_s = t;
-                admitted.push_back(waiting.front());
+                admitted[c].push_back(waiting.front());
                 waiting.pop_front();
                 progress = true;
             }
-            if (!admitted.empty() && (int)inflight.size() < cfg.max_inflight) {
+            for (int c = 0; c < ns; c++) {
+                if (admitted[c].empty() || (int)inflight[c].size() >= cfg.max_inflight) continue;
                 Batch b;
                 int64_t px = 0;
                 size_t k = 0;
-                for (; k < admitted.size() && (int)b.reqs.size() < cfg.max_batch_views; k++) {
-                    int64_t p = (int64_t)reqs[admitted[k]].width * reqs[admitted[k]].height;
+                for (; k < admitted[c].size() && (int)b.reqs.size() < cfg.max_batch_views; k++) {
+                    int64_t p = (int64_t)reqs[admitted[c][k]].width * reqs[admitted[c][k]].height;
                     if (!b.reqs.empty() && px + p > cfg.max_batch_pixels) break;
-                    b.reqs.push_back(admitted[k]);
+                    b.reqs.push_back(admitted[c][k]);
                     px += p;
                 }
-                admitted.erase(admitted.begin(), admitted.begin() + k);
-                if ((rc = launch(b.reqs, &b.fence))) return rc;
+                admitted[c].erase(admitted[c].begin(), admitted[c].begin() + k);
+                if ((rc = launch(c, b.reqs, &b.fence))) return rc;
                 double tl = now();
                 for (int i : b.reqs) {
                     res[i].launched_s = tl;
@@ -262,13 +291,15 @@ struct psserve_server {
                 st.views += (int64_t)b.reqs.size();
                 st.pixels += px;
                 st.max_batch_views_seen = std::max<int>(st.max_batch_views_seen, (int)b.reqs.size());
-                inflight.push_back(std::move(b));
+                inflight[c].push_back(std::move(b));
                 progress = true;
             }
             if (!progress) {
                 // idle until the next arrival or a completion; short sleeps only
                 // when nothing is on the GPU and the next arrival is far
-                if (inflight.empty() && next < n && reqs[next].arrival_s - now() > 200e-6)
+                bool busy = false;
+                for (int c = 0; c < ns; c++) busy = busy || !inflight[c].empty();
+                if (!busy && next < n && reqs[next].arrival_s - now() > 200e-6)
                     std::this_thread::sleep_for(std::chrono::microseconds(50));
                 else
                     std::this_thread::yield();
@@ -281,9 +312,13 @@ struct psserve_server {
     // after a run: targets still held (a failed run) go back to the pools
     int settle()
     {
-        int rc = psgpu(psgpu_sync(ctx));
+        int rc = PSGPU_OK;
+        for (int c = 0; c < (int)ctxs.size(); c++) {
+            int e = psgpu(psgpu_sync(ctxs[c]), c);
+            if (!rc) rc = e;
+        }
         for (Slot& s : held)
-            if (s.colour >= 0) idle[std::make_pair(s.cw, s.ch)].push_back(s);
+            if (s.colour >= 0) idle[std::make_tuple(s.ctx, s.cw, s.ch)].push_back(s);
         held.clear();
         inflight_scratch = 0;
         return rc;
@@ -306,7 +341,8 @@ namespace {
 int check_config(const psserve_config* cfg)
 {
     if (!cfg) return fail(PSGPU_ERR_INVALID, "null configuration");
-    if (cfg->max_batch_views < 1 || cfg->max_inflight < 1 || cfg->max_batch_pixels < 1 || cfg->scratch_bytes_per_px < 0)
+    if (cfg->max_batch_views < 1 || cfg->max_inflight < 1 || cfg->max_batch_pixels < 1 || cfg->scratch_bytes_per_px < 0 ||
+        cfg->render_streams < 0)
         return fail(PSGPU_ERR_INVALID, "bad configuration");
     return PSGPU_OK;
 }
@@ -315,22 +351,29 @@ int check_config(const psserve_config* cfg)
 
 extern "C" {
 
-int psserve_create(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_config* cfg,
-                   psserve_server** out)
+int psserve_create_multi(psgpu_ctx* const* ctxs, int n_ctx, const psserve_scene* scenes, int n_scenes,
+                         const psserve_config* cfg, psserve_server** out)
 {
     if (!out) return fail(PSGPU_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (!ctx || !scenes || n_scenes < 1) return fail(PSGPU_ERR_INVALID, "null argument");
+    if (!ctxs || n_ctx < 1 || !scenes || n_scenes < 1) return fail(PSGPU_ERR_INVALID, "null argument");
+    for (int c = 0; c < n_ctx; c++)
+        if (!ctxs[c]) return fail(PSGPU_ERR_INVALID, "null context %d", c);
     int rc = check_config(cfg);
     if (rc) return rc;
     psserve_server* s = new psserve_server();
-    s->ctx = ctx;
-    s->scene_list.assign(scenes, scenes + n_scenes);
-    s->scenes = s->scene_list.data();
+    s->ctxs.assign(ctxs, ctxs + n_ctx);
+    s->scene_list.assign(scenes, scenes + (size_t)n_ctx * n_scenes);
     s->n_scenes = n_scenes;
     s->cfg = *cfg;
     *out = s;
     return PSGPU_OK;
+}
+
+int psserve_create(psgpu_ctx* ctx, const psserve_scene* scenes, int n_scenes, const psserve_config* cfg,
+                   psserve_server** out)
+{
+    return psserve_create_multi(&ctx, 1, scenes, n_scenes, cfg, out);
 }
 
 int psserve_set_config(psserve_server* s, const psserve_config* cfg)

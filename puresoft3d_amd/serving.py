@@ -24,7 +24,7 @@ class Request(C.Structure):
 class Config(C.Structure):
     _fields_ = [("program", C.c_int), ("hbm_budget", C.c_size_t), ("scratch_bytes_per_px", C.c_int),
                 ("max_batch_views", C.c_int), ("max_batch_pixels", C.c_int64), ("max_inflight", C.c_int),
-                ("clear_bgra", C.c_uint32), ("deliver_to_host", C.c_int)]
+                ("clear_bgra", C.c_uint32), ("deliver_to_host", C.c_int), ("render_streams", C.c_int)]
 
 
 class Result(C.Structure):
@@ -50,6 +50,9 @@ def lib():
         L = C.CDLL(HOST_LIB)
         L.psserve_create.argtypes = [C.c_void_p, C.POINTER(Scene), C.c_int, C.POINTER(Config), C.POINTER(C.c_void_p)]
         L.psserve_create.restype = C.c_int
+        L.psserve_create_multi.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Scene), C.c_int, C.POINTER(Config),
+                                           C.POINTER(C.c_void_p)]
+        L.psserve_create_multi.restype = C.c_int
         L.psserve_set_config.argtypes = [C.c_void_p, C.POINTER(Config)]
         L.psserve_set_config.restype = C.c_int
         L.psserve_destroy.argtypes = [C.c_void_p]
@@ -63,28 +66,38 @@ def lib():
 
 
 class RenderServer:
-    """Scenes resident on one context; serve() runs a request stream.  The
-    native server keeps its pooled render targets between runs; close() it
-    before closing the context."""
+    """Scenes resident on one context, or on each of several contexts of the
+    device (render streams: ctx plus extra_ctxs); serve() runs a request
+    stream.  The native server keeps its pooled render targets between runs;
+    close() it before closing the contexts."""
 
-    def __init__(self, ctx, assets, gen_mips=True):
+    def __init__(self, ctx, assets, gen_mips=True, extra_ctxs=()):
         self.ctx, self.assets = ctx, assets
-        self.scenes = (Scene * len(assets))()
-        for k, a in enumerate(assets):
-            ids = []
-            for t in a.textures:
-                idx = ctx.texture_create(t.width, t.height, 4, wrap=t.wrap, mip_levels=t.levels, pixels=t.images[0])
-                if t.levels > 1 and gen_mips:
-                    ctx.texture_generate_mips(idx)
-                ids.append(idx)
-            v = ctx.vao_create()
-            for s, arr in a.mesh.items():
-                ctx.vao_attach(v, s, ctx.vbo_create(arr))
-            self.scenes[k] = Scene(v, ids[0], ids[1])
-        self.prog = ctx.program_create(N.PROC_BLINN)
+        self.ctxs = [ctx] + list(extra_ctxs)
+        self.all_scenes = (Scene * (len(assets) * len(self.ctxs)))()
+        progs = []
+        for c, cx in enumerate(self.ctxs):
+            for k, a in enumerate(assets):
+                ids = []
+                for t in a.textures:
+                    idx = cx.texture_create(t.width, t.height, 4, wrap=t.wrap, mip_levels=t.levels, pixels=t.images[0])
+                    if t.levels > 1 and gen_mips:
+                        cx.texture_generate_mips(idx)
+                    ids.append(idx)
+                v = cx.vao_create()
+                for s, arr in a.mesh.items():
+                    cx.vao_attach(v, s, cx.vbo_create(arr))
+                self.all_scenes[c * len(assets) + k] = Scene(v, ids[0], ids[1])
+            progs.append(cx.program_create(N.PROC_BLINN))
+        if len(set(progs)) != 1:
+            raise RuntimeError("the Blinn-Phong programme has different ids on the contexts: %s" % progs)
+        self.prog = progs[0]
+        self.scenes = self.all_scenes   # scene k of the first context at [k]
         L = lib()
         h = C.c_void_p()
-        self._check(L.psserve_create(ctx.h, self.scenes, len(assets), C.byref(self._config()), C.byref(h)))
+        ctxs = (C.c_void_p * len(self.ctxs))(*[cx.h for cx in self.ctxs])
+        self._check(L.psserve_create_multi(ctxs, len(self.ctxs), self.all_scenes, len(assets),
+                                           C.byref(self._config()), C.byref(h)))
         self.h = h
 
     @staticmethod
@@ -93,9 +106,9 @@ class RenderServer:
             raise RuntimeError("psserve failed (%d): %s" % (rc, lib().psserve_last_error().decode()))
 
     def _config(self, hbm_budget=8 << 30, max_batch_views=64, max_batch_pixels=16 << 20, max_inflight=2,
-                scratch_bytes_per_px=48, clear_bgra=0x00302010, to_host=True):
+                scratch_bytes_per_px=48, clear_bgra=0x00302010, to_host=True, streams=0):
         return Config(self.prog, hbm_budget, scratch_bytes_per_px, max_batch_views, max_batch_pixels, max_inflight,
-                      clear_bgra, 1 if to_host else 0)
+                      clear_bgra, 1 if to_host else 0, streams)
 
     def close(self):
         """frees the server's pooled targets (the scenes stay on the context)"""
@@ -117,18 +130,19 @@ class RenderServer:
         return reqs
 
     def serve(self, views, arrivals, hbm_budget=8 << 30, max_batch_views=64, max_batch_pixels=16 << 20,
-              max_inflight=2, scratch_bytes_per_px=48, clear_bgra=0x00302010, keep=None, to_host=True):
+              max_inflight=2, scratch_bytes_per_px=48, clear_bgra=0x00302010, keep=None, to_host=True, streams=0):
         """Runs the stream in real time.  keep: indices whose images to return
         (needs to_host).  to_host=False leaves images in HBM: a request is done
-        when its view is rendered.  Returns (per-request result dict of arrays,
-        stats dict, images dict)."""
+        when its view is rendered.  streams: render streams (contexts) used,
+        0 for all.  Returns (per-request result dict of arrays, stats dict,
+        images dict)."""
         if keep is not None and not to_host:
             raise ValueError("images are returned only when delivered to the host")
         L = lib()
         n = len(views)
         reqs = self.requests(views, arrivals)
         cfg = self._config(hbm_budget, max_batch_views, max_batch_pixels, max_inflight, scratch_bytes_per_px,
-                           clear_bgra, to_host)
+                           clear_bgra, to_host, streams)
         self._check(L.psserve_set_config(self.h, C.byref(cfg)))
         res = (Result * n)()
         st = Stats()

@@ -33,7 +33,8 @@ def test_library_is_gfx950_code_object():
 def test_serving_library_exports_psserve_header():
     hdr = open(os.path.join(ROOT, "include", "psserve.h")).read()
     names = set(re.findall(r"^\s*(?:int|const char\*)\s+(psserve_\w+)\s*\(", hdr, re.M))
-    assert names == {"psserve_create", "psserve_set_config", "psserve_destroy", "psserve_run", "psserve_last_error"}
+    assert names == {"psserve_create", "psserve_create_multi", "psserve_set_config", "psserve_destroy", "psserve_run",
+                     "psserve_last_error"}
     lib = os.path.join(ROOT, "puresoft3d_amd", "libpuresoft3d.so")
     if not os.path.exists(lib):
         pytest.skip("host library not built")

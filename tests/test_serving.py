@@ -37,6 +37,11 @@ def test_serve_rejects_bad_arguments():
     bad = serving.Config(0, 1 << 30, 16, 0, 1 << 20, 2, 0, 1)      # no views per batch
     assert L.psserve_set_config(None, C.byref(bad)) == N.ERR_INVALID
     assert L.psserve_destroy(None) == 0
+    ctxs = (C.c_void_p * 2)(None, None)
+    assert L.psserve_create_multi(ctxs, 2, None, 1, C.byref(cfg), C.byref(h)) == N.ERR_INVALID and not h.value
+    assert L.psserve_create_multi(None, 0, None, 1, C.byref(cfg), C.byref(h)) == N.ERR_INVALID
+    neg = serving.Config(0, 1 << 30, 16, 8, 1 << 20, 2, 0, 1, -1)   # negative stream count
+    assert L.psserve_set_config(None, C.byref(neg)) == N.ERR_INVALID
 
 
 def _small_server(ctx):
@@ -70,6 +75,32 @@ def test_served_images_equal_alone_and_oracle(oracle):
     for i in order[:3]:
         f, _ = oracle_view(oracle, S.view_as_scene(assets, views[i]), CLEAR)
         np.testing.assert_array_equal(imgs[i], f.colour)
+
+
+@pytest.mark.gpu
+def test_two_render_streams_serve_identical_images():
+    """batches spread over two contexts (render streams) give the same images
+    as one stream and as each view rendered alone"""
+    from puresoft3d_amd import serving
+    ctx, ctx2 = N.Context(64, 64), N.Context(64, 64)
+    try:
+        assets = S.serving_assets(n_scenes=4, seed=31, tri_lo=500, tri_hi=20000, tex_lo=64, tex_hi=256)
+        srv = serving.RenderServer(ctx, assets, extra_ctxs=[ctx2])
+        views, arr = S.poisson_requests(assets, n=48, rate=20000.0, seed=13, side_lo=24, side_hi=320)
+        announce("serve 48 views over two render streams")
+        res, st, imgs = srv.serve(views, arr, max_batch_views=8, max_inflight=2, keep=range(len(views)))
+        assert st["views"] == len(views) and st["batches"] > 2
+        assert (res["done_s"] >= res["launched_s"]).all() and (res["launched_s"] >= res["arrival_s"]).all()
+        res1, st1, imgs1 = srv.serve(views, arr, max_batch_views=8, max_inflight=2, keep=range(len(views)),
+                                     streams=1)
+        alone = [srv.render_alone(v) for v in views]
+        for i in range(len(views)):
+            np.testing.assert_array_equal(imgs[i], alone[i], err_msg="view %d" % i)
+            np.testing.assert_array_equal(imgs1[i], alone[i], err_msg="view %d, one stream" % i)
+        srv.close()
+    finally:
+        ctx2.close()
+        ctx.close()
 
 
 @pytest.mark.gpu
