@@ -226,13 +226,14 @@ PS_DEV void ps_span_varyings(const PsSpanGeom& g, const float* u0, const float* 
     }
 }
 
-// the same for a run-time number of varying floats (nv <= PS_MAX_NV)
+// the same for a run-time number of varying floats (nv <= NMAX)
+template <int NMAX = PS_MAX_NV>
 PS_DEV void ps_span_varyings_n(const PsSpanGeom& g, const float* u0, const float* u1, const float* u2, float* start,
                                float* step, int nv)
 {
     float rsc = g.step_count == 0 ? 1.0f : 1.0f / (float)g.step_count;
 #pragma unroll
-    for (int i = 0; i < PS_MAX_NV; i++) {
+    for (int i = 0; i < NMAX; i++) {
         if (i >= nv) break;
         float a0 = u0[i] * g.ccL[0], a1 = u1[i] * g.ccL[1], a2 = u2[i] * g.ccL[2];
         float s = (a0 + a1) + a2;
@@ -244,12 +245,12 @@ PS_DEV void ps_span_varyings_n(const PsSpanGeom& g, const float* u0, const float
     if (g.skip > 0) {
         if (g.skip == 1) {
 #pragma unroll
-            for (int i = 0; i < PS_MAX_NV; i++)
+            for (int i = 0; i < NMAX; i++)
                 if (i < nv) start[i] = start[i] + step[i];
         } else {
             float n = (float)g.skip;
 #pragma unroll
-            for (int i = 0; i < PS_MAX_NV; i++)
+            for (int i = 0; i < NMAX; i++)
                 if (i < nv) start[i] = start[i] + step[i] * n;
         }
     }
