@@ -973,33 +973,35 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nro
 // that window go round again from the lowest of them, so every round
 // resolves at least one lane and runs of triangles without rows are jumped.
 // All lanes of the wave call it; i0 is the wave's first item.
-__device__ __forceinline__ int64_t ps_row_tri(const uint32_t* __restrict__ row_off, int64_t T, int64_t R, int64_t i0)
+// The search runs only for flushes of up to 2^20 triangles (larger ones
+// read k_expand's table), and R < 2^26, so its arithmetic is 32-bit.
+__device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, int T, uint32_t R, uint32_t i0)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t i = i0 + lane;
-    int64_t found = -1;
+    const uint32_t i = i0 + lane;
+    int found = -1;
     uint64_t pend = __ballot(i < R);
     while (pend) {
-        const int64_t tgt = i0 + __builtin_ctzll(pend);
-        int64_t lo = 0, hi = T;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
+        const uint32_t tgt = i0 + __builtin_ctzll(pend);
+        int lo = 0, hi = T;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
         while (hi - lo > 1) {
-            int64_t step = (hi - lo + 63) / 64;
-            int64_t p = lo + lane * step;
-            bool le = p < hi && (int64_t)row_off[p] <= tgt;
-            int c = __popcll(__ballot(le));
-            int64_t nhi = lo + c * step;
-            lo += (int64_t)(c - 1) * step;
+            const int step = (hi - lo + 63) >> 6;
+            const int p = lo + lane * step;
+            const bool le = p < hi && row_off[p] <= tgt;
+            const int c = __popcll(__ballot(le));
+            const int nhi = lo + c * step;
+            lo += (c - 1) * step;
             hi = nhi < hi ? nhi : hi;
         }
-        int64_t we = lo + 1 + lane;
-        int64_t e = we <= T ? (int64_t)row_off[we] : INT64_MAX;
+        const int we = lo + 1 + lane;
+        const uint32_t e = we <= T ? row_off[we] : 0xffffffffu;   // above every item (R < 2^26)
         int pos = 0;   // row starts of the window at or below i: 0..63, or 64 past it
 #pragma unroll
         for (int st = 32; st >= 1; st >>= 1) {
-            int64_t v = __shfl(e, pos + st - 1, 64);
+            const uint32_t v = (uint32_t)__shfl((int)e, pos + st - 1, 64);
             if (v <= i) pos += st;
         }
-        if (__shfl(e, 63, 64) <= i) pos = 64;
+        if ((uint32_t)__builtin_amdgcn_readlane((int)e, 63) <= i) pos = 64;
         if (found < 0 && i < R && i >= tgt && pos < 64) found = lo + pos;
         pend = __ballot(found < 0 && i < R);
     }
@@ -1036,7 +1038,7 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         uint32_t nspan = 0, nfrag = 0;
         // the item's triangle: from k_expand's table, or searched in row_off
         int64_t tw = -1;
-        if constexpr (SEARCH) tw = ps_row_tri(row_off, T, R, base + (threadIdx.x & ~63));
+        if constexpr (SEARCH) tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)(base + (threadIdx.x & ~63)));
         PsSpanTiles st = {};
         bool has = false;
         if (i < R) {
