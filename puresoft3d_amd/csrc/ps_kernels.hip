@@ -889,7 +889,8 @@ struct PsRowWave {
 };
 __global__ void __launch_bounds__(1024) k_scan_parts(const uint32_t* __restrict__ in, int64_t n,
                                                      const uint32_t* __restrict__ partial, int pgran,
-                                                     uint32_t* __restrict__ out, uint32_t* total_out)
+                                                     uint32_t* __restrict__ out, uint32_t* total_out,
+                                                     uint32_t* __restrict__ bstart, int64_t bcap)
 {
     __shared__ PsRowWave xw[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -925,6 +926,15 @@ __global__ void __launch_bounds__(1024) k_scan_parts(const uint32_t* __restrict_
     uint32_t o[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) { o[j] = r; r += v[j]; }
+    // the triangle of every 64th row item: item 64b lies in item i + j's rows
+    if (bstart) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (i + j >= n) break;
+            const int64_t b1 = min(((int64_t)o[j] + v[j] + 63) >> 6, bcap);
+            for (int64_t b = ((int64_t)o[j] + 63) >> 6; b < b1; b++) bstart[b] = (uint32_t)(i + j);
+        }
+    }
     if (i + 4 <= n) {
         *(uint4*)(out + i) = make_uint4(o[0], o[1], o[2], o[3]);
     } else {
@@ -975,15 +985,21 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nro
 // All lanes of the wave call it; i0 is the wave's first item.
 // The search runs only for flushes of up to 2^20 triangles (larger ones
 // read k_expand's table), and R < 2^26, so its arithmetic is 32-bit.
-__device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, int T, uint32_t R, uint32_t i0)
+// hint, when not negative, is the triangle of item i0 (the row scan's table
+// of every 64th item): the first round then takes the 64 row starts after it
+// without a search.
+__device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, int T, uint32_t R, uint32_t i0, int hint)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t i = i0 + lane;
     int found = -1;
     uint64_t pend = __ballot(i < R);
+    const int lo0 = hint > 0 ? hint : 0;
+    bool guess = hint >= 0;
     while (pend) {
         const uint32_t tgt = i0 + __builtin_ctzll(pend);
-        int lo = 0, hi = T;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
+        int lo = lo0, hi = guess ? lo0 + 1 : T;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
+        guess = false;
         while (hi - lo > 1) {
             const int step = (hi - lo + 63) >> 6;
             const int p = lo + lane * step;
@@ -1014,7 +1030,8 @@ __device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, 
 // Also counts, per tile, the span pieces in it and the waves (of 64 span
 // items) that have pieces there: tcnt[t] += pieces | waves << 32.
 template <bool SEARCH>
-__global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const int* __restrict__ span_tri, const uint32_t* __restrict__ row_off,
+__global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const int* __restrict__ span_tri,
+                                              const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ bstart,
                                               int64_t T, int64_t cap,
                                               PsSpan* __restrict__ spans, unsigned long long* __restrict__ tcnt,
                                               const PsView* __restrict__ views, int n_views, int chunk_w, int tile_rows,
@@ -1038,7 +1055,11 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         uint32_t nspan = 0, nfrag = 0;
         // the item's triangle: from k_expand's table, or searched in row_off
         int64_t tw = -1;
-        if constexpr (SEARCH) tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)(base + (threadIdx.x & ~63)));
+        if constexpr (SEARCH) {
+            const int64_t i0 = base + (threadIdx.x & ~63);
+            tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)i0,
+                            bstart && i0 < R ? (int)bstart[i0 >> 6] : -1);
+        }
         PsSpanTiles st = {};
         bool has = false;
         if (i < R) {
@@ -2115,9 +2136,10 @@ void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t 
 }
 
 void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int pgran,
-                    uint32_t* total_out)
+                    uint32_t* total_out, uint32_t* bstart, int64_t bcap)
 {
-    hipLaunchKernelGGL(k_scan_parts, dim3(psk_scan_blocks(n)), dim3(1024), 0, s, in, n, parts, pgran, out, total_out);
+    hipLaunchKernelGGL(k_scan_parts, dim3(psk_scan_blocks(n)), dim3(1024), 0, s, in, n, parts, pgran, out, total_out,
+                       bstart, bcap);
 }
 
 int psk_heavy_cap() { return PS_HEAVY_CAP; }
@@ -2142,18 +2164,18 @@ void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, i
     hipLaunchKernelGGL(k_expand, dim3(gsz(T, 256)), dim3(256), 0, s, nrows, row_off, T, span_tri, cap, cnt);
 }
 
-void psk_span(hipStream_t s, const PsTri* tris, const int* span_tri, const uint32_t* row_off, int64_t T, PsSpan* spans,
-              unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w, int tile_rows, int64_t cap,
-              uint32_t* heavy, PsCounters* cnt)
+void psk_span(hipStream_t s, const PsTri* tris, const int* span_tri, const uint32_t* row_off, const uint32_t* bstart,
+              int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w,
+              int tile_rows, int64_t cap, uint32_t* heavy, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 2048) g = 2048;
     const int tile_cols = 64 / tile_rows;
     if (span_tri)
-        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, T, cap, spans, tcnt, views,
-                           n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
+        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, bstart, T, cap, spans, tcnt,
+                           views, n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
     else
-        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, T, cap, spans, tcnt, views,
-                           n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
+        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, bstart, T, cap, spans, tcnt,
+                           views, n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,

@@ -144,6 +144,7 @@ struct FlushBufs {
     DevBuf<float> grad;   // per triangle: screen gradients for the mip level (8 floats)
     DevBuf<uint32_t> nrows, rowoff, partial;
     DevBuf<int> spantri;
+    DevBuf<uint32_t> bstart;   // triangle of every 64th row item (flushes that search rows, of one programme)
     DevBuf<PsSpan> spans;
     // tile lists: per-tile piece and run counts (zero between flushes: the
     // piece kernel counts them back down), their scan, the scan's block sums,
@@ -559,7 +560,8 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
         return fail(c, PSGPU_ERR_NOMEM, "more than %lld spans or %lld pieces in one flush", (long long)c->max_spans,
                     (long long)c->max_segs);
     if ((rc = ensure(c, b.partial, row_partials(T))) ||
-        (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) || (rc = ensure(c, b.spans, b.span_cap)) ||
+        (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) ||
+        (T <= kSpanSearchTris && (rc = ensure(c, b.bstart, b.span_cap / 64 + 2))) || (rc = ensure(c, b.spans, b.span_cap)) ||
         (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)) ||
         (rc = ensure(c, b.runs, (size_t)b.seg_cap * 2)) || (rc = ensure(c, b.runs2, (size_t)b.seg_cap * 2)) ||
         (rc = ensure(c, b.tstart, n_tiles + 1)) || (rc = ensure(c, b.tpartial, psk_scan_blocks(n_tiles) + 1)) ||
@@ -627,7 +629,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         // largest row total of the log is known even after an overflow
         const int sb = psk_setup_block();
         if (row_part)
-            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, sb, c->d_sizes + 0);
+            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, sb, c->d_sizes + 0,
+                           T > kSpanSearchTris ? nullptr : c->fb.bstart.p, c->fb.span_cap / 64 + 2);
         else
             psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
     }
@@ -648,7 +651,8 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     uint32_t* heavy = imm ? nullptr : c->fb.heavy.p;
     {
         ProfScope p(c, "span");
-        psk_span(s, c->fb.tris.p, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p, T, c->fb.spans.p, c->fb.tcnt.p, dv,
+        psk_span(s, c->fb.tris.p, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p,
+                 row_part && !expand ? c->fb.bstart.p : nullptr, T, c->fb.spans.p, c->fb.tcnt.p, dv,
                  f.n_views, f.chunk_w, f.tile_rows, c->fb.span_cap, heavy, c->d_cnt);
     }
     {
@@ -1232,7 +1236,8 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
         FlushBufs x = b;
         x.span_cap = spans;
         x.seg_cap = spans * 3 / 2;
-        if ((!expand || t.take(x.spantri, x.span_cap)) && t.take(x.spans, x.span_cap) && t.take(x.segs, x.seg_cap) &&
+        if ((!expand || t.take(x.spantri, x.span_cap)) && (expand || t.take(x.bstart, x.span_cap / 64 + 2)) &&
+            t.take(x.spans, x.span_cap) && t.take(x.segs, x.seg_cap) &&
             t.take(x.ck, (size_t)x.seg_cap * sh.nvs) && t.take(x.runs, (size_t)x.seg_cap * 2) &&
             t.take(x.runs2, (size_t)x.seg_cap * 2)) {
             a = t;
@@ -1316,7 +1321,7 @@ int psgpu_destroy(psgpu_ctx* c)
     for (auto& t : c->tex) fr(t.d);
     for (auto& v : c->vbo) fr(v.d);
     fr(c->d_blob.p); fr(c->fb.tris.p); fr(c->fb.vary.p); fr(c->fb.nrows.p); fr(c->fb.rowoff.p); fr(c->fb.partial.p);
-    fr(c->fb.spantri.p); fr(c->fb.spans.p); fr(c->fb.tcnt.p); fr(c->fb.tstart.p); fr(c->fb.tpartial.p);
+    fr(c->fb.spantri.p); fr(c->fb.bstart.p); fr(c->fb.spans.p); fr(c->fb.tcnt.p); fr(c->fb.tstart.p); fr(c->fb.tpartial.p);
     fr(c->fb.runs.p); fr(c->fb.runs2.p); fr(c->fb.heavy.p); fr(c->fb.winner.p);
     fr(c->fb.segs.p); fr(c->fb.ck.p); fr(c->fb.grad.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);
