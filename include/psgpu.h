@@ -105,7 +105,9 @@ void* psgpu_stream(psgpu_ctx* ctx);
 
 /* ---- textures: createTexture / destroyTexture / getTexture (tex.cpp:4-59) ---- */
 /* pixels may be NULL (contents then undefined, as with image->pixels == NULL).
- * mip_levels >= 1; levels beyond 0 are filled by psgpu_texture_generate_mips. */
+ * mip_levels >= 1; levels beyond 0 are filled by psgpu_texture_generate_mips.
+ * width and height at most 2^22 texels and scanline below 2^24 bytes
+ * (PSGPU_ERR_RANGE beyond: texel addressing uses 24-bit multiplies). */
 int psgpu_texture_create(psgpu_ctx* ctx, int width, int scanline, int height, int elem_len,
                          int extra_layers, int wrap_mode, int mip_levels, const void* pixels, int* out_idx);
 int psgpu_texture_upload(psgpu_ctx* ctx, int tex, int layer, int level, const void* pixels, size_t bytes);

@@ -266,11 +266,13 @@ PS_DEV int ps_lvl_h(const PsTex& t, int l) { int h = t.height >> l; return h < 1
 // 2^21 * 1.5 * 2^-23 < 1, so the floored estimate is within one of the true
 // quotient and a single correction each way makes the remainder exact.
 // Larger |v| (and INT_MIN, the conversion of NaN) take the integer remainder.
+// Textures have at most 2^22 texels a side (psgpu_texture_create), so m and
+// |q| fit the 24-bit multiplier, whose low 32 bits equal the 32-bit product.
 PS_DEV int ps_floor_mod(int v, int m, float rm)
 {
     if ((unsigned)v + (1u << 21) < (2u << 21)) {
         int q = (int)floorf((float)v * rm);
-        int r = v - q * m;
+        int r = v - __mul24(q, m);
         r = r < 0 ? r + m : r;
         return r >= m ? r - m : r;
     }
@@ -416,7 +418,8 @@ PS_DEV void ps_sample_bilinear(const PsTex& t, int level, float u, float v, floa
     int scan = level == 0 ? t.scanline : w * t.elem_len;
     // a layer stays below 4 GiB (psgpu_texture_create): 32-bit texel offsets
     const uint32_t lo = (uint32_t)t.level_off[level];
-    const uint32_t q0 = lo + (uint32_t)r0 * (uint32_t)scan, q1 = lo + (uint32_t)r1 * (uint32_t)scan;
+    // rows below 2^22 and rows of fewer than 2^24 bytes (psgpu_texture_create): 24-bit multiplies
+    const uint32_t q0 = lo + __umul24((uint32_t)r0, (uint32_t)scan), q1 = lo + __umul24((uint32_t)r1, (uint32_t)scan);
     uint32_t a = ps_texel(t.data, q0 + (uint32_t)c0 * 4u);
     uint32_t b = ps_texel(t.data, q0 + (uint32_t)c1 * 4u);
     uint32_t c = ps_texel(t.data, q1 + (uint32_t)c0 * 4u);
@@ -514,7 +517,8 @@ PS_DEV void ps_sample_bilinear_pair(const PsTex& t, const uint8_t* data1, int le
     int scan = level == 0 ? t.scanline : w * t.elem_len;
     // a layer stays below 4 GiB (psgpu_texture_create): 32-bit texel offsets
     const uint32_t lo = (uint32_t)t.level_off[level];
-    const uint32_t q0 = lo + (uint32_t)r0 * (uint32_t)scan, q1 = lo + (uint32_t)r1 * (uint32_t)scan;
+    // rows below 2^22 and rows of fewer than 2^24 bytes (psgpu_texture_create): 24-bit multiplies
+    const uint32_t q0 = lo + __umul24((uint32_t)r0, (uint32_t)scan), q1 = lo + __umul24((uint32_t)r1, (uint32_t)scan);
     const uint32_t o00 = q0 + (uint32_t)c0 * 4u, o01 = q0 + (uint32_t)c1 * 4u;
     const uint32_t o10 = q1 + (uint32_t)c0 * 4u, o11 = q1 + (uint32_t)c1 * 4u;
     const uint8_t* ds[2] = {t.data, data1};

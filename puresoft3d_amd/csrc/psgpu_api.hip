@@ -1355,6 +1355,9 @@ int psgpu_texture_create(psgpu_ctx* c, int width, int scanline, int height, int 
     if (need_ctx(c) || !out_idx) return PSGPU_ERR_INVALID;
     if (elem_len != 1 && elem_len != 4) return fail(c, PSGPU_ERR_INVALID, "elemLen must be 1 or 4 (fbo.cpp:21-24)");
     if (width < 1 || height < 1 || scanline < width * elem_len) return fail(c, PSGPU_ERR_INVALID, "bad texture size");
+    if (width > (1 << 22) || height > (1 << 22) || scanline >= (1 << 24))
+        return fail(c, PSGPU_ERR_RANGE, "texture of %dx%d texels, %d-byte rows: at most 2^22 texels a side and rows "
+                    "below 2^24 bytes", width, height, scanline);
     if (extra_layers < 0 || extra_layers > PS_MAX_LAYERS - 1) return fail(c, PSGPU_ERR_RANGE, "extraLayers out of range");
     if (wrap_mode != PSGPU_CLAMP && wrap_mode != PSGPU_WRAP) return fail(c, PSGPU_ERR_INVALID, "bad wrap mode");
     int maxl = 1;

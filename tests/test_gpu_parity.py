@@ -219,6 +219,10 @@ def test_extra_fbo_slots_are_recorded_only(oracle):
         for bad in ((t, 16), (t, -1), (t + 100, 3)):
             with pytest.raises(IndexError):   # PSGPU_ERR_RANGE
                 ctx.target(*bad)
+        # texel addressing limits: 2^22 texels a side, rows below 2^24 bytes
+        for w, h, scan in (((1 << 22) + 1, 1, None), (1, (1 << 22) + 1, None), (4, 4, 1 << 24)):
+            with pytest.raises(IndexError):   # PSGPU_ERR_RANGE, refused before any allocation
+                ctx.texture_create(w, h, 4, scanline=scan)
         gs.frame()
         col = ctx.read_colour()
         np.testing.assert_array_equal(ctx.texture_download(t), px.reshape(16, 64))
