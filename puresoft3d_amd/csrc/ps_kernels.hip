@@ -983,22 +983,24 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nro
 // that window go round again from the lowest of them, so every round
 // resolves at least one lane and runs of triangles without rows are jumped.
 // All lanes of the wave call it; i0 is the wave's first item.
-// The search runs only for flushes of up to 2^20 triangles (larger ones
-// read k_expand's table), and R < 2^26, so its arithmetic is 32-bit.
-// hint, when not negative, is the triangle of item i0 (the row scan's table
-// of every 64th item): the first round then takes the 64 row starts after it
-// without a search.
-__device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, int T, uint32_t R, uint32_t i0, int hint)
+// Flushes that search hold fewer than 2^31 triangles and R < 2^26 row items,
+// so the arithmetic is 32-bit.  hint, when not negative, is the triangle of
+// item i0 and hint_hi that of item i0 + 64 (the row scan's table of every
+// 64th item's triangle): the first round then takes the 64 row starts after
+// hint without a search, and a later one searches only up to hint_hi.
+__device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, int T, uint32_t R, uint32_t i0, int hint,
+                                          int hint_hi)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t i = i0 + lane;
     int found = -1;
     uint64_t pend = __ballot(i < R);
     const int lo0 = hint > 0 ? hint : 0;
+    const int hi0 = hint >= 0 && hint_hi >= 0 ? hint_hi + 1 : T;
     bool guess = hint >= 0;
     while (pend) {
         const uint32_t tgt = i0 + __builtin_ctzll(pend);
-        int lo = lo0, hi = guess ? lo0 + 1 : T;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
+        int lo = lo0, hi = guess ? lo0 + 1 : hi0;   // row_off[lo] <= tgt; hi == T or row_off[hi] > tgt
         guess = false;
         while (hi - lo > 1) {
             const int step = (hi - lo + 63) >> 6;
@@ -1057,8 +1059,9 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
         int64_t tw = -1;
         if constexpr (SEARCH) {
             const int64_t i0 = base + (threadIdx.x & ~63);
-            tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)i0,
-                            bstart && i0 < R ? (int)bstart[i0 >> 6] : -1);
+            const bool tab = bstart && i0 < R;
+            tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)i0, tab ? (int)bstart[i0 >> 6] : -1,
+                            tab && i0 + 64 < R ? (int)bstart[(i0 >> 6) + 1] : -1);
         }
         PsSpanTiles st = {};
         bool has = false;

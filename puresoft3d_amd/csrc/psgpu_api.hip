@@ -561,7 +561,7 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
                     (long long)c->max_segs);
     if ((rc = ensure(c, b.partial, row_partials(T))) ||
         (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) ||
-        (T <= kSpanSearchTris && (rc = ensure(c, b.bstart, b.span_cap / 64 + 2))) || (rc = ensure(c, b.spans, b.span_cap)) ||
+        (rc = ensure(c, b.bstart, b.span_cap / 64 + 2)) || (rc = ensure(c, b.spans, b.span_cap)) ||
         (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)) ||
         (rc = ensure(c, b.runs, (size_t)b.seg_cap * 2)) || (rc = ensure(c, b.runs2, (size_t)b.seg_cap * 2)) ||
         (rc = ensure(c, b.tstart, n_tiles + 1)) || (rc = ensure(c, b.tpartial, psk_scan_blocks(n_tiles) + 1)) ||
@@ -629,15 +629,17 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
         // largest row total of the log is known even after an overflow
         const int sb = psk_setup_block();
         if (row_part)
-            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, sb, c->d_sizes + 0,
-                           T > kSpanSearchTris ? nullptr : c->fb.bstart.p, c->fb.span_cap / 64 + 2);
+            psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, sb, c->d_sizes + 0, c->fb.bstart.p,
+                           c->fb.span_cap / 64 + 2);
         else
             psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
     }
-    // up to kSpanSearchTris triangles k_span finds each row's triangle by a
-    // search in the row offsets; beyond, where views leave long runs of
-    // triangles without rows, k_expand writes the table out first
-    const bool expand = T > kSpanSearchTris;
+    // k_span finds each row's triangle by a search in the row offsets,
+    // started from the row scan's table of every 64th item's triangle when
+    // there is one (flushes of one programme); without it, beyond
+    // kSpanSearchTris triangles, where views leave long runs of triangles
+    // without rows, k_expand writes the row -> triangle table out first
+    const bool expand = T > kSpanSearchTris && !row_part;
     if (expand) {
         ProfScope p(c, "expand");
         psk_expand(s, c->fb.nrows.p, c->fb.rowoff.p, T, c->fb.spantri.p, c->fb.span_cap, c->d_cnt);
@@ -652,7 +654,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     {
         ProfScope p(c, "span");
         psk_span(s, c->fb.tris.p, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p,
-                 row_part && !expand ? c->fb.bstart.p : nullptr, T, c->fb.spans.p, c->fb.tcnt.p, dv,
+                 row_part ? c->fb.bstart.p : nullptr, T, c->fb.spans.p, c->fb.tcnt.p, dv,
                  f.n_views, f.chunk_w, f.tile_rows, c->fb.span_cap, heavy, c->d_cnt);
     }
     {
@@ -1236,7 +1238,7 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
         FlushBufs x = b;
         x.span_cap = spans;
         x.seg_cap = spans * 3 / 2;
-        if ((!expand || t.take(x.spantri, x.span_cap)) && (expand || t.take(x.bstart, x.span_cap / 64 + 2)) &&
+        if ((!expand || t.take(x.spantri, x.span_cap)) && t.take(x.bstart, x.span_cap / 64 + 2) &&
             t.take(x.spans, x.span_cap) && t.take(x.segs, x.seg_cap) &&
             t.take(x.ck, (size_t)x.seg_cap * sh.nvs) && t.take(x.runs, (size_t)x.seg_cap * 2) &&
             t.take(x.runs2, (size_t)x.seg_cap * 2)) {
