@@ -53,7 +53,7 @@ def kernel_alg_bytes(name, scene):
 
 
 PROFILE_NAMES = {"setup": "k_setup<7, false>", "raster": "k_raster<false, 2>", "shade": "k_shade<7>",
-                 "span": "k_span<true>", "segments": "k_segments<14>", "expand": "k_expand"}
+                 "span": "k_span", "segments": "k_segments<14>"}
 
 
 def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "traffic_terrain1080p.md")):

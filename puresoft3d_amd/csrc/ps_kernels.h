@@ -13,16 +13,14 @@ int psk_setup_block();
 void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd, int64_t T, PsTri* tris, float* vary, int nvs,
                float* grads, uint32_t* nrows, uint32_t* row_part, PsCounters* cnt, const PsView* views);
 int psk_scan_blocks(int64_t cap);
-// exclusive scan; out[n] and *total_out (if given) receive the total
-void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
-              uint32_t* partial, uint32_t* total_out, const PsCounters* c);
-// the same scan from given partial sums of pgran items each (pgran divides 4096)
-// bstart (optional, bcap entries): receives the triangle of every 64th row item
+// exclusive scan of the row counts from given partial sums of pgran items
+// each (pgran divides 4096); out[n] and *total_out receive the total, and
+// bstart (bcap entries) the triangle of every 64th row item
 void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int pgran,
                     uint32_t* total_out, uint32_t* bstart, int64_t bcap);
-// row item -> triangle table for k_span (large flushes; k_span searches row_off without it)
-void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
-                PsCounters* cnt);
+// the same from the counts alone (block totals into partial first)
+void psk_scan_rows(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, uint32_t* partial, uint32_t* total_out,
+                   uint32_t* bstart, int64_t bcap);
 // exclusive scan of the per-tile counts (pieces | runs << 32) into tstart
 // (n + 1 entries); the piece total into *total_out, its running maximum at
 // total_out[2]
@@ -31,9 +29,9 @@ int psk_heavy_cap();
 void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, unsigned long long* tstart,
                     unsigned long long* partial, uint32_t* total_out, const PsCounters* c, uint32_t* heavy);
 // spans of every row item; tcnt[t] += the pieces in tile t | the waves with pieces there << 32
-void psk_span(hipStream_t s, const PsTri* tris, const int* span_tri, const uint32_t* row_off, const uint32_t* bstart,
-              int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w,
-              int tile_rows, int64_t cap, uint32_t* heavy, PsCounters* cnt);
+void psk_span(hipStream_t s, const PsTri* tris, const uint32_t* row_off, const uint32_t* bstart, int64_t T,
+              PsSpan* spans, unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w, int tile_rows,
+              int64_t cap, uint32_t* heavy, PsCounters* cnt);
 // the pieces, into runs of the tile lists f.tstart / f.runs; counts tcnt back down to zero.
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,
                   const PsTri* tris, const PsDraw* draws, const float* vary, int nvs, unsigned long long* tcnt, PsSeg* segs,

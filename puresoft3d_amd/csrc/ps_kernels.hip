@@ -3,8 +3,10 @@
 //  setup   : one thread per triangle: vertex twin, 1/w divide, back-face cull,
 //            z reject, pushVertex/pushTriangle row-range and edge choice
 //            (drawvao.cpp:36-63, vertthrd.cpp, rasterizer.cpp:122-283)
-//  expand  : one wave per 64 triangles writes the triangle id of every row
-//  span    : one thread per (triangle,row): span ends, clamping and the
+//  rows    : exclusive scan of the row counts, and the triangle of every
+//            64th row item
+//  span    : one thread per (triangle,row), its triangle searched from that
+//            table: span ends, clamping and the
 //            depth / correction-factor start and step (drawvao.cpp:66-110,
 //            interp.cpp:26-80); counts, per tile, the pieces (bin chunks) of
 //            the spans and the waves that have pieces there
@@ -945,35 +947,6 @@ __global__ void __launch_bounds__(1024) k_scan_parts(const uint32_t* __restrict_
     if (blockIdx.x == gridDim.x - 1 && wave == 0) ps_scan_total(before + btot, n, n, out, total_out, (const PsCounters*)nullptr);
 }
 
-// ---------------------------------------------------------------------------
-// expand: triangle id of every drawn row
-// ---------------------------------------------------------------------------
-// row item -> triangle, written out: one wave per 64 triangles stores each
-// triangle's index once per row (used for flushes of many triangles, where
-// the search in k_span would cross long runs of triangles without rows)
-__global__ void __launch_bounds__(256) k_expand(const uint32_t* __restrict__ nrows, const uint32_t* __restrict__ row_off,
-                                                int64_t T, int* __restrict__ span_tri, int64_t cap, PsCounters* cnt)
-{
-    // row_off[T] holds the number of rows R
-    if (ps_poisoned(cnt)) return;
-    int lane = threadIdx.x & 63;
-    int64_t t0 = (blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63));
-    if (t0 >= T) return;
-    int64_t R = row_off[T];
-    if (R > cap) {
-        if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&cnt->overflow, 1);
-        return;
-    }
-    int64_t mt = t0 + lane;
-    uint32_t myn = mt < T ? nrows[mt] : 0;
-    uint32_t myo = mt < T ? row_off[mt] : 0;
-    for (int j = 0; j < 64; j++) {
-        uint32_t n = __builtin_amdgcn_readlane(myn, j);
-        uint32_t o = __builtin_amdgcn_readlane(myo, j);
-        for (uint32_t r = lane; r < n; r += 64) span_tri[o + r] = (int)(t0 + j);
-    }
-}
-
 // Triangle of row item i: the t with row_off[t] <= i < row_off[t + 1]
 // (row_off is the exclusive scan of the row counts, row_off[T] = R).  The
 // wave finds the triangle of its lowest unresolved item with a 64-way search
@@ -1030,9 +1003,10 @@ __device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, 
 // span setup
 // ---------------------------------------------------------------------------
 // Also counts, per tile, the span pieces in it and the waves (of 64 span
-// items) that have pieces there: tcnt[t] += pieces | waves << 32.
-template <bool SEARCH>
-__global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, const int* __restrict__ span_tri,
+// items) that have pieces there: tcnt[t] += pieces | waves << 32.  Eight
+// waves per SIMD (64 VGPRs, no spill): at seven the 256-view batch's span
+// pass took 1.55 ms instead of 1.21.
+__global__ void __launch_bounds__(256, 8) k_span(const PsTri* __restrict__ tris,
                                               const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ bstart,
                                               int64_t T, int64_t cap,
                                               PsSpan* __restrict__ spans, unsigned long long* __restrict__ tcnt,
@@ -1055,18 +1029,15 @@ __global__ void __launch_bounds__(256) k_span(const PsTri* __restrict__ tris, co
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < R; base += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = base + threadIdx.x;
         uint32_t nspan = 0, nfrag = 0;
-        // the item's triangle: from k_expand's table, or searched in row_off
-        int64_t tw = -1;
-        if constexpr (SEARCH) {
-            const int64_t i0 = base + (threadIdx.x & ~63);
-            const bool tab = bstart && i0 < R;
-            tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)i0, tab ? (int)bstart[i0 >> 6] : -1,
-                            tab && i0 + 64 < R ? (int)bstart[(i0 >> 6) + 1] : -1);
-        }
+        // the item's triangle, searched in row_off from the row scan's table
+        const int64_t i0 = base + (threadIdx.x & ~63);
+        const bool tab = i0 < R;
+        const int tw = ps_row_tri(row_off, (int)T, (uint32_t)R, (uint32_t)i0, tab ? (int)bstart[i0 >> 6] : -1,
+                                  tab && i0 + 64 < R ? (int)bstart[(i0 >> 6) + 1] : -1);
         PsSpanTiles st = {};
         bool has = false;
         if (i < R) {
-            int t = SEARCH ? (int)tw : span_tri[i];
+            const int t = tw;
             const PsTri tr = tris[t];
             int64_t vtb = tb1;
             int vntx = ntx1, W = W1;
@@ -2129,14 +2100,6 @@ void psk_setup(hipStream_t s, int prog, bool share, const PsDraw* draws, int nd,
 
 int psk_scan_blocks(int64_t cap) { return gsz(cap, SCAN_BLOCK); }
 
-void psk_scan(hipStream_t s, const uint32_t* in, const uint32_t* n_dev, int64_t n_host, int64_t cap, uint32_t* out,
-              uint32_t* partial, uint32_t* total_out, const PsCounters* c)
-{
-    int nb = psk_scan_blocks(cap);
-    hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, c);
-    hipLaunchKernelGGL(k_scan_final<uint32_t>, dim3(nb), dim3(256), 0, s, in, n_dev, n_host, partial, nb, SCAN_BLOCK, out,
-                       total_out, cap, c, (uint32_t*)nullptr);
-}
 
 void psk_scan_parts(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, const uint32_t* parts, int pgran,
                     uint32_t* total_out, uint32_t* bstart, int64_t bcap)
@@ -2161,24 +2124,23 @@ void psk_scan_tiles(hipStream_t s, const unsigned long long* tcnt, int64_t n, un
                        SCAN_BLOCK, tstart, total_out, n, c, heavy);
 }
 
-void psk_expand(hipStream_t s, const uint32_t* nrows, const uint32_t* row_off, int64_t T, int* span_tri, int64_t cap,
-                PsCounters* cnt)
-{
-    hipLaunchKernelGGL(k_expand, dim3(gsz(T, 256)), dim3(256), 0, s, nrows, row_off, T, span_tri, cap, cnt);
-}
 
-void psk_span(hipStream_t s, const PsTri* tris, const int* span_tri, const uint32_t* row_off, const uint32_t* bstart,
-              int64_t T, PsSpan* spans, unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w,
-              int tile_rows, int64_t cap, uint32_t* heavy, PsCounters* cnt)
+void psk_span(hipStream_t s, const PsTri* tris, const uint32_t* row_off, const uint32_t* bstart, int64_t T,
+              PsSpan* spans, unsigned long long* tcnt, const PsView* views, int n_views, int chunk_w, int tile_rows,
+              int64_t cap, uint32_t* heavy, PsCounters* cnt)
 {
     int g = gsz(cap, 256); if (g > 2048) g = 2048;
     const int tile_cols = 64 / tile_rows;
-    if (span_tri)
-        hipLaunchKernelGGL(k_span<false>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, bstart, T, cap, spans, tcnt,
-                           views, n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
-    else
-        hipLaunchKernelGGL(k_span<true>, dim3(g), dim3(256), 0, s, tris, span_tri, row_off, bstart, T, cap, spans, tcnt,
-                           views, n_views, chunk_w, tile_rows, tile_cols, heavy, cnt);
+    hipLaunchKernelGGL(k_span, dim3(g), dim3(256), 0, s, tris, row_off, bstart, T, cap, spans, tcnt, views, n_views,
+                       chunk_w, tile_rows, tile_cols, heavy, cnt);
+}
+
+void psk_scan_rows(hipStream_t s, const uint32_t* in, int64_t n, uint32_t* out, uint32_t* partial, uint32_t* total_out,
+                   uint32_t* bstart, int64_t bcap)
+{
+    hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3(psk_scan_blocks(n)), dim3(256), 0, s, in, (const uint32_t*)nullptr, n,
+                       partial, (const PsCounters*)nullptr);
+    psk_scan_parts(s, in, n, out, partial, SCAN_BLOCK, total_out, bstart, bcap);
 }
 
 void psk_segments(hipStream_t s, const PsSpan* spans, const uint32_t* row_off, int64_t T, const PsFrame& f,

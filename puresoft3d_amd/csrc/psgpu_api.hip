@@ -35,7 +35,6 @@ namespace {
 const int kMaxSlots = 16;      // config.h:4
 const int kMaxUniforms = 1024; // config.h:5
 const int64_t kMaxFlushTris = (int64_t)8 << 20;  // triangles queued before a flush is forced
-const int64_t kSpanSearchTris = (int64_t)1 << 20; // larger flushes expand rows to triangles first
 // piece records hold a column in 16 bits and span records a row in 24 bits
 const int kMaxViewW = PSGPU_MAX_VIEW_WIDTH, kMaxViewH = PSGPU_MAX_VIEW_HEIGHT;
 
@@ -143,8 +142,7 @@ struct FlushBufs {
     DevBuf<float> vary;
     DevBuf<float> grad;   // per triangle: screen gradients for the mip level (8 floats)
     DevBuf<uint32_t> nrows, rowoff, partial;
-    DevBuf<int> spantri;
-    DevBuf<uint32_t> bstart;   // triangle of every 64th row item (flushes that search rows, of one programme)
+    DevBuf<uint32_t> bstart;   // triangle of every 64th row item, written by the row scan
     DevBuf<PsSpan> spans;
     // tile lists: per-tile piece and run counts (zero between flushes: the
     // piece kernel counts them back down), their scan, the scan's block sums,
@@ -560,7 +558,6 @@ int size_flush_scratch(psgpu_ctx* c, int64_t T, int nvs, int64_t n_tiles)
         return fail(c, PSGPU_ERR_NOMEM, "more than %lld spans or %lld pieces in one flush", (long long)c->max_spans,
                     (long long)c->max_segs);
     if ((rc = ensure(c, b.partial, row_partials(T))) ||
-        (T > kSpanSearchTris && (rc = ensure(c, b.spantri, b.span_cap))) ||
         (rc = ensure(c, b.bstart, b.span_cap / 64 + 2)) || (rc = ensure(c, b.spans, b.span_cap)) ||
         (rc = ensure(c, b.segs, b.seg_cap)) || (rc = ensure(c, b.ck, (size_t)b.seg_cap * nvs)) ||
         (rc = ensure(c, b.runs, (size_t)b.seg_cap * 2)) || (rc = ensure(c, b.runs2, (size_t)b.seg_cap * 2)) ||
@@ -632,18 +629,13 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
             psk_scan_parts(s, c->fb.nrows.p, T, c->fb.rowoff.p, row_part, sb, c->d_sizes + 0, c->fb.bstart.p,
                            c->fb.span_cap / 64 + 2);
         else
-            psk_scan(s, c->fb.nrows.p, nullptr, T, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, nullptr);
+            psk_scan_rows(s, c->fb.nrows.p, T, c->fb.rowoff.p, c->fb.partial.p, c->d_sizes + 0, c->fb.bstart.p,
+                          c->fb.span_cap / 64 + 2);
     }
     // k_span finds each row's triangle by a search in the row offsets,
-    // started from the row scan's table of every 64th item's triangle when
-    // there is one (flushes of one programme); without it, beyond
-    // kSpanSearchTris triangles, where views leave long runs of triangles
-    // without rows, k_expand writes the row -> triangle table out first
-    const bool expand = T > kSpanSearchTris && !row_part;
-    if (expand) {
-        ProfScope p(c, "expand");
-        psk_expand(s, c->fb.nrows.p, c->fb.rowoff.p, T, c->fb.spantri.p, c->fb.span_cap, c->d_cnt);
-    }
+    // started from the row scan's table of every 64th item's triangle and
+    // bounded by the next entry, so runs of triangles without rows (culled,
+    // off screen or sub-pixel) cost nothing
     int imm = 0;   // a flush of one programme that shades in the raster walk
     {
         const PsDraw* hd = (const PsDraw*)blob.bytes.data();
@@ -653,8 +645,7 @@ int run_flush(psgpu_ctx* c, const Cmd& cmd, uint8_t* staged)
     uint32_t* heavy = imm ? nullptr : c->fb.heavy.p;
     {
         ProfScope p(c, "span");
-        psk_span(s, c->fb.tris.p, expand ? c->fb.spantri.p : nullptr, c->fb.rowoff.p,
-                 row_part ? c->fb.bstart.p : nullptr, T, c->fb.spans.p, c->fb.tcnt.p, dv,
+        psk_span(s, c->fb.tris.p, c->fb.rowoff.p, c->fb.bstart.p, T, c->fb.spans.p, c->fb.tcnt.p, dv,
                  f.n_views, f.chunk_w, f.tile_rows, c->fb.span_cap, heavy, c->d_cnt);
     }
     {
@@ -1226,10 +1217,9 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
         !a.take(b.tstart, sh.n_tiles + 1) || !a.take(b.tpartial, psk_scan_blocks(sh.n_tiles) + 1) ||
         !a.take(b.heavy, psk_heavy_cap() + 1))
         return false;
-    const bool expand = T > kSpanSearchTris;
-    // bytes per span: record (+ row table), and 1.5 pieces of record,
-    // checkpoint and two run records
-    const double per_span = 32 + (expand ? 4 : 0) + 1.5 * (sizeof(PsSeg) + 4.0 * sh.nvs + 16);
+    // bytes per span: record and row-start table entry, and 1.5 pieces of
+    // record, checkpoint and two run records
+    const double per_span = 32 + 4.0 / 64 + 1.5 * (sizeof(PsSeg) + 4.0 * sh.nvs + 16);
     size_t left = a.cap - a.used;
     int64_t spans = (int64_t)((double)left / per_span);
     spans = std::min<int64_t>(spans, (int64_t)1 << 26);
@@ -1238,7 +1228,7 @@ bool carve_flush(Arena& a, const FlushShape& sh, FlushBufs& b)
         FlushBufs x = b;
         x.span_cap = spans;
         x.seg_cap = spans * 3 / 2;
-        if ((!expand || t.take(x.spantri, x.span_cap)) && t.take(x.bstart, x.span_cap / 64 + 2) &&
+        if (t.take(x.bstart, x.span_cap / 64 + 2) &&
             t.take(x.spans, x.span_cap) && t.take(x.segs, x.seg_cap) &&
             t.take(x.ck, (size_t)x.seg_cap * sh.nvs) && t.take(x.runs, (size_t)x.seg_cap * 2) &&
             t.take(x.runs2, (size_t)x.seg_cap * 2)) {
@@ -1323,7 +1313,7 @@ int psgpu_destroy(psgpu_ctx* c)
     for (auto& t : c->tex) fr(t.d);
     for (auto& v : c->vbo) fr(v.d);
     fr(c->d_blob.p); fr(c->fb.tris.p); fr(c->fb.vary.p); fr(c->fb.nrows.p); fr(c->fb.rowoff.p); fr(c->fb.partial.p);
-    fr(c->fb.spantri.p); fr(c->fb.bstart.p); fr(c->fb.spans.p); fr(c->fb.tcnt.p); fr(c->fb.tstart.p); fr(c->fb.tpartial.p);
+    fr(c->fb.bstart.p); fr(c->fb.spans.p); fr(c->fb.tcnt.p); fr(c->fb.tstart.p); fr(c->fb.tpartial.p);
     fr(c->fb.runs.p); fr(c->fb.runs2.p); fr(c->fb.heavy.p); fr(c->fb.winner.p);
     fr(c->fb.segs.p); fr(c->fb.ck.p); fr(c->fb.grad.p);
     fr(c->dbg_last); fr(c->dbg_rcount); fr(c->dbg_rhash); fr(c->dbg_pcount); fr(c->dbg_phash);

@@ -256,10 +256,11 @@ def test_random_scene_sweep(oracle, seed):
     assert_same(sc, oracle, tiling=(cw, tr, wv))
 
 
-def test_large_flush_expands_rows(oracle):
-    """a flush of more than 2^20 triangles takes the row table written by
-    k_expand instead of k_span's search (1.1 M triangles of a dense sphere at
-    320x240: most are sub-pixel and have no rows)"""
+def test_large_flush_of_rowless_triangles(oracle):
+    """a flush of more than 2^20 triangles, most of them sub-pixel and without
+    rows (1.1 M triangles of a dense sphere at 320x240): k_span's search
+    starts from the row scan's table of every 64th row item's triangle and
+    stops at the next entry, however long the runs of row-less triangles"""
     sc = S.sphere_scene(width=320, height=240, tex_size=64)
     sc.meshes["sphere"] = S.uv_sphere(600, 920, 1.0, seed=1)
     sc.name = "uv_sphere_1.1M_320x240"
