@@ -335,9 +335,11 @@ __device__ __forceinline__ int ps_find_draw_wave(const PsDraw* __restrict__ draw
 }
 
 // row_part (flushes of one programme): the block's row total, for the scan
-// of the row counts
+// of the row counts.  At least three waves per SIMD: a 1080p frame's 200k
+// triangles make only about three per SIMD, and the bound of four left the
+// Blinn-Phong instantiation at 128 VGPRs with a 12-byte spill.
 template <int PROG, bool SHARE>
-__global__ void __launch_bounds__(128, 4) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
+__global__ void __launch_bounds__(128, 3) k_setup(const PsDraw* __restrict__ draws, int nd, int64_t T, PsTri* __restrict__ tris,
                                                float* __restrict__ vary, int nvs, float* __restrict__ grads,
                                                uint32_t* __restrict__ nrows, uint32_t* __restrict__ row_part,
                                                PsCounters* cnt, const PsView* __restrict__ views)
