@@ -1262,7 +1262,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                         sg.cst = sp.cf2_step;
                         sg.pst = sp.pz_step;
                         sg.y = y;
-                        sg.pad = 0;
+                        sg.win = need_ck && xa > sp.x1 ? (s | PS_SEG_CK) : (uint32_t)sp.tri;
                         segs[s] = sg;
                         if (need_ck && xa > sp.x1) {
                             float* c = ck + (int64_t)s * f.ck_stride;
@@ -1454,7 +1454,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
     uint32_t* wl = lds + (size_t)wave * RA_WORDS(KCW ? 0 : CW);
     float* dep = (float*)wl;                       // LDS columns: chunks wider than 8 only
     int* win = (int*)(dep + (KCW ? 0 : CW) * 64);
-    uint32_t* r_s = (uint32_t*)(win + (KCW ? 0 : CW) * 64);   // segment index
+    uint32_t* r_s = (uint32_t*)(win + (KCW ? 0 : CW) * 64);   // winner code (triangle, or checkpoint | piece)
     uint32_t* r_fl = r_s + RA_CHUNK;               // behaviour | writes colour (PsSeg.si high bits)
     uint32_t* r_x = r_fl + RA_CHUNK;               // xa | xb << 16
     float* r_cf2 = (float*)(r_x + RA_CHUNK);
@@ -1587,7 +1587,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
                 uint32_t e = cb + k * 64 + lane;
                 if (e < ce) {
                     uint32_t q = loff[key[k]] + rank[k];
-                    r_s[q] = v[k];
+                    r_s[q] = rb[k].w;
                     r_fl[q] = ra[k].x & ~PS_SEG_SPAN_MASK;
                     r_x[q] = ra[k].y;
                     r_cf2[q] = __uint_as_float(ra[k].z);
@@ -1600,7 +1600,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
             if (inside) {
                 uint32_t l0 = loff[lane], l1 = l0 + lcnt[lane];
                 for (uint32_t q = l0; q < l1; q++) {
-                    int sidx = (int)r_s[q];
+                    const uint32_t wcode = r_s[q];
                     uint32_t fl = r_fl[q];
                     int beh = (fl >> 26) & 3;
                     bool wc = (fl >> 28) & 1;
@@ -1611,8 +1611,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
                     uint32_t th = 0;
                     int tid = 0;
                     if (DEBUG) {
-                        int si = (int)(segs[sidx].si & PS_SEG_SPAN_MASK);
-                        int tri = spans[si].tri;
+                        int tri = (wcode & PS_SEG_CK) ? spans[segs[wcode & ~PS_SEG_CK].si & PS_SEG_SPAN_MASK].tri : (int)wcode;
                         const PsTri tr = tris[tri];
                         const PsDraw& d = draws[tr.code >> 8];
                         tid = (int)(d.id_base + (tri - d.tri_start));
@@ -1638,7 +1637,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
                             if (-1.0f < z && (z - cur < -0.0001f)) {
                                 passes++;
                                 if (beh & PSGPU_UPDATE_DEPTH) { dv[k] = z; dirty = true; }
-                                if (wc) wv[k] = sidx;
+                                if (wc) wv[k] = (int)wcode;
                                 if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
                             }
                         }
@@ -1657,7 +1656,7 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
                             if (-1.0f < z && (z - cur < -0.0001f)) {
                                 passes++;
                                 if (beh & PSGPU_UPDATE_DEPTH) { dep[c * 64 + lane] = z; dirty = true; }
-                                if (wc) win[c * 64 + lane] = sidx;
+                                if (wc) win[c * 64 + lane] = (int)wcode;
                                 if (DEBUG) { f.dbg_pcount[pix] += 1; f.dbg_phash[pix] ^= th; f.dbg_last[pix] = tid; }
                             }
                         }
@@ -1937,6 +1936,10 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
 // ---------------------------------------------------------------------------
 // shade: deferred FP of the last colour-writing pass per pixel
 // ---------------------------------------------------------------------------
+// The winner plane names the triangle, so most pixels go winner -> triangle
+// -> (draw, varyings) and re-derive the span's start (x1, cf2) from the
+// triangle exactly as k_span did; only pixels of a piece with a varying
+// checkpoint read that piece and its span first.
 template <int PROG>
 __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spans, const PsSeg* __restrict__ segs,
                                                const float* __restrict__ ck, const PsTri* __restrict__ tris,
@@ -1955,13 +1958,23 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
     const int64_t crow = V.colour_top_down ? (int64_t)(V.colour_h - 1 - y) : (int64_t)y;
     int w = V.winner[(int64_t)y * V.W + x];
     if (poisoned) return;
-    if (w < 0) {   // no fragment colours the pixel: a fused clear of its row, if any, is all it gets
+    if (w == -1) {   // no fragment colours the pixel: a fused clear of its row, if any, is all it gets
         if (clears && crow < V.cclear_rows) V.colour[crow * V.colour_scan + x] = V.cclear_value;
         return;
     }
-    const PsSeg sg = segs[w];
-    const PsSpan sp = spans[sg.si & PS_SEG_SPAN_MASK];
-    const PsTri tr = tris[sp.tri];
+    // a piece with a checkpoint: its first pixel xa carries the accumulators
+    // as the serial walk from x1 leaves them
+    const bool from_ck = (uint32_t)w & PS_SEG_CK;
+    const uint32_t wi = (uint32_t)w & ~PS_SEG_CK;
+    int tri = (int)wi, xa = 0;
+    float cfa = 0.0f;
+    if (from_ck) {
+        const PsSeg sg = segs[wi];
+        tri = spans[sg.si & PS_SEG_SPAN_MASK].tri;
+        xa = (int)(sg.xab & 0xffffu);
+        cfa = sg.cf2;
+    }
+    const PsTri tr = tris[tri];
     ps_with_draw(draws, (int)(tr.code >> 8), [&](const PsDraw& d) {
     if (d.prog != PROG) return;
     PsEdge L, R;
@@ -1970,23 +1983,23 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
     int right = ps_cvt_i32(ps_edge_eval(R, (float)y) + 0.5f);
     PsSpanGeom g;
     ps_span_geom(tr, y, left, right, L, R, g);
-    const float* u0 = vary + (int64_t)ps_vary_index(d, sp.tri) * 3 * nvs;
+    float cf2, cf2_step, pz, pz_step;
+    ps_span_vis(tr, g, cf2, cf2_step, pz, pz_step);
+    const float* u0 = vary + (int64_t)ps_vary_index(d, tri) * 3 * nvs;
     float vs[P::NV > 0 ? P::NV : 1], vt[P::NV > 0 ? P::NV : 1];
     ps_span_varyings<P::NV>(g, u0, u0 + nvs, u0 + 2 * nvs, vs, vt);
-    // the piece's first pixel carries the accumulators as the serial walk from
-    // x1 leaves them (checkpoint); walk on from there to the pixel
-    int xs = sp.x1;
-    float cf2 = sp.cf2;
-    int xa = (int)(sg.xab & 0xffffu);
-    if (xa > sp.x1 && ps_span_has_ck(sp)) {
-        const float* c = ck + (int64_t)w * f.ck_stride;
+    // from the span's first pixel (leftClamped), or from the checkpoint,
+    // the serial walk on to the pixel
+    int xs = left < 0 ? 0 : left;
+    if (from_ck) {
+        const float* c = ck + (int64_t)wi * f.ck_stride;
 #pragma unroll
         for (int i = 0; i < P::NV; i++) vs[i] = c[i];
         xs = xa;
-        cf2 = sg.cf2;
+        cf2 = cfa;
     }
     for (int k = xs; k < x; k++) {
-        cf2 += sp.cf2_step;
+        cf2 += cf2_step;
 #pragma unroll
         for (int i = 0; i < P::NV; i++) vs[i] = vs[i] + vt[i];
     }
@@ -2004,7 +2017,7 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
     }
     float grad[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (P::GRAD) {
-        const float4* gi = (const float4*)(grads + (int64_t)sp.tri * 8);
+        const float4* gi = (const float4*)(grads + (int64_t)tri * 8);
         float4 g0 = gi[0], g1 = gi[1];
         grad[0] = g0.x; grad[1] = g0.y; grad[2] = g0.z; grad[3] = g0.w; grad[4] = g1.x; grad[5] = g1.y;
     }

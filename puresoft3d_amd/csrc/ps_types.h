@@ -86,14 +86,17 @@ struct PsSpan {
 // The piece of a span inside one bin chunk, with the serial accumulators
 // (correction factor, depth numerator) as they stand at its first pixel xa.
 // si packs the span index (bits 0-25), the depth behaviour bits (26-27) and
-// whether the draw writes colour (28).
+// whether the draw writes colour (28).  win is what the winner plane keeps
+// for the piece: the span's triangle, or PS_SEG_CK | the piece's own index
+// when the piece keeps a varying checkpoint.
 #define PS_SEG_SPAN_MASK 0x3ffffffu
+#define PS_SEG_CK 0x80000000u
 struct PsSeg {
     uint32_t si;
     uint32_t xab;   // xa | xb << 16 (first and last pixel of the piece)
     float cf2, pz, cst, pst;
     int y;          // row
-    uint32_t pad;
+    uint32_t win;
 };
 
 // One view of a flush: a viewport and its colour/depth targets.  A flush
@@ -107,7 +110,10 @@ struct PsView {
     int64_t stile_base;             // first shading block of the view
     uint32_t* colour; int colour_scan; int colour_top_down; int colour_h;  // scan in uint32
     float* depth; int depth_scan;   // scan in floats
-    int* winner;                    // W*H: segment index of the last colour-writing pass
+    // W*H: what shading needs of the last colour-writing pass: its triangle,
+    // or PS_SEG_CK | its piece when the piece keeps a varying checkpoint;
+    // -1 for none
+    int* winner;
     float* gbuf;                    // optional W*H*12 floats: N.xyz 1 | wp.xyz 0 | u v 0 0 of the winner
     // clears that precede the flush's draws, applied by the raster and shade
     // kernels instead of a fill: the whole depth buffer (W x H, no padding)

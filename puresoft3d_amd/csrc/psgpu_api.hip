@@ -210,9 +210,10 @@ struct psgpu_ctx {
     FlushBufs fb;
     bool fixed_scratch = false;   // fb is carved from the static array: never reallocate it
     // most spans and pieces one flush may hold (pieces keep their span's index
-    // in 26 bits, run records the index of its wave of 64 below 2^20 - 1); a
+    // in 26 bits, run records the index of its wave of 64 below 2^20 - 1, the
+    // winner plane a piece's index in 31 bits, all ones meaning none); a
     // flush that needs more is split in two and re-run
-    int64_t max_spans = ((int64_t)1 << 26) - 64, max_segs = (int64_t)1 << 31;
+    int64_t max_spans = ((int64_t)1 << 26) - 64, max_segs = ((int64_t)1 << 31) - 64;
     uint32_t* d_sizes = nullptr;
     uint32_t* h_sizes = nullptr;
     // per logged command: the overflow flag as it stood after the command ran
