@@ -126,24 +126,29 @@ PS_DEV PsEdge ps_edge(const float* vx, const float* vy, int i0, int i1)
 PS_DEV float ps_edge_eval(const PsEdge& e, float y) { return e.dx * (y - e.y0) / e.dy + e.x0; }
 
 // Left/right edges of row `row` of a set-up triangle (processTriangle calls of
-// pushTriangle / processStandingTriangle, rasterizer.cpp:170-280).
+// pushTriangle / processStandingTriangle, rasterizer.cpp:170-280).  The case
+// analysis picks vertex indices only (selects, no branches), and each edge
+// is then built once: with an edge built per case the compiler emitted
+// divergent branches that moved every field of every case.
 PS_DEV void ps_row_edges(const PsTri& t, int row, PsEdge& L, PsEdge& R)
 {
-    uint32_t c = t.code;
-    int a = c & 3, b = (c >> 2) & 3, th = (c >> 4) & 3;
-    if (!(c & 64u)) {                 // flat top/bottom: (a, apex) and (b, apex)
-        L = ps_edge(t.vx, t.vy, a, th);
-        R = ps_edge(t.vx, t.vy, b, th);
-        return;
+    const uint32_t c = t.code;
+    const int a = c & 3, b = (c >> 2) & 3, th = (c >> 4) & 3;
+    const bool standing = (c & 64u) != 0;
+    const bool left = (c & 128u) != 0;    // standing: a = top, b = bottom, th = third
+    const bool lower = row <= t.split;    // standing: the lower half, written last
+    // flat top/bottom: (a, apex) and (b, apex)
+    // standing, lower half: left ? (th, b) | (a, b) : (a, b) | (th, b)
+    // standing, upper half: left ? (a, th) | (a, b) : (a, b) | (a, th)
+    int l0 = a, l1 = th, r0 = b, r1 = th;
+    if (standing) {
+        l0 = lower ? (left ? th : a) : a;
+        l1 = lower ? b : (left ? th : b);
+        r0 = lower ? (left ? a : th) : a;
+        r1 = lower ? b : (left ? b : th);
     }
-    bool left = (c & 128u) != 0;      // a = top, b = bottom, th = third
-    if (row <= t.split) {             // lower half, written last
-        if (left) { L = ps_edge(t.vx, t.vy, th, b); R = ps_edge(t.vx, t.vy, a, b); }
-        else      { L = ps_edge(t.vx, t.vy, a, b);  R = ps_edge(t.vx, t.vy, th, b); }
-    } else {                          // upper half
-        if (left) { L = ps_edge(t.vx, t.vy, a, th); R = ps_edge(t.vx, t.vy, a, b); }
-        else      { L = ps_edge(t.vx, t.vy, a, b);  R = ps_edge(t.vx, t.vy, a, th); }
-    }
+    L = ps_edge(t.vx, t.vy, l0, l1);
+    R = ps_edge(t.vx, t.vy, r0, r1);
 }
 
 // ---- interpolater (interp.cpp) ----------------------------------------
@@ -158,7 +163,9 @@ PS_DEV void ps_line_contrib(const float* vx, const float* vy, int v1, int v2, fl
 {
     float x1 = ps_sel3(vx, v1), x2 = ps_sel3(vx, v2), y1 = ps_sel3(vy, v1), y2 = ps_sel3(vy, v2);
     float dx = x1 - x2, dy = y1 - y2;
-    float cv = fabsf(dx) > fabsf(dy) ? ((x - x2) / dx) : ((y - y2) / dy);
+    // one division of the chosen operands
+    const bool by_x = fabsf(dx) > fabsf(dy);
+    float cv = (by_x ? (x - x2) : (y - y2)) / (by_x ? dx : dy);
     float o = 1.0f - cv;
     // c[v1] = cv, c[v2] = 1 - cv, c[other] = 0
     c[0] = (v1 == 0) ? cv : ((v2 == 0) ? o : 0.0f);
