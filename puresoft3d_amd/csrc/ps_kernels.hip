@@ -186,15 +186,20 @@ struct PsSpanTiles {
     int cx0, cx1, tx0;
 };
 
+// The chunk width, tile rows and tile columns are powers of two
+// (psgpu_tiling_set) and y, x1, x2 are not negative, so the divisions are
+// shifts (a division by a run-time divisor costs a dozen instructions).
 __device__ __forceinline__ PsSpanTiles ps_span_tiles(int64_t tile_base, int n_tiles_x, int tile_rows, int tile_cols,
                                                      int chunk_w, int y, int x1, int x2)
 {
+    const int cs = __builtin_ctz((unsigned)chunk_w), ts = __builtin_ctz((unsigned)tile_cols);
+    const int rs = __builtin_ctz((unsigned)tile_rows);
     PsSpanTiles s;
-    s.cx0 = x1 / chunk_w;
-    s.cx1 = x2 / chunk_w;
-    s.tx0 = s.cx0 / tile_cols;
-    s.g0 = (uint32_t)(tile_base + (int64_t)(y / tile_rows) * n_tiles_x + s.tx0);
-    s.g1 = s.g0 + (uint32_t)(s.cx1 / tile_cols - s.tx0);
+    s.cx0 = x1 >> cs;
+    s.cx1 = x2 >> cs;
+    s.tx0 = s.cx0 >> ts;
+    s.g0 = (uint32_t)(tile_base + (int64_t)(y >> rs) * n_tiles_x + s.tx0);
+    s.g1 = s.g0 + (uint32_t)((s.cx1 >> ts) - s.tx0);
     return s;
 }
 
@@ -1098,16 +1103,6 @@ __device__ __forceinline__ bool ps_span_has_ck(const PsSpan& sp) { return sp.x2 
 // ---------------------------------------------------------------------------
 // segments: (bin key, span) pairs
 // ---------------------------------------------------------------------------
-// tile of a (row, chunk) bin, and the bin's lane inside the tile's wave
-__device__ __forceinline__ uint32_t ps_bin_tile(const PsFrame& f, const PsView& v, int y, int cx)
-{
-    return (uint32_t)(v.tile_base + (y / f.tile_rows) * v.n_tiles_x + cx / f.tile_cols);
-}
-__device__ __forceinline__ uint32_t ps_bin_lane(const PsFrame& f, int y, int cx)
-{
-    return (uint32_t)((y % f.tile_rows) * f.tile_cols + cx % f.tile_cols);
-}
-
 // One thread per span: the span's pieces per bin chunk, in order.  The walk
 // from x1 applies the serial `+= step` of the pixel loop (interp.cpp:82-92)
 // pixel by pixel, so each piece starts with exactly the accumulators the
@@ -1242,7 +1237,7 @@ __global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spa
                 const uint32_t run0 = __shfl(first, jj < 0 ? 0 : jj, PS_WAVE);
                 if (q < ntb && run0 != 0xffffffffu) {
                     uint32_t s = run0 + rel[jj][lane];
-                    const int hi = min(st.cx1, (cx / TC) * TC + TC - 1);   // the tile's last chunk of the span
+                    const int hi = min(st.cx1, cx | (TC - 1));   // the tile's last chunk of the span (cx >= 0, TC a power of two)
                     for (; cx <= hi; cx++, s++) {
                         const int xa = max(sp.x1, cx * CW), xb = min(sp.x2, cx * CW + CW - 1);
                         for (; x < xa; x++) {
@@ -1467,8 +1462,10 @@ __global__ void __launch_bounds__(512, 5) k_raster(const PsSpan* __restrict__ sp
     const int cw_shift = __builtin_ctz((unsigned)CW), tc_shift = __builtin_ctz((unsigned)f.tile_cols);
     const PsView& V = f.views[ps_view_of_tile(f, tile)];
     int64_t ltile = tile - V.tile_base;
-    int ty = (int)(ltile / V.n_tiles_x), tx = (int)(ltile - (int64_t)ty * V.n_tiles_x);
-    int ry = lane / f.tile_cols, rx = lane - ry * f.tile_cols;
+    // a view has fewer than 2^31 tiles; tile_cols is a power of two
+    const uint32_t lt32 = (uint32_t)ltile, ntx = (uint32_t)V.n_tiles_x;
+    int ty = (int)(lt32 / ntx), tx = (int)(lt32 - (uint32_t)ty * ntx);
+    int ry = lane >> tc_shift, rx = lane & (f.tile_cols - 1);
     int y = ty * f.tile_rows + ry;
     int cx = tx * f.tile_cols + rx;
     int X0 = cx * CW;
@@ -1720,8 +1717,10 @@ __global__ void __launch_bounds__(512) k_raster_imm(const PsSpan* __restrict__ s
     const int cw_shift = __builtin_ctz((unsigned)CW), tc_shift = __builtin_ctz((unsigned)f.tile_cols);
     const PsView& V = f.views[ps_view_of_tile(f, tile)];
     int64_t ltile = tile - V.tile_base;
-    int ty = (int)(ltile / V.n_tiles_x), tx = (int)(ltile - (int64_t)ty * V.n_tiles_x);
-    int ry = lane / f.tile_cols, rx = lane - ry * f.tile_cols;
+    // a view has fewer than 2^31 tiles; tile_cols is a power of two
+    const uint32_t lt32 = (uint32_t)ltile, ntx = (uint32_t)V.n_tiles_x;
+    int ty = (int)(lt32 / ntx), tx = (int)(lt32 - (uint32_t)ty * ntx);
+    int ry = lane >> tc_shift, rx = lane & (f.tile_cols - 1);
     int y = ty * f.tile_rows + ry;
     int cx = tx * f.tile_cols + rx;
     int X0 = cx * CW;
@@ -1951,7 +1950,9 @@ __global__ void __launch_bounds__(256, 7) k_shade(const PsSpan* __restrict__ spa
     const int64_t lb = ps_xcd_remap<4>(blockIdx.x, gridDim.x);
     const PsView& V = f.views[ps_view_of_stile(f, lb)];
     int64_t st = lb - V.stile_base;
-    int sy = (int)(st / V.n_stiles_x), sx = (int)(st - (int64_t)sy * V.n_stiles_x);
+    // a view has fewer than 2^31 shading blocks
+    const uint32_t st32 = (uint32_t)st, nsx = (uint32_t)V.n_stiles_x;
+    int sy = (int)(st32 / nsx), sx = (int)(st32 - (uint32_t)sy * nsx);
     int x = sx * 16 + (threadIdx.x & 15);
     int y = sy * 16 + (threadIdx.x >> 4);
     if (x >= V.W || y >= V.H) return;
