@@ -1009,6 +1009,7 @@ __device__ __forceinline__ int ps_row_tri(const uint32_t* __restrict__ row_off, 
 // ---------------------------------------------------------------------------
 // span setup
 // ---------------------------------------------------------------------------
+#define SPAN_BINS 512
 // Also counts, per tile, the span pieces in it and the waves (of 64 span
 // items) that have pieces there: tcnt[t] += pieces | waves << 32.  Eight
 // waves per SIMD (64 VGPRs, no spill): at seven the 256-view batch's span
@@ -1020,6 +1021,10 @@ __global__ void __launch_bounds__(256, 8) k_span(const PsTri* __restrict__ tris,
                                               const PsView* __restrict__ views, int n_views, int chunk_w, int tile_rows,
                                               int tile_cols, uint32_t* heavy, PsCounters* cnt)
 {
+    // per wave: piece counts of the tiles [kmin, kmin + SPAN_BINS) its spans touch
+    __shared__ uint32_t bins_all[4][SPAN_BINS];
+    uint32_t* bins = bins_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & (PS_WAVE - 1);
     const bool poisoned = ps_poisoned(cnt);
     const int64_t R = row_off[T];
     // the only view of a one-view flush, read once (otherwise each triangle's)
@@ -1075,15 +1080,46 @@ __global__ void __launch_bounds__(256, 8) k_span(const PsTri* __restrict__ tris,
             }
             spans[i] = sp;
         }
-        // per tile of the wave's spans: its pieces there and one run (lane j
-        // adds the j-th tile's counts, so a batch of tiles is one atomic)
+        // per tile of the wave's spans: its pieces there and one run.  When
+        // the wave's tiles lie within SPAN_BINS of each other (about nine
+        // tiles of one or two tile rows, typically), the lanes add their
+        // pieces into LDS bins and one atomic per touched bin follows;
+        // otherwise the wave visits its tiles in rounds (lane j adds the
+        // j-th tile's counts, so a batch of tiles is one atomic).
         const uint32_t npieces = has ? (uint32_t)(st.cx1 - st.cx0 + 1) : 0u;
-        uint32_t cur = has ? st.g0 : 0xffffffffu;
-        for (;;) {
-            uint32_t bk, bn;
-            const int nb = ps_wave_tile_batch(st, tile_cols, cur, bk, bn);
-            if ((int)(threadIdx.x & (PS_WAVE - 1)) < nb) atomicAdd(&tcnt[bk], (unsigned long long)bn | (1ull << 32));
-            if (nb < PS_WAVE) break;
+        const uint32_t kmin = ps_wave_min(has ? st.g0 : 0xffffffffu);
+        const uint32_t kmax = ps_wave_max(has ? st.g1 : 0u);
+        if (kmin != 0xffffffffu && kmax - kmin < (uint32_t)SPAN_BINS) {
+            const uint32_t nbin = kmax - kmin + 1;
+            for (uint32_t b = lane; b < nbin; b += PS_WAVE) bins[b] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (has)
+                for (uint32_t k = st.g0; k <= st.g1; k++) {
+                    int lo, hi;
+                    ps_span_tile_chunks(st, tile_cols, k, lo, hi);
+                    atomicAdd(&bins[k - kmin], (uint32_t)(hi - lo + 1));
+                }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t b = lane; b < nbin; b += PS_WAVE) {
+                const uint32_t c = bins[b];
+                if (c) atomicAdd(&tcnt[kmin + b], (unsigned long long)c | (1ull << 32));
+            }
+            // the next iteration clears the bins only after every lane read them
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            uint32_t cur = has ? st.g0 : 0xffffffffu;
+            for (;;) {
+                uint32_t bk, bn;
+                const int nb = ps_wave_tile_batch(st, tile_cols, cur, bk, bn);
+                if (lane < nb) atomicAdd(&tcnt[bk], (unsigned long long)bn | (1ull << 32));
+                if (nb < PS_WAVE) break;
+            }
         }
         all_spans += nspan;
         all_pieces += npieces;
