@@ -1156,9 +1156,11 @@ __device__ __forceinline__ bool ps_span_has_ck(const PsSpan& sp) { return sp.x2 
 // floats); the accumulators live in registers across the walk, so the
 // narrowest instantiation that fits keeps more waves resident (a 1080p
 // Blinn-Phong flush: 117 VGPRs and four waves per SIMD at NVC 14 against
-// 149 and three at 20, pieces 29.2 -> 24.8 us)
+// 149 and three at 20, pieces 29.2 -> 24.8 us).  Up to NVC 14 the kernel
+// fits 96 VGPRs without spilling, five waves per SIMD (23.6 -> 22.4 us);
+// NVC 20 keeps its registers (a bound of four waves spilled).
 template <int NVC>
-__global__ void __launch_bounds__(256) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ row_off,
+__global__ void __launch_bounds__(256, NVC <= 14 ? 5 : 1) k_segments(const PsSpan* __restrict__ spans, const uint32_t* __restrict__ row_off,
                                                   int64_t T, PsFrame f, const PsTri* __restrict__ tris,
                                                   const PsDraw* __restrict__ draws, const float* __restrict__ vary, int nvs,
                                                   unsigned long long* __restrict__ tcnt, PsSeg* __restrict__ segs,
